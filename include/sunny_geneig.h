@@ -1,0 +1,132 @@
+/* C ABI of libsunny_geneig.so: batched generalized Hermitian-definite
+ * eigensolve H u = lambda S u on AMD Instinct MI355X (gfx950).
+ *
+ * Drop-in for the reference's src/common/c_api.h: the first eleven entry
+ * points and the four enums keep the reference's names, signatures, values
+ * and semantics (each declaration cites the line it replaces). Device
+ * pointers and the stream cross the boundary as uintptr_t: a HIP device
+ * pointer and a hipStream_t (0 = default stream).
+ *
+ * All functions are nothrow. A failure returns a status (or NULL) and leaves
+ * a message in sunny_geneig_last_error() (thread-local).
+ *
+ * Buffers (caller-owned, column-major, element (i,j) of matrix b at
+ * b*n*n + j*n + i):
+ *   d_H, d_S : complex128 [batch][n][n], Hermitian; read only.
+ *   d_W      : float64    [batch][n], eigenvalues ascending.
+ *   d_U      : complex128 [batch][n][n], eigenvectors, U^H S U = I.
+ *   d_info   : int32      [batch]; 0 = success, n + k = leading minor k of S
+ *              not positive definite (zhegv INFO), 1..n-1 = QL did not
+ *              converge (that many off-diagonals left).
+ */
+#ifndef SUNNY_GENEIG_H
+#define SUNNY_GENEIG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sunny_geneig_solver* sunny_geneig_solver_t;
+
+/* Reference: src/common/c_api.h:24-30. */
+typedef enum {
+    SUNNY_GENEIG_OK            =  0,
+    SUNNY_GENEIG_INVALID_ARG   = -1,
+    SUNNY_GENEIG_DEVICE_ERROR  = -2,
+    SUNNY_GENEIG_COMPILE_ERROR = -3,
+    SUNNY_GENEIG_LAUNCH_ERROR  = -4
+} sunny_geneig_status_t;
+
+/* Reference: src/common/c_api.h:32-36. AUTO = in-process + on-disk code-object
+ * cache; NODISK = in-process only; NOCACHE = compile every time. */
+typedef enum {
+    SUNNY_GENEIG_CACHE_AUTO    = 0,
+    SUNNY_GENEIG_CACHE_NODISK  = 1,
+    SUNNY_GENEIG_CACHE_NOCACHE = 2
+} sunny_geneig_cache_mode_t;
+
+/* Reference: src/common/c_api.h:38-41. SUGGESTED = built-in workgroup size
+ * model; AUTOTUNE = timed, correctness-gated sweep (winner persisted). */
+typedef enum {
+    SUNNY_GENEIG_TUNING_SUGGESTED = 0,
+    SUNNY_GENEIG_TUNING_AUTOTUNE  = 1
+} sunny_geneig_tuning_mode_t;
+
+/* Reference: src/common/c_api.h:43-46. UNIFIED = one fused kernel per batch;
+ * HYBRID = split mode: reduce kernel, batched heev kernel, batched back-trsm
+ * kernel (all written for gfx950 by this project), captured in a hipGraph. */
+typedef enum {
+    SUNNY_GENEIG_KERNEL_UNIFIED = 0,
+    SUNNY_GENEIG_KERNEL_HYBRID  = 1
+} sunny_geneig_kernel_mode_t;
+
+/* Reference: src/common/c_api.h:50-55 (c_api.cpp:76-107). Returns NULL on
+ * failure. batches_per_block_request <= 0 selects automatically. */
+sunny_geneig_solver_t sunny_geneig_create(
+    int                         n,
+    int                         device_id,
+    int                         batches_per_block_request,
+    sunny_geneig_tuning_mode_t  tuning_mode,
+    sunny_geneig_cache_mode_t   cache_mode);
+
+/* Reference: src/common/c_api.h:57-69 (c_api.cpp:109-137). */
+sunny_geneig_solver_t sunny_geneig_create_ex(
+    int                         n,
+    int                         device_id,
+    int                         batches_per_block_request,
+    sunny_geneig_tuning_mode_t  tuning_mode,
+    sunny_geneig_cache_mode_t   cache_mode,
+    sunny_geneig_kernel_mode_t  kernel_mode);
+
+/* Reference: src/common/c_api.h:71-72 (c_api.cpp:139-145). NULL-safe. */
+void sunny_geneig_destroy(sunny_geneig_solver_t solver);
+
+/* Reference: src/common/c_api.h:74-85 (c_api.cpp:147-186). Asynchronous on
+ * `stream`. NULL solver or batch_size < 0 -> INVALID_ARG; batch_size == 0 is a
+ * no-op. */
+sunny_geneig_status_t sunny_geneig_launch(
+    sunny_geneig_solver_t solver,
+    uintptr_t             d_H,
+    uintptr_t             d_S,
+    uintptr_t             d_W,
+    uintptr_t             d_U,
+    uintptr_t             d_info,
+    int                   batch_size,
+    uintptr_t             stream);
+
+/* Reference: src/common/c_api.h:87-91 (c_api.cpp:188-206). -1 / 0 on NULL. */
+int          sunny_geneig_matrix_size      (sunny_geneig_solver_t solver);
+int          sunny_geneig_batches_per_block(sunny_geneig_solver_t solver);
+int          sunny_geneig_block_dim_x      (sunny_geneig_solver_t solver);
+unsigned int sunny_geneig_shared_mem_bytes (sunny_geneig_solver_t solver);
+
+/* Reference: src/common/c_api.h:93-95 (c_api.cpp:208-210). Never NULL. */
+const char*  sunny_geneig_last_error(void);
+
+/* Reference: src/common/c_api.h:97-101 (c_api.cpp:212-220). "unified",
+ * "hybrid", or "" for NULL; kernel_mode returns -1 for NULL. */
+const char*                sunny_geneig_path_name  (sunny_geneig_solver_t solver);
+sunny_geneig_kernel_mode_t sunny_geneig_kernel_mode(sunny_geneig_solver_t solver);
+
+/* ---- additive entry points (no counterpart in the reference ABI) --------- */
+
+/* Compile (hipRTC) and store in the on-disk cache the code object for order n
+ * with `block_dim_x` threads (0 = suggested) for `arch` (e.g. "gfx950"),
+ * without touching a device. Returns 0 or a status. Lets a build step warm the
+ * cache so the first create on a fresh machine skips compilation. */
+int sunny_geneig_precompile(int n, int block_dim_x, const char* arch);
+
+/* Where the cache layer that served construction came from: "in-process",
+ * "disk" or "compile" (reference: unified_solver.hpp:129-136). "" for NULL. */
+const char* sunny_geneig_cache_layer(sunny_geneig_solver_t solver);
+
+/* Resolved on-disk cache directory (reference: unified_solver.cpp:403-420). */
+const char* sunny_geneig_cache_dir(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SUNNY_GENEIG_H */

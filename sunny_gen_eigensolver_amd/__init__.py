@@ -1,0 +1,228 @@
+"""Python host side of libsunny_geneig.so (MI355X / gfx950).
+
+Mirrors the reference's Julia wrapper SunnyGeneigBench
+(julia/SunnyGeneigBench/src/SunnyGeneigBench.jl): a `Solver(n; device_id,
+bpb_request, tuning, cache, kernel)` object over the C ABI, `run_kernel`
+(run_kernel! at :104-171) and `run_kernel_chunked` (:191-251). Device memory
+and streams are torch tensors / streams; every call goes through the C ABI
+into the HIP kernels. There is no CPU fallback: if the library is missing the
+import of this package raises.
+
+Array convention: a batch of B matrices of order n is a complex128 tensor of
+shape (B, n, n) holding each matrix column-major, i.e. T[b, j, i] = A_b(i, j)
+(the memory layout of the reference's (n, n, B) Julia arrays). W is float64
+(B, n); info is int32 (B,).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("SUNNY_GENEIG_LIBPATH", os.path.join(_PKG, "lib", "libsunny_geneig.so"))
+DEFAULT_CACHE_DIR = os.path.join(_PKG, "_cocache")
+
+# Status / mode constants: values are ABI (include/sunny_geneig.h).
+STATUS_OK = 0
+STATUS_INVALID_ARG = -1
+STATUS_DEVICE_ERROR = -2
+STATUS_COMPILE_ERROR = -3
+STATUS_LAUNCH_ERROR = -4
+TUNING_SUGGESTED = 0
+TUNING_AUTOTUNE = 1
+CACHE_AUTO = 0
+CACHE_NODISK = 1
+CACHE_NOCACHE = 2
+KERNEL_UNIFIED = 0
+KERNEL_HYBRID = 1
+
+EXPORTED_SYMBOLS = (
+    "sunny_geneig_create", "sunny_geneig_create_ex", "sunny_geneig_destroy",
+    "sunny_geneig_launch", "sunny_geneig_matrix_size", "sunny_geneig_batches_per_block",
+    "sunny_geneig_block_dim_x", "sunny_geneig_shared_mem_bytes", "sunny_geneig_last_error",
+    "sunny_geneig_path_name", "sunny_geneig_kernel_mode", "sunny_geneig_precompile",
+    "sunny_geneig_cache_layer", "sunny_geneig_cache_dir",
+)
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libsunny_geneig.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libsunny_geneig.so not found at {LIB_PATH}; run "
+                          "`python -m sunny_gen_eigensolver_amd.build` first")
+    # The in-tree code-object cache ships warm; use it unless told otherwise.
+    os.environ.setdefault("SUNNY_GENEIG_CACHE_DIR", DEFAULT_CACHE_DIR)
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ci, cu = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64
+    L.sunny_geneig_create.argtypes = [ci, ci, ci, ci, ci]
+    L.sunny_geneig_create.restype = vp
+    L.sunny_geneig_create_ex.argtypes = [ci, ci, ci, ci, ci, ci]
+    L.sunny_geneig_create_ex.restype = vp
+    L.sunny_geneig_destroy.argtypes = [vp]
+    L.sunny_geneig_destroy.restype = None
+    L.sunny_geneig_launch.argtypes = [vp, cu, cu, cu, cu, cu, ci, cu]
+    L.sunny_geneig_launch.restype = ci
+    for name in ("sunny_geneig_matrix_size", "sunny_geneig_batches_per_block",
+                 "sunny_geneig_block_dim_x", "sunny_geneig_kernel_mode"):
+        getattr(L, name).argtypes = [vp]
+        getattr(L, name).restype = ci
+    L.sunny_geneig_shared_mem_bytes.argtypes = [vp]
+    L.sunny_geneig_shared_mem_bytes.restype = ctypes.c_uint
+    L.sunny_geneig_last_error.argtypes = []
+    L.sunny_geneig_last_error.restype = ctypes.c_char_p
+    L.sunny_geneig_path_name.argtypes = [vp]
+    L.sunny_geneig_path_name.restype = ctypes.c_char_p
+    L.sunny_geneig_precompile.argtypes = [ci, ci, ctypes.c_char_p]
+    L.sunny_geneig_precompile.restype = ci
+    L.sunny_geneig_cache_layer.argtypes = [vp]
+    L.sunny_geneig_cache_layer.restype = ctypes.c_char_p
+    L.sunny_geneig_cache_dir.argtypes = []
+    L.sunny_geneig_cache_dir.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return lib().sunny_geneig_last_error().decode()
+
+
+class GeneigError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"status {status}: {msg}")
+        self.status = status
+
+
+class Solver:
+    """Solver(n; device_id=0, bpb_request=0, tuning='suggested', cache='auto',
+    kernel='unified') -- SunnyGeneigBench.jl:37-57."""
+
+    def __init__(self, n: int, device_id: int = 0, bpb_request: int = 0,
+                 tuning: str = "suggested", cache: str = "auto", kernel: str = "unified"):
+        L = lib()
+        t = TUNING_AUTOTUNE if tuning == "autotune" else TUNING_SUGGESTED
+        c = {"nodisk": CACHE_NODISK, "nocache": CACHE_NOCACHE}.get(cache, CACHE_AUTO)
+        k = KERNEL_HYBRID if kernel == "hybrid" else KERNEL_UNIFIED
+        h = L.sunny_geneig_create_ex(n, device_id, bpb_request, t, c, k)
+        if not h:
+            raise GeneigError(STATUS_INVALID_ARG, f"sunny_geneig_create_ex failed: {last_error()}")
+        self.handle = h
+        self.n = n
+        self.kernel = kernel
+        self.device_id = device_id
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().sunny_geneig_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # Introspection (SunnyGeneigBench.jl:69-92).
+    @property
+    def block_dim_x(self) -> int:
+        return lib().sunny_geneig_block_dim_x(self.handle)
+
+    @property
+    def batches_per_block(self) -> int:
+        return lib().sunny_geneig_batches_per_block(self.handle)
+
+    @property
+    def shared_mem_bytes(self) -> int:
+        return lib().sunny_geneig_shared_mem_bytes(self.handle)
+
+    @property
+    def path_name(self) -> str:
+        return lib().sunny_geneig_path_name(self.handle).decode()
+
+    @property
+    def kernel_mode(self) -> str:
+        return "hybrid" if lib().sunny_geneig_kernel_mode(self.handle) == KERNEL_HYBRID else "unified"
+
+    @property
+    def cache_layer(self) -> str:
+        return lib().sunny_geneig_cache_layer(self.handle).decode()
+
+    def launch_raw(self, d_H: int, d_S: int, d_W: int, d_U: int, d_info: int, batch: int,
+                   stream: int = 0) -> None:
+        st = lib().sunny_geneig_launch(self.handle, d_H, d_S, d_W, d_U, d_info, batch, stream)
+        if st != STATUS_OK:
+            raise GeneigError(st, last_error())
+
+
+def _stream_handle(stream) -> int:
+    if stream is None:
+        import torch
+        return int(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return stream
+    return int(stream.cuda_stream)
+
+
+def run_kernel(solver: Solver, H, S, W=None, U=None, info=None, stream=None):
+    """run_kernel!(solver, H, S; W, U, info, stream) -- SunnyGeneigBench.jl:104-171.
+
+    H, S: complex128 cuda tensors (B, n, n) in column-major-per-matrix layout.
+    Returns (W, U, info); pass output tensors to reuse them across calls."""
+    import torch
+    batch, n, n2 = H.shape
+    if n != solver.n or n2 != solver.n:
+        raise ValueError(f"matrix size mismatch: solver.n={solver.n}, H={tuple(H.shape)}")
+    if tuple(S.shape) != tuple(H.shape):
+        raise ValueError("S size must match H")
+    for name, t in (("H", H), ("S", S)):
+        if t.dtype != torch.complex128 or not t.is_cuda or not t.is_contiguous():
+            raise ValueError(f"{name} must be a contiguous complex128 device tensor")
+    dev = H.device
+    W = torch.empty((batch, n), dtype=torch.float64, device=dev) if W is None else W
+    U = torch.empty((batch, n, n), dtype=torch.complex128, device=dev) if U is None else U
+    info = torch.zeros((batch,), dtype=torch.int32, device=dev) if info is None else info
+    solver.launch_raw(H.data_ptr(), S.data_ptr(), W.data_ptr(), U.data_ptr(), info.data_ptr(),
+                      batch, _stream_handle(stream))
+    return W, U, info
+
+
+def run_kernel_chunked(solver: Solver, H, S, W, U, info, chunk_size: int, streams):
+    """run_kernel_chunked! (SunnyGeneigBench.jl:191-251): launch contiguous
+    chunks round-robin over `streams`, then synchronise each stream."""
+    batch = H.shape[0]
+    n = solver.n
+    mat = n * n * 16
+    launched = 0
+    s = 0
+    while launched < batch:
+        cnt = min(chunk_size, batch - launched)
+        st = streams[s % len(streams)]
+        solver.launch_raw(H.data_ptr() + launched * mat, S.data_ptr() + launched * mat,
+                          W.data_ptr() + launched * n * 8, U.data_ptr() + launched * mat,
+                          info.data_ptr() + launched * 4, cnt, _stream_handle(st))
+        launched += cnt
+        s += 1
+    for st in streams:
+        st.synchronize()
+    return W, U, info
+
+
+def precompile(n: int, block_dim_x: int = 0, arch: str = "gfx950") -> None:
+    st = lib().sunny_geneig_precompile(n, block_dim_x, arch.encode())
+    if st != STATUS_OK:
+        raise GeneigError(st, last_error())
+
+
+def cache_dir() -> str:
+    return lib().sunny_geneig_cache_dir().decode()

@@ -1,0 +1,107 @@
+"""Build libsunny_geneig.so in-tree and warm its code-object cache.
+
+Steps:
+  1. embed kernels/geneig_kernels.hip as a C string (csrc/_gen/kernel_source.inc)
+     for the run-time hipRTC compiler;
+  2. compile the host runtime + C ABI with g++ against the HIP runtime and
+     hipRTC -> sunny_gen_eigensolver_amd/lib/libsunny_geneig.so;
+  3. compile the kernel source ahead of time with hipcc for gfx950 (a build
+     check that also prints the kernels' register / LDS budgets);
+  4. compile the code objects for the common orders through the library's own
+     sunny_geneig_precompile into the in-tree cache directory, so a fresh
+     machine loads them from disk instead of running hipRTC.
+
+Run: python -m sunny_gen_eigensolver_amd.build [--no-warm]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+GEN = os.path.join(CSRC, "_gen")
+LIBDIR = os.path.join(PKG, "lib")
+BUILD = os.path.join(PKG, "_build")
+CACHE = os.path.join(PKG, "_cocache")
+KERNEL = os.path.join(CSRC, "kernels", "geneig_kernels.hip")
+LIB = os.path.join(LIBDIR, "libsunny_geneig.so")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIP_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+# Orders whose code objects ship warm: the reference fixtures' LDS-resident
+# sizes and the serving sizes.
+WARM_ORDERS = [16, 24, 32, 48, 54, 64]
+
+
+def _run(cmd, **kw):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, **kw)
+
+
+def embed_kernel_source() -> str:
+    os.makedirs(GEN, exist_ok=True)
+    src = open(KERNEL).read()
+    delim = "GENEIG_SRC"
+    assert (")" + delim + '"') not in src
+    out = os.path.join(GEN, "kernel_source.inc")
+    text = ("// Generated by sunny_gen_eigensolver_amd/build.py from kernels/geneig_kernels.hip.\n"
+            f'static const char kGeneigKernelSource[] = R"{delim}(' + src + f'){delim}";\n')
+    old = open(out).read() if os.path.exists(out) else None
+    if old != text:
+        open(out, "w").write(text)
+    return out
+
+
+def build_library() -> str:
+    embed_kernel_source()
+    os.makedirs(LIBDIR, exist_ok=True)
+    srcs = [os.path.join(CSRC, f) for f in ("codeobj.cpp", "solver.cpp", "c_api.cpp")]
+    tmp = LIB + ".tmp"
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall",
+          "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", f"-I{os.path.join(REPO, 'include')}",
+          f"-I{CSRC}", *srcs, "-o", tmp, f"-L{ROCM}/lib", "-lamdhip64", "-lhiprtc",
+          f"-Wl,-rpath,{ROCM}/lib"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def aot_check() -> None:
+    os.makedirs(BUILD, exist_ok=True)
+    _run(["hipcc", "--offload-arch=gfx950", "-O3", "-std=c++17", *HIP_FLAGS, "-c", KERNEL,
+          "-o", os.path.join(BUILD, "geneig_kernels.o")])
+
+
+def warm_cache(orders=WARM_ORDERS) -> None:
+    os.makedirs(CACHE, exist_ok=True)
+    os.environ["SUNNY_GENEIG_CACHE_DIR"] = CACHE
+    lib = ctypes.CDLL(LIB)
+    lib.sunny_geneig_precompile.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_char_p]
+    lib.sunny_geneig_last_error.restype = ctypes.c_char_p
+    for n in orders:
+        for bd in sorted({0, 64, 128, 256, 512}):
+            rc = lib.sunny_geneig_precompile(n, bd, b"gfx950")
+            if rc != 0:
+                raise RuntimeError(f"precompile N={n} wg={bd} failed: "
+                                   f"{lib.sunny_geneig_last_error().decode()}")
+        print(f"warmed N={n}", flush=True)
+
+
+def build(warm: bool = True) -> None:
+    build_library()
+    aot_check()
+    if warm:
+        # A child process: the cache directory is read from the environment.
+        _run([sys.executable, "-c",
+              "import sunny_gen_eigensolver_amd.build as b; b.warm_cache()"], cwd=REPO)
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--no-warm", action="store_true")
+    a = ap.parse_args()
+    build(warm=not a.no_warm)

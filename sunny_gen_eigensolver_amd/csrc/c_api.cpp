@@ -1,0 +1,171 @@
+// C ABI of libsunny_geneig.so (declared in include/sunny_geneig.h). Wraps the
+// HIP solver in an opaque handle and turns C++ exceptions into status codes
+// with a thread-local message, as the reference's src/common/c_api.cpp does.
+#include "sunny_geneig.h"
+
+#include "solver.hpp"
+
+#include <exception>
+#include <memory>
+#include <new>
+#include <string>
+
+struct sunny_geneig_solver {
+    std::unique_ptr<sgeig::Solver> impl;
+    sunny_geneig_kernel_mode_t mode = SUNNY_GENEIG_KERNEL_UNIFIED;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+thread_local std::string g_cache_dir;
+
+void set_error(const std::string& m) { g_last_error = m; }
+void clear_error() { g_last_error.clear(); }
+
+sgeig::CacheMode to_cache(sunny_geneig_cache_mode_t c) {
+    switch (c) {
+        case SUNNY_GENEIG_CACHE_NODISK: return sgeig::CacheMode::NoDisk;
+        case SUNNY_GENEIG_CACHE_NOCACHE: return sgeig::CacheMode::NoCache;
+        default: return sgeig::CacheMode::Auto;
+    }
+}
+
+sunny_geneig_solver_t create_impl(int n, int device_id, int bpb, sunny_geneig_tuning_mode_t tuning,
+                                  sunny_geneig_cache_mode_t cache, sunny_geneig_kernel_mode_t mode,
+                                  const char* fn) {
+    clear_error();
+    try {
+        auto h = std::make_unique<sunny_geneig_solver>();
+        h->mode = mode;
+        h->impl = std::make_unique<sgeig::Solver>(
+            n, device_id, bpb,
+            tuning == SUNNY_GENEIG_TUNING_AUTOTUNE ? sgeig::Tuning::Autotune : sgeig::Tuning::Suggested,
+            to_cache(cache),
+            mode == SUNNY_GENEIG_KERNEL_HYBRID ? sgeig::Mode::Hybrid : sgeig::Mode::Unified);
+        return h.release();
+    } catch (const std::bad_alloc&) {
+        set_error(std::string("out of memory in ") + fn);
+    } catch (const std::exception& e) {
+        set_error(std::string(fn) + ": " + e.what());
+    } catch (...) {
+        set_error(std::string("unknown C++ exception in ") + fn);
+    }
+    return nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+sunny_geneig_solver_t sunny_geneig_create(int n, int device_id, int batches_per_block_request,
+                                          sunny_geneig_tuning_mode_t tuning_mode,
+                                          sunny_geneig_cache_mode_t cache_mode) {
+    return create_impl(n, device_id, batches_per_block_request, tuning_mode, cache_mode,
+                       SUNNY_GENEIG_KERNEL_UNIFIED, "sunny_geneig_create");
+}
+
+sunny_geneig_solver_t sunny_geneig_create_ex(int n, int device_id, int batches_per_block_request,
+                                             sunny_geneig_tuning_mode_t tuning_mode,
+                                             sunny_geneig_cache_mode_t cache_mode,
+                                             sunny_geneig_kernel_mode_t kernel_mode) {
+    if (kernel_mode != SUNNY_GENEIG_KERNEL_UNIFIED && kernel_mode != SUNNY_GENEIG_KERNEL_HYBRID) {
+        set_error("sunny_geneig_create_ex: unknown kernel_mode");
+        return nullptr;
+    }
+    // As in the reference, the split mode ignores tuning and batches-per-block.
+    if (kernel_mode == SUNNY_GENEIG_KERNEL_HYBRID)
+        return create_impl(n, device_id, 0, SUNNY_GENEIG_TUNING_SUGGESTED, cache_mode, kernel_mode,
+                           "sunny_geneig_create_ex");
+    return create_impl(n, device_id, batches_per_block_request, tuning_mode, cache_mode, kernel_mode,
+                       "sunny_geneig_create_ex");
+}
+
+void sunny_geneig_destroy(sunny_geneig_solver_t solver) {
+    try {
+        delete solver;
+    } catch (...) {
+    }
+}
+
+sunny_geneig_status_t sunny_geneig_launch(sunny_geneig_solver_t solver, uintptr_t d_H, uintptr_t d_S,
+                                          uintptr_t d_W, uintptr_t d_U, uintptr_t d_info,
+                                          int batch_size, uintptr_t stream) {
+    clear_error();
+    if (!solver) {
+        set_error("sunny_geneig_launch: solver is NULL");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    if (batch_size < 0) {
+        set_error("sunny_geneig_launch: batch_size < 0");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    if (batch_size > 0 && (!d_H || !d_S || !d_W || !d_U || !d_info)) {
+        set_error("sunny_geneig_launch: NULL device buffer");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    try {
+        solver->impl->launch(reinterpret_cast<const void*>(d_H), reinterpret_cast<const void*>(d_S),
+                             reinterpret_cast<double*>(d_W), reinterpret_cast<void*>(d_U),
+                             reinterpret_cast<int*>(d_info), batch_size,
+                             reinterpret_cast<hipStream_t>(stream));
+        return SUNNY_GENEIG_OK;
+    } catch (const std::exception& e) {
+        set_error(std::string("sunny_geneig_launch: ") + e.what());
+    } catch (...) {
+        set_error("unknown C++ exception in sunny_geneig_launch");
+    }
+    return SUNNY_GENEIG_LAUNCH_ERROR;
+}
+
+int sunny_geneig_matrix_size(sunny_geneig_solver_t s) { return s ? s->impl->n() : -1; }
+int sunny_geneig_batches_per_block(sunny_geneig_solver_t s) {
+    return s ? s->impl->batches_per_block() : -1;
+}
+int sunny_geneig_block_dim_x(sunny_geneig_solver_t s) { return s ? s->impl->block_dim_x() : -1; }
+unsigned int sunny_geneig_shared_mem_bytes(sunny_geneig_solver_t s) {
+    return s ? s->impl->shared_mem_bytes() : 0u;
+}
+
+const char* sunny_geneig_last_error(void) { return g_last_error.c_str(); }
+
+const char* sunny_geneig_path_name(sunny_geneig_solver_t s) {
+    if (!s) return "";
+    return s->mode == SUNNY_GENEIG_KERNEL_HYBRID ? "hybrid" : "unified";
+}
+
+sunny_geneig_kernel_mode_t sunny_geneig_kernel_mode(sunny_geneig_solver_t s) {
+    if (!s) return static_cast<sunny_geneig_kernel_mode_t>(-1);
+    return s->mode;
+}
+
+int sunny_geneig_precompile(int n, int block_dim_x, const char* arch) {
+    clear_error();
+    try {
+        if (n < 1 || n > 64) throw std::invalid_argument("n out of range for the LDS-resident kernel");
+        sgeig::ModuleKey key{n, block_dim_x > 0 ? block_dim_x : sgeig::Solver::suggested_block_dim(n),
+                             arch && arch[0] ? arch : "gfx950"};
+        auto code = sgeig::compile_code_object(key);
+        sgeig::write_disk(key, code);
+        return SUNNY_GENEIG_OK;
+    } catch (const sgeig::CompileError& e) {
+        set_error(e.what());
+        return SUNNY_GENEIG_COMPILE_ERROR;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+}
+
+const char* sunny_geneig_cache_layer(sunny_geneig_solver_t s) { return s ? s->impl->cache_layer() : ""; }
+
+const char* sunny_geneig_cache_dir(void) {
+    try {
+        g_cache_dir = sgeig::cache_dir();
+    } catch (...) {
+        g_cache_dir.clear();
+    }
+    return g_cache_dir.c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,72 @@
+// Host runtime of the batched generalized eigensolver: the fused ("unified")
+// and split ("hybrid") modes behind the C ABI.
+//
+// Reference counterparts: UnifiedGeneigSolver (src/unified/unified_solver.hpp:81-217)
+// and HybridGeneigSolver (src/hybrid/hybrid_solver.hpp:34-89).
+#pragma once
+
+#include "codeobj.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+
+namespace sgeig {
+
+enum class Tuning { Suggested, Autotune };
+enum class Mode { Unified, Hybrid };
+
+class Solver {
+public:
+    Solver(int n, int device_id, int bpb_request, Tuning tuning, CacheMode cache, Mode mode);
+    ~Solver();
+    Solver(const Solver&) = delete;
+    Solver& operator=(const Solver&) = delete;
+
+    // Asynchronous on `stream`; batch == 0 is a no-op.
+    void launch(const void* H, const void* S, double* W, void* U, int* info, int batch,
+                hipStream_t stream);
+
+    int n() const { return n_; }
+    int batches_per_block() const { return bpb_; }
+    int block_dim_x() const { return block_dim_; }
+    unsigned shared_mem_bytes() const { return lds_bytes_; }
+    Mode mode() const { return mode_; }
+    const char* cache_layer() const { return layer_name(layer_); }
+    double acquire_ms() const { return acquire_ms_; }
+    bool was_autotuned() const { return autotuned_; }
+
+    // Suggested workgroup size for order n (a model, no measurement).
+    static int suggested_block_dim(int n);
+
+private:
+    void bind_module_(int block_dim, CacheMode cache);
+    int autotune_(CacheMode cache);
+    void launch_unified_(const void* H, const void* S, double* W, void* U, int* info, int batch,
+                         hipStream_t stream);
+    void launch_hybrid_(const void* H, const void* S, double* W, void* U, int* info, int batch,
+                        hipStream_t stream);
+    void ensure_workspace_(size_t batch);
+
+    int n_ = 0;
+    int device_ = 0;
+    int bpb_ = 1;
+    int block_dim_ = 0;
+    unsigned lds_bytes_ = 0;
+    Mode mode_ = Mode::Unified;
+    CacheLayer layer_ = CacheLayer::Compile;
+    double acquire_ms_ = 0.0;
+    bool autotuned_ = false;
+    std::string arch_;
+    std::shared_ptr<LoadedModule> module_;
+    hipFunction_t fused_ = nullptr;
+    hipFunction_t k_reduce_ = nullptr;
+    hipFunction_t k_heev_ = nullptr;
+    hipFunction_t k_back_ = nullptr;
+    // Split-mode workspace: L factors, one per matrix.
+    void* lbuf_ = nullptr;
+    size_t lbuf_batch_ = 0;
+};
+
+}  // namespace sgeig

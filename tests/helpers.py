@@ -1,0 +1,70 @@
+"""Shared test helpers: golden fixtures, input generation, GPU runs through
+the C ABI, and the reference's comparison metrics (via the oracle)."""
+import json
+import os
+
+import numpy as np
+
+from oracle import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+EPS = 2.0 ** -52
+TOL_EIG = 1e-10        # reference bar (tests/static/test_full_n54.cpp:144-182)
+TOL_PHASE = 1e-10
+DEGEN_GAP = 1e-6
+RESID_FACTOR = 50.0    # residual, S-orthogonality <= 50 N eps
+
+
+def fixtures_index():
+    with open(os.path.join(GOLDEN, "fixtures.json")) as f:
+        return json.load(f)
+
+
+def golden_arrays():
+    return np.load(os.path.join(GOLDEN, "fixtures.npz"), allow_pickle=False)
+
+
+def fixture_inputs(name):
+    """Regenerate a reference fixture's H, S with the oracle's generator."""
+    meta = fixtures_index()[name]
+    mode = "random" if meta["s_mode"] == "random" else "near"
+    H, S = O.gen_reference(meta["n"], meta["batch"], meta["seed"], mode)
+    return meta, H, S
+
+
+def run_gpu(H, S, kernel="unified", solver=None, n=None):
+    """Solve a host batch on cuda:0 through the C ABI; returns W, U, info (numpy)."""
+    import torch
+    import sunny_gen_eigensolver_amd as sg
+    b, n = H.shape[0], H.shape[1]
+    own = solver is None
+    if own:
+        solver = sg.Solver(n, kernel=kernel)
+    dev = torch.device("cuda", 0)
+    dH = torch.from_numpy(np.ascontiguousarray(H)).to(dev)
+    dS = torch.from_numpy(np.ascontiguousarray(S)).to(dev)
+    W, U, info = sg.run_kernel(solver, dH, dS)
+    torch.cuda.synchronize()
+    out = W.cpu().numpy(), U.cpu().numpy(), info.cpu().numpy()
+    if own:
+        solver.close()
+    return out
+
+
+def metrics(H, S, W, U, W_ref=None, U_ref=None):
+    return O.check_batch(H, S, W, U, W_ref, U_ref, DEGEN_GAP, threads=os.cpu_count() or 1)
+
+
+def assert_parity(H, S, W, U, info, W_ref, U_ref, tag=""):
+    n = H.shape[1]
+    assert (info == 0).all(), f"{tag}: nonzero info {info[info != 0][:8]}"
+    m = metrics(H, S, W, U, W_ref, U_ref)
+    bar = RESID_FACTOR * n * EPS
+    print(f"{tag}: max eig_rel {m[:,0].max():.3e} phase {m[:,1].max():.3e} "
+          f"resid/(N eps) {m[:,3].max()/(n*EPS):.3f} s_orth/(N eps) {m[:,4].max()/(n*EPS):.3f}")
+    assert m[:, 0].max() < TOL_EIG, f"{tag}: eig_rel {m[:,0].max():.3e}"
+    assert m[:, 1].max() < TOL_PHASE, f"{tag}: phase {m[:,1].max():.3e}"
+    assert m[:, 3].max() <= bar, f"{tag}: residual {m[:,3].max():.3e} > {bar:.3e}"
+    assert m[:, 4].max() <= bar, f"{tag}: S-orthogonality {m[:,4].max():.3e} > {bar:.3e}"
+    return m
