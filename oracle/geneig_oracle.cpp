@@ -29,7 +29,7 @@ inline size_t idx(int n, int i, int j) { return (size_t)j * n + i; }
 // ---------------------------------------------------------------------------
 
 // Hermitian H = A + A^H, A complex Gaussian; one fresh normal_distribution per
-// call and (re, im) drawn in that order, column-major.
+// call, column-major.
 // Follows tests/common/lapack_reference.cpp:86-101.
 void gen_hermitian(int n, std::mt19937_64& rng, cd* H) {
     std::normal_distribution<double> g(0.0, 1.0);
@@ -320,14 +320,17 @@ void rs_hetrd(int n, cd* A, double* d, double* e, cd* tau) {
 // Martin, Reinsch & Wilkinson, "The QR and QL algorithms for symmetric
 // matrices" (Handbook for Automatic Computation II/3). Returns 0, or the
 // number of off-diagonals that failed to converge within 30 sweeps per
-// eigenvalue (zheev's INFO meaning).
+// eigenvalue (zheev's INFO meaning). The GPU computes rotation lengths with a
+// Newton-refined reciprocal square root instead of hypot and a division;
+// results agree to rounding.
 int rs_tql(int n, double* d, double* e, double* Z) {
     for (int j = 0; j < n; ++j)
         for (int i = 0; i < n; ++i) Z[idx(n, i, j)] = (i == j) ? 1.0 : 0.0;
     if (n <= 1) return 0;
     e[n - 1] = 0.0;
     const double eps = 0x1p-53;
-    for (int l = 0; l < n; ++l) {
+    int info = 0;
+    for (int l = 0; l < n && info == 0; ++l) {
         int iter = 0;
         for (;;) {
             int m = l;
@@ -337,9 +340,8 @@ int rs_tql(int n, double* d, double* e, double* Z) {
             }
             if (m == l) break;
             if (iter++ == 30) {
-                int cnt = 0;
-                for (int i = l; i < n - 1; ++i) cnt += (e[i] != 0.0);
-                return cnt;
+                for (int i = l; i < n - 1; ++i) info += (e[i] != 0.0);
+                break;
             }
             double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
             double r = std::hypot(g, 1.0);
@@ -378,7 +380,7 @@ int rs_tql(int n, double* d, double* e, double* Z) {
             e[m] = 0.0;
         }
     }
-    // selection sort, ascending
+    // ascending order (also after a convergence failure, as the GPU does)
     for (int i = 0; i < n - 1; ++i) {
         int k = i;
         double p = d[i];
@@ -390,7 +392,7 @@ int rs_tql(int n, double* d, double* e, double* Z) {
             for (int r = 0; r < n; ++r) std::swap(Z[idx(n, r, i)], Z[idx(n, r, k)]);
         }
     }
-    return 0;
+    return info;
 }
 
 // V <- Q V with Q = H_0 H_1 ... H_{n-2} from rs_hetrd (applied last-first).

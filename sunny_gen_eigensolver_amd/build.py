@@ -78,6 +78,10 @@ def aot_check() -> None:
 
 def warm_cache(orders=WARM_ORDERS) -> None:
     os.makedirs(CACHE, exist_ok=True)
+    # Entries are keyed by a hash of source + options: drop stale ones.
+    for f in os.listdir(CACHE):
+        if f.endswith(".hsaco") or f == "autotune.tsv":
+            os.remove(os.path.join(CACHE, f))
     os.environ["SUNNY_GENEIG_CACHE_DIR"] = CACHE
     lib = ctypes.CDLL(LIB)
     lib.sunny_geneig_precompile.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_char_p]

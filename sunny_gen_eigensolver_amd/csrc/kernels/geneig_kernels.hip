@@ -286,103 +286,136 @@ __device__ void stash_reflectors(Work& w) {
 }
 
 // ---------------------------------------------------------------------------
-// Stage 4: implicit QL with Wilkinson-type shift on (d, e), rotations applied
-// to Z (real, in B's storage). Run by wave 0: every lane computes the same
-// scalar recurrence, lane r owns row r of Z and carries the column value that
-// the next rotation of the sweep touches. Then selection sort ascending.
-// Method: Bowdler, Martin, Reinsch & Wilkinson, "The QR and QL algorithms for
-// symmetric matrices" (Handbook II/3). Returns zheev-style INFO (number of
-// off-diagonals left unconverged) and writes sorted eigenvalues to W_out.
+// Stage 4: implicit QL with Wilkinson-type shift on the real symmetric
+// tridiagonal (d, e), one wave per matrix. Lane j holds d[j] and e[j]; the
+// scalar recurrence runs redundantly in every lane, reading entries with
+// v_readlane and writing them back with a lane select. Lane r holds row r of
+// Z in registers: the sweep is unrolled over the static column index, each
+// step predicated on the active window [l, m), so Z never leaves the VGPRs.
+// Rotation lengths use a reciprocal square root refined by two Newton steps
+// instead of a correctly rounded sqrt and division (c^2 + s^2 = 1 to a few
+// ulp). Convergence scan: every lane tests its own off-diagonal, one ballot
+// finds the first negligible one. Method: Bowdler, Martin, Reinsch &
+// Wilkinson, "The QR and QL algorithms for symmetric matrices" (Handbook
+// II/3). Returns zheev-style INFO (off-diagonals left unconverged, 0 = ok).
 // ---------------------------------------------------------------------------
-__device__ int tql(Work& w, double* __restrict__ W_out) {
+__device__ __forceinline__ double sel_lane(double v, double x, int target) {
+    return (lane_id() == target) ? x : v;
+}
+
+__device__ __forceinline__ double rsqrt_nr(double t) {
+    double y = __builtin_amdgcn_rsq(t);
+    const double h = 0.5 * t;
+    double r = fma(-(h * y), y, 0.5);
+    y = fma(y, r, y);
+    r = fma(-(h * y), y, 0.5);
+    y = fma(y, r, y);
+    return y;
+}
+
+__device__ int ql_wave(double& dreg, double& ereg, double (&z)[N]) {
     const int lane = lane_id();
-    double* Z = reinterpret_cast<double*>(w.B);
-    double* d = w.d;
-    double* e = w.e;
-    const bool own = lane < N;
     const double eps = 0x1p-53;
-    int info = 0;
-    for (int l = 0; l < N && info == 0; ++l) {
-        int iter = 0;
-        for (;;) {
-            int m = l;
-            for (; m < N - 1; ++m) {
-                const double dd = fabs(d[m]) + fabs(d[m + 1]);
-                if (fabs(e[m]) <= eps * dd) break;
-            }
-            if (m == l) break;
-            if (iter++ == QL_MAXIT) {
-                int cnt = 0;
-                for (int i = l; i < N - 1; ++i) cnt += (e[i] != 0.0);
-                info = cnt;
-                break;
-            }
-            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-            double r = sqrt(g * g + 1.0);
-            g = d[m] - d[l] + e[l] / (g + copysign(r, g));
-            double s = 1.0, c = 1.0, p = 0.0;
-            double zc = own ? Z[m * LDZ + lane] : 0.0;
-            bool underflow = false;
-            double dn = d[m - 1], en = e[m - 1];
-            for (int i = m - 1; i >= l; --i) {
-                const double di = dn, ei = en;
-                if (i > l) { dn = d[i - 1]; en = e[i - 1]; }
-                const double f = s * ei;
-                const double b = c * ei;
-                r = sqrt(f * f + g * g);
-                e[i + 1] = r;
-                if (r == 0.0) {
-                    d[i + 1] -= p;
-                    e[m] = 0.0;
-                    if (own) Z[(i + 1) * LDZ + lane] = zc;
-                    underflow = true;
-                    break;
-                }
-                const double rinv = 1.0 / r;
-                s = f * rinv;
-                c = g * rinv;
-                g = d[i + 1] - p;
-                r = (di - g) * s + 2.0 * c * b;
-                p = s * r;
-                d[i + 1] = g + p;
-                g = c * r - b;
-                const double zi = own ? Z[i * LDZ + lane] : 0.0;
-                if (own) Z[(i + 1) * LDZ + lane] = s * zi + c * zc;
-                zc = c * zi - s * zc;
-            }
-            if (underflow) continue;
-            if (own) Z[l * LDZ + lane] = zc;
-            d[l] -= p;
-            e[l] = g;
-            e[m] = 0.0;
+    int l = 0, iter = 0;
+    while (l < N) {
+        const double dnext = __shfl_down(dreg, 1, 64);
+        const bool small = fabs(ereg) <= eps * (fabs(dreg) + fabs(dnext));
+        unsigned long long mask = __ballot(small && lane >= l && lane < N - 1);
+        mask |= 1ull << (N - 1);
+        mask &= ~((1ull << l) - 1ull);
+        const int m = __ffsll((long long)mask) - 1;
+        if (m == l) {
+            ++l;
+            iter = 0;
+            continue;
         }
-    }
-    // Selection sort, ascending, first minimum wins (lane j holds d[j]).
-    double dj = own ? d[lane] : __builtin_inf();
-    for (int i = 0; i < N - 1; ++i) {
-        double best = (lane >= i) ? dj : __builtin_inf();
-        int bi = lane;
+        if (iter++ == QL_MAXIT) {
+            const unsigned long long nz = __ballot(ereg != 0.0 && lane >= l && lane < N - 1);
+            return __popcll(nz);
+        }
+        const double d_l = bcast(dreg, l);
+        const double e_l = bcast(ereg, l);
+        double g = (bcast(dreg, l + 1) - d_l) / (2.0 * e_l);
+        double r = sqrt(g * g + 1.0);
+        g = bcast(dreg, m) - d_l + e_l / (g + copysign(r, g));
+        double s = 1.0, c = 1.0, p = 0.0;
+        double d_up = bcast(dreg, m);      // original d[i+1] for the current i
+        bool underflow = false;
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            const double ob = __shfl_xor(best, off, 64);
-            const int oi = __shfl_xor(bi, off, 64);
-            if (ob < best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-        }
-        const int k = bi;
-        if (k != i) {
-            const double di = bcast(dj, i);
-            if (lane == i) dj = best;
-            else if (lane == k) dj = di;
-            if (own) {
-                const double zi = Z[i * LDZ + lane];
-                const double zk = Z[k * LDZ + lane];
-                Z[i * LDZ + lane] = zk;
-                Z[k * LDZ + lane] = zi;
+        for (int i = N - 2; i >= 0; --i) {
+            if (i < m && i >= l && !underflow) {
+                const double e_i = bcast(ereg, i);
+                const double d_i = bcast(dreg, i);
+                const double f = s * e_i;
+                const double b = c * e_i;
+                const double t = fma(f, f, g * g);
+                if (t == 0.0) {
+                    ereg = sel_lane(ereg, 0.0, i + 1);
+                    dreg = sel_lane(dreg, d_up - p, i + 1);
+                    ereg = sel_lane(ereg, 0.0, m);
+                    underflow = true;
+                } else {
+                    const double y = rsqrt_nr(t);
+                    ereg = sel_lane(ereg, t * y, i + 1);
+                    s = f * y;
+                    c = g * y;
+                    g = d_up - p;
+                    const double r2 = fma(d_i - g, s, 2.0 * c * b);
+                    p = s * r2;
+                    dreg = sel_lane(dreg, g + p, i + 1);
+                    g = fma(c, r2, -b);
+                    const double zi = z[i], zi1 = z[i + 1];
+                    z[i + 1] = fma(s, zi, c * zi1);
+                    z[i] = fma(c, zi, -(s * zi1));
+                    d_up = d_i;
+                }
             }
         }
+        if (underflow) continue;
+        dreg = sel_lane(dreg, d_l - p, l);
+        ereg = sel_lane(ereg, g, l);
+        ereg = sel_lane(ereg, 0.0, m);
     }
-    if (own) W_out[lane] = dj;
+    return 0;
+}
+
+// Ascending position of lane j's eigenvalue (ties by index).
+__device__ int ascending_rank(double dreg) {
+    const int lane = lane_id();
+    int rank = 0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const double dk = bcast(dreg, k);
+        rank += (dk < dreg || (dk == dreg && k < lane)) ? 1 : 0;
+    }
+    return rank;
+}
+
+// Wave-level eigen-solve of the tridiagonal held in dsrc/esrc (LDS): writes the
+// ascending eigenvalues to W_out and leaves Z (columns in ascending order) in
+// `z` (lane r = row r) plus its rank permutation in `rank`.
+__device__ int tridiag_eig_wave(const double* dsrc, const double* esrc, double* __restrict__ W_out,
+                                double (&z)[N], int& rank) {
+    const int lane = lane_id();
+    double dreg = (lane < N) ? dsrc[lane] : 0.0;
+    double ereg = (lane < N - 1) ? esrc[lane] : 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) z[j] = (lane == j) ? 1.0 : 0.0;
+    const int info = ql_wave(dreg, ereg, z);
+    rank = ascending_rank(lane < N ? dreg : __builtin_inf());
+    if (lane < N) W_out[rank] = dreg;
     return info;
+}
+
+// Write this wave's Z rows into Zs (real, leading dimension LDZ) with columns
+// permuted to ascending eigenvalue order.
+__device__ void store_z_sorted(double* Zs, const double (&z)[N], int rank) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const int pos = __builtin_amdgcn_readlane(rank, j);
+        if (lane < N) Zs[pos * LDZ + lane] = z[j];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -483,45 +516,116 @@ __device__ void fail_store(int code, double* __restrict__ Wb, c64* __restrict__ 
     if (threadIdx.x == 0) *info_b = code;
 }
 
+// ---------------------------------------------------------------------------
+// Stash of one matrix's factor data in its own U slot (global memory) while
+// the workgroup moves on to the next matrix, in the layout of A after
+// stash_reflectors: L on and below the diagonal with 1/L(k,k) in the
+// diagonal's imaginary part, tau_k on the first superdiagonal (k, k+1), and
+// reflector entries v_k(i) at (k, i), i >= k+2.
+// ---------------------------------------------------------------------------
+__device__ void stash_factors(const Work& w, c64* __restrict__ Ub) {
+    for (int e = threadIdx.x; e < N * N; e += NT) {
+        const int i = e % N, j = e / N;
+        c64 v;
+        if (i > j) v = w.A[j * LDA + i];
+        else if (i == j) v = cm(w.A[j * LDA + j].x, w.invL[j]);
+        else if (i == j - 1) v = w.tau[i];
+        else v = w.B[i * LDA + j];
+        Ub[e] = v;
+    }
+}
+
+__device__ void unstash_factors(Work& w, const c64* __restrict__ Ub) {
+    for (int e = threadIdx.x; e < N * N; e += NT) {
+        const int i = e % N, j = e / N;
+        const c64 v = Ub[e];
+        if (i == j) {
+            w.A[j * LDA + j] = cm(v.x, 0.0);
+            w.invL[j] = v.y;
+        } else {
+            w.A[j * LDA + i] = v;
+            if (i == j - 1) w.tau[i] = v;
+        }
+    }
+    if (threadIdx.x == 0) w.tau[N - 1] = cm(0.0, 0.0);
+}
+
+// Stage data kept per matrix of a workgroup between the front and back halves.
+struct Staged {
+    double d[NW][N];
+    double e[NW][N];
+    int status[NW];        // 0 = ok, otherwise the matrix is finished (failed)
+    int qinfo[NW];
+};
+
 }  // namespace gk
 
 using gk::c64;
 
-// Fused solve, one workgroup per matrix.
+// ---------------------------------------------------------------------------
+// Fused solve. A workgroup of NW waves takes NW consecutive matrices: it runs
+// the dense front half (Cholesky, two-sided solve, tridiagonalisation) for
+// each in turn with all waves, stashing the factors in the matrix's own U
+// slot; then every wave runs the QL of one matrix at the same time (Z in its
+// registers); then the dense back half (V = Q Z, U = L^-H V) for each in turn.
+// ---------------------------------------------------------------------------
 extern "C" __global__ void __launch_bounds__(GENEIG_NT)
 geneig_fused(const c64* __restrict__ H, const c64* __restrict__ S, double* __restrict__ W,
              c64* __restrict__ U, int* __restrict__ info, int batch) {
     using namespace gk;
     __shared__ Work w;
-    const int b = blockIdx.x;
-    if (b >= batch) return;
-    const size_t off = (size_t)b * N * N;
-    load_pair(w, H + off, S + off);
-    const int pinfo = potrf(w);
-    if (pinfo) {
-        fail_store(N + pinfo, W + (size_t)b * N, U + off, info + b);
-        return;
+    __shared__ Staged st;
+    const int lane = lane_id(), wave = wave_id();
+    const int b0 = blockIdx.x * NW;
+    for (int q = 0; q < NW; ++q) {
+        const int b = b0 + q;
+        if (b >= batch) break;
+        const size_t off = (size_t)b * N * N;
+        load_pair(w, H + off, S + off);
+        const int pinfo = potrf(w);
+        if (pinfo) {
+            fail_store(N + pinfo, W + (size_t)b * N, U + off, info + b);
+            if (threadIdx.x == 0) st.status[q] = 1;
+            __syncthreads();
+            continue;
+        }
+        trsm_left(w);
+        __syncthreads();
+        trsm_right(w);
+        __syncthreads();
+        hetrd(w);
+        stash_factors(w, U + off);
+        if (threadIdx.x < N) {
+            st.d[q][threadIdx.x] = w.d[threadIdx.x];
+            st.e[q][threadIdx.x] = w.e[threadIdx.x];
+        }
+        if (threadIdx.x == 0) st.status[q] = 0;
+        __syncthreads();
     }
-    trsm_left(w);
-    __syncthreads();
-    trsm_right(w);
-    __syncthreads();
-    hetrd(w);
-    stash_reflectors(w);
+    double z[N];
+    int rank = 0;
     {
-        double* Z = reinterpret_cast<double*>(w.B);
-        for (int e = threadIdx.x; e < N * LDZ; e += NT) Z[e] = ((e % LDZ) == (e / LDZ)) ? 1.0 : 0.0;
+        const int b = b0 + wave;
+        if (b < batch && st.status[wave] == 0) {
+            const int qi = tridiag_eig_wave(st.d[wave], st.e[wave], W + (size_t)b * N, z, rank);
+            if (lane == 0) st.qinfo[wave] = qi;
+        }
     }
     __syncthreads();
-    if (wave_id() == 0) {
-        const int qinfo = tql(w, W + (size_t)b * N);
-        if (lane_id() == 0) w.flag[0] = qinfo;
+    for (int q = 0; q < NW; ++q) {
+        const int b = b0 + q;
+        if (b >= batch) break;
+        if (st.status[q] != 0) continue;
+        const size_t off = (size_t)b * N * N;
+        if (wave == q) store_z_sorted(reinterpret_cast<double*>(w.B), z, rank);
+        unstash_factors(w, U + off);
+        __syncthreads();
+        apply_q(w);
+        __syncthreads();
+        backtrsm_store(w, U + off);
+        if (threadIdx.x == 0) info[b] = st.qinfo[q];
+        __syncthreads();
     }
-    __syncthreads();
-    apply_q(w);
-    __syncthreads();
-    backtrsm_store(w, U + off);
-    if (threadIdx.x == 0) info[b] = w.flag[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -533,7 +637,8 @@ geneig_fused(const c64* __restrict__ H, const c64* __restrict__ S, double* __res
 // Lw holds L's lower triangle with 1/L(k,k) in the imaginary part of the
 // diagonal, so the back substitution uses the same reciprocal pivots as the
 // fused kernel. A matrix whose Cholesky failed carries info > N and is skipped
-// by the later kernels.
+// by the later kernels. The reduce and back kernels take one matrix per
+// workgroup; the heev kernel takes NW, like the fused kernel.
 // ---------------------------------------------------------------------------
 extern "C" __global__ void __launch_bounds__(GENEIG_NT)
 geneig_split_reduce(const c64* __restrict__ H, const c64* __restrict__ S, double* __restrict__ W,
@@ -567,28 +672,62 @@ extern "C" __global__ void __launch_bounds__(GENEIG_NT)
 geneig_split_heev(double* __restrict__ W, c64* __restrict__ U, int* __restrict__ info, int batch) {
     using namespace gk;
     __shared__ Work w;
-    const int b = blockIdx.x;
-    if (b >= batch) return;
-    if (info[b] != 0) return;
-    const size_t off = (size_t)b * N * N;
-    for (int e = threadIdx.x; e < N * N; e += NT) w.B[(e / N) * LDA + (e % N)] = U[off + e];
-    __syncthreads();
-    hetrd(w);
-    stash_reflectors(w);
+    __shared__ Staged st;
+    const int lane = lane_id(), wave = wave_id();
+    const int b0 = blockIdx.x * NW;
+    for (int q = 0; q < NW; ++q) {
+        const int b = b0 + q;
+        if (b >= batch) break;
+        if (info[b] != 0) {
+            if (threadIdx.x == 0) st.status[q] = 1;
+            continue;
+        }
+        const size_t off = (size_t)b * N * N;
+        for (int e = threadIdx.x; e < N * N; e += NT) w.B[(e / N) * LDA + (e % N)] = U[off + e];
+        __syncthreads();
+        hetrd(w);
+        // Reflectors and tau into U's strict upper part (the lower part is unused).
+        for (int e = threadIdx.x; e < N * N; e += NT) {
+            const int i = e % N, j = e / N;
+            if (i == j - 1) U[off + e] = w.tau[i];
+            else if (i < j - 1) U[off + e] = w.B[i * LDA + j];
+        }
+        if (threadIdx.x < N) {
+            st.d[q][threadIdx.x] = w.d[threadIdx.x];
+            st.e[q][threadIdx.x] = w.e[threadIdx.x];
+        }
+        if (threadIdx.x == 0) st.status[q] = 0;
+        __syncthreads();
+    }
+    double z[N];
+    int rank = 0;
     {
-        double* Z = reinterpret_cast<double*>(w.B);
-        for (int e = threadIdx.x; e < N * LDZ; e += NT) Z[e] = ((e % LDZ) == (e / LDZ)) ? 1.0 : 0.0;
+        const int b = b0 + wave;
+        if (b < batch && st.status[wave] == 0) {
+            const int qi = tridiag_eig_wave(st.d[wave], st.e[wave], W + (size_t)b * N, z, rank);
+            if (lane == 0) st.qinfo[wave] = qi;
+        }
     }
     __syncthreads();
-    if (wave_id() == 0) {
-        const int qinfo = tql(w, W + (size_t)b * N);
-        if (lane_id() == 0) w.flag[0] = qinfo;
+    for (int q = 0; q < NW; ++q) {
+        const int b = b0 + q;
+        if (b >= batch) break;
+        if (st.status[q] != 0) continue;
+        const size_t off = (size_t)b * N * N;
+        if (wave == q) store_z_sorted(reinterpret_cast<double*>(w.B), z, rank);
+        for (int e = threadIdx.x; e < N * N; e += NT) {
+            const int i = e % N, j = e / N;
+            if (i < j - 1) w.A[j * LDA + i] = U[off + e];
+            else if (i == j - 1) w.tau[i] = U[off + e];
+        }
+        if (threadIdx.x == 0) w.tau[N - 1] = cm(0.0, 0.0);
+        __syncthreads();
+        apply_q(w);
+        __syncthreads();
+        for (int e = threadIdx.x; e < N * N; e += NT) U[off + e] = w.B[(e / N) * LDA + (e % N)];
+        if (threadIdx.x == 0) info[b] = st.qinfo[q];
+        __syncthreads();
     }
-    __syncthreads();
-    apply_q(w);
-    __syncthreads();
-    for (int e = threadIdx.x; e < N * N; e += NT) U[off + e] = w.B[(e / N) * LDA + (e % N)];
-    if (threadIdx.x == 0) info[b] = w.flag[0];
 }
 
 extern "C" __global__ void __launch_bounds__(GENEIG_NT)

@@ -98,17 +98,18 @@ Solver::Solver(int n, int device_id, int bpb_request, Tuning tuning, CacheMode c
         throw std::invalid_argument("matrix size " + std::to_string(n) +
                                     " exceeds the LDS-resident kernel limit of " +
                                     std::to_string(kMaxLdsOrder));
-    if (bpb_request > 1)
-        throw std::invalid_argument("batches_per_block > 1 is not supported by this build");
+    if (bpb_request > 16)
+        throw std::invalid_argument("batches_per_block must be <= 16 (one wave per matrix)");
     int count = 0;
     SG_HIP(hipGetDeviceCount(&count));
     if (device_id < 0 || device_id >= count)
         throw std::invalid_argument("device_id " + std::to_string(device_id) + " out of range");
     SG_HIP(hipSetDevice(device_id));
     arch_ = device_arch(device_id);
-    bpb_ = 1;
-    int bd = suggested_block_dim(n);
-    if (mode == Mode::Unified && tuning == Tuning::Autotune) {
+    // A workgroup of W waves solves W matrices (one QL chain per wave), so a
+    // requested matrices-per-block count fixes the workgroup size.
+    int bd = (bpb_request > 0) ? 64 * bpb_request : suggested_block_dim(n);
+    if (mode == Mode::Unified && tuning == Tuning::Autotune && bpb_request <= 0) {
         bd = autotune_(cache);
         autotuned_ = true;
     }
@@ -129,6 +130,7 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
     ModuleKey key{n_, block_dim, arch_};
     module_ = acquire_module(key, cache, &layer_, &acquire_ms_);
     block_dim_ = block_dim;
+    bpb_ = block_dim / 64;
     fused_ = module_->get("geneig_fused");
     k_reduce_ = module_->get("geneig_split_reduce");
     k_heev_ = module_->get("geneig_split_heev");
@@ -238,8 +240,9 @@ void Solver::launch(const void* H, const void* S, double* W, void* U, int* info,
 void Solver::launch_unified_(const void* H, const void* S, double* W, void* U, int* info, int batch,
                              hipStream_t stream) {
     void* args[] = {&H, &S, &W, &U, &info, &batch};
-    SG_HIP(hipModuleLaunchKernel(fused_, (unsigned)batch, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream,
-                                 args, nullptr));
+    const unsigned grid = (unsigned)((batch + bpb_ - 1) / bpb_);
+    SG_HIP(hipModuleLaunchKernel(fused_, grid, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream, args,
+                                 nullptr));
 }
 
 void Solver::ensure_workspace_(size_t batch) {
@@ -262,8 +265,9 @@ void Solver::launch_hybrid_(const void* H, const void* S, double* W, void* U, in
     SG_HIP(hipModuleLaunchKernel(k_reduce_, (unsigned)batch, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream,
                                  a1, nullptr));
     void* a2[] = {&W, &U, &info, &batch};
-    SG_HIP(hipModuleLaunchKernel(k_heev_, (unsigned)batch, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream,
-                                 a2, nullptr));
+    const unsigned grid2 = (unsigned)((batch + bpb_ - 1) / bpb_);
+    SG_HIP(hipModuleLaunchKernel(k_heev_, grid2, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream, a2,
+                                 nullptr));
     void* a3[] = {&U, &info, &L, &batch};
     SG_HIP(hipModuleLaunchKernel(k_back_, (unsigned)batch, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream,
                                  a3, nullptr));

@@ -51,7 +51,7 @@ private:
 
     int n_ = 0;
     int device_ = 0;
-    int bpb_ = 1;
+    int bpb_ = 1;            // matrices per workgroup = waves per workgroup
     int block_dim_ = 0;
     unsigned lds_bytes_ = 0;
     Mode mode_ = Mode::Unified;
