@@ -87,7 +87,7 @@ def warm_cache(orders=WARM_ORDERS) -> None:
     lib.sunny_geneig_precompile.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_char_p]
     lib.sunny_geneig_last_error.restype = ctypes.c_char_p
     for n in orders:
-        for bd in sorted({0, 64, 128, 256, 512}):
+        for bd in sorted({0, 64, 128, 256}):
             rc = lib.sunny_geneig_precompile(n, bd, b"gfx950")
             if rc != 0:
                 raise RuntimeError(f"precompile N={n} wg={bd} failed: "

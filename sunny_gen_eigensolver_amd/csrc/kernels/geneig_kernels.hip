@@ -95,6 +95,8 @@ struct Work {
     c64 tau[N];
     c64 vbuf[N];
     c64 wbuf[N];
+    c64 vb[2][N];          // Householder vector / Cholesky pivot column, by step parity
+    c64 wb[NW][N];         // per-wave copy of the rank-2 vector w
     double invL[N];        // 1 / L(k,k)
     double d[N];
     double e[N];
@@ -109,32 +111,52 @@ struct Work {
 // previous pivot column, so no thread reads a value another thread writes in
 // the same step; one barrier per step.
 // ---------------------------------------------------------------------------
-__device__ int potrf(Work& w) {
+__device__ __forceinline__ int potrf(Work& w) {
+    // Column j belongs to wave j % NW. The owner of column k finishes it
+    // (pivot check, scaling) right after its own trailing updates of step k-1
+    // and publishes it in vb[k & 1]; all waves then update their own columns.
+    // One barrier per step.
     const int lane = lane_id(), wave = wave_id();
-    double prev_rinv = 0.0;
-    for (int k = 0; k < N; ++k) {
-        const double dkk = w.A[k * LDA + k].x;
-        if (!(dkk > 0.0)) return k + 1;
-        const double rinv = 1.0 / sqrt(dkk);
-        const double rinv2 = rinv * rinv;
-        const c64 aik = (lane < N) ? w.A[k * LDA + lane] : cm(0.0, 0.0);
-        if (k > 0 && wave == (k - 1) % NW && lane >= k - 1 && lane < N)
-            w.A[(k - 1) * LDA + lane] = w.A[(k - 1) * LDA + lane] * prev_rinv;
-        for (int j = k + 1 + wave; j < N; j += NW) {
-            const c64 ajk = bcast(aik, j);
-            if (lane >= j && lane < N) {
-                c64 t = w.A[j * LDA + lane];
-                t = t - cmulc(aik, ajk) * rinv2;
-                w.A[j * LDA + lane] = t;
-            }
+    auto finish_column = [&](int k) {
+        const c64 a = (lane >= k && lane < N) ? w.A[k * LDA + lane] : cm(0.0, 0.0);
+        const double dkk = bcast(a.x, k);
+        if (!(dkk > 0.0)) {
+            if (lane == 0) w.flag[1] = k + 1;
+            return;
         }
-        if (threadIdx.x == 0) w.invL[k] = rinv;
-        prev_rinv = rinv;
+        const double rinv = 1.0 / sqrt(dkk);
+        const c64 l = a * rinv;
+        if (lane >= k && lane < N) w.A[k * LDA + lane] = l;
+        if (lane < N) w.vb[k & 1][lane] = l;
+        if (lane == 0) w.invL[k] = rinv;
+    };
+    if (threadIdx.x == 0) w.flag[1] = 0;
+    __syncthreads();
+    if (wave == 0) finish_column(0);
+    __syncthreads();
+    for (int k = 0; k < N; ++k) {
+        const int fail = w.flag[1];
+        if (fail) return fail;
+        const int p = k & 1;
+        const int lr = lane < N ? lane : N - 1;
+        const c64 li = w.vb[p][lr];
+        // All loads first (clamped addresses), then masked stores: no branch
+        // separates the LDS reads, so their latencies overlap.
+        c64 a[CPW], lj[CPW];
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) {
+            const int j = (wave + NW * c < N) ? wave + NW * c : N - 1;
+            a[c] = w.A[j * LDA + lr];
+            lj[c] = w.vb[p][j];
+        }
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) {
+            const int j = wave + NW * c;
+            if (j > k && j < N && lane >= j && lane < N) w.A[j * LDA + lane] = a[c] - cmulc(li, lj[c]);
+        }
+        if (k + 1 < N && wave == (k + 1) % NW) finish_column(k + 1);
         __syncthreads();
     }
-    if (wave == (N - 1) % NW && lane == N - 1)
-        w.A[(N - 1) * LDA + N - 1] = w.A[(N - 1) * LDA + N - 1] * prev_rinv;
-    __syncthreads();
     return 0;
 }
 
@@ -143,55 +165,54 @@ __device__ int potrf(Work& w) {
 // columns, lane = row), then right solve row by row (each wave owns rows,
 // lane = column). Pivot values travel by readlane: no barriers inside.
 // ---------------------------------------------------------------------------
-__device__ void trsm_left(Work& w) {
+__device__ __forceinline__ void trsm_left(Work& w) {
+    // X = L^-1 B, column j owned by wave j % NW (lane = row). Step k reads the
+    // pivot entry B(k, j) with a broadcast LDS read; a wave's own writes are
+    // visible to its later reads, so no barrier is needed between steps.
     const int lane = lane_id(), wave = wave_id();
-    c64 b[CPW];
-#pragma unroll
-    for (int c = 0; c < CPW; ++c) {
-        const int j = wave + NW * c;
-        b[c] = (j < N && lane < N) ? w.B[j * LDA + lane] : cm(0.0, 0.0);
-    }
-#pragma unroll 2
+    const int lr = lane < N ? lane : N - 1;
     for (int k = 0; k < N; ++k) {
-        const c64 lik = (lane < N) ? w.A[k * LDA + lane] : cm(0.0, 0.0);
         const double inv = w.invL[k];
+        const c64 lik = w.A[k * LDA + lr];
+        c64 pv[CPW], bv[CPW];
 #pragma unroll
         for (int c = 0; c < CPW; ++c) {
-            const c64 xk = bcast(b[c], k) * inv;
-            if (lane == k) b[c] = xk;
-            else if (lane > k) b[c] = b[c] - cmul(lik, xk);
+            const int j = (wave + NW * c < N) ? wave + NW * c : N - 1;
+            pv[c] = w.B[j * LDA + k];
+            bv[c] = w.B[j * LDA + lr];
         }
-    }
 #pragma unroll
-    for (int c = 0; c < CPW; ++c) {
-        const int j = wave + NW * c;
-        if (j < N && lane < N) w.B[j * LDA + lane] = b[c];
+        for (int c = 0; c < CPW; ++c) {
+            const int j = wave + NW * c;
+            const c64 x = pv[c] * inv;
+            const c64 nb = (lane == k) ? x : bv[c] - cmul(lik, x);
+            if (j < N && lane >= k && lane < N) w.B[j * LDA + lane] = nb;
+        }
     }
 }
 
-__device__ void trsm_right(Work& w) {
+__device__ __forceinline__ void trsm_right(Work& w) {
+    // Y L^H = X row by row, row i owned by wave i % NW (lane = column):
+    // Y(i,k) = X(i,k) / L(k,k), then X(i,j) -= Y(i,k) conj(L(j,k)), j > k.
     const int lane = lane_id(), wave = wave_id();
-    c64 b[CPW];
-#pragma unroll
-    for (int c = 0; c < CPW; ++c) {
-        const int i = wave + NW * c;
-        b[c] = (i < N && lane < N) ? w.B[lane * LDA + i] : cm(0.0, 0.0);
-    }
-#pragma unroll 2
+    const int lr = lane < N ? lane : N - 1;
     for (int k = 0; k < N; ++k) {
-        const c64 ljk = (lane < N) ? w.A[k * LDA + lane] : cm(0.0, 0.0);
         const double inv = w.invL[k];
+        const c64 ljk = w.A[k * LDA + lr];
+        c64 pv[CPW], bv[CPW];
 #pragma unroll
         for (int c = 0; c < CPW; ++c) {
-            const c64 yk = bcast(b[c], k) * inv;
-            if (lane == k) b[c] = yk;
-            else if (lane > k) b[c] = b[c] - cmulc(yk, ljk);
+            const int i = (wave + NW * c < N) ? wave + NW * c : N - 1;
+            pv[c] = w.B[k * LDA + i];
+            bv[c] = w.B[lr * LDA + i];
         }
-    }
 #pragma unroll
-    for (int c = 0; c < CPW; ++c) {
-        const int i = wave + NW * c;
-        if (i < N && lane < N) w.B[lane * LDA + i] = b[c];
+        for (int c = 0; c < CPW; ++c) {
+            const int i = wave + NW * c;
+            const c64 y = pv[c] * inv;
+            const c64 nb = (lane == k) ? y : bv[c] - cmulc(y, ljk);
+            if (i < N && lane >= k && lane < N) w.B[lane * LDA + i] = nb;
+        }
     }
 }
 
@@ -201,71 +222,97 @@ __device__ void trsm_right(Work& w) {
 // B(k+2.., k); reflector generation follows LAPACK zlarfg (beta real,
 // beta = -sign(Re alpha) * ||(alpha, x)||). Three barriers per step.
 // ---------------------------------------------------------------------------
-__device__ void hetrd(Work& w) {
+// Householder vector for column k, computed by the wave that owns column k:
+// v -> vb[k & 1] (zeros above row k+1) and B(k+2.., k); e[k], d[k], tau[k].
+__device__ __forceinline__ void house(Work& w, int k) {
+    const int lane = lane_id();
+    const int m0 = k + 1;
+    const c64 x = (lane >= m0 && lane < N) ? w.B[k * LDA + lane] : cm(0.0, 0.0);
+    const c64 alpha = bcast(x, m0);
+    const double xn2 = wave_sum(lane > m0 ? cabs2(x) : 0.0);
+    c64 t = cm(0.0, 0.0);
+    double beta = alpha.x;
+    c64 v = cm(0.0, 0.0);
+    if (xn2 == 0.0 && alpha.y == 0.0) {
+        if (lane == m0) v = cm(1.0, 0.0);
+    } else {
+        const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+        beta = (alpha.x >= 0.0) ? -nrm : nrm;
+        t = cm((beta - alpha.x) / beta, -alpha.y / beta);
+        const c64 den = cm(alpha.x - beta, alpha.y);
+        const double dd = 1.0 / cabs2(den);
+        const c64 scal = cm(den.x * dd, -den.y * dd);   // 1 / (alpha - beta)
+        if (lane == m0) v = cm(1.0, 0.0);
+        else if (lane > m0) v = cmul(x, scal);
+    }
+    if (lane > m0 && lane < N) w.B[k * LDA + lane] = v;
+    if (lane < N) w.vb[k & 1][lane] = v;
+    if (lane == 0) {
+        w.e[k] = beta;
+        w.d[k] = w.B[k * LDA + k].x;
+        w.tau[k] = t;
+    }
+}
+
+__device__ __forceinline__ void hetrd(Work& w) {
+    // Column j belongs to wave j % NW. Step k: every wave forms partial sums
+    // of p = M v over its own columns (barrier), combines them into w in its
+    // own lanes, applies the rank-2 update to its own columns, and the owner of
+    // column k+1 then builds the next Householder vector (barrier).
     const int lane = lane_id(), wave = wave_id();
+    if (wave == 0) house(w, 0);
+    __syncthreads();
     for (int k = 0; k < N - 1; ++k) {
         const int m0 = k + 1;
-        if (wave == 0) {
-            const c64 x = (lane >= m0 && lane < N) ? w.B[k * LDA + lane] : cm(0.0, 0.0);
-            const c64 alpha = bcast(x, m0);
-            const double xn2 = wave_sum(lane > m0 ? cabs2(x) : 0.0);
-            c64 t = cm(0.0, 0.0);
-            double beta = alpha.x;
-            c64 v = cm(0.0, 0.0);
-            if (xn2 == 0.0 && alpha.y == 0.0) {
-                if (lane == m0) v = cm(1.0, 0.0);
-            } else {
-                const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
-                beta = (alpha.x >= 0.0) ? -nrm : nrm;
-                t = cm((beta - alpha.x) / beta, -alpha.y / beta);
-                const c64 den = cm(alpha.x - beta, alpha.y);
-                const double dd = 1.0 / cabs2(den);
-                const c64 scal = cm(den.x * dd, -den.y * dd);   // 1 / (alpha - beta)
-                if (lane == m0) v = cm(1.0, 0.0);
-                else if (lane > m0) v = cmul(x, scal);
-            }
-            if (lane > m0 && lane < N) w.B[k * LDA + lane] = v;
-            if (lane < N) w.vbuf[lane] = v;
-            if (lane == 0) {
-                w.e[k] = beta;
-                w.d[k] = w.B[k * LDA + k].x;
-                w.tau[k] = t;
-            }
-        }
-        __syncthreads();
+        const int p = k & 1;
         const c64 t = w.tau[k];
-        if (t.x == 0.0 && t.y == 0.0) continue;
-        if (wave == 0) {
-            // p_i = t * sum_j M(i,j) v_j, reading column i (M Hermitian).
-            c64 acc = cm(0.0, 0.0);
-            if (lane >= m0 && lane < N) {
-                for (int j = m0; j < N; ++j) {
-                    const c64 mji = w.B[lane * LDA + j];
-                    acc = acc + cconjmul(mji, w.vbuf[j]);
-                }
+        if (t.x != 0.0 || t.y != 0.0) {
+            const int lr = lane < N ? lane : N - 1;
+            const c64 vl = (lane < N) ? w.vb[p][lr] : cm(0.0, 0.0);
+            c64 a[CPW], vj[CPW];
+#pragma unroll
+            for (int c = 0; c < CPW; ++c) {
+                const int j = (wave + NW * c < N) ? wave + NW * c : N - 1;
+                a[c] = w.B[j * LDA + lr];
+                vj[c] = w.vb[p][j];
             }
-            const c64 p = cmul(t, acc);
-            const c64 v = (lane < N) ? w.vbuf[lane] : cm(0.0, 0.0);
-            const c64 pvl = cconjmul(p, v);
+            c64 acc = cm(0.0, 0.0);
+#pragma unroll
+            for (int c = 0; c < CPW; ++c) {
+                const int j = wave + NW * c;
+                const c64 prod = cmul(a[c], vj[c]);
+                if (j >= m0 && j < N) acc = acc + prod;
+            }
+            if (lane < N) w.part[0][wave * N + lane] = acc;
+            __syncthreads();
+            c64 pi = cm(0.0, 0.0);
+            if (lane < N) {
+#pragma unroll
+                for (int q = 0; q < NW; ++q) pi = pi + w.part[0][q * N + lane];
+            }
+            pi = cmul(t, pi);
+            const c64 pvl = cconjmul(pi, vl);
             const c64 pv = cm(wave_sum(pvl.x), wave_sum(pvl.y));
             const c64 cc = cmul(t, pv) * -0.5;
-            const c64 wv = p + cmul(cc, v);
-            if (lane < N) w.wbuf[lane] = wv;
-        }
-        __syncthreads();
-        {
-            const c64 vi = (lane < N) ? w.vbuf[lane] : cm(0.0, 0.0);
-            const c64 wi = (lane < N) ? w.wbuf[lane] : cm(0.0, 0.0);
-            for (int j = m0 + wave; j < N; j += NW) {
-                const c64 vj = w.vbuf[j];
-                const c64 wj = w.wbuf[j];
-                if (lane >= m0 && lane < N) {
-                    c64 a = w.B[j * LDA + lane];
-                    a = a - cmulc(vi, wj) - cmulc(wi, vj);
-                    w.B[j * LDA + lane] = a;
-                }
+            const c64 wi = pi + cmul(cc, vl);
+            if (lane < N) w.wb[wave][lane] = wi;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            c64 wj[CPW];
+#pragma unroll
+            for (int c = 0; c < CPW; ++c) {
+                const int j = (wave + NW * c < N) ? wave + NW * c : N - 1;
+                wj[c] = w.wb[wave][j];
+            }
+#pragma unroll
+            for (int c = 0; c < CPW; ++c) {
+                const int j = wave + NW * c;
+                if (j >= m0 && j < N && lane >= m0 && lane < N)
+                    w.B[j * LDA + lane] = a[c] - cmulc(vl, wj[c]) - cmulc(wi, vj[c]);
             }
         }
+        if (k + 1 < N - 1 && wave == (k + 1) % NW) house(w, k + 1);
         __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -278,7 +325,7 @@ __device__ void hetrd(Work& w) {
 
 // Move the Householder vectors from B's lower part to A's strict upper
 // triangle (A(k, i) = v_k(i), i >= k+2) so that B can hold Z and V.
-__device__ void stash_reflectors(Work& w) {
+__device__ __forceinline__ void stash_reflectors(Work& w) {
     const int lane = lane_id(), wave = wave_id();
     for (int k = wave; k < N - 2; k += NW)
         if (lane >= k + 2 && lane < N) w.A[lane * LDA + k] = w.B[k * LDA + lane];
@@ -313,7 +360,7 @@ __device__ __forceinline__ double rsqrt_nr(double t) {
     return y;
 }
 
-__device__ int ql_wave(double& dreg, double& ereg, double (&z)[N]) {
+__device__ __forceinline__ int ql_wave(double& dreg, double& ereg, double (&z)[N]) {
     const int lane = lane_id();
     const double eps = 0x1p-53;
     int l = 0, iter = 0;
@@ -380,7 +427,7 @@ __device__ int ql_wave(double& dreg, double& ereg, double (&z)[N]) {
 }
 
 // Ascending position of lane j's eigenvalue (ties by index).
-__device__ int ascending_rank(double dreg) {
+__device__ __forceinline__ int ascending_rank(double dreg) {
     const int lane = lane_id();
     int rank = 0;
 #pragma unroll
@@ -394,7 +441,7 @@ __device__ int ascending_rank(double dreg) {
 // Wave-level eigen-solve of the tridiagonal held in dsrc/esrc (LDS): writes the
 // ascending eigenvalues to W_out and leaves Z (columns in ascending order) in
 // `z` (lane r = row r) plus its rank permutation in `rank`.
-__device__ int tridiag_eig_wave(const double* dsrc, const double* esrc, double* __restrict__ W_out,
+__device__ __forceinline__ int tridiag_eig_wave(const double* dsrc, const double* esrc, double* __restrict__ W_out,
                                 double (&z)[N], int& rank) {
     const int lane = lane_id();
     double dreg = (lane < N) ? dsrc[lane] : 0.0;
@@ -409,7 +456,7 @@ __device__ int tridiag_eig_wave(const double* dsrc, const double* esrc, double* 
 
 // Write this wave's Z rows into Zs (real, leading dimension LDZ) with columns
 // permuted to ascending eigenvalue order.
-__device__ void store_z_sorted(double* Zs, const double (&z)[N], int rank) {
+__device__ __forceinline__ void store_z_sorted(double* Zs, const double (&z)[N], int rank) {
     const int lane = lane_id();
 #pragma unroll
     for (int j = 0; j < N; ++j) {
@@ -427,7 +474,7 @@ __device__ void store_z_sorted(double* Zs, const double (&z)[N], int rank) {
 // reflectors so a fast wave never overwrites sums a slow wave still reads.
 // Z is read from B's storage as real; V is written back to B (complex).
 // ---------------------------------------------------------------------------
-__device__ void apply_q(Work& w) {
+__device__ __forceinline__ void apply_q(Work& w) {
     const int lane = lane_id(), wave = wave_id();
     const double* Z = reinterpret_cast<const double*>(w.B);
     c64 b[CPW];
@@ -473,7 +520,7 @@ __device__ void apply_q(Work& w) {
 }
 
 // Stage 6: U = L^-H V column by column (lane = row), written straight to HBM.
-__device__ void backtrsm_store(Work& w, c64* __restrict__ Ub) {
+__device__ __forceinline__ void backtrsm_store(Work& w, c64* __restrict__ Ub) {
     const int lane = lane_id(), wave = wave_id();
     c64 b[CPW];
 #pragma unroll
@@ -500,7 +547,7 @@ __device__ void backtrsm_store(Work& w, c64* __restrict__ Ub) {
     }
 }
 
-__device__ void load_pair(Work& w, const c64* __restrict__ Hb, const c64* __restrict__ Sb) {
+__device__ __forceinline__ void load_pair(Work& w, const c64* __restrict__ Hb, const c64* __restrict__ Sb) {
     for (int e = threadIdx.x; e < N * N; e += NT) {
         const int i = e % N, j = e / N;
         w.A[j * LDA + i] = Sb[e];
@@ -509,7 +556,7 @@ __device__ void load_pair(Work& w, const c64* __restrict__ Hb, const c64* __rest
     __syncthreads();
 }
 
-__device__ void fail_store(int code, double* __restrict__ Wb, c64* __restrict__ Ub,
+__device__ __forceinline__ void fail_store(int code, double* __restrict__ Wb, c64* __restrict__ Ub,
                            int* __restrict__ info_b) {
     for (int e = threadIdx.x; e < N * N; e += NT) Ub[e] = cm(0.0, 0.0);
     for (int e = threadIdx.x; e < N; e += NT) Wb[e] = __builtin_nan("");
@@ -523,7 +570,7 @@ __device__ void fail_store(int code, double* __restrict__ Wb, c64* __restrict__ 
 // diagonal's imaginary part, tau_k on the first superdiagonal (k, k+1), and
 // reflector entries v_k(i) at (k, i), i >= k+2.
 // ---------------------------------------------------------------------------
-__device__ void stash_factors(const Work& w, c64* __restrict__ Ub) {
+__device__ __forceinline__ void stash_factors(const Work& w, c64* __restrict__ Ub) {
     for (int e = threadIdx.x; e < N * N; e += NT) {
         const int i = e % N, j = e / N;
         c64 v;
@@ -535,7 +582,7 @@ __device__ void stash_factors(const Work& w, c64* __restrict__ Ub) {
     }
 }
 
-__device__ void unstash_factors(Work& w, const c64* __restrict__ Ub) {
+__device__ __forceinline__ void unstash_factors(Work& w, const c64* __restrict__ Ub) {
     for (int e = threadIdx.x; e < N * N; e += NT) {
         const int i = e % N, j = e / N;
         const c64 v = Ub[e];
@@ -548,6 +595,222 @@ __device__ void unstash_factors(Work& w, const c64* __restrict__ Ub) {
         }
     }
     if (threadIdx.x == 0) w.tau[N - 1] = cm(0.0, 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// Back half for one matrix, run by a single wave with lane j holding column j
+// of V in registers (rows statically indexed; every step of the unrolled row
+// loops is predicated on a wave-uniform condition):
+//   Z (ascending columns) -> V = Q Z (reflectors applied last first: per
+//   reflector a column-local dot product and update, no cross-lane traffic)
+//   -> U = L^-H V (backward substitution, pivots local to each lane).
+// Reflectors, tau, L and 1/L(k,k) are read from the stash layout in global
+// memory (wave-uniform addresses). Z arrives in `zbuf` (this wave's LDS slice,
+// N x N doubles, column-major) written by store_z_slice.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void store_z_slice(double* zbuf, const double (&z)[N], int rank) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const int pos = __builtin_amdgcn_readlane(rank, j);
+        if (lane < N) zbuf[pos * N + lane] = z[j];
+    }
+}
+
+__device__ __forceinline__ void load_z_columns(const double* zbuf, c64 (&b)[N]) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int i = 0; i < N; ++i) b[i] = cm((lane < N) ? zbuf[lane * N + i] : 0.0, 0.0);
+}
+
+// Packed per-wave copies of the factors, staged in the wave's LDS slice:
+//   reflectors: v_k(i), i = k+2..N-1, at RPK(k) + (i - k - 2); tau_k after them;
+//   L by rows:  L(k, i), i <= k, at k(k+1)/2 + i, the diagonal slot holding
+//               (1/L(k,k), 0).
+constexpr int NREFL = (N - 1) * (N - 2) / 2;
+constexpr int NLPK = N * (N + 1) / 2;
+__device__ __forceinline__ int rpk(int k) { return k * (2 * N - k - 3) / 2; }
+
+constexpr int RB = 8;                           // rows per branch-free block
+constexpr int NRB = (N + RB - 1) / RB;
+
+// Stage reflectors + tau (from the stash in global memory) into `sl`.
+__device__ __forceinline__ void stage_reflectors(c64* sl, const c64* __restrict__ Ub) {
+    const int lane = lane_id();
+    for (int e = lane; e < N * N; e += 64) {
+        const int i = e % N, j = e / N;           // stash element (row i, col j)
+        if (i < j - 1) sl[rpk(i) + (j - i - 2)] = Ub[e];
+        else if (i == j - 1) sl[NREFL + i] = Ub[e];
+    }
+}
+
+// Stage L (rows packed, 1/L(k,k) on the diagonal) into `sl`.
+__device__ __forceinline__ void stage_lower(c64* sl, const c64* __restrict__ Ub) {
+    const int lane = lane_id();
+    for (int e = lane; e < N * N; e += 64) {
+        const int i = e % N, j = e / N;
+        if (i > j) sl[i * (i + 1) / 2 + j] = Ub[e];
+        else if (i == j) sl[i * (i + 1) / 2 + i] = cm(Ub[e].y, 0.0);
+    }
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// V <- Q V for this lane's column; reflectors from the staged slice.
+__device__ __forceinline__ void apply_q_wave(const c64* sl, c64 (&b)[N]) {
+    for (int k = N - 2; k >= 0; --k) {
+        const c64 t = sl[NREFL + k];
+        if (t.x == 0.0 && t.y == 0.0) continue;
+        const int base = rpk(k) - k - 2;
+        c64 acc = cm(0.0, 0.0);
+#pragma unroll
+        for (int blk = 0; blk < NRB; ++blk) {
+            if (blk * RB + RB - 1 < k + 1) continue;
+            c64 vv[RB];
+#pragma unroll
+            for (int ii = 0; ii < RB; ++ii) {
+                const int i = blk * RB + ii;
+                if (i < N) {
+                    const c64 r = sl[(i > k + 1) ? base + i : 0];
+                    vv[ii] = (i > k + 1) ? r : cm((i == k + 1) ? 1.0 : 0.0, 0.0);
+                }
+            }
+#pragma unroll
+            for (int ii = 0; ii < RB; ++ii) {
+                const int i = blk * RB + ii;
+                if (i < N) acc = acc + cconjmul(vv[ii], b[i]);
+            }
+        }
+        const c64 y = cmul(t, acc);
+#pragma unroll
+        for (int blk = 0; blk < NRB; ++blk) {
+            if (blk * RB + RB - 1 < k + 1) continue;
+            c64 vv[RB];
+#pragma unroll
+            for (int ii = 0; ii < RB; ++ii) {
+                const int i = blk * RB + ii;
+                if (i < N) {
+                    const c64 r = sl[(i > k + 1) ? base + i : 0];
+                    vv[ii] = (i > k + 1) ? r : cm((i == k + 1) ? 1.0 : 0.0, 0.0);
+                }
+            }
+#pragma unroll
+            for (int ii = 0; ii < RB; ++ii) {
+                const int i = blk * RB + ii;
+                if (i < N) b[i] = b[i] - cmul(vv[ii], y);
+            }
+        }
+    }
+}
+
+// Value barrier: the compiler may not see through it, so a select between
+// statically indexed registers is never rewritten as a dynamically indexed
+// (scratch) array access.
+__device__ __forceinline__ double opq(double x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+__device__ __forceinline__ c64 opq(c64 a) { return cm(opq(a.x), opq(a.y)); }
+
+// U = L^-H V for this lane's column; L from the staged slice (rows packed).
+// Row blocks entirely above the pivot are untouched, blocks entirely below it
+// take a plain update, and only the pivot's block uses selects.
+__device__ __forceinline__ void backtrsm_wave(const c64* sl, c64 (&b)[N]) {
+    for (int k = N - 1; k >= 0; --k) {
+        const int row = k * (k + 1) / 2;
+        const double inv = sl[row + k].x;
+        const int kb = k / RB;
+        c64 bk = cm(0.0, 0.0);
+#pragma unroll
+        for (int blk = 0; blk < NRB; ++blk) {
+            if (blk != kb) continue;
+#pragma unroll
+            for (int ii = 0; ii < RB; ++ii) {
+                const int i = blk * RB + ii;
+                if (i < N) bk = (i == k) ? opq(b[i]) : bk;
+            }
+        }
+        const c64 u = bk * inv;
+#pragma unroll
+        for (int blk = 0; blk < NRB; ++blk) {
+            if (blk > kb) continue;
+            c64 ll[RB];
+#pragma unroll
+            for (int ii = 0; ii < RB; ++ii) {
+                const int i = blk * RB + ii;
+                if (i < N) ll[ii] = sl[row + ((i < k) ? i : 0)];
+            }
+            if (blk < kb) {
+#pragma unroll
+                for (int ii = 0; ii < RB; ++ii) {
+                    const int i = blk * RB + ii;
+                    if (i < N) b[i] = b[i] - cconjmul(ll[ii], u);
+                }
+            } else {
+#pragma unroll
+                for (int ii = 0; ii < RB; ++ii) {
+                    const int i = blk * RB + ii;
+                    if (i < N) {
+                        const c64 nb = b[i] - cconjmul(ll[ii], u);
+                        b[i] = (i < k) ? nb : ((i == k) ? opq(u) : b[i]);
+                    }
+                }
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void store_columns(c64* __restrict__ Ub, const c64 (&b)[N]) {
+    const int lane = lane_id();
+    if (lane < N) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) Ub[lane * N + i] = b[i];
+    }
+}
+
+// Per-wave LDS slice: a Z transpose buffer, then the staged reflectors, then L.
+constexpr size_t SLICE_BYTES =
+    ((((size_t)N * N * 8 > (size_t)NLPK * 16 ? (size_t)N * N * 8 : (size_t)NLPK * 16) >
+      (size_t)(NREFL + N) * 16
+          ? ((size_t)N * N * 8 > (size_t)NLPK * 16 ? (size_t)N * N * 8 : (size_t)NLPK * 16)
+          : (size_t)(NREFL + N) * 16) + 15) / 16 * 16;
+constexpr int ZSLICES = (int)(sizeof(Work) / SLICE_BYTES) < NW ? (int)(sizeof(Work) / SLICE_BYTES) : NW;
+static_assert(ZSLICES >= 1, "work area must hold one slice");
+
+// Back half of every wave's matrix: Z -> V = Q Z (-> U = L^-H V when `with_l`)
+// -> Ub. Waves whose slices do not fit at once take turns; every wave passes
+// the same barriers.
+__device__ __forceinline__ void back_half_waves(Work& w, bool active, const double (&z)[N], int rank,
+                                                c64* __restrict__ Ub, const c64* __restrict__ Lb,
+                                                bool with_l) {
+    const int wave = wave_id();
+    char* base = reinterpret_cast<char*>(&w);
+    for (int r = 0; r < (NW + ZSLICES - 1) / ZSLICES; ++r) {
+        if (active && wave / ZSLICES == r) {
+            c64* sl = reinterpret_cast<c64*>(base + (size_t)(wave % ZSLICES) * SLICE_BYTES);
+            double* zbuf = reinterpret_cast<double*>(sl);
+            c64 b[N];
+            store_z_slice(zbuf, z, rank);
+            wave_lds_sync();
+            load_z_columns(zbuf, b);
+            wave_lds_sync();
+            stage_reflectors(sl, Ub);
+            wave_lds_sync();
+            apply_q_wave(sl, b);
+            if (with_l) {
+                wave_lds_sync();
+                stage_lower(sl, Lb);
+                wave_lds_sync();
+                backtrsm_wave(sl, b);
+            }
+            store_columns(Ub, b);
+        }
+        __syncthreads();
+    }
 }
 
 // Stage data kept per matrix of a workgroup between the front and back halves.
@@ -612,21 +875,15 @@ geneig_fused(const c64* __restrict__ H, const c64* __restrict__ S, double* __res
         }
     }
     __syncthreads();
-    for (int q = 0; q < NW; ++q) {
-        const int b = b0 + q;
-        if (b >= batch) break;
-        if (st.status[q] != 0) continue;
-        const size_t off = (size_t)b * N * N;
-        if (wave == q) store_z_sorted(reinterpret_cast<double*>(w.B), z, rank);
-        unstash_factors(w, U + off);
-        __syncthreads();
-        apply_q(w);
-        __syncthreads();
-        backtrsm_store(w, U + off);
-        if (threadIdx.x == 0) info[b] = st.qinfo[q];
-        __syncthreads();
+    {
+        const int b = b0 + wave;
+        const bool active = b < batch && st.status[wave] == 0;
+        const size_t off = active ? (size_t)b * N * N : 0;
+        back_half_waves(w, active, z, rank, U + off, U + off, true);
+        if (active && lane == 0) info[b] = st.qinfo[wave];
     }
 }
+
 
 // ---------------------------------------------------------------------------
 // Split mode (reference: src/hybrid/hybrid_solver.cpp:295-393), three kernels
@@ -709,47 +966,45 @@ geneig_split_heev(double* __restrict__ W, c64* __restrict__ U, int* __restrict__
         }
     }
     __syncthreads();
-    for (int q = 0; q < NW; ++q) {
-        const int b = b0 + q;
-        if (b >= batch) break;
-        if (st.status[q] != 0) continue;
-        const size_t off = (size_t)b * N * N;
-        if (wave == q) store_z_sorted(reinterpret_cast<double*>(w.B), z, rank);
-        for (int e = threadIdx.x; e < N * N; e += NT) {
-            const int i = e % N, j = e / N;
-            if (i < j - 1) w.A[j * LDA + i] = U[off + e];
-            else if (i == j - 1) w.tau[i] = U[off + e];
-        }
-        if (threadIdx.x == 0) w.tau[N - 1] = cm(0.0, 0.0);
-        __syncthreads();
-        apply_q(w);
-        __syncthreads();
-        for (int e = threadIdx.x; e < N * N; e += NT) U[off + e] = w.B[(e / N) * LDA + (e % N)];
-        if (threadIdx.x == 0) info[b] = st.qinfo[q];
-        __syncthreads();
+    {
+        const int b = b0 + wave;
+        const bool active = b < batch && st.status[wave] == 0;
+        const size_t off = active ? (size_t)b * N * N : 0;
+        back_half_waves(w, active, z, rank, U + off, nullptr, false);
+        if (active && lane == 0) info[b] = st.qinfo[wave];
     }
 }
+
+
+namespace gk {
+// LDS slices for geneig_split_back: as many waves as fit take turns.
+constexpr int BSLICES = (150 * 1024) / (NLPK * 16) < NW ? (150 * 1024) / (NLPK * 16) : NW;
+static_assert(BSLICES >= 1, "one L slice must fit");
+
+}  // namespace gk (split-back helpers)
 
 extern "C" __global__ void __launch_bounds__(GENEIG_NT)
 geneig_split_back(c64* __restrict__ U, const int* __restrict__ info, const c64* __restrict__ Lw,
                   int batch) {
     using namespace gk;
-    __shared__ Work w;
-    const int b = blockIdx.x;
-    if (b >= batch) return;
-    if (info[b] > N) return;
-    const size_t off = (size_t)b * N * N;
-    for (int e = threadIdx.x; e < N * N; e += NT) {
-        const int i = e % N, j = e / N;
-        const c64 l = Lw[off + e];
-        if (i == j) {
-            w.A[j * LDA + i] = cm(l.x, 0.0);
-            w.invL[j] = l.y;
-        } else {
-            w.A[j * LDA + i] = l;
+    // One wave per matrix: V's columns in lanes, L staged (rows packed) in the
+    // wave's LDS slice; waves beyond the slice count take turns.
+    __shared__ c64 lsl[BSLICES][NLPK];
+    const int wave = wave_id(), lane = lane_id();
+    const int b = blockIdx.x * NW + wave;
+    const bool active = b < batch && info[b] <= N;
+    for (int r = 0; r < (NW + BSLICES - 1) / BSLICES; ++r) {
+        if (active && wave / BSLICES == r) {
+            const size_t off = (size_t)b * N * N;
+            c64* sl = lsl[wave % BSLICES];
+            c64 v[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) v[i] = (lane < N) ? U[off + lane * N + i] : cm(0.0, 0.0);
+            stage_lower(sl, Lw + off);
+            wave_lds_sync();
+            backtrsm_wave(sl, v);
+            store_columns(U + off, v);
         }
-        w.B[j * LDA + i] = U[off + e];
+        __syncthreads();
     }
-    __syncthreads();
-    backtrsm_store(w, U + off);
 }

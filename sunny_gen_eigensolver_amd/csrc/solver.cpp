@@ -98,8 +98,9 @@ Solver::Solver(int n, int device_id, int bpb_request, Tuning tuning, CacheMode c
         throw std::invalid_argument("matrix size " + std::to_string(n) +
                                     " exceeds the LDS-resident kernel limit of " +
                                     std::to_string(kMaxLdsOrder));
-    if (bpb_request > 16)
-        throw std::invalid_argument("batches_per_block must be <= 16 (one wave per matrix)");
+    if (bpb_request > 4)
+        throw std::invalid_argument("batches_per_block must be <= 4 (one wave per matrix, "
+                                    "LDS-resident work area)");
     int count = 0;
     SG_HIP(hipGetDeviceCount(&count));
     if (device_id < 0 || device_id >= count)
@@ -157,7 +158,7 @@ int Solver::autotune_(CacheMode cache) {
         if (it != g_tune.end()) return it->second;
     }
     const int suggested = suggested_block_dim(n_);
-    std::vector<int> cands = {64, 128, 256, 512};
+    std::vector<int> cands = {64, 128, 256};
     const int batch = 1024;
     const size_t nn = (size_t)n_ * n_;
     std::vector<std::complex<double>> H, S;
@@ -269,8 +270,8 @@ void Solver::launch_hybrid_(const void* H, const void* S, double* W, void* U, in
     SG_HIP(hipModuleLaunchKernel(k_heev_, grid2, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream, a2,
                                  nullptr));
     void* a3[] = {&U, &info, &L, &batch};
-    SG_HIP(hipModuleLaunchKernel(k_back_, (unsigned)batch, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream,
-                                 a3, nullptr));
+    SG_HIP(hipModuleLaunchKernel(k_back_, grid2, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream, a3,
+                                 nullptr));
 }
 
 }  // namespace sgeig
