@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-split", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=4096)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="launch one device copy of this many MiB first (HBM counter calibration)")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -100,6 +102,15 @@ def main():
     torch.cuda.synchronize()
     log(f"generated {B} x N={n} inputs on device in {time.perf_counter() - t0:.1f}s")
 
+    if a.copy_probe_mib > 0:
+        # An elementwise kernel (not a DMA copy) that reads and writes exactly
+        # copy_probe_mib MiB each: the reference point for HBM byte counters.
+        src = torch.ones(a.copy_probe_mib * (1 << 20) // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     W = torch.empty((B, n), dtype=torch.float64, device=dev)
     U = torch.empty((B, n, n), dtype=torch.complex128, device=dev)
     info = torch.zeros((B,), dtype=torch.int32, device=dev)
