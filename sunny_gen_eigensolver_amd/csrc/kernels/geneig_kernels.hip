@@ -82,10 +82,24 @@ __device__ __forceinline__ double bcast(double v, int src) {
 }
 __device__ __forceinline__ c64 bcast(c64 v, int src) { return cm(bcast(v.x, src), bcast(v.y, src)); }
 
+// DPP move of a double between lanes (both halves with the same control).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 64 lanes, returned in every lane: a butterfly inside each
+// 16-lane row with DPP (quad_perm [1,0,3,2], quad_perm [2,3,0,1],
+// row_half_mirror, row_mirror), then the four row sums via v_readlane.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += dpp_f64<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);   // row_half_mirror
+    v += dpp_f64<0x140>(v);   // row_mirror
+    return (bcast(v, 0) + bcast(v, 16)) + (bcast(v, 32) + bcast(v, 48));
 }
 
 // Shared-memory work area of one matrix.
@@ -387,38 +401,32 @@ __device__ __forceinline__ int ql_wave(double& dreg, double& ereg, double (&z)[N
         g = bcast(dreg, m) - d_l + e_l / (g + copysign(r, g));
         double s = 1.0, c = 1.0, p = 0.0;
         double d_up = bcast(dreg, m);      // original d[i+1] for the current i
-        bool underflow = false;
 #pragma unroll
         for (int i = N - 2; i >= 0; --i) {
-            if (i < m && i >= l && !underflow) {
+            if (i < m && i >= l) {
                 const double e_i = bcast(ereg, i);
                 const double d_i = bcast(dreg, i);
                 const double f = s * e_i;
                 const double b = c * e_i;
-                const double t = fma(f, f, g * g);
-                if (t == 0.0) {
-                    ereg = sel_lane(ereg, 0.0, i + 1);
-                    dreg = sel_lane(dreg, d_up - p, i + 1);
-                    ereg = sel_lane(ereg, 0.0, m);
-                    underflow = true;
-                } else {
-                    const double y = rsqrt_nr(t);
-                    ereg = sel_lane(ereg, t * y, i + 1);
-                    s = f * y;
-                    c = g * y;
-                    g = d_up - p;
-                    const double r2 = fma(d_i - g, s, 2.0 * c * b);
-                    p = s * r2;
-                    dreg = sel_lane(dreg, g + p, i + 1);
-                    g = fma(c, r2, -b);
-                    const double zi = z[i], zi1 = z[i + 1];
-                    z[i + 1] = fma(s, zi, c * zi1);
-                    z[i] = fma(c, zi, -(s * zi1));
-                    d_up = d_i;
-                }
+                // t > 0: every e_i in [l, m) is above the deflation threshold,
+                // so f = s e_i != 0 (s starts at 1 and stays nonzero). A
+                // floor keeps t finite even for inputs scaled near underflow.
+                const double t = fmax(fma(f, f, g * g), 0x1p-1022);
+                const double y = rsqrt_nr(t);
+                ereg = sel_lane(ereg, t * y, i + 1);
+                s = f * y;
+                c = g * y;
+                g = d_up - p;
+                const double r2 = fma(d_i - g, s, 2.0 * c * b);
+                p = s * r2;
+                dreg = sel_lane(dreg, g + p, i + 1);
+                g = fma(c, r2, -b);
+                const double zi = z[i], zi1 = z[i + 1];
+                z[i + 1] = fma(s, zi, c * zi1);
+                z[i] = fma(c, zi, -(s * zi1));
+                d_up = d_i;
             }
         }
-        if (underflow) continue;
         dreg = sel_lane(dreg, d_l - p, l);
         ereg = sel_lane(ereg, g, l);
         ereg = sel_lane(ereg, 0.0, m);
