@@ -80,6 +80,7 @@ def main():
     ap.add_argument("--seed", type=lambda s: int(s, 0), default=0x5EED0002)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-split", action="store_true")
+    ap.add_argument("--bpb", type=int, default=0, help="matrices per workgroup (0 = solver default)")
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="launch one device copy of this many MiB first (HBM counter calibration)")
@@ -129,7 +130,7 @@ def main():
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / steps   # ms per step
 
-    solver = sg.Solver(n, kernel="unified")
+    solver = sg.Solver(n, kernel="unified", bpb_request=a.bpb)
     log(f"unified: wg={solver.block_dim_x} lds={solver.shared_mem_bytes} B cache={solver.cache_layer}")
     t_wall = time.perf_counter()
     ms = timed(solver, a.steps, a.warmup)

@@ -146,7 +146,7 @@ __device__ __forceinline__ int potrf(Work& w) {
     };
     if (threadIdx.x == 0) w.flag[1] = 0;
     __syncthreads();
-    if (wave == 0) finish_column(0);
+    if (wave_id() == 0) finish_column(0);
     __syncthreads();
     for (int k = 0; k < N; ++k) {
         const int fail = w.flag[1];
@@ -821,6 +821,370 @@ __device__ __forceinline__ void back_half_waves(Work& w, bool active, const doub
     }
 }
 
+// ===========================================================================
+// Packed front half (used by geneig_fused). Hermitian and triangular N x N
+// matrices are kept as their lower triangle, column-major packed: element
+// (i, j), i >= j, at pk(j) + i - j. Two such triangles (L then M, and
+// W = L^-1) plus an 8-column block buffer make the whole LDS footprint, so
+// two workgroups share a CU.
+// ===========================================================================
+constexpr int NPK = N * (N + 1) / 2;
+constexpr int CB = 8;                              // columns per branch-free chunk
+constexpr int ZB = 8;                              // columns per block of M = W H W^H
+constexpr int NZB = (N + ZB - 1) / ZB;
+constexpr int ZPW = (ZB + NW - 1) / NW;            // block columns per wave
+__device__ __forceinline__ int pk(int j) { return j * N - (j * (j - 1)) / 2; }
+
+// A copy of `x` the compiler cannot prove loop-invariant: per-column LDS
+// addresses derived from it are recomputed each step instead of being held
+// live (dozens of VGPRs) across a whole loop.
+__device__ __forceinline__ int fresh(int x) {
+    x = __builtin_amdgcn_readfirstlane(x);   // wave-uniform by construction
+    asm volatile("" : "+s"(x));
+    return x;
+}
+
+struct WorkP {
+    c64 L[NPK];            // S -> L; then M -> T (reflectors below the subdiagonal)
+    c64 Wi[NPK];           // W = L^-1
+    union {
+        c64 zj[N * ZB];    // Z block of H W^H
+        struct {
+            c64 vb[2][N];
+            c64 wb[NW][N];
+            c64 part[NW * N];
+        } h;
+    } u;
+    c64 tau[N];
+    double invL[N];
+    double d[N];
+    double e[N];
+    int flag[2];
+};
+
+__device__ __forceinline__ void load_lower_packed(c64* P, const c64* __restrict__ src) {
+    for (int e = threadIdx.x; e < N * N; e += NT) {
+        const int i = e % N, j = e / N;
+        if (i >= j) P[pk(j) + i - j] = src[e];
+    }
+}
+
+// Cholesky on the packed lower triangle: column j belongs to wave j % NW; the
+// owner of column k+1 finishes it (pivot check, scaling, publication in
+// vb[(k+1) & 1]) right after its own updates of step k. One barrier per step.
+__device__ __forceinline__ int potrf_packed(WorkP& w) {
+    const int lane = lane_id();
+    const int lr = lane < N ? lane : N - 1;
+    auto finish_column = [&](int k) {
+        const c64 a = (lane >= k && lane < N) ? w.L[pk(k) + lane - k] : cm(0.0, 0.0);
+        const double dkk = bcast(a.x, k);
+        if (!(dkk > 0.0)) {
+            if (lane == 0) w.flag[1] = k + 1;
+            return;
+        }
+        const double rinv = 1.0 / sqrt(dkk);
+        const c64 l = a * rinv;
+        if (lane >= k && lane < N) w.L[pk(k) + lane - k] = l;
+        if (lane < N) w.u.h.vb[k & 1][lane] = l;
+        if (lane == 0) w.invL[k] = rinv;
+    };
+    if (threadIdx.x == 0) w.flag[1] = 0;
+    __syncthreads();
+    if (wave_id() == 0) finish_column(0);
+    __syncthreads();
+    for (int k = 0; k < N; ++k) {
+        const int fail = w.flag[1];
+        if (fail) return fail;
+        const int p = k & 1;
+        const int wave = fresh(wave_id());
+        const c64 li = w.u.h.vb[p][lr];
+#pragma unroll
+        for (int c0 = 0; c0 < CPW; c0 += CB) {
+            __builtin_amdgcn_sched_barrier(0);   // keep chunks from interleaving (register pressure)
+            c64 a[CB], lj[CB];
+#pragma unroll
+            for (int cc = 0; cc < CB; ++cc) {
+                const int c = c0 + cc;
+                const int j = (c < CPW && wave + NW * c < N) ? wave + NW * c : N - 1;
+                a[cc] = w.L[pk(j) + (lr >= j ? lr - j : 0)];
+                lj[cc] = w.u.h.vb[p][j];
+            }
+#pragma unroll
+            for (int cc = 0; cc < CB; ++cc) {
+                const int c = c0 + cc;
+                const int j = wave + NW * c;
+                if (c < CPW && j > k && j < N && lane >= j && lane < N)
+                    w.L[pk(j) + lane - j] = a[cc] - cmulc(li, lj[cc]);
+            }
+        }
+        if (k + 1 < N && wave == (k + 1) % NW) finish_column(k + 1);
+        __syncthreads();
+    }
+    return 0;
+}
+
+// W = L^-1 by forward substitution on the identity, column j owned by wave
+// j % NW (lane = row); L is read only, so no barrier is needed between steps.
+__device__ __forceinline__ void trtri_packed(WorkP& w) {
+    const int lane = lane_id();
+    const int lr = lane < N ? lane : N - 1;
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+        const int j = wave_id() + NW * c;
+        if (j < N && lane >= j && lane < N) w.Wi[pk(j) + lane - j] = cm(lane == j ? 1.0 : 0.0, 0.0);
+    }
+    for (int k = 0; k < N; ++k) {
+        const int wave = fresh(wave_id());
+        const double inv = w.invL[k];
+        const c64 lik = (lr > k) ? w.L[pk(k) + lr - k] : cm(0.0, 0.0);
+#pragma unroll
+        for (int c0 = 0; c0 < CPW; c0 += CB) {
+            __builtin_amdgcn_sched_barrier(0);   // keep chunks from interleaving (register pressure)
+            c64 pv[CB], bv[CB];
+#pragma unroll
+            for (int cc = 0; cc < CB; ++cc) {
+                const int j0 = wave + NW * (c0 + cc);
+                const int j = (c0 + cc < CPW && j0 <= k) ? j0 : 0;
+                pv[cc] = w.Wi[pk(j) + k - j];
+                bv[cc] = w.Wi[pk(j) + (lr >= j ? lr - j : 0)];
+            }
+#pragma unroll
+            for (int cc = 0; cc < CB; ++cc) {
+                const int j = wave + NW * (c0 + cc);
+                const c64 x = pv[cc] * inv;
+                const c64 nb = (lane == k) ? x : bv[cc] - cmul(lik, x);
+                if (c0 + cc < CPW && j <= k && lane >= k && lane < N) w.Wi[pk(j) + lane - j] = nb;
+            }
+        }
+    }
+}
+
+// M = W H W^H (lower triangle, packed into w.L), eight columns at a time:
+// Z = H (W rows J)^H with H read from global memory (coalesced columns), then
+// M(:, J) = W Z on rows i >= j. Column c of a block belongs to wave c % NW.
+__device__ __forceinline__ void form_m(WorkP& w, const c64* __restrict__ Hb) {
+    const int lane = lane_id(), wave = wave_id();
+    const int lr = lane < N ? lane : N - 1;
+#pragma unroll
+    for (int jb = 0; jb < NZB; ++jb) {
+        const int j0 = jb * ZB;
+        c64 acc[ZPW];
+#pragma unroll
+        for (int cc = 0; cc < ZPW; ++cc) acc[cc] = cm(0.0, 0.0);
+        const int kend = (j0 + ZB < N) ? j0 + ZB : N;
+#pragma unroll 4
+        for (int k = 0; k < kend; ++k) {
+            const c64 h = Hb[k * N + lr];
+#pragma unroll
+            for (int cc = 0; cc < ZPW; ++cc) {
+                const int j = j0 + wave + NW * cc;
+                const bool on = (j < N) && (k <= j) && (wave + NW * cc < ZB);
+                const c64 wjk = w.Wi[on ? pk(k) + j - k : 0];
+                acc[cc] = acc[cc] + cmulc(h, on ? wjk : cm(0.0, 0.0));
+            }
+        }
+#pragma unroll
+        for (int cc = 0; cc < ZPW; ++cc) {
+            const int c = wave + NW * cc;
+            if (c < ZB && lane < N) w.u.zj[c * N + lane] = acc[cc];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int cc = 0; cc < ZPW; ++cc) acc[cc] = cm(0.0, 0.0);
+#pragma unroll 4
+        for (int k = 0; k < N; ++k) {
+            const c64 wik = (lr >= k) ? w.Wi[pk(k) + lr - k] : cm(0.0, 0.0);
+#pragma unroll
+            for (int cc = 0; cc < ZPW; ++cc) {
+                const int c = wave + NW * cc;
+                const c64 z = w.u.zj[(c < ZB ? c : 0) * N + k];
+                acc[cc] = acc[cc] + cmul(wik, z);
+            }
+        }
+#pragma unroll
+        for (int cc = 0; cc < ZPW; ++cc) {
+            const int c = wave + NW * cc;
+            const int j = j0 + c;
+            if (c < ZB && j < N && lane >= j && lane < N) w.L[pk(j) + lane - j] = acc[cc];
+        }
+        __syncthreads();
+    }
+}
+
+// Householder vector for column k (packed M), by the wave owning column k.
+__device__ __forceinline__ void house_packed(WorkP& w, int k) {
+    const int lane = lane_id();
+    const int m0 = k + 1;
+    const c64 x = (lane >= m0 && lane < N) ? w.L[pk(k) + lane - k] : cm(0.0, 0.0);
+    const c64 alpha = bcast(x, m0);
+    const double xn2 = wave_sum(lane > m0 ? cabs2(x) : 0.0);
+    c64 t = cm(0.0, 0.0);
+    double beta = alpha.x;
+    c64 v = cm(0.0, 0.0);
+    if (xn2 == 0.0 && alpha.y == 0.0) {
+        if (lane == m0) v = cm(1.0, 0.0);
+    } else {
+        const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+        beta = (alpha.x >= 0.0) ? -nrm : nrm;
+        t = cm((beta - alpha.x) / beta, -alpha.y / beta);
+        const c64 den = cm(alpha.x - beta, alpha.y);
+        const double dd = 1.0 / cabs2(den);
+        const c64 scal = cm(den.x * dd, -den.y * dd);   // 1 / (alpha - beta)
+        if (lane == m0) v = cm(1.0, 0.0);
+        else if (lane > m0) v = cmul(x, scal);
+    }
+    if (lane > m0 && lane < N) w.L[pk(k) + lane - k] = v;
+    if (lane < N) w.u.h.vb[k & 1][lane] = v;
+    if (lane == 0) {
+        w.e[k] = beta;
+        w.d[k] = w.L[pk(k)].x;
+        w.tau[k] = t;
+    }
+}
+
+// Tridiagonalisation of the packed Hermitian M (see hetrd for the schedule).
+// Lane i reads row i of the full matrix from the lower triangle: M(i, j) for
+// i >= j, conj(M(j, i)) for i < j; the rank-2 update touches the lower part.
+__device__ __forceinline__ void hetrd_packed(WorkP& w) {
+    const int lane = lane_id();
+    const int lr = lane < N ? lane : N - 1;
+    if (wave_id() == 0) house_packed(w, 0);
+    __syncthreads();
+    for (int k = 0; k < N - 1; ++k) {
+        const int m0 = k + 1;
+        const int p = k & 1;
+        const int wave = fresh(wave_id());
+        const c64 t = w.tau[k];
+        if (t.x != 0.0 || t.y != 0.0) {
+            const c64 vl = (lane < N) ? w.u.h.vb[p][lr] : cm(0.0, 0.0);
+            c64 acc = cm(0.0, 0.0);
+#pragma unroll
+            for (int c0 = 0; c0 < CPW; c0 += CB) {
+                __builtin_amdgcn_sched_barrier(0);   // keep chunks from interleaving (register pressure)
+                c64 a[CB], vj[CB];
+#pragma unroll
+                for (int cc = 0; cc < CB; ++cc) {
+                    const int c = c0 + cc;
+                    const int j = (c < CPW && wave + NW * c < N) ? wave + NW * c : N - 1;
+                    const c64 m = w.L[(lr >= j) ? pk(j) + lr - j : pk(lr) + j - lr];
+                    a[cc] = (lr >= j) ? m : cconj(m);
+                    vj[cc] = w.u.h.vb[p][j];
+                }
+#pragma unroll
+                for (int cc = 0; cc < CB; ++cc) {
+                    const int c = c0 + cc;
+                    const int j = wave + NW * c;
+                    const c64 prod = cmul(a[cc], vj[cc]);
+                    if (c < CPW && j >= m0 && j < N) acc = acc + prod;
+                }
+            }
+            if (lane < N) w.u.h.part[wave * N + lane] = acc;
+            __syncthreads();
+            c64 pi = cm(0.0, 0.0);
+            if (lane < N) {
+#pragma unroll
+                for (int q = 0; q < NW; ++q) pi = pi + w.u.h.part[q * N + lane];
+            }
+            pi = cmul(t, pi);
+            const c64 pvl = cconjmul(pi, vl);
+            const c64 pv = cm(wave_sum(pvl.x), wave_sum(pvl.y));
+            const c64 cc = cmul(t, pv) * -0.5;
+            const c64 wi = pi + cmul(cc, vl);
+            if (lane < N) w.u.h.wb[wave][lane] = wi;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int c0 = 0; c0 < CPW; c0 += CB) {
+                __builtin_amdgcn_sched_barrier(0);   // keep chunks from interleaving (register pressure)
+                c64 a[CB], vj[CB], wj[CB];
+#pragma unroll
+                for (int cc = 0; cc < CB; ++cc) {
+                    const int c = c0 + cc;
+                    const int j = (c < CPW && wave + NW * c < N) ? wave + NW * c : N - 1;
+                    a[cc] = w.L[pk(j) + (lr >= j ? lr - j : 0)];
+                    vj[cc] = w.u.h.vb[p][j];
+                    wj[cc] = w.u.h.wb[wave][j];
+                }
+#pragma unroll
+                for (int cc = 0; cc < CB; ++cc) {
+                    const int c = c0 + cc;
+                    const int j = wave + NW * c;
+                    if (c < CPW && j >= m0 && j < N && lane >= j && lane < N)
+                        w.L[pk(j) + lane - j] = a[cc] - cmulc(vl, wj[cc]) - cmulc(wi, vj[cc]);
+                }
+            }
+        }
+        if (k + 1 < N - 1 && wave == (k + 1) % NW) house_packed(w, k + 1);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        w.d[N - 1] = w.L[pk(N - 1)].x;
+        w.e[N - 1] = 0.0;
+        w.tau[N - 1] = cm(0.0, 0.0);
+    }
+    __syncthreads();
+}
+
+// Stash in the matrix's U slot: W on and below the diagonal, tau_k at
+// (k, k+1), reflector entries v_k(i) at (k, i), i >= k+2.
+__device__ __forceinline__ void stash_packed(const WorkP& w, c64* __restrict__ Ub) {
+    for (int e = threadIdx.x; e < N * N; e += NT) {
+        const int i = e % N, j = e / N;
+        c64 v;
+        if (i >= j) v = w.Wi[pk(j) + i - j];
+        else if (i == j - 1) v = w.tau[i];
+        else v = w.L[pk(i) + j - i];
+        Ub[e] = v;
+    }
+}
+
+// Stage W (columns packed) from the stash into a wave's slice.
+__device__ __forceinline__ void stage_w(c64* sl, const c64* __restrict__ Ub) {
+    const int lane = lane_id();
+    for (int e = lane; e < N * N; e += 64) {
+        const int i = e % N, j = e / N;
+        if (i >= j) sl[pk(j) + i - j] = Ub[e];
+    }
+}
+
+// U = W^H V for this lane's column, in place in ascending row order:
+// u_i = sum_{k >= i} conj(W(k, i)) v_k.
+__device__ __forceinline__ void trmm_wh_wave(const c64* sl, c64 (&b)[N]) {
+    for (int i = 0; i < N; ++i) {
+        const int col = pk(i) - i;            // W(k, i) at col + k, k >= i
+        c64 acc = cm(0.0, 0.0);
+#pragma unroll
+        for (int blk = 0; blk < NRB; ++blk) {
+            if (blk * RB + RB - 1 < i) continue;
+            c64 ww[RB];
+#pragma unroll
+            for (int ii = 0; ii < RB; ++ii) {
+                const int k = blk * RB + ii;
+                if (k < N) ww[ii] = sl[(k >= i) ? col + k : 0];
+            }
+#pragma unroll
+            for (int ii = 0; ii < RB; ++ii) {
+                const int k = blk * RB + ii;
+                if (k < N) {
+                    const c64 t = cconjmul(ww[ii], b[k]);
+                    if (k >= i) acc = acc + t;
+                }
+            }
+        }
+        const int ib = i / RB;
+#pragma unroll
+        for (int blk = 0; blk < NRB; ++blk) {
+            if (blk != ib) continue;
+#pragma unroll
+            for (int ii = 0; ii < RB; ++ii) {
+                const int k = blk * RB + ii;
+                if (k < N) b[k] = (k == i) ? opq(acc) : b[k];
+            }
+        }
+    }
+}
+
 // Stage data kept per matrix of a workgroup between the front and back halves.
 struct Staged {
     double d[NW][N];
@@ -832,6 +1196,39 @@ struct Staged {
 }  // namespace gk
 
 using gk::c64;
+
+namespace gk {
+// Back half of the fused kernel: each wave finishes its own matrix in its
+// LDS slice of the (then idle) work area: Z -> V = Q Z -> U = W^H V.
+constexpr int PSLICES = (int)(sizeof(WorkP) / SLICE_BYTES) < NW ? (int)(sizeof(WorkP) / SLICE_BYTES) : NW;
+static_assert(PSLICES >= 1, "packed work area must hold one slice");
+
+__device__ __forceinline__ void back_half_packed(WorkP& w, bool active, const double (&z)[N], int rank,
+                                                 c64* __restrict__ Ub) {
+    const int wave = wave_id();
+    char* base = reinterpret_cast<char*>(&w);
+    for (int r = 0; r < (NW + PSLICES - 1) / PSLICES; ++r) {
+        if (active && wave / PSLICES == r) {
+            c64* sl = reinterpret_cast<c64*>(base + (size_t)(wave % PSLICES) * SLICE_BYTES);
+            double* zbuf = reinterpret_cast<double*>(sl);
+            c64 b[N];
+            store_z_slice(zbuf, z, rank);
+            wave_lds_sync();
+            load_z_columns(zbuf, b);
+            wave_lds_sync();
+            stage_reflectors(sl, Ub);
+            wave_lds_sync();
+            apply_q_wave(sl, b);
+            wave_lds_sync();
+            stage_w(sl, Ub);
+            wave_lds_sync();
+            trmm_wh_wave(sl, b);
+            store_columns(Ub, b);
+        }
+        __syncthreads();
+    }
+}
+}  // namespace gk
 
 // ---------------------------------------------------------------------------
 // Fused solve. A workgroup of NW waves takes NW consecutive matrices: it runs
@@ -894,123 +1291,101 @@ geneig_fused(const c64* __restrict__ H, const c64* __restrict__ S, double* __res
 
 
 // ---------------------------------------------------------------------------
-// Split mode (reference: src/hybrid/hybrid_solver.cpp:295-393), three kernels
-// over the same device stages:
-//   geneig_split_reduce : potrf + two-sided solve; L -> Lw, M -> U (scratch)
-//   geneig_split_heev   : tridiagonalise M, QL, sort, V = Q Z -> U, W
-//   geneig_split_back   : U = L^-H V
-// Lw holds L's lower triangle with 1/L(k,k) in the imaginary part of the
-// diagonal, so the back substitution uses the same reciprocal pivots as the
-// fused kernel. A matrix whose Cholesky failed carries info > N and is skipped
-// by the later kernels. The reduce and back kernels take one matrix per
-// workgroup; the heev kernel takes NW, like the fused kernel.
+// Split mode (reference: src/hybrid/hybrid_solver.cpp:295-393): the same
+// stages as three kernels, each launched with the occupancy its register and
+// LDS footprint allow.
+//   geneig_split_front : Cholesky, W = L^-1, M = W H W^H, tridiagonalisation
+//                        (packed LDS, one matrix per workgroup). The stash
+//                        (W, reflectors, tau) goes to U, the diagonal of T to
+//                        W and its off-diagonal to E.
+//   geneig_split_ql    : one wave per matrix, no LDS: QL on (d, e), sorted
+//                        eigenvalues to W, Z (ascending columns) to Zw.
+//   geneig_split_back  : one wave per matrix: V = Q Z, U = W^H V.
+// A matrix whose Cholesky failed carries info > N and is skipped downstream.
 // ---------------------------------------------------------------------------
-extern "C" __global__ void __launch_bounds__(GENEIG_NT)
-geneig_split_reduce(const c64* __restrict__ H, const c64* __restrict__ S, double* __restrict__ W,
-                    c64* __restrict__ U, int* __restrict__ info, c64* __restrict__ Lw, int batch) {
+extern "C" __global__ void __launch_bounds__(GENEIG_NT, 2)
+geneig_split_front(const c64* __restrict__ H, const c64* __restrict__ S, double* __restrict__ W,
+                   c64* __restrict__ U, int* __restrict__ info, double* __restrict__ E, int batch) {
     using namespace gk;
-    __shared__ Work w;
+    __shared__ WorkP w;
     const int b = blockIdx.x;
     if (b >= batch) return;
     const size_t off = (size_t)b * N * N;
-    load_pair(w, H + off, S + off);
-    const int pinfo = potrf(w);
+    load_lower_packed(w.L, S + off);
+    const int pinfo = potrf_packed(w);
     if (pinfo) {
         fail_store(N + pinfo, W + (size_t)b * N, U + off, info + b);
         return;
     }
-    trsm_left(w);
+    trtri_packed(w);
     __syncthreads();
-    trsm_right(w);
-    __syncthreads();
-    for (int e = threadIdx.x; e < N * N; e += NT) {
-        const int i = e % N, j = e / N;
-        c64 l = (i >= j) ? w.A[j * LDA + i] : cm(0.0, 0.0);
-        if (i == j) l.y = w.invL[j];
-        Lw[off + e] = l;
-        U[off + e] = w.B[j * LDA + i];
+    form_m(w, H + off);
+    hetrd_packed(w);
+    stash_packed(w, U + off);
+    if (threadIdx.x < N) {
+        W[(size_t)b * N + threadIdx.x] = w.d[threadIdx.x];
+        E[(size_t)b * N + threadIdx.x] = w.e[threadIdx.x];
     }
     if (threadIdx.x == 0) info[b] = 0;
 }
 
 extern "C" __global__ void __launch_bounds__(GENEIG_NT)
-geneig_split_heev(double* __restrict__ W, c64* __restrict__ U, int* __restrict__ info, int batch) {
+geneig_split_ql(double* __restrict__ W, const double* __restrict__ E, double* __restrict__ Zw,
+                int* __restrict__ info, int batch) {
     using namespace gk;
-    __shared__ Work w;
-    __shared__ Staged st;
-    const int lane = lane_id(), wave = wave_id();
-    const int b0 = blockIdx.x * NW;
-    for (int q = 0; q < NW; ++q) {
-        const int b = b0 + q;
-        if (b >= batch) break;
-        if (info[b] != 0) {
-            if (threadIdx.x == 0) st.status[q] = 1;
-            continue;
-        }
-        const size_t off = (size_t)b * N * N;
-        for (int e = threadIdx.x; e < N * N; e += NT) w.B[(e / N) * LDA + (e % N)] = U[off + e];
-        __syncthreads();
-        hetrd(w);
-        // Reflectors and tau into U's strict upper part (the lower part is unused).
-        for (int e = threadIdx.x; e < N * N; e += NT) {
-            const int i = e % N, j = e / N;
-            if (i == j - 1) U[off + e] = w.tau[i];
-            else if (i < j - 1) U[off + e] = w.B[i * LDA + j];
-        }
-        if (threadIdx.x < N) {
-            st.d[q][threadIdx.x] = w.d[threadIdx.x];
-            st.e[q][threadIdx.x] = w.e[threadIdx.x];
-        }
-        if (threadIdx.x == 0) st.status[q] = 0;
-        __syncthreads();
-    }
+    const int b = blockIdx.x * NW + wave_id();
+    if (b >= batch) return;
+    if (info[b] != 0) return;
+    const int lane = lane_id();
+    double* Wb = W + (size_t)b * N;
+    const double* Eb = E + (size_t)b * N;
+    double dreg = (lane < N) ? Wb[lane] : 0.0;
+    double ereg = (lane < N - 1) ? Eb[lane] : 0.0;
     double z[N];
-    int rank = 0;
-    {
-        const int b = b0 + wave;
-        if (b < batch && st.status[wave] == 0) {
-            const int qi = tridiag_eig_wave(st.d[wave], st.e[wave], W + (size_t)b * N, z, rank);
-            if (lane == 0) st.qinfo[wave] = qi;
-        }
+#pragma unroll
+    for (int j = 0; j < N; ++j) z[j] = (lane == j) ? 1.0 : 0.0;
+    const int qi = ql_wave(dreg, ereg, z);
+    const int rank = ascending_rank(lane < N ? dreg : __builtin_inf());
+    if (lane < N) Wb[rank] = dreg;
+    double* Zb = Zw + (size_t)b * N * N;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const int pos = __builtin_amdgcn_readlane(rank, j);
+        if (lane < N) Zb[pos * N + lane] = z[j];
     }
-    __syncthreads();
-    {
-        const int b = b0 + wave;
-        const bool active = b < batch && st.status[wave] == 0;
-        const size_t off = active ? (size_t)b * N * N : 0;
-        back_half_waves(w, active, z, rank, U + off, nullptr, false);
-        if (active && lane == 0) info[b] = st.qinfo[wave];
-    }
+    if (lane == 0) info[b] = qi;
 }
 
-
 namespace gk {
-// LDS slices for geneig_split_back: as many waves as fit take turns.
-constexpr int BSLICES = (150 * 1024) / (NLPK * 16) < NW ? (150 * 1024) / (NLPK * 16) : NW;
-static_assert(BSLICES >= 1, "one L slice must fit");
-
-}  // namespace gk (split-back helpers)
+constexpr int BACK_SLICES = (150 * 1024) / (int)SLICE_BYTES < NW ? (150 * 1024) / (int)SLICE_BYTES : NW;
+static_assert(BACK_SLICES >= 1, "one back-half slice must fit");
+}  // namespace gk
 
 extern "C" __global__ void __launch_bounds__(GENEIG_NT)
-geneig_split_back(c64* __restrict__ U, const int* __restrict__ info, const c64* __restrict__ Lw,
+geneig_split_back(c64* __restrict__ U, const double* __restrict__ Zw, const int* __restrict__ info,
                   int batch) {
     using namespace gk;
-    // One wave per matrix: V's columns in lanes, L staged (rows packed) in the
-    // wave's LDS slice; waves beyond the slice count take turns.
-    __shared__ c64 lsl[BSLICES][NLPK];
+    // One wave per matrix: lane j holds column j of V; reflectors and W are
+    // staged (packed) in the wave's LDS slice; waves beyond the slice count
+    // take turns.
+    __shared__ __attribute__((aligned(16))) char slices[BACK_SLICES][SLICE_BYTES];
     const int wave = wave_id(), lane = lane_id();
     const int b = blockIdx.x * NW + wave;
     const bool active = b < batch && info[b] <= N;
-    for (int r = 0; r < (NW + BSLICES - 1) / BSLICES; ++r) {
-        if (active && wave / BSLICES == r) {
+    for (int r = 0; r < (NW + BACK_SLICES - 1) / BACK_SLICES; ++r) {
+        if (active && wave / BACK_SLICES == r) {
             const size_t off = (size_t)b * N * N;
-            c64* sl = lsl[wave % BSLICES];
+            c64* sl = reinterpret_cast<c64*>(slices[wave % BACK_SLICES]);
             c64 v[N];
 #pragma unroll
-            for (int i = 0; i < N; ++i) v[i] = (lane < N) ? U[off + lane * N + i] : cm(0.0, 0.0);
-            stage_lower(sl, Lw + off);
+            for (int i = 0; i < N; ++i) v[i] = cm((lane < N) ? Zw[off + lane * N + i] : 0.0, 0.0);
+            stage_reflectors(sl, U + off);
             wave_lds_sync();
-            backtrsm_wave(sl, v);
+            apply_q_wave(sl, v);
+            wave_lds_sync();
+            stage_w(sl, U + off);
+            wave_lds_sync();
+            trmm_wh_wave(sl, v);
             store_columns(U + off, v);
         }
         __syncthreads();

@@ -133,8 +133,8 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
     block_dim_ = block_dim;
     bpb_ = block_dim / 64;
     fused_ = module_->get("geneig_fused");
-    k_reduce_ = module_->get("geneig_split_reduce");
-    k_heev_ = module_->get("geneig_split_heev");
+    k_reduce_ = module_->get("geneig_split_front");
+    k_heev_ = module_->get("geneig_split_ql");
     k_back_ = module_->get("geneig_split_back");
     int lds = 0;
     SG_HIP(hipFuncGetAttribute(&lds, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, fused_));
@@ -254,22 +254,26 @@ void Solver::ensure_workspace_(size_t batch) {
         lbuf_ = nullptr;
         lbuf_batch_ = 0;
     }
-    SG_HIP(hipMalloc(&lbuf_, batch * (size_t)n_ * n_ * 16));
+    // Per matrix: the tridiagonal's off-diagonal (N doubles) and Z (N x N doubles).
+    SG_HIP(hipMalloc(&lbuf_, batch * ((size_t)n_ + (size_t)n_ * n_) * 8));
     lbuf_batch_ = batch;
 }
 
+// Split mode: three kernels (front half, QL, back half), each at the
+// occupancy its own footprint allows (reference: hybrid_solver.cpp:295-393).
 void Solver::launch_hybrid_(const void* H, const void* S, double* W, void* U, int* info, int batch,
                             hipStream_t stream) {
     ensure_workspace_((size_t)batch);
-    void* L = lbuf_;
-    void* a1[] = {&H, &S, &W, &U, &info, &L, &batch};
+    double* E = static_cast<double*>(lbuf_);
+    double* Zw = E + (size_t)batch * n_;
+    void* a1[] = {&H, &S, &W, &U, &info, &E, &batch};
     SG_HIP(hipModuleLaunchKernel(k_reduce_, (unsigned)batch, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream,
                                  a1, nullptr));
-    void* a2[] = {&W, &U, &info, &batch};
     const unsigned grid2 = (unsigned)((batch + bpb_ - 1) / bpb_);
+    void* a2[] = {&W, &E, &Zw, &info, &batch};
     SG_HIP(hipModuleLaunchKernel(k_heev_, grid2, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream, a2,
                                  nullptr));
-    void* a3[] = {&U, &info, &L, &batch};
+    void* a3[] = {&U, &Zw, &info, &batch};
     SG_HIP(hipModuleLaunchKernel(k_back_, grid2, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream, a3,
                                  nullptr));
 }

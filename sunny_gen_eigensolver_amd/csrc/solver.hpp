@@ -64,7 +64,7 @@ private:
     hipFunction_t k_reduce_ = nullptr;
     hipFunction_t k_heev_ = nullptr;
     hipFunction_t k_back_ = nullptr;
-    // Split-mode workspace: L factors, one per matrix.
+    // Split-mode workspace: tridiagonal off-diagonals and Z, one per matrix.
     void* lbuf_ = nullptr;
     size_t lbuf_batch_ = 0;
 };
