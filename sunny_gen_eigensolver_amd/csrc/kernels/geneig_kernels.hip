@@ -1139,6 +1139,245 @@ __device__ __forceinline__ void stash_packed(const WorkP& w, c64* __restrict__ U
     }
 }
 
+// ===========================================================================
+// Register-resident front half (split mode): each wave keeps its own columns
+// (j = wave + NW c) in registers, lane = row, so the Cholesky updates and the
+// tridiagonalisation's matrix-vector products and rank-2 updates never read
+// the matrix from LDS; LDS carries only the published pivot column / v / w.
+// ===========================================================================
+
+// Cholesky with the lower part of owned columns in registers; L is written to
+// the packed LDS triangle at the end (for the triangular inverse).
+__device__ __forceinline__ int potrf_reg(WorkP& w, const c64* __restrict__ Sb) {
+    const int lane = lane_id(), wave = wave_id();
+    const int lr = lane < N ? lane : N - 1;
+    c64 a[CPW];
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+        const int j = wave + NW * c;
+        a[c] = (j < N && lane >= j && lane < N) ? Sb[(size_t)j * N + lane] : cm(0.0, 0.0);
+    }
+    if (threadIdx.x == 0) w.flag[1] = 0;
+    // Finish column k (pivot, scaling, publication) in the owning wave.
+    auto finish = [&](int k) {
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) {
+            if (wave + NW * c == k) {
+                const double dkk = bcast(a[c].x, k);
+                if (!(dkk > 0.0)) {
+                    if (lane == 0) w.flag[1] = k + 1;
+                } else {
+                    const double rinv = 1.0 / sqrt(dkk);
+                    a[c] = a[c] * rinv;
+                    if (lane < N) w.u.h.vb[k & 1][lane] = a[c];
+                    if (lane == 0) w.invL[k] = rinv;
+                }
+            }
+        }
+    };
+    if (wave == 0) finish(0);
+    __syncthreads();
+    for (int k = 0; k < N; ++k) {
+        const int fail = w.flag[1];
+        if (fail) return fail;
+        const int p = k & 1;
+        const c64 li = w.u.h.vb[p][lr];
+        c64 lj[CPW];
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) {
+            const int j = wave + NW * c;
+            lj[c] = w.u.h.vb[p][j < N ? j : N - 1];
+        }
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) {
+            const int j = wave + NW * c;
+            const c64 na = a[c] - cmulc(li, lj[c]);
+            if (j > k && j < N && lane >= j && lane < N) a[c] = na;
+        }
+        if (k + 1 < N && wave == (k + 1) % NW) finish(k + 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+        const int j = wave + NW * c;
+        if (j < N && lane >= j && lane < N) w.L[pk(j) + lane - j] = a[c];
+    }
+    return 0;
+}
+
+// M = W H W^H, full columns, straight into the owning waves' registers
+// (column j = j0 + wave + NW cc of block jb is owned by wave j % NW, slot
+// j / NW). Z blocks go through LDS as in form_m.
+__device__ __forceinline__ void form_m_reg(WorkP& w, const c64* __restrict__ Hb, c64 (&m)[CPW]) {
+    const int lane = lane_id(), wave = wave_id();
+    const int lr = lane < N ? lane : N - 1;
+    static_assert(ZB % NW == 0, "block columns must map onto owning waves");
+#pragma unroll
+    for (int jb = 0; jb < NZB; ++jb) {
+        const int j0 = jb * ZB;
+        c64 acc[ZPW];
+#pragma unroll
+        for (int cc = 0; cc < ZPW; ++cc) acc[cc] = cm(0.0, 0.0);
+        const int kend = (j0 + ZB < N) ? j0 + ZB : N;
+#pragma unroll 4
+        for (int k = 0; k < kend; ++k) {
+            const c64 h = Hb[k * N + lr];
+#pragma unroll
+            for (int cc = 0; cc < ZPW; ++cc) {
+                const int j = j0 + wave + NW * cc;
+                const bool on = (j < N) && (k <= j);
+                const c64 wjk = w.Wi[on ? pk(k) + j - k : 0];
+                acc[cc] = acc[cc] + cmulc(h, on ? wjk : cm(0.0, 0.0));
+            }
+        }
+#pragma unroll
+        for (int cc = 0; cc < ZPW; ++cc) {
+            const int c = wave + NW * cc;
+            if (lane < N) w.u.zj[c * N + lane] = acc[cc];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int cc = 0; cc < ZPW; ++cc) acc[cc] = cm(0.0, 0.0);
+#pragma unroll 4
+        for (int k = 0; k < N; ++k) {
+            const c64 wik = (lr >= k) ? w.Wi[pk(k) + lr - k] : cm(0.0, 0.0);
+#pragma unroll
+            for (int cc = 0; cc < ZPW; ++cc) {
+                const int c = wave + NW * cc;
+                acc[cc] = acc[cc] + cmul(wik, w.u.zj[c * N + k]);
+            }
+        }
+#pragma unroll
+        for (int cc = 0; cc < ZPW; ++cc) {
+            const int slot = jb * (ZB / NW) + cc;      // (j0 + wave + NW cc) / NW
+            if (slot < CPW) m[slot] = (lane < N) ? acc[cc] : cm(0.0, 0.0);
+        }
+        __syncthreads();
+    }
+}
+
+// Householder vector for column k held in registers (slot c) of its owner.
+__device__ __forceinline__ void house_reg(WorkP& w, int k, c64& col) {
+    const int lane = lane_id();
+    const int m0 = k + 1;
+    const c64 x = (lane >= m0 && lane < N) ? col : cm(0.0, 0.0);
+    const c64 alpha = bcast(x, m0);
+    const double xn2 = wave_sum(lane > m0 ? cabs2(x) : 0.0);
+    c64 t = cm(0.0, 0.0);
+    double beta = alpha.x;
+    c64 v = cm(0.0, 0.0);
+    if (xn2 == 0.0 && alpha.y == 0.0) {
+        if (lane == m0) v = cm(1.0, 0.0);
+    } else {
+        const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+        beta = (alpha.x >= 0.0) ? -nrm : nrm;
+        t = cm((beta - alpha.x) / beta, -alpha.y / beta);
+        const c64 den = cm(alpha.x - beta, alpha.y);
+        const double dd = 1.0 / cabs2(den);
+        const c64 scal = cm(den.x * dd, -den.y * dd);   // 1 / (alpha - beta)
+        if (lane == m0) v = cm(1.0, 0.0);
+        else if (lane > m0) v = cmul(x, scal);
+    }
+    const double dk = bcast(col.x, k);
+    if (lane > m0 && lane < N) col = v;                 // reflector kept in the column
+    if (lane < N) w.u.h.vb[k & 1][lane] = v;
+    if (lane == 0) {
+        w.e[k] = beta;
+        w.d[k] = dk;
+        w.tau[k] = t;
+    }
+}
+
+// Tridiagonalisation with the full Hermitian M in the owning waves'
+// registers (schedule as in hetrd: two barriers per step).
+__device__ __forceinline__ void hetrd_reg(WorkP& w, c64 (&m)[CPW]) {
+    const int lane = lane_id(), wave = wave_id();
+    const int lr = lane < N ? lane : N - 1;
+    if (wave == 0) house_reg(w, 0, m[0]);
+    __syncthreads();
+    for (int k = 0; k < N - 1; ++k) {
+        const int m0 = k + 1;
+        const int p = k & 1;
+        const c64 t = w.tau[k];
+        if (t.x != 0.0 || t.y != 0.0) {
+            const c64 vl = (lane < N) ? w.u.h.vb[p][lr] : cm(0.0, 0.0);
+            c64 vj[CPW];
+#pragma unroll
+            for (int c = 0; c < CPW; ++c) {
+                const int j = wave + NW * c;
+                vj[c] = w.u.h.vb[p][j < N ? j : N - 1];
+            }
+            c64 acc = cm(0.0, 0.0);
+#pragma unroll
+            for (int c = 0; c < CPW; ++c) {
+                const int j = wave + NW * c;
+                const c64 prod = cmul(m[c], vj[c]);
+                if (j >= m0 && j < N) acc = acc + prod;
+            }
+            if (lane < N) w.u.h.part[wave * N + lane] = acc;
+            __syncthreads();
+            c64 pi = cm(0.0, 0.0);
+            if (lane < N) {
+#pragma unroll
+                for (int q = 0; q < NW; ++q) pi = pi + w.u.h.part[q * N + lane];
+            }
+            pi = cmul(t, pi);
+            const c64 pvl = cconjmul(pi, vl);
+            const c64 pv = cm(wave_sum(pvl.x), wave_sum(pvl.y));
+            const c64 cc = cmul(t, pv) * -0.5;
+            const c64 wi = pi + cmul(cc, vl);
+            if (lane < N) w.u.h.wb[wave][lane] = wi;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int c = 0; c < CPW; ++c) {
+                const int j = wave + NW * c;
+                const c64 wj = w.u.h.wb[wave][j < N ? j : N - 1];
+                const c64 na = m[c] - cmulc(vl, wj) - cmulc(wi, vj[c]);
+                if (j >= m0 && j < N && lane >= m0 && lane < N) m[c] = na;
+            }
+        }
+        if (k + 1 < N - 1 && wave == (k + 1) % NW) {
+#pragma unroll
+            for (int c = 0; c < CPW; ++c)
+                if (wave + NW * c == k + 1) house_reg(w, k + 1, m[c]);
+        }
+        __syncthreads();
+    }
+    if (wave == (N - 1) % NW) {
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) {
+            if (wave + NW * c == N - 1) {
+                const double dl = bcast(m[c].x, N - 1);
+                if (lane == 0) {
+                    w.d[N - 1] = dl;
+                    w.e[N - 1] = 0.0;
+                    w.tau[N - 1] = cm(0.0, 0.0);
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Stash: W (LDS) on and below the diagonal, tau on the superdiagonal, and the
+// reflector entries from the owning waves' registers (column k, rows >= k+2,
+// go to (k, i) of the stash).
+__device__ __forceinline__ void stash_reg(const WorkP& w, const c64 (&m)[CPW], c64* __restrict__ Ub) {
+    const int lane = lane_id(), wave = wave_id();
+    for (int e = threadIdx.x; e < N * N; e += NT) {
+        const int i = e % N, j = e / N;
+        if (i >= j) Ub[e] = w.Wi[pk(j) + i - j];
+        else if (i == j - 1) Ub[e] = w.tau[i];
+    }
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+        const int k = wave + NW * c;
+        if (k < N && lane >= k + 2 && lane < N) Ub[(size_t)lane * N + k] = m[c];
+    }
+}
+
 // Stage W (columns packed) from the stash into a wave's slice.
 __device__ __forceinline__ void stage_w(c64* sl, const c64* __restrict__ Ub) {
     const int lane = lane_id();
@@ -1311,17 +1550,18 @@ geneig_split_front(const c64* __restrict__ H, const c64* __restrict__ S, double*
     const int b = blockIdx.x;
     if (b >= batch) return;
     const size_t off = (size_t)b * N * N;
-    load_lower_packed(w.L, S + off);
-    const int pinfo = potrf_packed(w);
+    const int pinfo = potrf_reg(w, S + off);
     if (pinfo) {
         fail_store(N + pinfo, W + (size_t)b * N, U + off, info + b);
         return;
     }
+    __syncthreads();
     trtri_packed(w);
     __syncthreads();
-    form_m(w, H + off);
-    hetrd_packed(w);
-    stash_packed(w, U + off);
+    c64 m[CPW];
+    form_m_reg(w, H + off, m);
+    hetrd_reg(w, m);
+    stash_reg(w, m, U + off);
     if (threadIdx.x < N) {
         W[(size_t)b * N + threadIdx.x] = w.d[threadIdx.x];
         E[(size_t)b * N + threadIdx.x] = w.e[threadIdx.x];
