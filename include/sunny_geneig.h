@@ -125,6 +125,44 @@ const char* sunny_geneig_cache_layer(sunny_geneig_solver_t solver);
 /* Resolved on-disk cache directory (reference: unified_solver.cpp:403-420). */
 const char* sunny_geneig_cache_dir(void);
 
+/* ---- uneven batches (no counterpart in the reference ABI) ----------------
+ * One call solves `count` pencils of different orders n[k] in [1, max_n].
+ * Every matrix goes to the smallest bucket order >= n[k]; each bucket is one
+ * grid of a kernel compiled for that order (the n x n pencil is embedded as
+ * (H (+) pad I, S (+) I), which leaves its eigenpairs unchanged).
+ *
+ * Packed buffers, matrix k column-major at element offset sum_{j<k} n[j]^2:
+ *   d_H, d_S, d_U : complex128, sum n[k]^2 elements;
+ *   d_W           : float64, sum n[k] elements (matrix k at sum_{j<k} n[j]);
+ *   d_info        : int32 [count], same codes as sunny_geneig_launch.
+ * bucket_orders: strictly ascending orders in [1, 64], the last >= max_n; NULL
+ * (or num_buckets == 0) = multiples of 4 below max_n, then max_n. */
+typedef struct sunny_geneig_varlen_solver* sunny_geneig_varlen_t;
+
+sunny_geneig_varlen_t sunny_geneig_varlen_create(int max_n, int device_id,
+                                                 const int* bucket_orders, int num_buckets,
+                                                 sunny_geneig_cache_mode_t cache_mode);
+
+void sunny_geneig_varlen_destroy(sunny_geneig_varlen_t solver);
+
+/* Asynchronous on `stream`; n_host (host memory, `count` orders) is read
+ * before return. A handle is not reentrant: one call at a time per handle.
+ * count == 0 is a no-op; an order outside [1, max_n] -> INVALID_ARG. */
+sunny_geneig_status_t sunny_geneig_varlen_launch(
+    sunny_geneig_varlen_t solver,
+    uintptr_t             d_H,
+    uintptr_t             d_S,
+    uintptr_t             d_W,
+    uintptr_t             d_U,
+    uintptr_t             d_info,
+    const int*            n_host,
+    int                   count,
+    uintptr_t             stream);
+
+/* Number of buckets and the order of bucket i (-1 on NULL / out of range). */
+int sunny_geneig_varlen_num_buckets (sunny_geneig_varlen_t solver);
+int sunny_geneig_varlen_bucket_order(sunny_geneig_varlen_t solver, int i);
+
 #ifdef __cplusplus
 }
 #endif

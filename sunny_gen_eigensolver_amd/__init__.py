@@ -43,6 +43,8 @@ EXPORTED_SYMBOLS = (
     "sunny_geneig_block_dim_x", "sunny_geneig_shared_mem_bytes", "sunny_geneig_last_error",
     "sunny_geneig_path_name", "sunny_geneig_kernel_mode", "sunny_geneig_precompile",
     "sunny_geneig_cache_layer", "sunny_geneig_cache_dir",
+    "sunny_geneig_varlen_create", "sunny_geneig_varlen_destroy", "sunny_geneig_varlen_launch",
+    "sunny_geneig_varlen_num_buckets", "sunny_geneig_varlen_bucket_order",
 )
 
 _lib = None
@@ -84,6 +86,17 @@ def lib() -> ctypes.CDLL:
     L.sunny_geneig_cache_layer.restype = ctypes.c_char_p
     L.sunny_geneig_cache_dir.argtypes = []
     L.sunny_geneig_cache_dir.restype = ctypes.c_char_p
+    pi = ctypes.POINTER(ctypes.c_int)
+    L.sunny_geneig_varlen_create.argtypes = [ci, ci, pi, ci, ci]
+    L.sunny_geneig_varlen_create.restype = vp
+    L.sunny_geneig_varlen_destroy.argtypes = [vp]
+    L.sunny_geneig_varlen_destroy.restype = None
+    L.sunny_geneig_varlen_launch.argtypes = [vp, cu, cu, cu, cu, cu, pi, ci, cu]
+    L.sunny_geneig_varlen_launch.restype = ci
+    L.sunny_geneig_varlen_num_buckets.argtypes = [vp]
+    L.sunny_geneig_varlen_num_buckets.restype = ci
+    L.sunny_geneig_varlen_bucket_order.argtypes = [vp, ci]
+    L.sunny_geneig_varlen_bucket_order.restype = ci
     _lib = L
     return L
 
@@ -226,3 +239,96 @@ def precompile(n: int, block_dim_x: int = 0, arch: str = "gfx950") -> None:
 
 def cache_dir() -> str:
     return lib().sunny_geneig_cache_dir().decode()
+
+
+class VarlenSolver:
+    """Uneven batches: one call solves pencils of different orders n_k <= max_n.
+
+    Matrices are packed back to back (matrix k column-major at element offset
+    sum_{j<k} n_j^2; eigenvalues at sum_{j<k} n_j); `pack` / `unpack` convert
+    from and to lists of (n_k, n_k) tensors in the (B, n, n) convention above."""
+
+    def __init__(self, max_n: int = 64, device_id: int = 0, buckets=None, cache: str = "auto"):
+        L = lib()
+        c = {"nodisk": CACHE_NODISK, "nocache": CACHE_NOCACHE}.get(cache, CACHE_AUTO)
+        if buckets:
+            arr = (ctypes.c_int * len(buckets))(*buckets)
+            h = L.sunny_geneig_varlen_create(max_n, device_id, arr, len(buckets), c)
+        else:
+            h = L.sunny_geneig_varlen_create(max_n, device_id, None, 0, c)
+        if not h:
+            raise GeneigError(STATUS_INVALID_ARG, f"sunny_geneig_varlen_create failed: {last_error()}")
+        self.handle = h
+        self.max_n = max_n
+
+    @property
+    def buckets(self):
+        L = lib()
+        return [L.sunny_geneig_varlen_bucket_order(self.handle, i)
+                for i in range(L.sunny_geneig_varlen_num_buckets(self.handle))]
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().sunny_geneig_varlen_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def launch_raw(self, d_H: int, d_S: int, d_W: int, d_U: int, d_info: int, orders,
+                   stream: int = 0) -> None:
+        arr = (ctypes.c_int * len(orders))(*[int(x) for x in orders])
+        st = lib().sunny_geneig_varlen_launch(self.handle, d_H, d_S, d_W, d_U, d_info, arr,
+                                              len(orders), stream)
+        if st != STATUS_OK:
+            raise GeneigError(st, last_error())
+
+    def run(self, H, S, orders, W=None, U=None, info=None, stream=None):
+        """H, S: packed complex128 device tensors (sum n_k^2,). Returns packed
+        (W, U, info)."""
+        import torch
+        orders = [int(x) for x in orders]
+        nsq = sum(x * x for x in orders)
+        if H.numel() != nsq or S.numel() != nsq:
+            raise ValueError(f"packed H/S must hold sum n_k^2 = {nsq} elements")
+        for name, t in (("H", H), ("S", S)):
+            if t.dtype != torch.complex128 or not t.is_cuda or not t.is_contiguous():
+                raise ValueError(f"{name} must be a contiguous complex128 device tensor")
+        dev = H.device
+        W = torch.empty((sum(orders),), dtype=torch.float64, device=dev) if W is None else W
+        U = torch.empty((nsq,), dtype=torch.complex128, device=dev) if U is None else U
+        info = torch.zeros((len(orders),), dtype=torch.int32, device=dev) if info is None else info
+        self.launch_raw(H.data_ptr(), S.data_ptr(), W.data_ptr(), U.data_ptr(), info.data_ptr(),
+                        orders, _stream_handle(stream))
+        return W, U, info
+
+
+def pack(mats):
+    """List of (n_k, n_k) tensors -> one flat packed tensor."""
+    import torch
+    return torch.cat([m.reshape(-1) for m in mats]).contiguous()
+
+
+def unpack_matrices(flat, orders):
+    out, o = [], 0
+    for n in orders:
+        out.append(flat[o:o + n * n].view(n, n))
+        o += n * n
+    return out
+
+
+def unpack_vectors(flat, orders):
+    out, o = [], 0
+    for n in orders:
+        out.append(flat[o:o + n])
+        o += n
+    return out

@@ -33,9 +33,11 @@ LIB = os.path.join(LIBDIR, "libsunny_geneig.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIP_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-# Orders whose code objects ship warm: the reference fixtures' LDS-resident
-# sizes and the serving sizes.
+# Orders whose code objects ship warm at every workgroup size: the reference
+# fixtures' LDS-resident sizes and the serving sizes. The uneven-batch buckets
+# (multiples of 4 up to 64) ship warm at their own workgroup size.
 WARM_ORDERS = [16, 24, 32, 48, 54, 64]
+BUCKET_ORDERS = list(range(4, 65, 4))
 
 
 def _run(cmd, **kw):
@@ -60,7 +62,7 @@ def embed_kernel_source() -> str:
 def build_library() -> str:
     embed_kernel_source()
     os.makedirs(LIBDIR, exist_ok=True)
-    srcs = [os.path.join(CSRC, f) for f in ("codeobj.cpp", "solver.cpp", "c_api.cpp")]
+    srcs = [os.path.join(CSRC, f) for f in ("codeobj.cpp", "solver.cpp", "varlen.cpp", "c_api.cpp")]
     tmp = LIB + ".tmp"
     _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall",
           "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", f"-I{os.path.join(REPO, 'include')}",
@@ -76,23 +78,31 @@ def aot_check() -> None:
           "-o", os.path.join(BUILD, "geneig_kernels.o")])
 
 
-def warm_cache(orders=WARM_ORDERS) -> None:
+def _precompile(job):
+    n, bd = job
+    lib = ctypes.CDLL(LIB)
+    lib.sunny_geneig_precompile.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_char_p]
+    lib.sunny_geneig_last_error.restype = ctypes.c_char_p
+    rc = lib.sunny_geneig_precompile(n, bd, b"gfx950")
+    return n, bd, rc, (lib.sunny_geneig_last_error().decode() if rc else "")
+
+
+def warm_cache(orders=WARM_ORDERS, buckets=BUCKET_ORDERS) -> None:
+    from concurrent.futures import ProcessPoolExecutor
     os.makedirs(CACHE, exist_ok=True)
     # Entries are keyed by a hash of source + options: drop stale ones.
     for f in os.listdir(CACHE):
         if f.endswith(".hsaco") or f == "autotune.tsv":
             os.remove(os.path.join(CACHE, f))
     os.environ["SUNNY_GENEIG_CACHE_DIR"] = CACHE
-    lib = ctypes.CDLL(LIB)
-    lib.sunny_geneig_precompile.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_char_p]
-    lib.sunny_geneig_last_error.restype = ctypes.c_char_p
-    for n in orders:
-        for bd in sorted({0, 64, 128, 256}):
-            rc = lib.sunny_geneig_precompile(n, bd, b"gfx950")
+    jobs = [(n, bd) for n in orders for bd in (64, 128, 256)]
+    jobs += [(n, 0) for n in buckets if n not in orders]
+    workers = max(1, min(8, len(os.sched_getaffinity(0))))
+    with ProcessPoolExecutor(workers) as ex:
+        for n, bd, rc, msg in ex.map(_precompile, jobs):
             if rc != 0:
-                raise RuntimeError(f"precompile N={n} wg={bd} failed: "
-                                   f"{lib.sunny_geneig_last_error().decode()}")
-        print(f"warmed N={n}", flush=True)
+                raise RuntimeError(f"precompile N={n} wg={bd} failed: {msg}")
+    print(f"warmed {len(jobs)} code objects", flush=True)
 
 
 def build(warm: bool = True) -> None:

@@ -4,15 +4,21 @@
 #include "sunny_geneig.h"
 
 #include "solver.hpp"
+#include "varlen.hpp"
 
 #include <exception>
 #include <memory>
 #include <new>
 #include <string>
+#include <vector>
 
 struct sunny_geneig_solver {
     std::unique_ptr<sgeig::Solver> impl;
     sunny_geneig_kernel_mode_t mode = SUNNY_GENEIG_KERNEL_UNIFIED;
+};
+
+struct sunny_geneig_varlen_solver {
+    std::unique_ptr<sgeig::VarlenSolver> impl;
 };
 
 namespace {
@@ -166,6 +172,76 @@ const char* sunny_geneig_cache_dir(void) {
         g_cache_dir.clear();
     }
     return g_cache_dir.c_str();
+}
+
+sunny_geneig_varlen_t sunny_geneig_varlen_create(int max_n, int device_id, const int* bucket_orders,
+                                                 int num_buckets, sunny_geneig_cache_mode_t cache_mode) {
+    clear_error();
+    try {
+        if (num_buckets < 0) throw std::invalid_argument("num_buckets < 0");
+        std::vector<int> buckets;
+        if (bucket_orders && num_buckets > 0) buckets.assign(bucket_orders, bucket_orders + num_buckets);
+        auto h = std::make_unique<sunny_geneig_varlen_solver>();
+        h->impl = std::make_unique<sgeig::VarlenSolver>(max_n, device_id, buckets, to_cache(cache_mode));
+        return h.release();
+    } catch (const std::bad_alloc&) {
+        set_error("out of memory in sunny_geneig_varlen_create");
+    } catch (const std::exception& e) {
+        set_error(std::string("sunny_geneig_varlen_create: ") + e.what());
+    } catch (...) {
+        set_error("unknown C++ exception in sunny_geneig_varlen_create");
+    }
+    return nullptr;
+}
+
+void sunny_geneig_varlen_destroy(sunny_geneig_varlen_t solver) {
+    try {
+        delete solver;
+    } catch (...) {
+    }
+}
+
+sunny_geneig_status_t sunny_geneig_varlen_launch(sunny_geneig_varlen_t solver, uintptr_t d_H,
+                                                 uintptr_t d_S, uintptr_t d_W, uintptr_t d_U,
+                                                 uintptr_t d_info, const int* n_host, int count,
+                                                 uintptr_t stream) {
+    clear_error();
+    if (!solver) {
+        set_error("sunny_geneig_varlen_launch: solver is NULL");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    if (count < 0) {
+        set_error("sunny_geneig_varlen_launch: count < 0");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    if (count > 0 && (!d_H || !d_S || !d_W || !d_U || !d_info || !n_host)) {
+        set_error("sunny_geneig_varlen_launch: NULL buffer");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    try {
+        solver->impl->launch(reinterpret_cast<const void*>(d_H), reinterpret_cast<const void*>(d_S),
+                             reinterpret_cast<double*>(d_W), reinterpret_cast<void*>(d_U),
+                             reinterpret_cast<int*>(d_info), n_host, count,
+                             reinterpret_cast<hipStream_t>(stream));
+        return SUNNY_GENEIG_OK;
+    } catch (const std::invalid_argument& e) {
+        set_error(std::string("sunny_geneig_varlen_launch: ") + e.what());
+        return SUNNY_GENEIG_INVALID_ARG;
+    } catch (const std::exception& e) {
+        set_error(std::string("sunny_geneig_varlen_launch: ") + e.what());
+    } catch (...) {
+        set_error("unknown C++ exception in sunny_geneig_varlen_launch");
+    }
+    return SUNNY_GENEIG_LAUNCH_ERROR;
+}
+
+int sunny_geneig_varlen_num_buckets(sunny_geneig_varlen_t s) {
+    return s ? (int)s->impl->buckets().size() : -1;
+}
+
+int sunny_geneig_varlen_bucket_order(sunny_geneig_varlen_t s, int i) {
+    if (!s || i < 0 || i >= (int)s->impl->buckets().size()) return -1;
+    return s->impl->buckets()[i];
 }
 
 }  // extern "C"

@@ -1,0 +1,172 @@
+#include "varlen.hpp"
+
+#include "solver.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+namespace sgeig {
+
+namespace {
+
+constexpr int kMaxVarlenOrder = 64;
+
+}  // namespace
+
+VarlenSolver::VarlenSolver(int max_n, int device_id, const std::vector<int>& buckets,
+                           CacheMode cache)
+    : max_n_(max_n), device_(device_id) {
+    if (max_n < 1 || max_n > kMaxVarlenOrder)
+        throw std::invalid_argument("max_n must be in [1, " + std::to_string(kMaxVarlenOrder) +
+                                    "] for the LDS-resident uneven-batch kernel");
+    if (buckets.empty()) {
+        for (int o = 4; o < max_n; o += 4) orders_.push_back(o);
+        orders_.push_back(max_n);
+    } else {
+        orders_ = buckets;
+        for (size_t i = 0; i < orders_.size(); ++i) {
+            if (orders_[i] < 1 || orders_[i] > kMaxVarlenOrder)
+                throw std::invalid_argument("bucket order out of range [1, 64]");
+            if (i && orders_[i] <= orders_[i - 1])
+                throw std::invalid_argument("bucket orders must be strictly ascending");
+        }
+        if (orders_.back() < max_n)
+            throw std::invalid_argument("largest bucket order is below max_n");
+    }
+    int count = 0;
+    SG_HIP(hipGetDeviceCount(&count));
+    if (device_id < 0 || device_id >= count)
+        throw std::invalid_argument("device_id " + std::to_string(device_id) + " out of range");
+    SG_HIP(hipSetDevice(device_id));
+    const std::string arch = device_arch(device_id);
+    for (int o : orders_) {
+        Bucket b;
+        b.order = o;
+        b.block_dim = Solver::suggested_block_dim(o);   // same warm code objects
+        b.nw = b.block_dim / 64;
+        CacheLayer layer;
+        double ms = 0.0;
+        b.module = acquire_module(ModuleKey{o, b.block_dim, arch}, cache, &layer, &ms);
+        b.fn = b.module->get("geneig_varlen");
+        buckets_.push_back(std::move(b));
+    }
+    SG_HIP(hipEventCreateWithFlags(&staged_, hipEventDisableTiming));
+    SG_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
+}
+
+VarlenSolver::~VarlenSolver() {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(device_);
+    if (done_) (void)hipEventSynchronize(done_);
+    if (staged_) (void)hipEventDestroy(staged_);
+    if (done_) (void)hipEventDestroy(done_);
+    if (host_desc_) (void)hipHostFree(host_desc_);
+    if (dev_desc_) (void)hipFree(dev_desc_);
+    if (scratch_) (void)hipFree(scratch_);
+    (void)hipSetDevice(cur);
+}
+
+void VarlenSolver::ensure_capacity_(int count) {
+    if ((size_t)count <= desc_count_) return;
+    // Earlier launches may still read the device descriptors.
+    SG_HIP(hipEventSynchronize(done_));
+    if (host_desc_) SG_HIP(hipHostFree(host_desc_));
+    if (dev_desc_) SG_HIP(hipFree(dev_desc_));
+    host_desc_ = dev_desc_ = nullptr;
+    desc_count_ = 0;
+    const size_t bytes = (size_t)count * 24;
+    SG_HIP(hipHostMalloc(&host_desc_, bytes, hipHostMallocDefault));
+    SG_HIP(hipMalloc(&dev_desc_, bytes));
+    desc_count_ = (size_t)count;
+}
+
+void VarlenSolver::launch(const void* H, const void* S, double* W, void* U, int* info, const int* n,
+                          int count, hipStream_t stream) {
+    if (count < 0) throw std::invalid_argument("count < 0");
+    if (count == 0) return;
+    if (!n) throw std::invalid_argument("order array is NULL");
+    const int nb = (int)buckets_.size();
+    std::vector<int> bucket_of(count), per_bucket(nb, 0);
+    for (int k = 0; k < count; ++k) {
+        if (n[k] < 1 || n[k] > max_n_)
+            throw std::invalid_argument("matrix " + std::to_string(k) + " has order " +
+                                        std::to_string(n[k]) + " outside [1, " +
+                                        std::to_string(max_n_) + "]");
+        const int b = (int)(std::lower_bound(orders_.begin(), orders_.end(), n[k]) - orders_.begin());
+        bucket_of[k] = b;
+        ++per_bucket[b];
+    }
+    int cur = 0;
+    SG_HIP(hipGetDevice(&cur));
+    if (cur != device_) SG_HIP(hipSetDevice(device_));
+
+    ensure_capacity_(count);
+    size_t scratch_need = 0;
+    for (int b = 0; b < nb; ++b) {
+        if (!per_bucket[b]) continue;
+        const Bucket& B = buckets_[b];
+        const size_t wgs = std::min<size_t>((per_bucket[b] + B.nw - 1) / B.nw, kMaxGrid);
+        scratch_need = std::max(scratch_need, wgs * B.nw * (size_t)B.order * B.order * 16);
+    }
+    if (scratch_need > scratch_bytes_) {
+        SG_HIP(hipEventSynchronize(done_));
+        if (scratch_) SG_HIP(hipFree(scratch_));
+        scratch_ = nullptr;
+        scratch_bytes_ = 0;
+        SG_HIP(hipMalloc(&scratch_, scratch_need));
+        scratch_bytes_ = scratch_need;
+    }
+
+    // The previous upload must have left the pinned staging area.
+    if (staged_pending_) SG_HIP(hipEventSynchronize(staged_));
+    int64_t* moff = static_cast<int64_t*>(host_desc_);
+    int64_t* woff = moff + count;
+    int* ids = reinterpret_cast<int*>(woff + count);
+    int* nn = ids + count;
+    int64_t mo = 0, wo = 0;
+    for (int k = 0; k < count; ++k) {
+        moff[k] = mo;
+        woff[k] = wo;
+        nn[k] = n[k];
+        mo += (int64_t)n[k] * n[k];
+        wo += n[k];
+    }
+    // Counting sort of matrix ids by bucket (stable).
+    std::vector<int> start(nb + 1, 0);
+    for (int b = 0; b < nb; ++b) start[b + 1] = start[b] + per_bucket[b];
+    {
+        std::vector<int> fill(start.begin(), start.end() - 1);
+        for (int k = 0; k < count; ++k) ids[fill[bucket_of[k]]++] = k;
+    }
+    // Descriptors and scratch are reused: order this call after the last one
+    // even when the caller switches streams.
+    SG_HIP(hipStreamWaitEvent(stream, done_, 0));
+    SG_HIP(hipMemcpyAsync(dev_desc_, host_desc_, (size_t)count * 24, hipMemcpyHostToDevice, stream));
+    SG_HIP(hipEventRecord(staged_, stream));
+    staged_pending_ = true;
+
+    const int64_t* d_moff = static_cast<const int64_t*>(dev_desc_);
+    const int64_t* d_woff = d_moff + count;
+    const int* d_ids = reinterpret_cast<const int*>(d_woff + count);
+    const int* d_nn = d_ids + count;
+    void* scratch = scratch_;
+    // Largest orders first: their grids are the longest, the small ones fill
+    // in behind them.
+    for (int b = nb - 1; b >= 0; --b) {
+        const Bucket& B = buckets_[b];
+        for (int off = 0; off < per_bucket[b]; off += kMaxGrid * B.nw) {
+            int cnt = std::min(per_bucket[b] - off, kMaxGrid * B.nw);
+            const int* list = d_ids + start[b] + off;
+            void* args[] = {&H, &S, &W, &U, &info, &list, &d_moff, &d_woff, &d_nn, &cnt, &scratch};
+            const unsigned grid = (unsigned)((cnt + B.nw - 1) / B.nw);
+            SG_HIP(hipModuleLaunchKernel(B.fn, grid, 1, 1, (unsigned)B.block_dim, 1, 1, 0, stream,
+                                         args, nullptr));
+        }
+    }
+    SG_HIP(hipEventRecord(done_, stream));
+    if (cur != device_) SG_HIP(hipSetDevice(cur));
+}
+
+}  // namespace sgeig

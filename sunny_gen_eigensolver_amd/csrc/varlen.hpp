@@ -1,0 +1,73 @@
+// Uneven batches: matrices of different orders solved by one call. Every
+// matrix goes to the smallest bucket order >= its own; each bucket is one
+// grid of the geneig_varlen kernel compiled for that order, which embeds an
+// n x n pencil in the bucket's order as (H (+) pad I, S (+) I). The host sorts
+// the work list by bucket, so each grid reads a contiguous run of it.
+//
+// No counterpart in the reference (its solvers take one order per handle,
+// src/unified/unified_solver.hpp:81-217).
+#pragma once
+
+#include "codeobj.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+namespace sgeig {
+
+class VarlenSolver {
+public:
+    // Buckets: ascending orders in [1, 64]; empty = multiples of 4 up to max_n
+    // (plus max_n itself).
+    VarlenSolver(int max_n, int device_id, const std::vector<int>& buckets, CacheMode cache);
+    ~VarlenSolver();
+    VarlenSolver(const VarlenSolver&) = delete;
+    VarlenSolver& operator=(const VarlenSolver&) = delete;
+
+    // Matrices packed back to back: matrix k (order n[k]) at element offset
+    // sum_{j<k} n[j]^2 of H, S and U, its eigenvalues at sum_{j<k} n[j] of W,
+    // its status at info[k]. Asynchronous on `stream`; `n` is read before
+    // return. Not reentrant for one handle (one staging buffer).
+    void launch(const void* H, const void* S, double* W, void* U, int* info, const int* n,
+                int count, hipStream_t stream);
+
+    int max_n() const { return max_n_; }
+    const std::vector<int>& buckets() const { return orders_; }
+    // Matrices per workgroup of each bucket's grid.
+    int bucket_waves(int i) const { return buckets_[i].nw; }
+
+    // Workgroups per grid launch; longer runs are split into several grids
+    // that reuse one scratch area.
+    static constexpr int kMaxGrid = 4096;
+
+private:
+    struct Bucket {
+        int order = 0;
+        int block_dim = 0;
+        int nw = 0;
+        std::shared_ptr<LoadedModule> module;
+        hipFunction_t fn = nullptr;
+    };
+    void ensure_capacity_(int count);
+
+    int max_n_ = 0;
+    int device_ = 0;
+    std::vector<int> orders_;
+    std::vector<Bucket> buckets_;
+    // Descriptor staging: ids | nn (int32) then moff | woff (int64), pinned on
+    // the host and mirrored on the device.
+    void* host_desc_ = nullptr;
+    void* dev_desc_ = nullptr;
+    size_t desc_count_ = 0;
+    hipEvent_t staged_ = nullptr;   // last upload from host_desc_ has finished
+    bool staged_pending_ = false;
+    hipEvent_t done_ = nullptr;     // last launch's grids have finished
+    // Per-workgroup factor stash, sized for the largest bucket's grid.
+    void* scratch_ = nullptr;
+    size_t scratch_bytes_ = 0;
+};
+
+}  // namespace sgeig

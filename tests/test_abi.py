@@ -102,3 +102,18 @@ def test_product_path_never_touches_the_oracle():
             if f.endswith((".py", ".cpp", ".hpp", ".hip", ".h")):
                 text = open(os.path.join(root, f)).read()
                 assert "oracle" not in text.replace("# oracle-free", ""), f
+
+
+def test_varlen_null_and_bad_arguments():
+    L = sg.lib()
+    assert L.sunny_geneig_varlen_num_buckets(None) == -1
+    assert L.sunny_geneig_varlen_bucket_order(None, 0) == -1
+    L.sunny_geneig_varlen_destroy(None)
+    orders = (ctypes.c_int * 1)(8)
+    st = L.sunny_geneig_varlen_launch(None, 0, 0, 0, 0, 0, orders, 1, 0)
+    assert st == sg.STATUS_INVALID_ARG and "NULL" in sg.last_error()
+    for max_n, buckets in ((0, None), (65, None), (16, [8, 4, 16]), (16, [4, 8]), (16, [0, 16])):
+        arr = (ctypes.c_int * len(buckets))(*buckets) if buckets else None
+        h = L.sunny_geneig_varlen_create(max_n, 0, arr, len(buckets) if buckets else 0, 0)
+        assert not h
+        assert "sunny_geneig_varlen_create" in sg.last_error()

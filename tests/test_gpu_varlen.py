@@ -1,0 +1,175 @@
+"""GPU parity of the uneven-batch path (sunny_geneig_varlen_*): pencils of
+different orders in one call, each checked against FP64 LAPACK zhegv on the
+same matrix with the fixed-order bars (eig_rel and phase < 1e-10, residual
+and S-orthogonality <= 50 N eps), plus zhegv's INFO codes for a non-HPD S in
+the mix, and the alone-versus-mixed property: a matrix's result does not
+depend on what else is in the call (bit-identical, since every matrix is
+solved by the same bucket kernel whatever its neighbours)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import helpers as T
+
+pytestmark = pytest.mark.gpu
+
+
+def _gen_mix(orders, seed):
+    """Per-matrix (H, S) from the reference generator, one seed per entry."""
+    Hs, Ss = [], []
+    for k, n in enumerate(orders):
+        H, S = O.gen_reference(n, 1, seed + k, "random")
+        Hs.append(H[0])
+        Ss.append(S[0])
+    return Hs, Ss
+
+
+def _solve(vs, Hs, Ss, orders):
+    import torch
+    import sunny_gen_eigensolver_amd as sg
+    dev = torch.device("cuda", 0)
+    dH = sg.pack([torch.from_numpy(np.ascontiguousarray(h)) for h in Hs]).to(dev)
+    dS = sg.pack([torch.from_numpy(np.ascontiguousarray(s)) for s in Ss]).to(dev)
+    W, U, info = vs.run(dH, dS, orders)
+    torch.cuda.synchronize()
+    Wl = [w.cpu().numpy() for w in sg.unpack_vectors(W, orders)]
+    Ul = [u.cpu().numpy() for u in sg.unpack_matrices(U, orders)]
+    return Wl, Ul, info.cpu().numpy()
+
+
+def _check_groups(Hs, Ss, W, U, info, orders, tag):
+    by_n = {}
+    for k, n in enumerate(orders):
+        by_n.setdefault(n, []).append(k)
+    for n, ks in sorted(by_n.items()):
+        H = np.stack([Hs[k] for k in ks])
+        S = np.stack([Ss[k] for k in ks])
+        Wg = np.stack([W[k] for k in ks])
+        Ug = np.stack([U[k] for k in ks])
+        ig = info[ks]
+        Wl, Ul, il = O.zhegv_batch(H, S, threads=8)
+        assert (il == 0).all()
+        T.assert_parity(H, S, Wg, Ug, ig, Wl, Ul, tag=f"{tag} n={n} x{len(ks)}")
+
+
+@pytest.fixture(scope="module")
+def vs64():
+    import sunny_gen_eigensolver_amd as sg
+    s = sg.VarlenSolver(64)
+    yield s
+    s.close()
+
+
+def test_default_buckets(vs64):
+    assert vs64.buckets == list(range(4, 65, 4))
+
+
+def test_mixed_orders_vs_lapack(vs64):
+    rng = np.random.default_rng(7)
+    orders = [1, 2, 3, 4, 5, 12, 16, 17, 31, 32, 33, 54, 63, 64]
+    orders += [int(x) for x in rng.integers(1, 65, size=50)]
+    rng.shuffle(orders)
+    Hs, Ss = _gen_mix(orders, 5000)
+    W, U, info = _solve(vs64, Hs, Ss, orders)
+    _check_groups(Hs, Ss, W, U, info, orders, "varlen")
+
+
+def test_log_uniform_mix_vs_lapack(vs64):
+    """The uneven-batch workload's order law (log-uniform), restricted to the
+    LDS-resident range 12..64."""
+    rng = np.random.default_rng(11)
+    orders = [int(x) for x in np.exp(rng.uniform(np.log(12), np.log(64.999), size=300))]
+    Hs, Ss = _gen_mix(orders, 9000)
+    W, U, info = _solve(vs64, Hs, Ss, orders)
+    _check_groups(Hs, Ss, W, U, info, orders, "varlen log-uniform")
+
+
+def test_alone_equals_mixed(vs64):
+    orders = [54, 7, 33, 54, 16, 64, 21, 54, 9]
+    Hs, Ss = _gen_mix(orders, 123)
+    W, U, info = _solve(vs64, Hs, Ss, orders)
+    for k in (0, 2, 5, 6):
+        Wa, Ua, ia = _solve(vs64, [Hs[k]], [Ss[k]], [orders[k]])
+        assert ia[0] == info[k] == 0
+        assert np.array_equal(Wa[0], W[k]), f"matrix {k}: eigenvalues differ alone vs mixed"
+        assert np.array_equal(Ua[0], U[k]), f"matrix {k}: eigenvectors differ alone vs mixed"
+
+
+def test_fixed_order_solver_agrees(vs64):
+    """A bucket-order matrix solved by the uneven path matches the fused
+    fixed-order kernel to the parity bars."""
+    meta, H, S = T.fixture_inputs("n32_b8_s42_random")
+    orders = [32] * 8
+    W, U, info = _solve(vs64, list(H), list(S), orders)
+    Wf, Uf, _ = T.run_gpu(H, S)
+    T.assert_parity(H, S, np.stack(W), np.stack(U), info, Wf, Uf, tag="varlen vs fused n=32")
+
+
+def test_non_hpd_in_mix(vs64):
+    """zhegv INFO = n + k for the first non-positive leading minor k of S; the
+    other matrices in the call are unaffected."""
+    orders = [12, 20, 12, 40]
+    Hs, Ss = _gen_mix(orders, 77)
+    bad = np.eye(20, dtype=np.complex128)
+    bad[6, 6] = -1.0
+    Ss[1] = bad.T.copy()
+    W, U, info = _solve(vs64, Hs, Ss, orders)
+    _, _, il = O.zhegv_batch(Hs[1][None], Ss[1][None])
+    assert int(info[1]) == 20 + 7 == int(il[0])
+    assert np.isnan(W[1]).all() and (U[1] == 0).all()
+    good = [0, 2, 3]
+    _check_groups([Hs[k] for k in good], [Ss[k] for k in good], [W[k] for k in good],
+                  [U[k] for k in good], info[good], [orders[k] for k in good], "varlen beside non-HPD")
+
+
+def test_multiple_grids_per_bucket():
+    """More matrices than one grid of a bucket holds (4096 workgroups x 1
+    matrix at order 4): the run is split over several grids sharing one
+    scratch area."""
+    import sunny_gen_eigensolver_amd as sg
+    n = 4
+    count = 4096 + 37
+    H, S = O.gen_reference(n, count, 31337, "random")
+    orders = [3 if k % 5 == 0 else 4 for k in range(count)]
+    Hs = [H[k][:o, :o].copy() if o < n else H[k] for k, o in enumerate(orders)]
+    Ss = [S[k][:o, :o].copy() if o < n else S[k] for k, o in enumerate(orders)]
+    with sg.VarlenSolver(8, buckets=[4, 8]) as vs:
+        W, U, info = _solve(vs, Hs, Ss, orders)
+    _check_groups(Hs, Ss, W, U, info, orders, "varlen multi-grid")
+
+
+def test_rejects_order_above_max():
+    import torch
+    import sunny_gen_eigensolver_amd as sg
+    with sg.VarlenSolver(16) as vs:
+        dev = torch.device("cuda", 0)
+        x = torch.zeros(17 * 17, dtype=torch.complex128, device=dev)
+        with pytest.raises(sg.GeneigError) as ei:
+            vs.run(x, x, [17])
+        assert ei.value.status == sg.STATUS_INVALID_ARG
+
+
+@pytest.mark.parametrize("bucket", list(range(4, 65, 4)))
+def test_every_default_bucket_every_slot(vs64, bucket):
+    """Each default bucket with its workgroups full: every wave slot holds a
+    matrix of the bucket's own order or just below it (padded), so the second
+    and later matrices of a workgroup are covered at every order."""
+    import sunny_gen_eigensolver_amd as sg
+    nw = (64 if bucket <= 16 else 128 if bucket <= 32 else 256) // 64
+    lo = max(1, bucket - 3)
+    orders = [bucket if k % 2 == 0 else max(lo, bucket - 1 - (k // 2) % 3) for k in range(4 * nw)]
+    Hs, Ss = _gen_mix(orders, 40000 + 100 * bucket)
+    W, U, info = _solve(vs64, Hs, Ss, orders)
+    _check_groups(Hs, Ss, W, U, info, orders, f"bucket {bucket}")
+
+
+@pytest.mark.parametrize("n", list(range(4, 65, 4)))
+def test_fused_at_bucket_orders(n):
+    """The fixed-order fused and split kernels at every bucket order (their
+    code objects ship warm with the buckets), several matrices per workgroup."""
+    H, S = O.gen_reference(n, 9, 50000 + n, "random")
+    Wl, Ul, il = O.zhegv_batch(H, S, threads=8)
+    assert (il == 0).all()
+    for kernel in ("unified", "hybrid"):
+        W, U, info = T.run_gpu(H, S, kernel=kernel)
+        T.assert_parity(H, S, W, U, info, Wl, Ul, tag=f"{kernel} n={n}")
