@@ -55,11 +55,16 @@ typedef enum {
 } sunny_geneig_tuning_mode_t;
 
 /* Reference: src/common/c_api.h:43-46. UNIFIED = one fused kernel per batch;
- * HYBRID = split mode: reduce kernel, batched heev kernel, batched back-trsm
- * kernel (all written for gfx950 by this project), captured in a hipGraph. */
+ * HYBRID = split mode: front-half kernel (Cholesky, reduction,
+ * tridiagonalisation), batched QL kernel, batched back-transform kernel (all
+ * written for gfx950 by this project). MIXED is additive (no counterpart in
+ * the reference): FP64 Cholesky, FP32 reduction / tridiagonalisation / QL /
+ * back-transform, then FP64 eigenpair refinement on the original pencil; same
+ * buffers and outputs as the other modes. */
 typedef enum {
     SUNNY_GENEIG_KERNEL_UNIFIED = 0,
-    SUNNY_GENEIG_KERNEL_HYBRID  = 1
+    SUNNY_GENEIG_KERNEL_HYBRID  = 1,
+    SUNNY_GENEIG_KERNEL_MIXED   = 2
 } sunny_geneig_kernel_mode_t;
 
 /* Reference: src/common/c_api.h:50-55 (c_api.cpp:76-107). Returns NULL on
@@ -106,15 +111,15 @@ unsigned int sunny_geneig_shared_mem_bytes (sunny_geneig_solver_t solver);
 const char*  sunny_geneig_last_error(void);
 
 /* Reference: src/common/c_api.h:97-101 (c_api.cpp:212-220). "unified",
- * "hybrid", or "" for NULL; kernel_mode returns -1 for NULL. */
+ * "hybrid", "mixed", or "" for NULL; kernel_mode returns -1 for NULL. */
 const char*                sunny_geneig_path_name  (sunny_geneig_solver_t solver);
 sunny_geneig_kernel_mode_t sunny_geneig_kernel_mode(sunny_geneig_solver_t solver);
 
 /* ---- additive entry points (no counterpart in the reference ABI) --------- */
 
-/* Compile (hipRTC) and store in the on-disk cache the code object for order n
- * with `block_dim_x` threads (0 = suggested) for `arch` (e.g. "gfx950"),
- * without touching a device. Returns 0 or a status. Lets a build step warm the
+/* Compile (hipRTC) and store in the on-disk cache the code objects for order n
+ * with `block_dim_x` threads (0 = suggested) for `arch` (e.g. "gfx950"): the
+ * FP64 module and the FP32 module of the mixed mode, without touching a device. Returns 0 or a status. Lets a build step warm the
  * cache so the first create on a fresh machine skips compilation. */
 int sunny_geneig_precompile(int n, int block_dim_x, const char* arch);
 

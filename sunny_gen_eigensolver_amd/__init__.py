@@ -36,6 +36,7 @@ CACHE_NODISK = 1
 CACHE_NOCACHE = 2
 KERNEL_UNIFIED = 0
 KERNEL_HYBRID = 1
+KERNEL_MIXED = 2
 
 EXPORTED_SYMBOLS = (
     "sunny_geneig_create", "sunny_geneig_create_ex", "sunny_geneig_destroy",
@@ -113,14 +114,15 @@ class GeneigError(RuntimeError):
 
 class Solver:
     """Solver(n; device_id=0, bpb_request=0, tuning='suggested', cache='auto',
-    kernel='unified') -- SunnyGeneigBench.jl:37-57."""
+    kernel='unified') -- SunnyGeneigBench.jl:37-57. kernel is 'unified' (fused
+    FP64), 'hybrid' (split FP64) or 'mixed' (FP32 stages + FP64 refinement)."""
 
     def __init__(self, n: int, device_id: int = 0, bpb_request: int = 0,
                  tuning: str = "suggested", cache: str = "auto", kernel: str = "unified"):
         L = lib()
         t = TUNING_AUTOTUNE if tuning == "autotune" else TUNING_SUGGESTED
         c = {"nodisk": CACHE_NODISK, "nocache": CACHE_NOCACHE}.get(cache, CACHE_AUTO)
-        k = KERNEL_HYBRID if kernel == "hybrid" else KERNEL_UNIFIED
+        k = {"hybrid": KERNEL_HYBRID, "mixed": KERNEL_MIXED}.get(kernel, KERNEL_UNIFIED)
         h = L.sunny_geneig_create_ex(n, device_id, bpb_request, t, c, k)
         if not h:
             raise GeneigError(STATUS_INVALID_ARG, f"sunny_geneig_create_ex failed: {last_error()}")
@@ -165,7 +167,8 @@ class Solver:
 
     @property
     def kernel_mode(self) -> str:
-        return "hybrid" if lib().sunny_geneig_kernel_mode(self.handle) == KERNEL_HYBRID else "unified"
+        return {KERNEL_HYBRID: "hybrid", KERNEL_MIXED: "mixed"}.get(
+            lib().sunny_geneig_kernel_mode(self.handle), "unified")
 
     @property
     def cache_layer(self) -> str:

@@ -73,9 +73,13 @@ def build_library() -> str:
 
 
 def aot_check() -> None:
+    """Ahead-of-time compile of both module precisions. The FP32 build must not
+    promote any arithmetic to FP64 (an unsuffixed literal would)."""
     os.makedirs(BUILD, exist_ok=True)
     _run(["hipcc", "--offload-arch=gfx950", "-O3", "-std=c++17", *HIP_FLAGS, "-c", KERNEL,
           "-o", os.path.join(BUILD, "geneig_kernels.o")])
+    _run(["hipcc", "--offload-arch=gfx950", "-O3", "-std=c++17", *HIP_FLAGS, "-DGENEIG_FP32",
+          "-Werror=double-promotion", "-c", KERNEL, "-o", os.path.join(BUILD, "geneig_kernels_fp32.o")])
 
 
 def _precompile(job):

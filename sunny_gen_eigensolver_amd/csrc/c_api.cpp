@@ -48,7 +48,9 @@ sunny_geneig_solver_t create_impl(int n, int device_id, int bpb, sunny_geneig_tu
             n, device_id, bpb,
             tuning == SUNNY_GENEIG_TUNING_AUTOTUNE ? sgeig::Tuning::Autotune : sgeig::Tuning::Suggested,
             to_cache(cache),
-            mode == SUNNY_GENEIG_KERNEL_HYBRID ? sgeig::Mode::Hybrid : sgeig::Mode::Unified);
+            mode == SUNNY_GENEIG_KERNEL_HYBRID  ? sgeig::Mode::Hybrid
+            : mode == SUNNY_GENEIG_KERNEL_MIXED ? sgeig::Mode::Mixed
+                                                : sgeig::Mode::Unified);
         return h.release();
     } catch (const std::bad_alloc&) {
         set_error(std::string("out of memory in ") + fn);
@@ -75,12 +77,14 @@ sunny_geneig_solver_t sunny_geneig_create_ex(int n, int device_id, int batches_p
                                              sunny_geneig_tuning_mode_t tuning_mode,
                                              sunny_geneig_cache_mode_t cache_mode,
                                              sunny_geneig_kernel_mode_t kernel_mode) {
-    if (kernel_mode != SUNNY_GENEIG_KERNEL_UNIFIED && kernel_mode != SUNNY_GENEIG_KERNEL_HYBRID) {
+    if (kernel_mode != SUNNY_GENEIG_KERNEL_UNIFIED && kernel_mode != SUNNY_GENEIG_KERNEL_HYBRID &&
+        kernel_mode != SUNNY_GENEIG_KERNEL_MIXED) {
         set_error("sunny_geneig_create_ex: unknown kernel_mode");
         return nullptr;
     }
-    // As in the reference, the split mode ignores tuning and batches-per-block.
-    if (kernel_mode == SUNNY_GENEIG_KERNEL_HYBRID)
+    // As in the reference, the split mode ignores tuning and batches-per-block;
+    // so does the mixed mode, which is built on the split kernels.
+    if (kernel_mode != SUNNY_GENEIG_KERNEL_UNIFIED)
         return create_impl(n, device_id, 0, SUNNY_GENEIG_TUNING_SUGGESTED, cache_mode, kernel_mode,
                            "sunny_geneig_create_ex");
     return create_impl(n, device_id, batches_per_block_request, tuning_mode, cache_mode, kernel_mode,
@@ -137,7 +141,9 @@ const char* sunny_geneig_last_error(void) { return g_last_error.c_str(); }
 
 const char* sunny_geneig_path_name(sunny_geneig_solver_t s) {
     if (!s) return "";
-    return s->mode == SUNNY_GENEIG_KERNEL_HYBRID ? "hybrid" : "unified";
+    return s->mode == SUNNY_GENEIG_KERNEL_HYBRID  ? "hybrid"
+           : s->mode == SUNNY_GENEIG_KERNEL_MIXED ? "mixed"
+                                                  : "unified";
 }
 
 sunny_geneig_kernel_mode_t sunny_geneig_kernel_mode(sunny_geneig_solver_t s) {
@@ -151,8 +157,9 @@ int sunny_geneig_precompile(int n, int block_dim_x, const char* arch) {
         if (n < 1 || n > 64) throw std::invalid_argument("n out of range for the LDS-resident kernel");
         sgeig::ModuleKey key{n, block_dim_x > 0 ? block_dim_x : sgeig::Solver::suggested_block_dim(n),
                              arch && arch[0] ? arch : "gfx950"};
-        auto code = sgeig::compile_code_object(key);
-        sgeig::write_disk(key, code);
+        sgeig::write_disk(key, sgeig::compile_code_object(key));
+        key.fp32 = true;
+        sgeig::write_disk(key, sgeig::compile_code_object(key));
         return SUNNY_GENEIG_OK;
     } catch (const sgeig::CompileError& e) {
         set_error(e.what());

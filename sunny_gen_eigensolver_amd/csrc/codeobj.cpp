@@ -63,7 +63,7 @@ uint64_t fnv1a(const void* data, size_t len, uint64_t h = 0xcbf29ce484222325ULL)
 }
 
 std::vector<std::string> compile_options(const ModuleKey& key) {
-    return {
+    std::vector<std::string> o = {
         "--offload-arch=" + key.arch,
         "-O3",
         "-std=c++17",
@@ -72,6 +72,8 @@ std::vector<std::string> compile_options(const ModuleKey& key) {
         // Keep the compiler from emitting packed FP32 instructions.
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     };
+    if (key.fp32) o.push_back("-DGENEIG_FP32");
+    return o;
 }
 
 int rtc_version() {
@@ -157,8 +159,8 @@ std::string cache_dir() {
 
 std::string code_object_path(const ModuleKey& key) {
     char name[160];
-    std::snprintf(name, sizeof(name), "geneig_n%d_wg%d_%s_%016llx.hsaco", key.n, key.block_dim,
-                  key.arch.c_str(), (unsigned long long)key_hash(key));
+    std::snprintf(name, sizeof(name), "geneig_n%d_wg%d_%s%s_%016llx.hsaco", key.n, key.block_dim,
+                  key.arch.c_str(), key.fp32 ? "_fp32" : "", (unsigned long long)key_hash(key));
     return cache_dir() + "/" + name;
 }
 

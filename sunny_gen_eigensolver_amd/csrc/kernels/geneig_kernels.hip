@@ -12,7 +12,7 @@
 //   ->  sort ascending  ->  V = Q Z  ->  U = L^-H V.
 //
 // Layout: column-major, element (i,j) of matrix b at b*N*N + j*N + i, complex
-// numbers as (re, im) double pairs. In LDS the two N x N complex work
+// numbers as (re, im) real pairs. In LDS the two N x N complex work
 // matrices use an odd leading dimension so that a wave reading one matrix
 // row (lane = column) touches 64 distinct 16-byte bank groups.
 //
@@ -48,27 +48,58 @@ constexpr int QL_MAXIT = 30;               // sweeps per eigenvalue before givin
 static_assert(NT % 64 == 0 && NW >= 1, "workgroup must be whole waves");
 static_assert(N >= 1 && N <= 64, "LDS-resident path holds one row per lane");
 
-struct __attribute__((aligned(16))) c64 {
-    double x, y;
+// Module precision: FP64 by default; -DGENEIG_FP32 builds the FP32 variant
+// of the split-mode stages used by the mixed-precision path.
+#ifdef GENEIG_FP32
+typedef float real;
+#else
+typedef double real;
+#endif
+
+struct __attribute__((aligned(2 * sizeof(real)))) cx {
+    real x, y;
 };
 
-__device__ __forceinline__ c64 cm(double a, double b) { c64 r; r.x = a; r.y = b; return r; }
-__device__ __forceinline__ c64 operator+(c64 a, c64 b) { return cm(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ c64 operator-(c64 a, c64 b) { return cm(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ c64 operator*(c64 a, double s) { return cm(a.x * s, a.y * s); }
-__device__ __forceinline__ c64 cmul(c64 a, c64 b) {
+__device__ __forceinline__ float sqrt_r(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double sqrt_r(double x) { return __builtin_sqrt(x); }
+
+#ifdef GENEIG_FP32
+constexpr real REAL_HALF_ULP = 0x1p-24f;   // unit roundoff
+constexpr real REAL_TINY = 0x1p-126f;      // smallest normal
+#else
+constexpr real REAL_HALF_ULP = 0x1p-53;
+constexpr real REAL_TINY = 0x1p-1022;
+#endif
+
+__device__ __forceinline__ cx cm(real a, real b) { cx r; r.x = a; r.y = b; return r; }
+__device__ __forceinline__ cx operator+(cx a, cx b) { return cm(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cx operator-(cx a, cx b) { return cm(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cx operator*(cx a, real s) { return cm(a.x * s, a.y * s); }
+__device__ __forceinline__ cx cmul(cx a, cx b) {
     return cm(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 // a * conj(b)
-__device__ __forceinline__ c64 cmulc(c64 a, c64 b) {
+__device__ __forceinline__ cx cmulc(cx a, cx b) {
     return cm(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 // conj(a) * b
-__device__ __forceinline__ c64 cconjmul(c64 a, c64 b) {
+__device__ __forceinline__ cx cconjmul(cx a, cx b) {
     return cm(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
 }
-__device__ __forceinline__ c64 cconj(c64 a) { return cm(a.x, -a.y); }
-__device__ __forceinline__ double cabs2(c64 a) { return a.x * a.x + a.y * a.y; }
+__device__ __forceinline__ cx cconj(cx a) { return cm(a.x, -a.y); }
+
+// Complex numbers as stored in global memory by the mixed-precision path:
+// complex128 inputs and outputs, complex64 hand-offs between its stages.
+struct __attribute__((aligned(16))) cd {
+    double x, y;
+};
+struct __attribute__((aligned(8))) cf {
+    float x, y;
+};
+__device__ __forceinline__ cx to_cx(const cd& a) { return cm((real)a.x, (real)a.y); }
+__device__ __forceinline__ cx to_cx(const cf& a) { return cm((real)a.x, (real)a.y); }
+__device__ __forceinline__ cx to_cx(const cx& a) { return a; }
+__device__ __forceinline__ real cabs2(cx a) { return a.x * a.x + a.y * a.y; }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 // Lane index, optionally behind a value barrier. With OPAQUE the per-lane
@@ -89,41 +120,58 @@ __device__ __forceinline__ double bcast(double v, int src) {
     int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
-__device__ __forceinline__ c64 bcast(c64 v, int src) { return cm(bcast(v.x, src), bcast(v.y, src)); }
+__device__ __forceinline__ float bcast(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ __forceinline__ cx bcast(cx v, int src) { return cm(bcast(v.x, src), bcast(v.y, src)); }
 
-// DPP move of a double between lanes (both halves with the same control).
+// DPP move of a value between lanes (each 32-bit half with the same control).
 template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double v) {
+__device__ __forceinline__ double dpp_mov(double v) {
     const long long b = __double_as_longlong(v);
     const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xf, 0xf, false);
     const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
 
 // Sum over the 64 lanes, returned in every lane: a butterfly inside each
 // 16-lane row with DPP (quad_perm [1,0,3,2], quad_perm [2,3,0,1],
 // row_half_mirror, row_mirror), then the four row sums via v_readlane.
-__device__ __forceinline__ double wave_sum(double v) {
-    v += dpp_f64<0xB1>(v);    // quad_perm [1,0,3,2]
-    v += dpp_f64<0x4E>(v);    // quad_perm [2,3,0,1]
-    v += dpp_f64<0x141>(v);   // row_half_mirror
-    v += dpp_f64<0x140>(v);   // row_mirror
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+    v += dpp_mov<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += dpp_mov<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += dpp_mov<0x141>(v);   // row_half_mirror
+    v += dpp_mov<0x140>(v);   // row_mirror
     return (bcast(v, 0) + bcast(v, 16)) + (bcast(v, 32) + bcast(v, 48));
+}
+
+template <class T>
+__device__ __forceinline__ T wave_max(T v) {
+    v = fmax(v, dpp_mov<0xB1>(v));
+    v = fmax(v, dpp_mov<0x4E>(v));
+    v = fmax(v, dpp_mov<0x141>(v));
+    v = fmax(v, dpp_mov<0x140>(v));
+    return fmax(fmax(bcast(v, 0), bcast(v, 16)), fmax(bcast(v, 32), bcast(v, 48)));
 }
 
 // Shared-memory work area of one matrix.
 struct Work {
-    c64 A[N * LDA];        // S -> L (lower); Householder vectors (strict upper, row k = v_k)
-    c64 B[N * LDA];        // H -> M -> T; then Z (real); then V
-    c64 tau[N];
-    c64 vbuf[N];
-    c64 wbuf[N];
-    c64 vb[2][N];          // Householder vector / Cholesky pivot column, by step parity
-    c64 wb[NW][N];         // per-wave copy of the rank-2 vector w
-    double invL[N];        // 1 / L(k,k)
-    double d[N];
-    double e[N];
-    c64 part[2][NW * N];   // reflector partial sums, double-buffered
+    cx A[N * LDA];        // S -> L (lower); Householder vectors (strict upper, row k = v_k)
+    cx B[N * LDA];        // H -> M -> T; then Z (real); then V
+    cx tau[N];
+    cx vbuf[N];
+    cx wbuf[N];
+    cx vb[2][N];          // Householder vector / Cholesky pivot column, by step parity
+    cx wb[NW][N];         // per-wave copy of the rank-2 vector w
+    real invL[N];        // 1 / L(k,k)
+    real d[N];
+    real e[N];
+    cx part[2][NW * N];   // reflector partial sums, real-buffered
     int flag[2];
 };
 
@@ -142,14 +190,14 @@ __device__ __forceinline__ int potrf(Work& w) {
     // One barrier per step.
     const int lane = call_lane<RELANE>(), wave = wave_id();
     auto finish_column = [&](int k) {
-        const c64 a = (lane >= k && lane < N) ? w.A[k * LDA + lane] : cm(0.0, 0.0);
-        const double dkk = bcast(a.x, k);
-        if (!(dkk > 0.0)) {
+        const cx a = (lane >= k && lane < N) ? w.A[k * LDA + lane] : cm(0.0, 0.0);
+        const real dkk = bcast(a.x, k);
+        if (!(dkk > real(0.0))) {
             if (lane == 0) w.flag[1] = k + 1;
             return;
         }
-        const double rinv = 1.0 / sqrt(dkk);
-        const c64 l = a * rinv;
+        const real rinv = real(1.0) / sqrt_r(dkk);
+        const cx l = a * rinv;
         if (lane >= k && lane < N) w.A[k * LDA + lane] = l;
         if (lane < N) w.vb[k & 1][lane] = l;
         if (lane == 0) w.invL[k] = rinv;
@@ -163,10 +211,10 @@ __device__ __forceinline__ int potrf(Work& w) {
         if (fail) return fail;
         const int p = k & 1;
         const int lr = lane < N ? lane : N - 1;
-        const c64 li = w.vb[p][lr];
+        const cx li = w.vb[p][lr];
         // All loads first (clamped addresses), then masked stores: no branch
         // separates the LDS reads, so their latencies overlap.
-        c64 a[CPW], lj[CPW];
+        cx a[CPW], lj[CPW];
 #pragma unroll
         for (int c = 0; c < CPW; ++c) {
             const int j = (wave + NW * c < N) ? wave + NW * c : N - 1;
@@ -197,9 +245,9 @@ __device__ __forceinline__ void trsm_left(Work& w) {
     const int lane = call_lane<RELANE>(), wave = wave_id();
     const int lr = lane < N ? lane : N - 1;
     for (int k = 0; k < N; ++k) {
-        const double inv = w.invL[k];
-        const c64 lik = w.A[k * LDA + lr];
-        c64 pv[CPW], bv[CPW];
+        const real inv = w.invL[k];
+        const cx lik = w.A[k * LDA + lr];
+        cx pv[CPW], bv[CPW];
 #pragma unroll
         for (int c = 0; c < CPW; ++c) {
             const int j = (wave + NW * c < N) ? wave + NW * c : N - 1;
@@ -209,8 +257,8 @@ __device__ __forceinline__ void trsm_left(Work& w) {
 #pragma unroll
         for (int c = 0; c < CPW; ++c) {
             const int j = wave + NW * c;
-            const c64 x = pv[c] * inv;
-            const c64 nb = (lane == k) ? x : bv[c] - cmul(lik, x);
+            const cx x = pv[c] * inv;
+            const cx nb = (lane == k) ? x : bv[c] - cmul(lik, x);
             if (j < N && lane >= k && lane < N) w.B[j * LDA + lane] = nb;
         }
     }
@@ -223,9 +271,9 @@ __device__ __forceinline__ void trsm_right(Work& w) {
     const int lane = call_lane<RELANE>(), wave = wave_id();
     const int lr = lane < N ? lane : N - 1;
     for (int k = 0; k < N; ++k) {
-        const double inv = w.invL[k];
-        const c64 ljk = w.A[k * LDA + lr];
-        c64 pv[CPW], bv[CPW];
+        const real inv = w.invL[k];
+        const cx ljk = w.A[k * LDA + lr];
+        cx pv[CPW], bv[CPW];
 #pragma unroll
         for (int c = 0; c < CPW; ++c) {
             const int i = (wave + NW * c < N) ? wave + NW * c : N - 1;
@@ -235,8 +283,8 @@ __device__ __forceinline__ void trsm_right(Work& w) {
 #pragma unroll
         for (int c = 0; c < CPW; ++c) {
             const int i = wave + NW * c;
-            const c64 y = pv[c] * inv;
-            const c64 nb = (lane == k) ? y : bv[c] - cmulc(y, ljk);
+            const cx y = pv[c] * inv;
+            const cx nb = (lane == k) ? y : bv[c] - cmulc(y, ljk);
             if (i < N && lane >= k && lane < N) w.B[lane * LDA + i] = nb;
         }
     }
@@ -254,21 +302,21 @@ template <bool RELANE = false>
 __device__ __forceinline__ void house(Work& w, int k) {
     const int lane = call_lane<RELANE>();
     const int m0 = k + 1;
-    const c64 x = (lane >= m0 && lane < N) ? w.B[k * LDA + lane] : cm(0.0, 0.0);
-    const c64 alpha = bcast(x, m0);
-    const double xn2 = wave_sum(lane > m0 ? cabs2(x) : 0.0);
-    c64 t = cm(0.0, 0.0);
-    double beta = alpha.x;
-    c64 v = cm(0.0, 0.0);
-    if (xn2 == 0.0 && alpha.y == 0.0) {
+    const cx x = (lane >= m0 && lane < N) ? w.B[k * LDA + lane] : cm(0.0, 0.0);
+    const cx alpha = bcast(x, m0);
+    const real xn2 = wave_sum(lane > m0 ? cabs2(x) : 0.0);
+    cx t = cm(0.0, 0.0);
+    real beta = alpha.x;
+    cx v = cm(0.0, 0.0);
+    if (xn2 == real(0.0) && alpha.y == real(0.0)) {
         if (lane == m0) v = cm(1.0, 0.0);
     } else {
-        const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
-        beta = (alpha.x >= 0.0) ? -nrm : nrm;
+        const real nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+        beta = (alpha.x >= real(0.0)) ? -nrm : nrm;
         t = cm((beta - alpha.x) / beta, -alpha.y / beta);
-        const c64 den = cm(alpha.x - beta, alpha.y);
-        const double dd = 1.0 / cabs2(den);
-        const c64 scal = cm(den.x * dd, -den.y * dd);   // 1 / (alpha - beta)
+        const cx den = cm(alpha.x - beta, alpha.y);
+        const real dd = real(1.0) / cabs2(den);
+        const cx scal = cm(den.x * dd, -den.y * dd);   // 1 / (alpha - beta)
         if (lane == m0) v = cm(1.0, 0.0);
         else if (lane > m0) v = cmul(x, scal);
     }
@@ -293,41 +341,41 @@ __device__ __forceinline__ void hetrd(Work& w) {
     for (int k = 0; k < N - 1; ++k) {
         const int m0 = k + 1;
         const int p = k & 1;
-        const c64 t = w.tau[k];
-        if (t.x != 0.0 || t.y != 0.0) {
+        const cx t = w.tau[k];
+        if (t.x != real(0.0) || t.y != real(0.0)) {
             const int lr = lane < N ? lane : N - 1;
-            const c64 vl = (lane < N) ? w.vb[p][lr] : cm(0.0, 0.0);
-            c64 a[CPW], vj[CPW];
+            const cx vl = (lane < N) ? w.vb[p][lr] : cm(0.0, 0.0);
+            cx a[CPW], vj[CPW];
 #pragma unroll
             for (int c = 0; c < CPW; ++c) {
                 const int j = (wave + NW * c < N) ? wave + NW * c : N - 1;
                 a[c] = w.B[j * LDA + lr];
                 vj[c] = w.vb[p][j];
             }
-            c64 acc = cm(0.0, 0.0);
+            cx acc = cm(0.0, 0.0);
 #pragma unroll
             for (int c = 0; c < CPW; ++c) {
                 const int j = wave + NW * c;
-                const c64 prod = cmul(a[c], vj[c]);
+                const cx prod = cmul(a[c], vj[c]);
                 if (j >= m0 && j < N) acc = acc + prod;
             }
             if (lane < N) w.part[0][wave * N + lane] = acc;
             __syncthreads();
-            c64 pi = cm(0.0, 0.0);
+            cx pi = cm(0.0, 0.0);
             if (lane < N) {
 #pragma unroll
                 for (int q = 0; q < NW; ++q) pi = pi + w.part[0][q * N + lane];
             }
             pi = cmul(t, pi);
-            const c64 pvl = cconjmul(pi, vl);
-            const c64 pv = cm(wave_sum(pvl.x), wave_sum(pvl.y));
-            const c64 cc = cmul(t, pv) * -0.5;
-            const c64 wi = pi + cmul(cc, vl);
+            const cx pvl = cconjmul(pi, vl);
+            const cx pv = cm(wave_sum(pvl.x), wave_sum(pvl.y));
+            const cx cc = cmul(t, pv) * -0.5;
+            const cx wi = pi + cmul(cc, vl);
             if (lane < N) w.wb[wave][lane] = wi;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            c64 wj[CPW];
+            cx wj[CPW];
 #pragma unroll
             for (int c = 0; c < CPW; ++c) {
                 const int j = (wave + NW * c < N) ? wave + NW * c : N - 1;
@@ -374,10 +422,16 @@ __device__ __forceinline__ void stash_reflectors(Work& w) {
 // Wilkinson, "The QR and QL algorithms for symmetric matrices" (Handbook
 // II/3). Returns zheev-style INFO (off-diagonals left unconverged, 0 = ok).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double sel_lane(double v, double x, int target) {
+__device__ __forceinline__ real sel_lane(real v, real x, int target) {
     return (lane_id() == target) ? x : v;
 }
 
+#ifdef GENEIG_FP32
+__device__ __forceinline__ float rsqrt_nr(float t) {
+    // v_rsq_f32 is accurate to about one ulp: no refinement needed in FP32.
+    return __builtin_amdgcn_rsqf(t);
+}
+#else
 __device__ __forceinline__ double rsqrt_nr(double t) {
     double y = __builtin_amdgcn_rsq(t);
     const double h = 0.5 * t;
@@ -387,13 +441,14 @@ __device__ __forceinline__ double rsqrt_nr(double t) {
     y = fma(y, r, y);
     return y;
 }
+#endif
 
-__device__ __forceinline__ int ql_wave(double& dreg, double& ereg, double (&z)[N]) {
+__device__ __forceinline__ int ql_wave(real& dreg, real& ereg, real (&z)[N]) {
     const int lane = lane_id();
-    const double eps = 0x1p-53;
+    const real eps = REAL_HALF_ULP;
     int l = 0, iter = 0;
     while (l < N) {
-        const double dnext = __shfl_down(dreg, 1, 64);
+        const real dnext = __shfl_down(dreg, 1, 64);
         const bool small = fabs(ereg) <= eps * (fabs(dreg) + fabs(dnext));
         unsigned long long mask = __ballot(small && lane >= l && lane < N - 1);
         mask |= 1ull << (N - 1);
@@ -405,37 +460,37 @@ __device__ __forceinline__ int ql_wave(double& dreg, double& ereg, double (&z)[N
             continue;
         }
         if (iter++ == QL_MAXIT) {
-            const unsigned long long nz = __ballot(ereg != 0.0 && lane >= l && lane < N - 1);
+            const unsigned long long nz = __ballot(ereg != real(0.0) && lane >= l && lane < N - 1);
             return __popcll(nz);
         }
-        const double d_l = bcast(dreg, l);
-        const double e_l = bcast(ereg, l);
-        double g = (bcast(dreg, l + 1) - d_l) / (2.0 * e_l);
-        double r = sqrt(g * g + 1.0);
+        const real d_l = bcast(dreg, l);
+        const real e_l = bcast(ereg, l);
+        real g = (bcast(dreg, l + 1) - d_l) / (real(2.0) * e_l);
+        real r = sqrt(g * g + real(1.0));
         g = bcast(dreg, m) - d_l + e_l / (g + copysign(r, g));
-        double s = 1.0, c = 1.0, p = 0.0;
-        double d_up = bcast(dreg, m);      // original d[i+1] for the current i
+        real s = 1.0, c = 1.0, p = 0.0;
+        real d_up = bcast(dreg, m);      // original d[i+1] for the current i
 #pragma unroll
         for (int i = N - 2; i >= 0; --i) {
             if (i < m && i >= l) {
-                const double e_i = bcast(ereg, i);
-                const double d_i = bcast(dreg, i);
-                const double f = s * e_i;
-                const double b = c * e_i;
+                const real e_i = bcast(ereg, i);
+                const real d_i = bcast(dreg, i);
+                const real f = s * e_i;
+                const real b = c * e_i;
                 // t > 0: every e_i in [l, m) is above the deflation threshold,
                 // so f = s e_i != 0 (s starts at 1 and stays nonzero). A
                 // floor keeps t finite even for inputs scaled near underflow.
-                const double t = fmax(fma(f, f, g * g), 0x1p-1022);
-                const double y = rsqrt_nr(t);
+                const real t = fmax(fma(f, f, g * g), REAL_TINY);
+                const real y = rsqrt_nr(t);
                 ereg = sel_lane(ereg, t * y, i + 1);
                 s = f * y;
                 c = g * y;
                 g = d_up - p;
-                const double r2 = fma(d_i - g, s, 2.0 * c * b);
+                const real r2 = fma(d_i - g, s, real(2.0) * c * b);
                 p = s * r2;
                 dreg = sel_lane(dreg, g + p, i + 1);
                 g = fma(c, r2, -b);
-                const double zi = z[i], zi1 = z[i + 1];
+                const real zi = z[i], zi1 = z[i + 1];
                 z[i + 1] = fma(s, zi, c * zi1);
                 z[i] = fma(c, zi, -(s * zi1));
                 d_up = d_i;
@@ -449,12 +504,12 @@ __device__ __forceinline__ int ql_wave(double& dreg, double& ereg, double (&z)[N
 }
 
 // Ascending position of lane j's eigenvalue (ties by index).
-__device__ __forceinline__ int ascending_rank(double dreg) {
+__device__ __forceinline__ int ascending_rank(real dreg) {
     const int lane = lane_id();
     int rank = 0;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        const double dk = bcast(dreg, k);
+        const real dk = bcast(dreg, k);
         rank += (dk < dreg || (dk == dreg && k < lane)) ? 1 : 0;
     }
     return rank;
@@ -463,11 +518,11 @@ __device__ __forceinline__ int ascending_rank(double dreg) {
 // Wave-level eigen-solve of the tridiagonal held in dsrc/esrc (LDS): writes the
 // ascending eigenvalues to W_out and leaves Z (columns in ascending order) in
 // `z` (lane r = row r) plus its rank permutation in `rank`.
-__device__ __forceinline__ int tridiag_eig_wave(const double* dsrc, const double* esrc, double* __restrict__ W_out,
-                                double (&z)[N], int& rank, int n_out = N) {
+__device__ __forceinline__ int tridiag_eig_wave(const real* dsrc, const real* esrc, real* __restrict__ W_out,
+                                real (&z)[N], int& rank, int n_out = N) {
     const int lane = lane_id();
-    double dreg = (lane < N) ? dsrc[lane] : 0.0;
-    double ereg = (lane < N - 1) ? esrc[lane] : 0.0;
+    real dreg = (lane < N) ? dsrc[lane] : 0.0;
+    real ereg = (lane < N - 1) ? esrc[lane] : 0.0;
 #pragma unroll
     for (int j = 0; j < N; ++j) z[j] = (lane == j) ? 1.0 : 0.0;
     const int info = ql_wave(dreg, ereg, z);
@@ -478,7 +533,7 @@ __device__ __forceinline__ int tridiag_eig_wave(const double* dsrc, const double
 
 // Write this wave's Z rows into Zs (real, leading dimension LDZ) with columns
 // permuted to ascending eigenvalue order.
-__device__ __forceinline__ void store_z_sorted(double* Zs, const double (&z)[N], int rank) {
+__device__ __forceinline__ void store_z_sorted(real* Zs, const real (&z)[N], int rank) {
     const int lane = lane_id();
 #pragma unroll
     for (int j = 0; j < N; ++j) {
@@ -498,8 +553,8 @@ __device__ __forceinline__ void store_z_sorted(double* Zs, const double (&z)[N],
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void apply_q(Work& w) {
     const int lane = lane_id(), wave = wave_id();
-    const double* Z = reinterpret_cast<const double*>(w.B);
-    c64 b[CPW];
+    const real* Z = reinterpret_cast<const real*>(w.B);
+    cx b[CPW];
 #pragma unroll
     for (int c = 0; c < CPW; ++c) {
         const int i = wave + NW * c;
@@ -507,11 +562,11 @@ __device__ __forceinline__ void apply_q(Work& w) {
     }
     int applied = 0;
     for (int k = N - 2; k >= 0; --k) {
-        const c64 t = w.tau[k];
-        if (t.x == 0.0 && t.y == 0.0) continue;
+        const cx t = w.tau[k];
+        if (t.x == real(0.0) && t.y == real(0.0)) continue;
         const int m0 = k + 1;
-        c64 v[CPW];
-        c64 acc = cm(0.0, 0.0);
+        cx v[CPW];
+        cx acc = cm(0.0, 0.0);
 #pragma unroll
         for (int c = 0; c < CPW; ++c) {
             const int i = wave + NW * c;
@@ -520,11 +575,11 @@ __device__ __forceinline__ void apply_q(Work& w) {
             else if (i > m0 && i < N) v[c] = w.A[i * LDA + k];
             acc = acc + cconjmul(v[c], b[c]);
         }
-        c64* part = w.part[applied & 1];
+        cx* part = w.part[applied & 1];
         ++applied;
         if (lane < N) part[wave * N + lane] = acc;
         __syncthreads();
-        c64 y = cm(0.0, 0.0);
+        cx y = cm(0.0, 0.0);
         if (lane < N) {
 #pragma unroll
             for (int q = 0; q < NW; ++q) y = y + part[q * N + lane];
@@ -542,9 +597,9 @@ __device__ __forceinline__ void apply_q(Work& w) {
 }
 
 // Stage 6: U = L^-H V column by column (lane = row), written straight to HBM.
-__device__ __forceinline__ void backtrsm_store(Work& w, c64* __restrict__ Ub) {
+__device__ __forceinline__ void backtrsm_store(Work& w, cx* __restrict__ Ub) {
     const int lane = lane_id(), wave = wave_id();
-    c64 b[CPW];
+    cx b[CPW];
 #pragma unroll
     for (int c = 0; c < CPW; ++c) {
         const int j = wave + NW * c;
@@ -553,11 +608,11 @@ __device__ __forceinline__ void backtrsm_store(Work& w, c64* __restrict__ Ub) {
 #pragma unroll 2
     for (int k = N - 1; k >= 0; --k) {
         // L(k, i) for lane i < k: column i, row k.
-        const c64 lki = (lane < k) ? w.A[lane * LDA + k] : cm(0.0, 0.0);
-        const double inv = w.invL[k];
+        const cx lki = (lane < k) ? w.A[lane * LDA + k] : cm(0.0, 0.0);
+        const real inv = w.invL[k];
 #pragma unroll
         for (int c = 0; c < CPW; ++c) {
-            const c64 uk = bcast(b[c], k) * inv;
+            const cx uk = bcast(b[c], k) * inv;
             if (lane == k) b[c] = uk;
             else if (lane < k) b[c] = b[c] - cconjmul(lki, uk);
         }
@@ -569,7 +624,7 @@ __device__ __forceinline__ void backtrsm_store(Work& w, c64* __restrict__ Ub) {
     }
 }
 
-__device__ __forceinline__ void load_pair(Work& w, const c64* __restrict__ Hb, const c64* __restrict__ Sb) {
+__device__ __forceinline__ void load_pair(Work& w, const cx* __restrict__ Hb, const cx* __restrict__ Sb) {
     for (int e = threadIdx.x; e < N * N; e += NT) {
         const int i = e % N, j = e / N;
         w.A[j * LDA + i] = Sb[e];
@@ -578,7 +633,7 @@ __device__ __forceinline__ void load_pair(Work& w, const c64* __restrict__ Hb, c
     __syncthreads();
 }
 
-__device__ __forceinline__ void fail_store(int code, double* __restrict__ Wb, c64* __restrict__ Ub,
+__device__ __forceinline__ void fail_store(int code, real* __restrict__ Wb, cx* __restrict__ Ub,
                            int* __restrict__ info_b) {
     for (int e = threadIdx.x; e < N * N; e += NT) Ub[e] = cm(0.0, 0.0);
     for (int e = threadIdx.x; e < N; e += NT) Wb[e] = __builtin_nan("");
@@ -592,10 +647,10 @@ __device__ __forceinline__ void fail_store(int code, double* __restrict__ Wb, c6
 // diagonal's imaginary part, tau_k on the first superdiagonal (k, k+1), and
 // reflector entries v_k(i) at (k, i), i >= k+2.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void stash_factors(const Work& w, c64* __restrict__ Ub) {
+__device__ __forceinline__ void stash_factors(const Work& w, cx* __restrict__ Ub) {
     for (int e = threadIdx.x; e < N * N; e += NT) {
         const int i = e % N, j = e / N;
-        c64 v;
+        cx v;
         if (i > j) v = w.A[j * LDA + i];
         else if (i == j) v = cm(w.A[j * LDA + j].x, w.invL[j]);
         else if (i == j - 1) v = w.tau[i];
@@ -604,10 +659,10 @@ __device__ __forceinline__ void stash_factors(const Work& w, c64* __restrict__ U
     }
 }
 
-__device__ __forceinline__ void unstash_factors(Work& w, const c64* __restrict__ Ub) {
+__device__ __forceinline__ void unstash_factors(Work& w, const cx* __restrict__ Ub) {
     for (int e = threadIdx.x; e < N * N; e += NT) {
         const int i = e % N, j = e / N;
-        const c64 v = Ub[e];
+        const cx v = Ub[e];
         if (i == j) {
             w.A[j * LDA + j] = cm(v.x, 0.0);
             w.invL[j] = v.y;
@@ -630,7 +685,7 @@ __device__ __forceinline__ void unstash_factors(Work& w, const c64* __restrict__
 // memory (wave-uniform addresses). Z arrives in `zbuf` (this wave's LDS slice,
 // N x N doubles, column-major) written by store_z_slice.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void store_z_slice(double* zbuf, const double (&z)[N], int rank) {
+__device__ __forceinline__ void store_z_slice(real* zbuf, const real (&z)[N], int rank) {
     const int lane = lane_id();
 #pragma unroll
     for (int j = 0; j < N; ++j) {
@@ -639,7 +694,7 @@ __device__ __forceinline__ void store_z_slice(double* zbuf, const double (&z)[N]
     }
 }
 
-__device__ __forceinline__ void load_z_columns(const double* zbuf, c64 (&b)[N]) {
+__device__ __forceinline__ void load_z_columns(const real* zbuf, cx (&b)[N]) {
     const int lane = lane_id();
 #pragma unroll
     for (int i = 0; i < N; ++i) b[i] = cm((lane < N) ? zbuf[lane * N + i] : 0.0, 0.0);
@@ -657,7 +712,7 @@ constexpr int RB = 8;                           // rows per branch-free block
 constexpr int NRB = (N + RB - 1) / RB;
 
 // Stage reflectors + tau (from the stash in global memory) into `sl`.
-__device__ __forceinline__ void stage_reflectors(c64* sl, const c64* __restrict__ Ub) {
+__device__ __forceinline__ void stage_reflectors(cx* sl, const cx* __restrict__ Ub) {
     const int lane = lane_id();
     for (int e = lane; e < N * N; e += 64) {
         const int i = e % N, j = e / N;           // stash element (row i, col j)
@@ -667,7 +722,7 @@ __device__ __forceinline__ void stage_reflectors(c64* sl, const c64* __restrict_
 }
 
 // Stage L (rows packed, 1/L(k,k) on the diagonal) into `sl`.
-__device__ __forceinline__ void stage_lower(c64* sl, const c64* __restrict__ Ub) {
+__device__ __forceinline__ void stage_lower(cx* sl, const cx* __restrict__ Ub) {
     const int lane = lane_id();
     for (int e = lane; e < N * N; e += 64) {
         const int i = e % N, j = e / N;
@@ -683,21 +738,21 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // V <- Q V for this lane's column; reflectors from the staged slice.
-__device__ __forceinline__ void apply_q_wave(const c64* sl, c64 (&b)[N]) {
+__device__ __forceinline__ void apply_q_wave(const cx* sl, cx (&b)[N]) {
     for (int k = N - 2; k >= 0; --k) {
-        const c64 t = sl[NREFL + k];
-        if (t.x == 0.0 && t.y == 0.0) continue;
+        const cx t = sl[NREFL + k];
+        if (t.x == real(0.0) && t.y == real(0.0)) continue;
         const int base = rpk(k) - k - 2;
-        c64 acc = cm(0.0, 0.0);
+        cx acc = cm(0.0, 0.0);
 #pragma unroll
         for (int blk = 0; blk < NRB; ++blk) {
             if (blk * RB + RB - 1 < k + 1) continue;
-            c64 vv[RB];
+            cx vv[RB];
 #pragma unroll
             for (int ii = 0; ii < RB; ++ii) {
                 const int i = blk * RB + ii;
                 if (i < N) {
-                    const c64 r = sl[(i > k + 1) ? base + i : 0];
+                    const cx r = sl[(i > k + 1) ? base + i : 0];
                     vv[ii] = (i > k + 1) ? r : cm((i == k + 1) ? 1.0 : 0.0, 0.0);
                 }
             }
@@ -707,16 +762,16 @@ __device__ __forceinline__ void apply_q_wave(const c64* sl, c64 (&b)[N]) {
                 if (i < N) acc = acc + cconjmul(vv[ii], b[i]);
             }
         }
-        const c64 y = cmul(t, acc);
+        const cx y = cmul(t, acc);
 #pragma unroll
         for (int blk = 0; blk < NRB; ++blk) {
             if (blk * RB + RB - 1 < k + 1) continue;
-            c64 vv[RB];
+            cx vv[RB];
 #pragma unroll
             for (int ii = 0; ii < RB; ++ii) {
                 const int i = blk * RB + ii;
                 if (i < N) {
-                    const c64 r = sl[(i > k + 1) ? base + i : 0];
+                    const cx r = sl[(i > k + 1) ? base + i : 0];
                     vv[ii] = (i > k + 1) ? r : cm((i == k + 1) ? 1.0 : 0.0, 0.0);
                 }
             }
@@ -732,21 +787,21 @@ __device__ __forceinline__ void apply_q_wave(const c64* sl, c64 (&b)[N]) {
 // Value barrier: the compiler may not see through it, so a select between
 // statically indexed registers is never rewritten as a dynamically indexed
 // (scratch) array access.
-__device__ __forceinline__ double opq(double x) {
+__device__ __forceinline__ real opq(real x) {
     asm volatile("" : "+v"(x));
     return x;
 }
-__device__ __forceinline__ c64 opq(c64 a) { return cm(opq(a.x), opq(a.y)); }
+__device__ __forceinline__ cx opq(cx a) { return cm(opq(a.x), opq(a.y)); }
 
 // U = L^-H V for this lane's column; L from the staged slice (rows packed).
 // Row blocks entirely above the pivot are untouched, blocks entirely below it
 // take a plain update, and only the pivot's block uses selects.
-__device__ __forceinline__ void backtrsm_wave(const c64* sl, c64 (&b)[N]) {
+__device__ __forceinline__ void backtrsm_wave(const cx* sl, cx (&b)[N]) {
     for (int k = N - 1; k >= 0; --k) {
         const int row = k * (k + 1) / 2;
-        const double inv = sl[row + k].x;
+        const real inv = sl[row + k].x;
         const int kb = k / RB;
-        c64 bk = cm(0.0, 0.0);
+        cx bk = cm(0.0, 0.0);
 #pragma unroll
         for (int blk = 0; blk < NRB; ++blk) {
             if (blk != kb) continue;
@@ -756,11 +811,11 @@ __device__ __forceinline__ void backtrsm_wave(const c64* sl, c64 (&b)[N]) {
                 if (i < N) bk = (i == k) ? opq(b[i]) : bk;
             }
         }
-        const c64 u = bk * inv;
+        const cx u = bk * inv;
 #pragma unroll
         for (int blk = 0; blk < NRB; ++blk) {
             if (blk > kb) continue;
-            c64 ll[RB];
+            cx ll[RB];
 #pragma unroll
             for (int ii = 0; ii < RB; ++ii) {
                 const int i = blk * RB + ii;
@@ -777,7 +832,7 @@ __device__ __forceinline__ void backtrsm_wave(const c64* sl, c64 (&b)[N]) {
                 for (int ii = 0; ii < RB; ++ii) {
                     const int i = blk * RB + ii;
                     if (i < N) {
-                        const c64 nb = b[i] - cconjmul(ll[ii], u);
+                        const cx nb = b[i] - cconjmul(ll[ii], u);
                         b[i] = (i < k) ? nb : ((i == k) ? opq(u) : b[i]);
                     }
                 }
@@ -786,7 +841,7 @@ __device__ __forceinline__ void backtrsm_wave(const c64* sl, c64 (&b)[N]) {
     }
 }
 
-__device__ __forceinline__ void store_columns(c64* __restrict__ Ub, const c64 (&b)[N]) {
+__device__ __forceinline__ void store_columns(cx* __restrict__ Ub, const cx (&b)[N]) {
     const int lane = lane_id();
     if (lane < N) {
 #pragma unroll
@@ -806,7 +861,7 @@ static_assert(ZSLICES >= 1, "work area must hold one slice");
 // Back half of every wave's matrix: Z -> V = Q Z (-> U = L^-H V when `with_l`)
 // -> Ub. Waves whose slices do not fit at once take turns; every wave passes
 // the same barriers.
-__device__ __forceinline__ void store_columns_n(c64* __restrict__ Ub, const c64 (&b)[N], int n) {
+__device__ __forceinline__ void store_columns_n(cx* __restrict__ Ub, const cx (&b)[N], int n) {
     const int lane = lane_id();
     if (lane < n) {
 #pragma unroll
@@ -815,17 +870,17 @@ __device__ __forceinline__ void store_columns_n(c64* __restrict__ Ub, const c64 
     }
 }
 
-__device__ __forceinline__ void back_half_waves(Work& w, bool active, const double (&z)[N], int rank,
-                                                c64* __restrict__ Ub, const c64* __restrict__ Lb,
-                                                bool with_l, c64* __restrict__ Uout = nullptr,
+__device__ __forceinline__ void back_half_waves(Work& w, bool active, const real (&z)[N], int rank,
+                                                cx* __restrict__ Ub, const cx* __restrict__ Lb,
+                                                bool with_l, cx* __restrict__ Uout = nullptr,
                                                 int n_out = N) {
     const int wave = wave_id();
     char* base = reinterpret_cast<char*>(&w);
     for (int r = 0; r < (NW + ZSLICES - 1) / ZSLICES; ++r) {
         if (active && wave / ZSLICES == r) {
-            c64* sl = reinterpret_cast<c64*>(base + (size_t)(wave % ZSLICES) * SLICE_BYTES);
-            double* zbuf = reinterpret_cast<double*>(sl);
-            c64 b[N];
+            cx* sl = reinterpret_cast<cx*>(base + (size_t)(wave % ZSLICES) * SLICE_BYTES);
+            real* zbuf = reinterpret_cast<real*>(sl);
+            cx b[N];
             store_z_slice(zbuf, z, rank);
             wave_lds_sync();
             load_z_columns(zbuf, b);
@@ -870,24 +925,24 @@ __device__ __forceinline__ int fresh(int x) {
 }
 
 struct WorkP {
-    c64 L[NPK];            // S -> L; then M -> T (reflectors below the subdiagonal)
-    c64 Wi[NPK];           // W = L^-1
+    cx L[NPK];            // S -> L; then M -> T (reflectors below the subdiagonal)
+    cx Wi[NPK];           // W = L^-1
     union {
-        c64 zj[N * ZB];    // Z block of H W^H
+        cx zj[N * ZB];    // Z block of H W^H
         struct {
-            c64 vb[2][N];
-            c64 wb[NW][N];
-            c64 part[NW * N];
+            cx vb[2][N];
+            cx wb[NW][N];
+            cx part[NW * N];
         } h;
     } u;
-    c64 tau[N];
-    double invL[N];
-    double d[N];
-    double e[N];
+    cx tau[N];
+    real invL[N];
+    real d[N];
+    real e[N];
     int flag[2];
 };
 
-__device__ __forceinline__ void load_lower_packed(c64* P, const c64* __restrict__ src) {
+__device__ __forceinline__ void load_lower_packed(cx* P, const cx* __restrict__ src) {
     for (int e = threadIdx.x; e < N * N; e += NT) {
         const int i = e % N, j = e / N;
         if (i >= j) P[pk(j) + i - j] = src[e];
@@ -901,14 +956,14 @@ __device__ __forceinline__ int potrf_packed(WorkP& w) {
     const int lane = lane_id();
     const int lr = lane < N ? lane : N - 1;
     auto finish_column = [&](int k) {
-        const c64 a = (lane >= k && lane < N) ? w.L[pk(k) + lane - k] : cm(0.0, 0.0);
-        const double dkk = bcast(a.x, k);
-        if (!(dkk > 0.0)) {
+        const cx a = (lane >= k && lane < N) ? w.L[pk(k) + lane - k] : cm(0.0, 0.0);
+        const real dkk = bcast(a.x, k);
+        if (!(dkk > real(0.0))) {
             if (lane == 0) w.flag[1] = k + 1;
             return;
         }
-        const double rinv = 1.0 / sqrt(dkk);
-        const c64 l = a * rinv;
+        const real rinv = real(1.0) / sqrt_r(dkk);
+        const cx l = a * rinv;
         if (lane >= k && lane < N) w.L[pk(k) + lane - k] = l;
         if (lane < N) w.u.h.vb[k & 1][lane] = l;
         if (lane == 0) w.invL[k] = rinv;
@@ -922,11 +977,11 @@ __device__ __forceinline__ int potrf_packed(WorkP& w) {
         if (fail) return fail;
         const int p = k & 1;
         const int wave = fresh(wave_id());
-        const c64 li = w.u.h.vb[p][lr];
+        const cx li = w.u.h.vb[p][lr];
 #pragma unroll
         for (int c0 = 0; c0 < CPW; c0 += CB) {
             __builtin_amdgcn_sched_barrier(0);   // keep chunks from interleaving (register pressure)
-            c64 a[CB], lj[CB];
+            cx a[CB], lj[CB];
 #pragma unroll
             for (int cc = 0; cc < CB; ++cc) {
                 const int c = c0 + cc;
@@ -960,12 +1015,12 @@ __device__ __forceinline__ void trtri_packed(WorkP& w) {
     }
     for (int k = 0; k < N; ++k) {
         const int wave = fresh(wave_id());
-        const double inv = w.invL[k];
-        const c64 lik = (lr > k) ? w.L[pk(k) + lr - k] : cm(0.0, 0.0);
+        const real inv = w.invL[k];
+        const cx lik = (lr > k) ? w.L[pk(k) + lr - k] : cm(0.0, 0.0);
 #pragma unroll
         for (int c0 = 0; c0 < CPW; c0 += CB) {
             __builtin_amdgcn_sched_barrier(0);   // keep chunks from interleaving (register pressure)
-            c64 pv[CB], bv[CB];
+            cx pv[CB], bv[CB];
 #pragma unroll
             for (int cc = 0; cc < CB; ++cc) {
                 const int j0 = wave + NW * (c0 + cc);
@@ -976,8 +1031,8 @@ __device__ __forceinline__ void trtri_packed(WorkP& w) {
 #pragma unroll
             for (int cc = 0; cc < CB; ++cc) {
                 const int j = wave + NW * (c0 + cc);
-                const c64 x = pv[cc] * inv;
-                const c64 nb = (lane == k) ? x : bv[cc] - cmul(lik, x);
+                const cx x = pv[cc] * inv;
+                const cx nb = (lane == k) ? x : bv[cc] - cmul(lik, x);
                 if (c0 + cc < CPW && j <= k && lane >= k && lane < N) w.Wi[pk(j) + lane - j] = nb;
             }
         }
@@ -987,24 +1042,24 @@ __device__ __forceinline__ void trtri_packed(WorkP& w) {
 // M = W H W^H (lower triangle, packed into w.L), eight columns at a time:
 // Z = H (W rows J)^H with H read from global memory (coalesced columns), then
 // M(:, J) = W Z on rows i >= j. Column c of a block belongs to wave c % NW.
-__device__ __forceinline__ void form_m(WorkP& w, const c64* __restrict__ Hb) {
+__device__ __forceinline__ void form_m(WorkP& w, const cx* __restrict__ Hb) {
     const int lane = lane_id(), wave = wave_id();
     const int lr = lane < N ? lane : N - 1;
 #pragma unroll
     for (int jb = 0; jb < NZB; ++jb) {
         const int j0 = jb * ZB;
-        c64 acc[ZPW];
+        cx acc[ZPW];
 #pragma unroll
         for (int cc = 0; cc < ZPW; ++cc) acc[cc] = cm(0.0, 0.0);
         const int kend = (j0 + ZB < N) ? j0 + ZB : N;
 #pragma unroll 4
         for (int k = 0; k < kend; ++k) {
-            const c64 h = Hb[k * N + lr];
+            const cx h = Hb[k * N + lr];
 #pragma unroll
             for (int cc = 0; cc < ZPW; ++cc) {
                 const int j = j0 + wave + NW * cc;
                 const bool on = (j < N) && (k <= j) && (wave + NW * cc < ZB);
-                const c64 wjk = w.Wi[on ? pk(k) + j - k : 0];
+                const cx wjk = w.Wi[on ? pk(k) + j - k : 0];
                 acc[cc] = acc[cc] + cmulc(h, on ? wjk : cm(0.0, 0.0));
             }
         }
@@ -1018,11 +1073,11 @@ __device__ __forceinline__ void form_m(WorkP& w, const c64* __restrict__ Hb) {
         for (int cc = 0; cc < ZPW; ++cc) acc[cc] = cm(0.0, 0.0);
 #pragma unroll 4
         for (int k = 0; k < N; ++k) {
-            const c64 wik = (lr >= k) ? w.Wi[pk(k) + lr - k] : cm(0.0, 0.0);
+            const cx wik = (lr >= k) ? w.Wi[pk(k) + lr - k] : cm(0.0, 0.0);
 #pragma unroll
             for (int cc = 0; cc < ZPW; ++cc) {
                 const int c = wave + NW * cc;
-                const c64 z = w.u.zj[(c < ZB ? c : 0) * N + k];
+                const cx z = w.u.zj[(c < ZB ? c : 0) * N + k];
                 acc[cc] = acc[cc] + cmul(wik, z);
             }
         }
@@ -1040,21 +1095,21 @@ __device__ __forceinline__ void form_m(WorkP& w, const c64* __restrict__ Hb) {
 __device__ __forceinline__ void house_packed(WorkP& w, int k) {
     const int lane = lane_id();
     const int m0 = k + 1;
-    const c64 x = (lane >= m0 && lane < N) ? w.L[pk(k) + lane - k] : cm(0.0, 0.0);
-    const c64 alpha = bcast(x, m0);
-    const double xn2 = wave_sum(lane > m0 ? cabs2(x) : 0.0);
-    c64 t = cm(0.0, 0.0);
-    double beta = alpha.x;
-    c64 v = cm(0.0, 0.0);
-    if (xn2 == 0.0 && alpha.y == 0.0) {
+    const cx x = (lane >= m0 && lane < N) ? w.L[pk(k) + lane - k] : cm(0.0, 0.0);
+    const cx alpha = bcast(x, m0);
+    const real xn2 = wave_sum(lane > m0 ? cabs2(x) : real(0.0));
+    cx t = cm(0.0, 0.0);
+    real beta = alpha.x;
+    cx v = cm(0.0, 0.0);
+    if (xn2 == real(0.0) && alpha.y == real(0.0)) {
         if (lane == m0) v = cm(1.0, 0.0);
     } else {
-        const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
-        beta = (alpha.x >= 0.0) ? -nrm : nrm;
+        const real nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+        beta = (alpha.x >= real(0.0)) ? -nrm : nrm;
         t = cm((beta - alpha.x) / beta, -alpha.y / beta);
-        const c64 den = cm(alpha.x - beta, alpha.y);
-        const double dd = 1.0 / cabs2(den);
-        const c64 scal = cm(den.x * dd, -den.y * dd);   // 1 / (alpha - beta)
+        const cx den = cm(alpha.x - beta, alpha.y);
+        const real dd = real(1.0) / cabs2(den);
+        const cx scal = cm(den.x * dd, -den.y * dd);   // 1 / (alpha - beta)
         if (lane == m0) v = cm(1.0, 0.0);
         else if (lane > m0) v = cmul(x, scal);
     }
@@ -1079,19 +1134,19 @@ __device__ __forceinline__ void hetrd_packed(WorkP& w) {
         const int m0 = k + 1;
         const int p = k & 1;
         const int wave = fresh(wave_id());
-        const c64 t = w.tau[k];
-        if (t.x != 0.0 || t.y != 0.0) {
-            const c64 vl = (lane < N) ? w.u.h.vb[p][lr] : cm(0.0, 0.0);
-            c64 acc = cm(0.0, 0.0);
+        const cx t = w.tau[k];
+        if (t.x != real(0.0) || t.y != real(0.0)) {
+            const cx vl = (lane < N) ? w.u.h.vb[p][lr] : cm(0.0, 0.0);
+            cx acc = cm(0.0, 0.0);
 #pragma unroll
             for (int c0 = 0; c0 < CPW; c0 += CB) {
                 __builtin_amdgcn_sched_barrier(0);   // keep chunks from interleaving (register pressure)
-                c64 a[CB], vj[CB];
+                cx a[CB], vj[CB];
 #pragma unroll
                 for (int cc = 0; cc < CB; ++cc) {
                     const int c = c0 + cc;
                     const int j = (c < CPW && wave + NW * c < N) ? wave + NW * c : N - 1;
-                    const c64 m = w.L[(lr >= j) ? pk(j) + lr - j : pk(lr) + j - lr];
+                    const cx m = w.L[(lr >= j) ? pk(j) + lr - j : pk(lr) + j - lr];
                     a[cc] = (lr >= j) ? m : cconj(m);
                     vj[cc] = w.u.h.vb[p][j];
                 }
@@ -1099,22 +1154,22 @@ __device__ __forceinline__ void hetrd_packed(WorkP& w) {
                 for (int cc = 0; cc < CB; ++cc) {
                     const int c = c0 + cc;
                     const int j = wave + NW * c;
-                    const c64 prod = cmul(a[cc], vj[cc]);
+                    const cx prod = cmul(a[cc], vj[cc]);
                     if (c < CPW && j >= m0 && j < N) acc = acc + prod;
                 }
             }
             if (lane < N) w.u.h.part[wave * N + lane] = acc;
             __syncthreads();
-            c64 pi = cm(0.0, 0.0);
+            cx pi = cm(0.0, 0.0);
             if (lane < N) {
 #pragma unroll
                 for (int q = 0; q < NW; ++q) pi = pi + w.u.h.part[q * N + lane];
             }
             pi = cmul(t, pi);
-            const c64 pvl = cconjmul(pi, vl);
-            const c64 pv = cm(wave_sum(pvl.x), wave_sum(pvl.y));
-            const c64 cc = cmul(t, pv) * -0.5;
-            const c64 wi = pi + cmul(cc, vl);
+            const cx pvl = cconjmul(pi, vl);
+            const cx pv = cm(wave_sum(pvl.x), wave_sum(pvl.y));
+            const cx cc = cmul(t, pv) * -0.5;
+            const cx wi = pi + cmul(cc, vl);
             if (lane < N) w.u.h.wb[wave][lane] = wi;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -1122,7 +1177,7 @@ __device__ __forceinline__ void hetrd_packed(WorkP& w) {
 #pragma unroll
             for (int c0 = 0; c0 < CPW; c0 += CB) {
                 __builtin_amdgcn_sched_barrier(0);   // keep chunks from interleaving (register pressure)
-                c64 a[CB], vj[CB], wj[CB];
+                cx a[CB], vj[CB], wj[CB];
 #pragma unroll
                 for (int cc = 0; cc < CB; ++cc) {
                     const int c = c0 + cc;
@@ -1153,10 +1208,10 @@ __device__ __forceinline__ void hetrd_packed(WorkP& w) {
 
 // Stash in the matrix's U slot: W on and below the diagonal, tau_k at
 // (k, k+1), reflector entries v_k(i) at (k, i), i >= k+2.
-__device__ __forceinline__ void stash_packed(const WorkP& w, c64* __restrict__ Ub) {
+__device__ __forceinline__ void stash_packed(const WorkP& w, cx* __restrict__ Ub) {
     for (int e = threadIdx.x; e < N * N; e += NT) {
         const int i = e % N, j = e / N;
-        c64 v;
+        cx v;
         if (i >= j) v = w.Wi[pk(j) + i - j];
         else if (i == j - 1) v = w.tau[i];
         else v = w.L[pk(i) + j - i];
@@ -1173,10 +1228,10 @@ __device__ __forceinline__ void stash_packed(const WorkP& w, c64* __restrict__ U
 
 // Cholesky with the lower part of owned columns in registers; L is written to
 // the packed LDS triangle at the end (for the triangular inverse).
-__device__ __forceinline__ int potrf_reg(WorkP& w, const c64* __restrict__ Sb) {
+__device__ __forceinline__ int potrf_reg(WorkP& w, const cx* __restrict__ Sb) {
     const int lane = lane_id(), wave = wave_id();
     const int lr = lane < N ? lane : N - 1;
-    c64 a[CPW];
+    cx a[CPW];
 #pragma unroll
     for (int c = 0; c < CPW; ++c) {
         const int j = wave + NW * c;
@@ -1188,11 +1243,11 @@ __device__ __forceinline__ int potrf_reg(WorkP& w, const c64* __restrict__ Sb) {
 #pragma unroll
         for (int c = 0; c < CPW; ++c) {
             if (wave + NW * c == k) {
-                const double dkk = bcast(a[c].x, k);
-                if (!(dkk > 0.0)) {
+                const real dkk = bcast(a[c].x, k);
+                if (!(dkk > real(0.0))) {
                     if (lane == 0) w.flag[1] = k + 1;
                 } else {
-                    const double rinv = 1.0 / sqrt(dkk);
+                    const real rinv = real(1.0) / sqrt_r(dkk);
                     a[c] = a[c] * rinv;
                     if (lane < N) w.u.h.vb[k & 1][lane] = a[c];
                     if (lane == 0) w.invL[k] = rinv;
@@ -1206,8 +1261,8 @@ __device__ __forceinline__ int potrf_reg(WorkP& w, const c64* __restrict__ Sb) {
         const int fail = w.flag[1];
         if (fail) return fail;
         const int p = k & 1;
-        const c64 li = w.u.h.vb[p][lr];
-        c64 lj[CPW];
+        const cx li = w.u.h.vb[p][lr];
+        cx lj[CPW];
 #pragma unroll
         for (int c = 0; c < CPW; ++c) {
             const int j = wave + NW * c;
@@ -1216,7 +1271,7 @@ __device__ __forceinline__ int potrf_reg(WorkP& w, const c64* __restrict__ Sb) {
 #pragma unroll
         for (int c = 0; c < CPW; ++c) {
             const int j = wave + NW * c;
-            const c64 na = a[c] - cmulc(li, lj[c]);
+            const cx na = a[c] - cmulc(li, lj[c]);
             if (j > k && j < N && lane >= j && lane < N) a[c] = na;
         }
         if (k + 1 < N && wave == (k + 1) % NW) finish(k + 1);
@@ -1233,25 +1288,26 @@ __device__ __forceinline__ int potrf_reg(WorkP& w, const c64* __restrict__ Sb) {
 // M = W H W^H, full columns, straight into the owning waves' registers
 // (column j = j0 + wave + NW cc of block jb is owned by wave j % NW, slot
 // j / NW). Z blocks go through LDS as in form_m.
-__device__ __forceinline__ void form_m_reg(WorkP& w, const c64* __restrict__ Hb, c64 (&m)[CPW]) {
+template <class HT>
+__device__ __forceinline__ void form_m_reg(WorkP& w, const HT* __restrict__ Hb, cx (&m)[CPW]) {
     const int lane = lane_id(), wave = wave_id();
     const int lr = lane < N ? lane : N - 1;
     static_assert(ZB % NW == 0, "block columns must map onto owning waves");
 #pragma unroll
     for (int jb = 0; jb < NZB; ++jb) {
         const int j0 = jb * ZB;
-        c64 acc[ZPW];
+        cx acc[ZPW];
 #pragma unroll
         for (int cc = 0; cc < ZPW; ++cc) acc[cc] = cm(0.0, 0.0);
         const int kend = (j0 + ZB < N) ? j0 + ZB : N;
 #pragma unroll 4
         for (int k = 0; k < kend; ++k) {
-            const c64 h = Hb[k * N + lr];
+            const cx h = to_cx(Hb[k * N + lr]);
 #pragma unroll
             for (int cc = 0; cc < ZPW; ++cc) {
                 const int j = j0 + wave + NW * cc;
                 const bool on = (j < N) && (k <= j);
-                const c64 wjk = w.Wi[on ? pk(k) + j - k : 0];
+                const cx wjk = w.Wi[on ? pk(k) + j - k : 0];
                 acc[cc] = acc[cc] + cmulc(h, on ? wjk : cm(0.0, 0.0));
             }
         }
@@ -1265,7 +1321,7 @@ __device__ __forceinline__ void form_m_reg(WorkP& w, const c64* __restrict__ Hb,
         for (int cc = 0; cc < ZPW; ++cc) acc[cc] = cm(0.0, 0.0);
 #pragma unroll 4
         for (int k = 0; k < N; ++k) {
-            const c64 wik = (lr >= k) ? w.Wi[pk(k) + lr - k] : cm(0.0, 0.0);
+            const cx wik = (lr >= k) ? w.Wi[pk(k) + lr - k] : cm(0.0, 0.0);
 #pragma unroll
             for (int cc = 0; cc < ZPW; ++cc) {
                 const int c = wave + NW * cc;
@@ -1282,28 +1338,28 @@ __device__ __forceinline__ void form_m_reg(WorkP& w, const c64* __restrict__ Hb,
 }
 
 // Householder vector for column k held in registers (slot c) of its owner.
-__device__ __forceinline__ void house_reg(WorkP& w, int k, c64& col) {
+__device__ __forceinline__ void house_reg(WorkP& w, int k, cx& col) {
     const int lane = lane_id();
     const int m0 = k + 1;
-    const c64 x = (lane >= m0 && lane < N) ? col : cm(0.0, 0.0);
-    const c64 alpha = bcast(x, m0);
-    const double xn2 = wave_sum(lane > m0 ? cabs2(x) : 0.0);
-    c64 t = cm(0.0, 0.0);
-    double beta = alpha.x;
-    c64 v = cm(0.0, 0.0);
-    if (xn2 == 0.0 && alpha.y == 0.0) {
+    const cx x = (lane >= m0 && lane < N) ? col : cm(0.0, 0.0);
+    const cx alpha = bcast(x, m0);
+    const real xn2 = wave_sum(lane > m0 ? cabs2(x) : real(0.0));
+    cx t = cm(0.0, 0.0);
+    real beta = alpha.x;
+    cx v = cm(0.0, 0.0);
+    if (xn2 == real(0.0) && alpha.y == real(0.0)) {
         if (lane == m0) v = cm(1.0, 0.0);
     } else {
-        const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
-        beta = (alpha.x >= 0.0) ? -nrm : nrm;
+        const real nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+        beta = (alpha.x >= real(0.0)) ? -nrm : nrm;
         t = cm((beta - alpha.x) / beta, -alpha.y / beta);
-        const c64 den = cm(alpha.x - beta, alpha.y);
-        const double dd = 1.0 / cabs2(den);
-        const c64 scal = cm(den.x * dd, -den.y * dd);   // 1 / (alpha - beta)
+        const cx den = cm(alpha.x - beta, alpha.y);
+        const real dd = real(1.0) / cabs2(den);
+        const cx scal = cm(den.x * dd, -den.y * dd);   // 1 / (alpha - beta)
         if (lane == m0) v = cm(1.0, 0.0);
         else if (lane > m0) v = cmul(x, scal);
     }
-    const double dk = bcast(col.x, k);
+    const real dk = bcast(col.x, k);
     if (lane > m0 && lane < N) col = v;                 // reflector kept in the column
     if (lane < N) w.u.h.vb[k & 1][lane] = v;
     if (lane == 0) {
@@ -1315,7 +1371,7 @@ __device__ __forceinline__ void house_reg(WorkP& w, int k, c64& col) {
 
 // Tridiagonalisation with the full Hermitian M in the owning waves'
 // registers (schedule as in hetrd: two barriers per step).
-__device__ __forceinline__ void hetrd_reg(WorkP& w, c64 (&m)[CPW]) {
+__device__ __forceinline__ void hetrd_reg(WorkP& w, cx (&m)[CPW]) {
     const int lane = lane_id(), wave = wave_id();
     const int lr = lane < N ? lane : N - 1;
     if (wave == 0) house_reg(w, 0, m[0]);
@@ -1323,34 +1379,34 @@ __device__ __forceinline__ void hetrd_reg(WorkP& w, c64 (&m)[CPW]) {
     for (int k = 0; k < N - 1; ++k) {
         const int m0 = k + 1;
         const int p = k & 1;
-        const c64 t = w.tau[k];
-        if (t.x != 0.0 || t.y != 0.0) {
-            const c64 vl = (lane < N) ? w.u.h.vb[p][lr] : cm(0.0, 0.0);
-            c64 vj[CPW];
+        const cx t = w.tau[k];
+        if (t.x != real(0.0) || t.y != real(0.0)) {
+            const cx vl = (lane < N) ? w.u.h.vb[p][lr] : cm(0.0, 0.0);
+            cx vj[CPW];
 #pragma unroll
             for (int c = 0; c < CPW; ++c) {
                 const int j = wave + NW * c;
                 vj[c] = w.u.h.vb[p][j < N ? j : N - 1];
             }
-            c64 acc = cm(0.0, 0.0);
+            cx acc = cm(0.0, 0.0);
 #pragma unroll
             for (int c = 0; c < CPW; ++c) {
                 const int j = wave + NW * c;
-                const c64 prod = cmul(m[c], vj[c]);
+                const cx prod = cmul(m[c], vj[c]);
                 if (j >= m0 && j < N) acc = acc + prod;
             }
             if (lane < N) w.u.h.part[wave * N + lane] = acc;
             __syncthreads();
-            c64 pi = cm(0.0, 0.0);
+            cx pi = cm(0.0, 0.0);
             if (lane < N) {
 #pragma unroll
                 for (int q = 0; q < NW; ++q) pi = pi + w.u.h.part[q * N + lane];
             }
             pi = cmul(t, pi);
-            const c64 pvl = cconjmul(pi, vl);
-            const c64 pv = cm(wave_sum(pvl.x), wave_sum(pvl.y));
-            const c64 cc = cmul(t, pv) * -0.5;
-            const c64 wi = pi + cmul(cc, vl);
+            const cx pvl = cconjmul(pi, vl);
+            const cx pv = cm(wave_sum(pvl.x), wave_sum(pvl.y));
+            const cx cc = cmul(t, pv) * -0.5;
+            const cx wi = pi + cmul(cc, vl);
             if (lane < N) w.u.h.wb[wave][lane] = wi;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -1358,8 +1414,8 @@ __device__ __forceinline__ void hetrd_reg(WorkP& w, c64 (&m)[CPW]) {
 #pragma unroll
             for (int c = 0; c < CPW; ++c) {
                 const int j = wave + NW * c;
-                const c64 wj = w.u.h.wb[wave][j < N ? j : N - 1];
-                const c64 na = m[c] - cmulc(vl, wj) - cmulc(wi, vj[c]);
+                const cx wj = w.u.h.wb[wave][j < N ? j : N - 1];
+                const cx na = m[c] - cmulc(vl, wj) - cmulc(wi, vj[c]);
                 if (j >= m0 && j < N && lane >= m0 && lane < N) m[c] = na;
             }
         }
@@ -1374,7 +1430,7 @@ __device__ __forceinline__ void hetrd_reg(WorkP& w, c64 (&m)[CPW]) {
 #pragma unroll
         for (int c = 0; c < CPW; ++c) {
             if (wave + NW * c == N - 1) {
-                const double dl = bcast(m[c].x, N - 1);
+                const real dl = bcast(m[c].x, N - 1);
                 if (lane == 0) {
                     w.d[N - 1] = dl;
                     w.e[N - 1] = 0.0;
@@ -1389,7 +1445,7 @@ __device__ __forceinline__ void hetrd_reg(WorkP& w, c64 (&m)[CPW]) {
 // Stash: W (LDS) on and below the diagonal, tau on the superdiagonal, and the
 // reflector entries from the owning waves' registers (column k, rows >= k+2,
 // go to (k, i) of the stash).
-__device__ __forceinline__ void stash_reg(const WorkP& w, const c64 (&m)[CPW], c64* __restrict__ Ub) {
+__device__ __forceinline__ void stash_reg(const WorkP& w, const cx (&m)[CPW], cx* __restrict__ Ub) {
     const int lane = lane_id(), wave = wave_id();
     for (int e = threadIdx.x; e < N * N; e += NT) {
         const int i = e % N, j = e / N;
@@ -1404,7 +1460,7 @@ __device__ __forceinline__ void stash_reg(const WorkP& w, const c64 (&m)[CPW], c
 }
 
 // Stage W (columns packed) from the stash into a wave's slice.
-__device__ __forceinline__ void stage_w(c64* sl, const c64* __restrict__ Ub) {
+__device__ __forceinline__ void stage_w(cx* sl, const cx* __restrict__ Ub) {
     const int lane = lane_id();
     for (int e = lane; e < N * N; e += 64) {
         const int i = e % N, j = e / N;
@@ -1414,14 +1470,14 @@ __device__ __forceinline__ void stage_w(c64* sl, const c64* __restrict__ Ub) {
 
 // U = W^H V for this lane's column, in place in ascending row order:
 // u_i = sum_{k >= i} conj(W(k, i)) v_k.
-__device__ __forceinline__ void trmm_wh_wave(const c64* sl, c64 (&b)[N]) {
+__device__ __forceinline__ void trmm_wh_wave(const cx* sl, cx (&b)[N]) {
     for (int i = 0; i < N; ++i) {
         const int col = pk(i) - i;            // W(k, i) at col + k, k >= i
-        c64 acc = cm(0.0, 0.0);
+        cx acc = cm(0.0, 0.0);
 #pragma unroll
         for (int blk = 0; blk < NRB; ++blk) {
             if (blk * RB + RB - 1 < i) continue;
-            c64 ww[RB];
+            cx ww[RB];
 #pragma unroll
             for (int ii = 0; ii < RB; ++ii) {
                 const int k = blk * RB + ii;
@@ -1431,7 +1487,7 @@ __device__ __forceinline__ void trmm_wh_wave(const c64* sl, c64 (&b)[N]) {
             for (int ii = 0; ii < RB; ++ii) {
                 const int k = blk * RB + ii;
                 if (k < N) {
-                    const c64 t = cconjmul(ww[ii], b[k]);
+                    const cx t = cconjmul(ww[ii], b[k]);
                     if (k >= i) acc = acc + t;
                 }
             }
@@ -1451,15 +1507,16 @@ __device__ __forceinline__ void trmm_wh_wave(const c64* sl, c64 (&b)[N]) {
 
 // Stage data kept per matrix of a workgroup between the front and back halves.
 struct Staged {
-    double d[NW][N];
-    double e[NW][N];
+    real d[NW][N];
+    real e[NW][N];
     int status[NW];        // 0 = ok, otherwise the matrix is finished (failed)
     int qinfo[NW];
 };
 
 }  // namespace gk
 
-using gk::c64;
+using gk::cx;
+using gk::real;
 
 namespace gk {
 // Back half of the fused kernel: each wave finishes its own matrix in its
@@ -1467,15 +1524,15 @@ namespace gk {
 constexpr int PSLICES = (int)(sizeof(WorkP) / SLICE_BYTES) < NW ? (int)(sizeof(WorkP) / SLICE_BYTES) : NW;
 static_assert(PSLICES >= 1, "packed work area must hold one slice");
 
-__device__ __forceinline__ void back_half_packed(WorkP& w, bool active, const double (&z)[N], int rank,
-                                                 c64* __restrict__ Ub) {
+__device__ __forceinline__ void back_half_packed(WorkP& w, bool active, const real (&z)[N], int rank,
+                                                 cx* __restrict__ Ub) {
     const int wave = wave_id();
     char* base = reinterpret_cast<char*>(&w);
     for (int r = 0; r < (NW + PSLICES - 1) / PSLICES; ++r) {
         if (active && wave / PSLICES == r) {
-            c64* sl = reinterpret_cast<c64*>(base + (size_t)(wave % PSLICES) * SLICE_BYTES);
-            double* zbuf = reinterpret_cast<double*>(sl);
-            c64 b[N];
+            cx* sl = reinterpret_cast<cx*>(base + (size_t)(wave % PSLICES) * SLICE_BYTES);
+            real* zbuf = reinterpret_cast<real*>(sl);
+            cx b[N];
             store_z_slice(zbuf, z, rank);
             wave_lds_sync();
             load_z_columns(zbuf, b);
@@ -1501,9 +1558,10 @@ __device__ __forceinline__ void back_half_packed(WorkP& w, bool active, const do
 // slot; then every wave runs the QL of one matrix at the same time (Z in its
 // registers); then the dense back half (V = Q Z, U = L^-H V) for each in turn.
 // ---------------------------------------------------------------------------
+#ifndef GENEIG_FP32
 extern "C" __global__ void __launch_bounds__(GENEIG_NT)
-geneig_fused(const c64* __restrict__ H, const c64* __restrict__ S, double* __restrict__ W,
-             c64* __restrict__ U, int* __restrict__ info, int batch) {
+geneig_fused(const cx* __restrict__ H, const cx* __restrict__ S, real* __restrict__ W,
+             cx* __restrict__ U, int* __restrict__ info, int batch) {
     using namespace gk;
     __shared__ Work w;
     __shared__ Staged st;
@@ -1534,7 +1592,7 @@ geneig_fused(const c64* __restrict__ H, const c64* __restrict__ S, double* __res
         if (threadIdx.x == 0) st.status[q] = 0;
         __syncthreads();
     }
-    double z[N];
+    real z[N];
     int rank = 0;
     {
         const int b = b0 + wave;
@@ -1570,8 +1628,8 @@ geneig_fused(const c64* __restrict__ H, const c64* __restrict__ S, double* __res
 namespace gk {
 constexpr double VARLEN_PAD = 0x1p600;
 
-__device__ __forceinline__ void load_pair_padded(Work& w, const c64* __restrict__ Hb,
-                                                 const c64* __restrict__ Sb, int n) {
+__device__ __forceinline__ void load_pair_padded(Work& w, const cx* __restrict__ Hb,
+                                                 const cx* __restrict__ Sb, int n) {
     for (int e = threadIdx.x; e < N * N; e += NT) {
         const int i = e % N, j = e / N;
         const bool in = i < n && j < n;
@@ -1581,8 +1639,8 @@ __device__ __forceinline__ void load_pair_padded(Work& w, const c64* __restrict_
     __syncthreads();
 }
 
-__device__ __forceinline__ void fail_store_n(int code, int n, double* __restrict__ Wb,
-                                             c64* __restrict__ Ub, int* __restrict__ info_b) {
+__device__ __forceinline__ void fail_store_n(int code, int n, real* __restrict__ Wb,
+                                             cx* __restrict__ Ub, int* __restrict__ info_b) {
     for (int e = threadIdx.x; e < n * n; e += NT) Ub[e] = cm(0.0, 0.0);
     for (int e = threadIdx.x; e < n; e += NT) Wb[e] = __builtin_nan("");
     if (threadIdx.x == 0) *info_b = code;
@@ -1590,17 +1648,17 @@ __device__ __forceinline__ void fail_store_n(int code, int n, double* __restrict
 }  // namespace gk
 
 extern "C" __global__ void __launch_bounds__(GENEIG_NT)
-geneig_varlen(const c64* __restrict__ H, const c64* __restrict__ S, double* __restrict__ W,
-              c64* __restrict__ U, int* __restrict__ info, const int* __restrict__ ids,
+geneig_varlen(const cx* __restrict__ H, const cx* __restrict__ S, real* __restrict__ W,
+              cx* __restrict__ U, int* __restrict__ info, const int* __restrict__ ids,
               const long long* __restrict__ moff, const long long* __restrict__ woff,
-              const int* __restrict__ nn, int count, c64* __restrict__ scratch) {
+              const int* __restrict__ nn, int count, cx* __restrict__ scratch) {
     using namespace gk;
     __shared__ Work w;
     __shared__ Staged st;
     __shared__ int qid[NW];
     __shared__ int qn[NW];
     const int lane = lane_id(), wave = wave_id();
-    c64* slot0 = scratch + (size_t)blockIdx.x * NW * N * N;
+    cx* slot0 = scratch + (size_t)blockIdx.x * NW * N * N;
     {
         const int base = blockIdx.x * NW;
         if (base >= count) return;
@@ -1643,7 +1701,7 @@ geneig_varlen(const c64* __restrict__ H, const c64* __restrict__ S, double* __re
         const int id = qid[wave];
         const int n = qn[wave];
         const bool active = id >= 0 && st.status[wave] == 0;
-        double z[N];
+        real z[N];
         int rank = 0;
         if (active) {
             const int qi = tridiag_eig_wave(st.d[wave], st.e[wave], W + woff[id], z, rank, n);
@@ -1653,7 +1711,7 @@ geneig_varlen(const c64* __restrict__ H, const c64* __restrict__ S, double* __re
         {
             // Back half into the scratch slot (N x N), then copy the leading
             // n x n block out.
-            c64* slot = slot0 + (size_t)wave * N * N;
+            cx* slot = slot0 + (size_t)wave * N * N;
             back_half_waves(w, active, z, rank, slot, slot, true, U + (active ? moff[id] : 0), n);
             if (active && lane == 0) info[id] = st.qinfo[wave];
         }
@@ -1675,8 +1733,8 @@ geneig_varlen(const c64* __restrict__ H, const c64* __restrict__ S, double* __re
 // A matrix whose Cholesky failed carries info > N and is skipped downstream.
 // ---------------------------------------------------------------------------
 extern "C" __global__ void __launch_bounds__(GENEIG_NT, 2)
-geneig_split_front(const c64* __restrict__ H, const c64* __restrict__ S, double* __restrict__ W,
-                   c64* __restrict__ U, int* __restrict__ info, double* __restrict__ E, int batch) {
+geneig_split_front(const cx* __restrict__ H, const cx* __restrict__ S, real* __restrict__ W,
+                   cx* __restrict__ U, int* __restrict__ info, real* __restrict__ E, int batch) {
     using namespace gk;
     __shared__ WorkP w;
     const int b = blockIdx.x;
@@ -1690,7 +1748,7 @@ geneig_split_front(const c64* __restrict__ H, const c64* __restrict__ S, double*
     __syncthreads();
     trtri_packed(w);
     __syncthreads();
-    c64 m[CPW];
+    cx m[CPW];
     form_m_reg(w, H + off, m);
     hetrd_reg(w, m);
     stash_reg(w, m, U + off);
@@ -1700,26 +1758,27 @@ geneig_split_front(const c64* __restrict__ H, const c64* __restrict__ S, double*
     }
     if (threadIdx.x == 0) info[b] = 0;
 }
+#endif  // !GENEIG_FP32
 
 extern "C" __global__ void __launch_bounds__(GENEIG_NT)
-geneig_split_ql(double* __restrict__ W, const double* __restrict__ E, double* __restrict__ Zw,
+geneig_split_ql(real* __restrict__ W, const real* __restrict__ E, real* __restrict__ Zw,
                 int* __restrict__ info, int batch) {
     using namespace gk;
     const int b = blockIdx.x * NW + wave_id();
     if (b >= batch) return;
     if (info[b] != 0) return;
     const int lane = lane_id();
-    double* Wb = W + (size_t)b * N;
-    const double* Eb = E + (size_t)b * N;
-    double dreg = (lane < N) ? Wb[lane] : 0.0;
-    double ereg = (lane < N - 1) ? Eb[lane] : 0.0;
-    double z[N];
+    real* Wb = W + (size_t)b * N;
+    const real* Eb = E + (size_t)b * N;
+    real dreg = (lane < N) ? Wb[lane] : 0.0;
+    real ereg = (lane < N - 1) ? Eb[lane] : 0.0;
+    real z[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) z[j] = (lane == j) ? 1.0 : 0.0;
     const int qi = ql_wave(dreg, ereg, z);
     const int rank = ascending_rank(lane < N ? dreg : __builtin_inf());
     if (lane < N) Wb[rank] = dreg;
-    double* Zb = Zw + (size_t)b * N * N;
+    real* Zb = Zw + (size_t)b * N * N;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
         const int pos = __builtin_amdgcn_readlane(rank, j);
@@ -1734,7 +1793,7 @@ static_assert(BACK_SLICES >= 1, "one back-half slice must fit");
 }  // namespace gk
 
 extern "C" __global__ void __launch_bounds__(GENEIG_NT)
-geneig_split_back(c64* __restrict__ U, const double* __restrict__ Zw, const int* __restrict__ info,
+geneig_split_back(cx* __restrict__ U, const real* __restrict__ Zw, const int* __restrict__ info,
                   int batch) {
     using namespace gk;
     // One wave per matrix: lane j holds column j of V; reflectors and W are
@@ -1747,8 +1806,8 @@ geneig_split_back(c64* __restrict__ U, const double* __restrict__ Zw, const int*
     for (int r = 0; r < (NW + BACK_SLICES - 1) / BACK_SLICES; ++r) {
         if (active && wave / BACK_SLICES == r) {
             const size_t off = (size_t)b * N * N;
-            c64* sl = reinterpret_cast<c64*>(slices[wave % BACK_SLICES]);
-            c64 v[N];
+            cx* sl = reinterpret_cast<cx*>(slices[wave % BACK_SLICES]);
+            cx v[N];
 #pragma unroll
             for (int i = 0; i < N; ++i) v[i] = cm((lane < N) ? Zw[off + lane * N + i] : 0.0, 0.0);
             stage_reflectors(sl, U + off);
@@ -1763,3 +1822,432 @@ geneig_split_back(c64* __restrict__ U, const double* __restrict__ Zw, const int*
         __syncthreads();
     }
 }
+
+// ---------------------------------------------------------------------------
+// Mixed precision (FP32 tridiagonalisation and eigenvectors, FP64 Cholesky
+// and refinement). Per matrix:
+//   geneig_mx_chol    (FP64 module): S = L L^H in FP64; L rounded to complex64
+//                     (packed lower triangle) into L32. A non-positive pivot
+//                     ends the matrix here with zhegv's INFO, as in the other
+//                     modes.
+//   geneig_mx_front   (FP32 module): W = L^-1, M = W H W^H (H read as complex128
+//                     and rounded), Householder tridiagonalisation; the stash
+//                     (W, reflectors, tau) goes to U32, T to W32 / E32.
+//   geneig_split_ql   (FP32 module): QL with FP32 unit roundoff, Z to Z32.
+//   geneig_split_back (FP32 module): U32 = W^H Q Z.
+//   geneig_mx_refine  (FP64 module): Ogita-Aishima refinement of (W32, U32)
+//                     on the original pencil (H, S) in FP64, then exact 2 x 2
+//                     Rayleigh-Ritz on pairs of close eigenvalues; W, U out.
+// ---------------------------------------------------------------------------
+#ifndef GENEIG_FP32
+extern "C" __global__ void __launch_bounds__(GENEIG_NT, 2)
+geneig_mx_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ U,
+               int* __restrict__ info, cx* __restrict__ Wd, int batch) {
+    using namespace gk;
+    __shared__ WorkP w;
+    const int b = blockIdx.x;
+    if (b >= batch) return;
+    const size_t off = (size_t)b * N * N;
+    const int pinfo = potrf_reg(w, S + off);
+    if (pinfo) {
+        fail_store(N + pinfo, W + (size_t)b * N, U + off, info + b);
+        return;
+    }
+    __syncthreads();
+    trtri_packed(w);
+    __syncthreads();
+    cx* Wb = Wd + (size_t)b * NPK;
+    for (int e = threadIdx.x; e < NPK; e += NT) Wb[e] = w.Wi[e];
+    if (threadIdx.x == 0) info[b] = 0;
+}
+#endif  // !GENEIG_FP32
+
+#ifdef GENEIG_FP32
+extern "C" __global__ void __launch_bounds__(GENEIG_NT, 2)
+geneig_mx_front(const gk::cd* __restrict__ H, const gk::cd* __restrict__ Wd, real* __restrict__ W32,
+                cx* __restrict__ U32, const int* __restrict__ info, real* __restrict__ E32, int batch) {
+    using namespace gk;
+    __shared__ WorkP w;
+    const int b = blockIdx.x;
+    if (b >= batch || info[b] != 0) return;
+    const size_t off = (size_t)b * N * N;
+    const cd* Wb = Wd + (size_t)b * NPK;
+    for (int e = threadIdx.x; e < NPK; e += NT) w.Wi[e] = to_cx(Wb[e]);
+    __syncthreads();
+    cx m[CPW];
+    form_m_reg(w, H + off, m);
+    hetrd_reg(w, m);
+    stash_reg(w, m, U32 + off);
+    if (threadIdx.x < N) {
+        W32[(size_t)b * N + threadIdx.x] = w.d[threadIdx.x];
+        E32[(size_t)b * N + threadIdx.x] = w.e[threadIdx.x];
+    }
+}
+
+// V = Q Z per wave (the FP32 back half without the W^H factor, which the FP64
+// refinement applies): reflectors staged from the stash in U32, V over it.
+extern "C" __global__ void __launch_bounds__(GENEIG_NT)
+geneig_mx_back(cx* __restrict__ U32, const real* __restrict__ Zw, const int* __restrict__ info, int batch) {
+    using namespace gk;
+    __shared__ __attribute__((aligned(16))) char slices[BACK_SLICES][SLICE_BYTES];
+    const int wave = wave_id(), lane = lane_id();
+    const int b = blockIdx.x * NW + wave;
+    const bool active = b < batch && info[b] <= N;
+    for (int r = 0; r < (NW + BACK_SLICES - 1) / BACK_SLICES; ++r) {
+        if (active && wave / BACK_SLICES == r) {
+            const size_t off = (size_t)b * N * N;
+            cx* sl = reinterpret_cast<cx*>(slices[wave % BACK_SLICES]);
+            cx v[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) v[i] = cm((lane < N) ? Zw[off + lane * N + i] : real(0.0), 0.0);
+            stage_reflectors(sl, U32 + off);
+            wave_lds_sync();
+            apply_q_wave(sl, v);
+            store_columns(U32 + off, v);
+        }
+        __syncthreads();
+    }
+}
+#endif  // GENEIG_FP32
+
+#ifndef GENEIG_FP32
+
+namespace gk {
+// Complex FP64 GEMMs of the refinement on the matrix cores:
+// v_mfma_f64_16x16x4f64 takes A(i, k) from lane i + 16 k and B(k, j) from lane
+// j + 16 k (i, j < 16, k < 4) and leaves D(i, j) in lane j + 16 (i % 4),
+// element i / 4. A 16 x 16 complex tile is a pair of such accumulators; a wave
+// owns one 16-column block of the result (all row tiles), so reductions over
+// a column stay inside the wave.
+typedef double d4 __attribute__((ext_vector_type(4)));
+constexpr int NB = (N + 15) / 16;          // 16-blocks per dimension
+constexpr int REF_NT = 64 * NB;            // one wave per column block
+constexpr int LDR = N | 1;
+
+struct ctile {
+    d4 re, im;
+};
+
+__device__ __forceinline__ void cmfma(ctile& c, cx a, cx b) {
+    c.re = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, c.re, 0, 0, 0);
+    c.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, c.re, 0, 0, 0);
+    c.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, c.im, 0, 0, 0);
+    c.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, c.im, 0, 0, 0);
+}
+__device__ __forceinline__ cx tile_get(const ctile& t, int v) { return cm(t.re[v], t.im[v]); }
+
+// Element (i, k) of an N x N column-major matrix (leading dimension ld), zero
+// outside; the address is clamped so the load is branch-free.
+__device__ __forceinline__ cx elem(const cx* M, int ld, int i, int k) {
+    const bool in = i < N && k < N;
+    const cx v = M[(in ? k : 0) * ld + (in ? i : 0)];
+    return in ? v : cm(0.0, 0.0);
+}
+
+// Element (i, k) of a lower-triangular matrix stored packed by columns in
+// global memory (zero above the diagonal and outside).
+__device__ __forceinline__ cx pelem(const cx* P, int i, int k) {
+    const bool in = i < N && k < N && i >= k;
+    const cx v = P[in ? pk(k) + i - k : 0];
+    return in ? v : cm(0.0, 0.0);
+}
+
+// acc[rt] (row tile rt, column block cb) = A B, A(i, k) = fa(i, k),
+// B(k, j) = fb(k, j), k < 16 NB (the fetchers return zero outside N).
+template <class FA, class FB>
+__device__ __forceinline__ void cgemm(ctile (&acc)[NB], FA fa, FB fb, int cb) {
+    const int l = lane_id();
+    const int c16 = l & 15, k4 = l >> 4;
+#pragma unroll
+    for (int rt = 0; rt < NB; ++rt) {
+        acc[rt].re = d4{0.0, 0.0, 0.0, 0.0};
+        acc[rt].im = d4{0.0, 0.0, 0.0, 0.0};
+    }
+#pragma unroll 4
+    for (int k0 = 0; k0 < 16 * NB; k0 += 4) {
+        const cx b = fb(k0 + k4, cb * 16 + c16);
+#pragma unroll
+        for (int rt = 0; rt < NB; ++rt) cmfma(acc[rt], fa(rt * 16 + c16, k0 + k4), b);
+    }
+}
+
+// acc[rt] = A Bt, where the right operand Bt is this wave's own column block
+// held in registers in the result layout: the result layout of row tile q is
+// the B-operand layout of the k-steps 16 q .. 16 q + 15 (element v of tile q
+// = rows 16 q + 4 v .. 16 q + 4 v + 3), so no LDS round trip is needed.
+template <class FA>
+__device__ __forceinline__ void cgemm_reg(ctile (&acc)[NB], FA fa, const ctile (&Bt)[NB]) {
+    const int l = lane_id();
+    const int c16 = l & 15, k4 = l >> 4;
+#pragma unroll
+    for (int rt = 0; rt < NB; ++rt) {
+        acc[rt].re = d4{0.0, 0.0, 0.0, 0.0};
+        acc[rt].im = d4{0.0, 0.0, 0.0, 0.0};
+    }
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const cx b = tile_get(Bt[q], v);
+            const int k = 16 * q + 4 * v + k4;
+#pragma unroll
+            for (int rt = 0; rt < NB; ++rt) cmfma(acc[rt], fa(rt * 16 + c16, k), b);
+        }
+    }
+}
+
+// Sum over the four lanes that hold one column (lanes c, c + 16, c + 32, c + 48).
+__device__ __forceinline__ real col_sum(real v) {
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
+struct RefWork {
+    cx U[N * LDR];        // eigenvector estimate V
+    cx B[N * LDR];        // H, then M = W H W^H
+    real lam[N];
+};
+
+// 2 x 2 Hermitian-definite pencil (T, G) (G close to I): eigenvalues
+// mu1 <= mu2 and G-orthonormal eigenvectors X = [x1 x2], by a Cholesky of G
+// and the closed-form eigensystem of C = L^-1 T L^-H. Arguments wave-uniform.
+__device__ __forceinline__ void eig2_pencil(real t11, real t22, cx t12, real g11, real g22, cx g12,
+                                            real& mu1, real& mu2, cx (&X)[2][2]) {
+    const real l11 = sqrt_r(g11);
+    const cx l21 = cm(g12.x / l11, -g12.y / l11);            // conj(g12) / l11
+    const real l22 = sqrt_r(g22 - cabs2(l21));
+    const cx m11 = cm(t11 / l11, 0.0), m12 = cm(t12.x / l11, t12.y / l11);
+    const cx m21 = (cm(t12.x, -t12.y) - cmul(l21, m11)) * (real(1.0) / l22);
+    const cx m22 = (cm(t22, 0.0) - cmul(l21, m12)) * (real(1.0) / l22);
+    const cx c21 = m21 * (real(1.0) / l11);
+    const real a = t11 / (l11 * l11);
+    const real c = (m22 - cmulc(c21, l21)).x / l22;
+    const cx bb = cm(c21.x, -c21.y);                          // C(1, 2)
+    const real d = real(0.5) * (c - a);
+    const real r = sqrt_r(d * d + cabs2(bb));
+    mu1 = real(0.5) * (a + c) - r;
+    mu2 = real(0.5) * (a + c) + r;
+    cx y1a, y1b;                                              // eigenvector of mu1
+    if (d >= real(0.0)) {
+        y1a = cm(d + r, 0.0);
+        y1b = cm(-bb.x, bb.y);
+    } else {
+        y1a = bb;
+        y1b = cm(d - r, 0.0);
+    }
+    const real nrm = sqrt_r(cabs2(y1a) + cabs2(y1b));
+    if (nrm > real(0.0)) {
+        y1a = y1a * (real(1.0) / nrm);
+        y1b = y1b * (real(1.0) / nrm);
+    } else {
+        y1a = cm(1.0, 0.0);
+        y1b = cm(0.0, 0.0);
+    }
+    const cx y2a = cm(-y1b.x, y1b.y), y2b = cm(y1a.x, -y1a.y);
+    const cx cl21 = cm(l21.x, -l21.y);
+    X[1][0] = y1b * (real(1.0) / l22);
+    X[0][0] = (y1a - cmul(cl21, X[1][0])) * (real(1.0) / l11);
+    X[1][1] = y2b * (real(1.0) / l22);
+    X[0][1] = (y2a - cmul(cl21, X[1][1])) * (real(1.0) / l11);
+}
+
+// Rayleigh-Ritz on columns (a, b) of V for the Hermitian M (held in r.B), by
+// one wave: the 2 x 2 projected pencil (V^H M V, V^H V) of the pair, then
+// V(:, [a b]) <- V(:, [a b]) X and lam[a], lam[b] <- the Ritz values.
+__device__ __forceinline__ void ritz_pair(RefWork& r, int a, int b) {
+    const int lane = lane_id();
+    const int lr = lane < N ? lane : N - 1;
+    const bool in = lane < N;
+    cx ha = cm(0.0, 0.0), hb = cm(0.0, 0.0);
+    for (int k = 0; k < N; ++k) {
+        const cx mk = r.B[k * LDR + lr];
+        ha = ha + cmul(mk, r.U[a * LDR + k]);
+        hb = hb + cmul(mk, r.U[b * LDR + k]);
+    }
+    const cx ua = in ? r.U[a * LDR + lr] : cm(0.0, 0.0);
+    const cx ub = in ? r.U[b * LDR + lr] : cm(0.0, 0.0);
+    const real t11 = wave_sum(in ? cconjmul(ua, ha).x : real(0.0));
+    const real t22 = wave_sum(in ? cconjmul(ub, hb).x : real(0.0));
+    const cx t12v = cconjmul(ua, hb);
+    const cx t12 = cm(wave_sum(in ? t12v.x : real(0.0)), wave_sum(in ? t12v.y : real(0.0)));
+    const real g11 = wave_sum(in ? cabs2(ua) : real(0.0));
+    const real g22 = wave_sum(in ? cabs2(ub) : real(0.0));
+    const cx g12v = cconjmul(ua, ub);
+    const cx g12 = cm(wave_sum(in ? g12v.x : real(0.0)), wave_sum(in ? g12v.y : real(0.0)));
+    real mu1, mu2;
+    cx X[2][2];
+    eig2_pencil(t11, t22, t12, g11, g22, g12, mu1, mu2, X);
+    if (in) {
+        r.U[a * LDR + lane] = cmul(ua, X[0][0]) + cmul(ub, X[1][0]);
+        r.U[b * LDR + lane] = cmul(ua, X[0][1]) + cmul(ub, X[1][1]);
+    }
+    if (lane == 0) {
+        r.lam[a] = mu1;
+        r.lam[b] = mu2;
+    }
+    wave_lds_sync();
+}
+}  // namespace gk
+
+// Refinement in the Cholesky frame: with W = L^-1 (FP64), M = W H W^H and the
+// FP32 eigenvectors V of M, iters Ogita-Aishima steps on the matrix cores
+// (T = V^H M V, G = V^H V, lambda_j = T_jj / G_jj, V <- V + V E with E as
+// described at its computation), Rayleigh-Ritz on every cluster (a run of
+// eigenvalues with adjacent gaps below delta: one exact 2 x 2 step for a
+// pair, Jacobi sweeps over all its pairs otherwise), then U = W^H V. In step
+// it (from 0), delta = 2^it delta_rel max |lambda|. U^H S U = V^H V up to the
+// accuracy of W, as in the FP64 modes.
+extern "C" __global__ void __launch_bounds__(gk::REF_NT)
+geneig_mx_refine(const cx* __restrict__ H, const cx* __restrict__ Wd, real* __restrict__ W,
+                 cx* __restrict__ U, const int* __restrict__ info, const gk::cf* __restrict__ V32,
+                 const float* __restrict__ W32, int batch, int iters, real delta_rel) {
+    using namespace gk;
+    __shared__ RefWork r;                      // r.U: V; r.B: H, then M
+    const int b = blockIdx.x;
+    if (b >= batch || info[b] > N) return;     // Cholesky failures keep their outputs
+    const int l = lane_id(), cb = wave_id();
+    const int g4 = l >> 4;
+    const int j = cb * 16 + (l & 15);          // this lane's column
+    const bool jin = j < N;
+    const size_t off = (size_t)b * N * N;
+    const cx* Hb = H + off;
+    const cx* Wb = Wd + (size_t)b * NPK;
+    for (int e = threadIdx.x; e < N * N; e += REF_NT) {
+        r.U[(e / N) * LDR + e % N] = to_cx(V32[off + e]);
+        r.B[(e / N) * LDR + e % N] = Hb[e];
+    }
+    if (threadIdx.x < N) r.lam[threadIdx.x] = (real)W32[(size_t)b * N + threadIdx.x];
+    __syncthreads();
+    auto fB = [&](int i, int k) { return elem(r.B, LDR, i, k); };
+    auto fV = [&](int i, int k) { return elem(r.U, LDR, i, k); };
+    auto fVh = [&](int i, int k) { return cconj(elem(r.U, LDR, k, i)); };
+    {
+        // M = W (H W^H), into B.
+        ctile Z[NB], M[NB];
+        cgemm(Z, fB, [&](int k, int jj) { return cconj(pelem(Wb, jj, k)); }, cb);
+        cgemm_reg(M, [&](int i, int k) { return pelem(Wb, i, k); }, Z);
+        __syncthreads();     // every wave is done reading H
+#pragma unroll
+        for (int rt = 0; rt < NB; ++rt) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = rt * 16 + g4 + 4 * v;
+                if (i < N && jin) r.B[j * LDR + i] = tile_get(M[rt], v);
+            }
+        }
+        __syncthreads();
+    }
+    real delta = real(0.0);
+    for (int it = 0; it < iters; ++it) {
+        ctile T[NB], G[NB];
+        {
+            ctile X[NB];
+            cgemm(X, fB, fV, cb);                // X = M V
+            cgemm_reg(T, fVh, X);                // T = V^H X
+        }
+        cgemm(G, fVh, fV, cb);                   // G = V^H V
+        // Diagonal: lambda_j = T_jj / G_jj (the lane holding (j, j) publishes).
+#pragma unroll
+        for (int rt = 0; rt < NB; ++rt) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = rt * 16 + g4 + 4 * v;
+                if (i == j && jin) r.lam[j] = T[rt].re[v] / G[rt].re[v];
+            }
+        }
+        __syncthreads();
+        // The threshold doubles every step, so a pair treated as close once
+        // stays close (its vectors are then mixed) and ends in Rayleigh-Ritz.
+        delta = delta_rel * (real)(1 << it) * wave_max(l < N ? fabs(r.lam[l < N ? l : 0]) : real(0.0));
+        // E = -(G - I) / 2 + A with A anti-Hermitian: A_ij = (T_ij - (lambda_i +
+        // lambda_j) G_ij / 2) / (lambda_j - lambda_i) for separated pairs, 0 for
+        // close ones (equal to Ogita-Aishima's (T_ij - lambda_j G_ij) /
+        // (lambda_j - lambda_i) in exact arithmetic; the split keeps V^H V = I
+        // free of rounding divided by small gaps).
+        const real lj = jin ? r.lam[j] : real(0.0);
+#pragma unroll
+        for (int rt = 0; rt < NB; ++rt) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = rt * 16 + g4 + 4 * v;
+                const real li = r.lam[i < N ? i : 0];
+                const cx t = tile_get(T[rt], v), g = tile_get(G[rt], v);
+                cx e = g * real(-0.5);
+                const real gap = lj - li;
+                if (i == j) e = cm(real(0.5) * (real(1.0) - g.x), 0.0);
+                else if (!(fabs(gap) < delta)) e = e + (t - g * (real(0.5) * (li + lj))) * (real(1.0) / gap);
+                if (i >= N || !jin) e = cm(0.0, 0.0);
+                T[rt].re[v] = e.x;
+                T[rt].im[v] = e.y;
+            }
+        }
+        cgemm_reg(G, fV, T);                     // V E
+#pragma unroll
+        for (int rt = 0; rt < NB; ++rt) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = rt * 16 + g4 + 4 * v;
+                const cx u = elem(r.U, LDR, i, j) + tile_get(G[rt], v);
+                G[rt].re[v] = u.x;
+                G[rt].im[v] = u.y;
+            }
+        }
+        __syncthreads();     // every wave is done reading V
+#pragma unroll
+        for (int rt = 0; rt < NB; ++rt) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = rt * 16 + g4 + 4 * v;
+                if (i < N && jin) r.U[j * LDR + i] = tile_get(G[rt], v);
+            }
+        }
+        __syncthreads();
+    }
+    // Clusters: runs [c0, c1] with adjacent gaps below delta (wave 0).
+    if (cb == 0 && iters > 0) {
+        int c0 = 0;
+        while (c0 < N - 1) {
+            int c1 = c0;
+            while (c1 + 1 < N && fabs(r.lam[c1 + 1] - r.lam[c1]) < delta) ++c1;
+            if (c1 > c0) {
+                const int sweeps = (c1 == c0 + 1) ? 1 : 4;
+                for (int sw = 0; sw < sweeps; ++sw)
+                    for (int a = c0; a < c1; ++a)
+                        for (int bb = a + 1; bb <= c1; ++bb) ritz_pair(r, a, bb);
+                // Keep the run ascending.
+                for (int a = c0 + 1; a <= c1; ++a)
+                    for (int q = a; q > c0 && r.lam[q] < r.lam[q - 1]; --q) {
+                        const real tl = r.lam[q];
+                        if (l < N) {
+                            const cx t0 = r.U[q * LDR + l];
+                            r.U[q * LDR + l] = r.U[(q - 1) * LDR + l];
+                            r.U[(q - 1) * LDR + l] = t0;
+                        }
+                        wave_lds_sync();
+                        if (l == 0) {
+                            r.lam[q] = r.lam[q - 1];
+                            r.lam[q - 1] = tl;
+                        }
+                        wave_lds_sync();
+                    }
+            }
+            c0 = c1 + 1;
+        }
+    }
+    __syncthreads();
+    // U = W^H V.
+    ctile Uo[NB];
+    cgemm(Uo, [&](int i, int k) { return cconj(pelem(Wb, k, i)); }, fV, cb);
+    cx* Ub = U + off;
+#pragma unroll
+    for (int rt = 0; rt < NB; ++rt) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = rt * 16 + g4 + 4 * v;
+            if (i < N && jin) Ub[(size_t)j * N + i] = tile_get(Uo[rt], v);
+        }
+    }
+    if (threadIdx.x < N) W[(size_t)b * N + threadIdx.x] = r.lam[threadIdx.x];
+}
+#endif  // !GENEIG_FP32

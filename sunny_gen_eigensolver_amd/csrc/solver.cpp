@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -118,11 +119,12 @@ Solver::Solver(int n, int device_id, int bpb_request, Tuning tuning, CacheMode c
 }
 
 Solver::~Solver() {
-    if (lbuf_) {
+    if (lbuf_ || mbuf_) {
         int cur = 0;
         (void)hipGetDevice(&cur);
         (void)hipSetDevice(device_);
-        (void)hipFree(lbuf_);
+        if (lbuf_) (void)hipFree(lbuf_);
+        if (mbuf_) (void)hipFree(mbuf_);
         (void)hipSetDevice(cur);
     }
 }
@@ -136,6 +138,21 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
     k_reduce_ = module_->get("geneig_split_front");
     k_heev_ = module_->get("geneig_split_ql");
     k_back_ = module_->get("geneig_split_back");
+    if (mode_ == Mode::Mixed) {
+        k_mx_chol_ = module_->get("geneig_mx_chol");
+        k_refine_ = module_->get("geneig_mx_refine");
+        ModuleKey k32 = key;
+        k32.fp32 = true;
+        CacheLayer l32;
+        double ms32 = 0.0;
+        module32_ = acquire_module(k32, cache, &l32, &ms32);
+        acquire_ms_ += ms32;
+        k_mx_front_ = module32_->get("geneig_mx_front");
+        k_ql32_ = module32_->get("geneig_split_ql");
+        k_back32_ = module32_->get("geneig_mx_back");
+        if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_ITERS")) refine_iters_ = std::max(0, std::atoi(v));
+        if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_DELTA")) refine_delta_ = std::atof(v);
+    }
     int lds = 0;
     SG_HIP(hipFuncGetAttribute(&lds, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, fused_));
     int max_lds = 0;
@@ -234,7 +251,8 @@ void Solver::launch(const void* H, const void* S, double* W, void* U, int* info,
     SG_HIP(hipGetDevice(&cur));
     if (cur != device_) SG_HIP(hipSetDevice(device_));
     if (mode_ == Mode::Unified) launch_unified_(H, S, W, U, info, batch, stream);
-    else launch_hybrid_(H, S, W, U, info, batch, stream);
+    else if (mode_ == Mode::Hybrid) launch_hybrid_(H, S, W, U, info, batch, stream);
+    else launch_mixed_(H, S, W, U, info, batch, stream);
     if (cur != device_) SG_HIP(hipSetDevice(cur));
 }
 
@@ -276,6 +294,53 @@ void Solver::launch_hybrid_(const void* H, const void* S, double* W, void* U, in
     void* a3[] = {&U, &Zw, &info, &batch};
     SG_HIP(hipModuleLaunchKernel(k_back_, grid2, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream, a3,
                                  nullptr));
+}
+
+void Solver::ensure_mixed_workspace_(size_t batch) {
+    if (batch <= mbuf_batch_) return;
+    if (mbuf_) {
+        SG_HIP(hipDeviceSynchronize());
+        SG_HIP(hipFree(mbuf_));
+        mbuf_ = nullptr;
+        mbuf_batch_ = 0;
+    }
+    const size_t n = (size_t)n_, npk = n * (n + 1) / 2;
+    // complex128 W = L^-1 (packed), complex64 stash / V, float eigenvalues,
+    // off-diagonals and Z.
+    SG_HIP(hipMalloc(&mbuf_, batch * (npk * 16 + n * n * 8 + n * 4 + n * 4 + n * n * 4)));
+    mbuf_batch_ = batch;
+}
+
+// Mixed precision: FP64 Cholesky and L^-1, FP32 reduction / tridiagonalisation
+// / QL / V = Q Z (the split kernels built in FP32), FP64 refinement of (M, V)
+// in the Cholesky frame and U = W^H V on the matrix cores.
+void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int* info, int batch,
+                           hipStream_t stream) {
+    ensure_mixed_workspace_((size_t)batch);
+    const size_t n = (size_t)n_, npk = n * (n + 1) / 2, B = (size_t)batch;
+    char* base = static_cast<char*>(mbuf_);
+    void* Wd = base;
+    void* V32 = base + B * npk * 16;
+    void* W32 = base + B * (npk * 16 + n * n * 8);
+    void* E32 = base + B * (npk * 16 + n * n * 8 + n * 4);
+    void* Z32 = base + B * (npk * 16 + n * n * 8 + n * 8);
+    const unsigned grid_b = (unsigned)batch;
+    const unsigned grid_w = (unsigned)((batch + bpb_ - 1) / bpb_);
+    const unsigned bd = (unsigned)block_dim_;
+    void* a1[] = {&S, &W, &U, &info, &Wd, &batch};
+    SG_HIP(hipModuleLaunchKernel(k_mx_chol_, grid_b, 1, 1, bd, 1, 1, 0, stream, a1, nullptr));
+    void* a2[] = {&H, &Wd, &W32, &V32, &info, &E32, &batch};
+    SG_HIP(hipModuleLaunchKernel(k_mx_front_, grid_b, 1, 1, bd, 1, 1, 0, stream, a2, nullptr));
+    void* a3[] = {&W32, &E32, &Z32, &info, &batch};
+    SG_HIP(hipModuleLaunchKernel(k_ql32_, grid_w, 1, 1, bd, 1, 1, 0, stream, a3, nullptr));
+    void* a4[] = {&V32, &Z32, &info, &batch};
+    SG_HIP(hipModuleLaunchKernel(k_back32_, grid_w, 1, 1, bd, 1, 1, 0, stream, a4, nullptr));
+    int iters = refine_iters_;
+    double delta = refine_delta_;
+    void* a5[] = {&H, &Wd, &W, &U, &info, &V32, &W32, &batch, &iters, &delta};
+    // The refinement runs one wave per 16-column block of the matrix.
+    const unsigned bd_ref = (unsigned)(64 * ((n_ + 15) / 16));
+    SG_HIP(hipModuleLaunchKernel(k_refine_, grid_b, 1, 1, bd_ref, 1, 1, 0, stream, a5, nullptr));
 }
 
 }  // namespace sgeig

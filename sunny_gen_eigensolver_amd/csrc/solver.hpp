@@ -15,7 +15,7 @@
 namespace sgeig {
 
 enum class Tuning { Suggested, Autotune };
-enum class Mode { Unified, Hybrid };
+enum class Mode { Unified, Hybrid, Mixed };
 
 class Solver {
 public:
@@ -47,7 +47,10 @@ private:
                          hipStream_t stream);
     void launch_hybrid_(const void* H, const void* S, double* W, void* U, int* info, int batch,
                         hipStream_t stream);
+    void launch_mixed_(const void* H, const void* S, double* W, void* U, int* info, int batch,
+                       hipStream_t stream);
     void ensure_workspace_(size_t batch);
+    void ensure_mixed_workspace_(size_t batch);
 
     int n_ = 0;
     int device_ = 0;
@@ -67,6 +70,23 @@ private:
     // Split-mode workspace: tridiagonal off-diagonals and Z, one per matrix.
     void* lbuf_ = nullptr;
     size_t lbuf_batch_ = 0;
+    // Mixed mode: the FP32 module and the FP32 hand-offs between its stages
+    // (L, stash then eigenvectors, eigenvalues, off-diagonals, Z), one set per
+    // matrix; refinement steps and the relative closeness threshold.
+    std::shared_ptr<LoadedModule> module32_;
+    hipFunction_t k_mx_chol_ = nullptr;
+    hipFunction_t k_mx_front_ = nullptr;
+    hipFunction_t k_ql32_ = nullptr;
+    hipFunction_t k_back32_ = nullptr;
+    hipFunction_t k_refine_ = nullptr;
+    void* mbuf_ = nullptr;
+    size_t mbuf_batch_ = 0;
+    int refine_iters_ = 2;
+    double refine_delta_ = 1e-4;
+
+public:
+    int refine_iters() const { return refine_iters_; }
+    double refine_delta() const { return refine_delta_; }
 };
 
 }  // namespace sgeig

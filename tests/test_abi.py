@@ -46,7 +46,9 @@ def test_enum_values_match_reference():
         "SUNNY_GENEIG_COMPILE_ERROR": -3, "SUNNY_GENEIG_LAUNCH_ERROR": -4,
         "SUNNY_GENEIG_CACHE_AUTO": 0, "SUNNY_GENEIG_CACHE_NODISK": 1, "SUNNY_GENEIG_CACHE_NOCACHE": 2,
         "SUNNY_GENEIG_TUNING_SUGGESTED": 0, "SUNNY_GENEIG_TUNING_AUTOTUNE": 1,
-        "SUNNY_GENEIG_KERNEL_UNIFIED": 0, "SUNNY_GENEIG_KERNEL_HYBRID": 1}
+        "SUNNY_GENEIG_KERNEL_UNIFIED": 0, "SUNNY_GENEIG_KERNEL_HYBRID": 1,
+        # additive: no counterpart in the reference
+        "SUNNY_GENEIG_KERNEL_MIXED": 2}
 
 
 def test_null_handle_accessors():
@@ -70,7 +72,7 @@ def test_null_launch_is_invalid_arg_with_message():
 
 def test_create_rejects_bad_arguments():
     L = sg.lib()
-    for n, kernel in ((0, 0), (-3, 0), (54, 7)):
+    for n, kernel in ((0, 0), (-3, 0), (54, 7), (54, 3)):
         h = L.sunny_geneig_create_ex(n, 0, 0, 0, 0, kernel)
         assert not h
         assert len(sg.last_error()) > 0
@@ -84,8 +86,9 @@ def test_precompile_writes_code_object(tmp_path):
     out = subprocess.run([sys.executable, "-c", code], env=env, cwd=REPO, check=True,
                          capture_output=True, text=True)
     assert out.stdout.strip() == str(tmp_path)
-    files = [f for f in os.listdir(tmp_path) if f.endswith(".hsaco")]
-    assert len(files) == 1 and files[0].startswith("geneig_n8_wg64_gfx950_")
+    files = sorted(f for f in os.listdir(tmp_path) if f.endswith(".hsaco"))
+    assert len(files) == 2, files
+    assert files[0].startswith("geneig_n8_wg64_gfx950_") and files[1].startswith("geneig_n8_wg64_gfx950_fp32_")
     with open(os.path.join(tmp_path, files[0]), "rb") as f:
         assert f.read(4) == b"\x7fELF"
 

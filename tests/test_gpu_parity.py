@@ -1,5 +1,6 @@
-"""GPU parity of the fused (unified) and split (hybrid) modes against FP64
-LAPACK zhegv, through the C ABI.
+"""GPU parity of the fused (unified), split (hybrid) and mixed-precision
+(FP32 stages + FP64 refinement) modes against FP64 LAPACK zhegv, through the
+C ABI.
 
 Mirrors the reference's tests: tests/unified/test_n54.cpp, test_multiN.cpp,
 tests/static/test_full_batched_n54.cpp (B in {1, 8, 32} on n54_b32_s123 and a
@@ -19,7 +20,10 @@ pytestmark = pytest.mark.gpu
 LDS_FIXTURES = [f"n{n}_b8_s42_{m}" for n in (16, 32, 48, 54, 64) for m in ("random", "nearid")]
 
 
-@pytest.mark.parametrize("kernel", ["unified", "hybrid"])
+MODES = ["unified", "hybrid", "mixed"]
+
+
+@pytest.mark.parametrize("kernel", MODES)
 @pytest.mark.parametrize("name", LDS_FIXTURES)
 def test_fixture_vs_lapack(name, kernel):
     meta, H, S = T.fixture_inputs(name)
@@ -33,7 +37,7 @@ def test_fixture_vs_lapack(name, kernel):
     assert rel.max() < T.TOL_EIG
 
 
-@pytest.mark.parametrize("kernel", ["unified", "hybrid"])
+@pytest.mark.parametrize("kernel", MODES)
 def test_fixture_vectors_vs_golden(kernel):
     """Phase check against the reference's stored eigenvectors."""
     g = T.golden_arrays()
@@ -53,7 +57,7 @@ def test_n54_b32_slices(batch):
     T.assert_parity(H[:batch], S[:batch], W, U, info, Wl, Ul, tag=f"B={batch}")
 
 
-@pytest.mark.parametrize("kernel", ["unified", "hybrid"])
+@pytest.mark.parametrize("kernel", MODES)
 def test_synthetic_1024(kernel):
     """tests/static/test_full_batched_n54.cpp:222-334: B=1024, seed 0xC0FFEE0042."""
     H, S = O.gen_reference(54, 1024, 0xC0FFEE0042, "random")
@@ -63,10 +67,34 @@ def test_synthetic_1024(kernel):
     T.assert_parity(H, S, W, U, info, Wl, Ul, tag=f"{kernel} B=1024")
 
 
-def test_baseline_generator_n54():
+@pytest.mark.parametrize("kernel", MODES)
+def test_baseline_generator_n54(kernel):
     """The benchmark's input family (Q diag(lambda) Q^H with near-degenerate
     pairs, S = B B^H + 0.05 N I) against zhegv."""
     H, S = O.gen_spectrum(54, 64, 0x5EED0002)
     Wl, Ul, il = O.zhegv_batch(H, S, threads=8)
-    W, U, info = T.run_gpu(H, S)
-    T.assert_parity(H, S, W, U, info, Wl, Ul, tag="baseline-gen N=54")
+    W, U, info = T.run_gpu(H, S, kernel=kernel)
+    T.assert_parity(H, S, W, U, info, Wl, Ul, tag=f"{kernel} baseline-gen N=54")
+
+
+@pytest.mark.parametrize("kernel", MODES)
+@pytest.mark.parametrize("n", [12, 54])
+def test_non_hpd_info_codes(kernel, n):
+    """zhegv's INFO for S that is not positive definite: n + k for the first
+    non-positive leading minor k (1-based), W = NaN and U = 0 for that matrix;
+    the other matrices of the batch are unaffected."""
+    H, S = O.gen_reference(n, 6, 900 + n, "random")
+    S = S.copy()
+    bad = {1: 0, 3: n // 2, 4: n - 1}               # matrix -> failing pivot (0-based)
+    for b, k in bad.items():
+        Sm = np.eye(n, dtype=np.complex128)
+        Sm[k, k] = -1.0
+        S[b] = Sm.T
+    Wl, Ul, il = O.zhegv_batch(H, S, threads=8)
+    W, U, info = T.run_gpu(H, S, kernel=kernel)
+    for b, k in bad.items():
+        assert int(info[b]) == n + k + 1 == int(il[b]), (b, info[b], il[b])
+        assert np.isnan(W[b]).all() and (U[b] == 0).all()
+    good = [b for b in range(6) if b not in bad]
+    T.assert_parity(H[good], S[good], W[good], U[good], info[good], Wl[good], Ul[good],
+                    tag=f"{kernel} beside non-HPD n={n}")

@@ -165,11 +165,13 @@ def test_every_default_bucket_every_slot(vs64, bucket):
 
 @pytest.mark.parametrize("n", list(range(4, 65, 4)))
 def test_fused_at_bucket_orders(n):
-    """The fixed-order fused and split kernels at every bucket order (their
-    code objects ship warm with the buckets), several matrices per workgroup."""
+    """The fixed-order fused, split and mixed modes at every bucket order (their
+    code objects ship warm with the buckets), several matrices per workgroup;
+    for the mixed mode's refinement these orders cover one to four 16-blocks
+    and partial blocks."""
     H, S = O.gen_reference(n, 9, 50000 + n, "random")
     Wl, Ul, il = O.zhegv_batch(H, S, threads=8)
     assert (il == 0).all()
-    for kernel in ("unified", "hybrid"):
+    for kernel in ("unified", "hybrid", "mixed"):
         W, U, info = T.run_gpu(H, S, kernel=kernel)
         T.assert_parity(H, S, W, U, info, Wl, Ul, tag=f"{kernel} n={n}")
