@@ -1886,27 +1886,26 @@ geneig_mx_front(const gk::cd* __restrict__ H, const gk::cd* __restrict__ Wd, rea
 
 // V = Q Z per wave (the FP32 back half without the W^H factor, which the FP64
 // refinement applies): reflectors staged from the stash in U32, V over it.
+namespace gk {
+constexpr size_t REFL_SLICE = ((size_t)(NREFL + N) * sizeof(cx) + 15) / 16 * 16;   // reflectors + tau
+}  // namespace gk
+
 extern "C" __global__ void __launch_bounds__(GENEIG_NT)
 geneig_mx_back(cx* __restrict__ U32, const real* __restrict__ Zw, const int* __restrict__ info, int batch) {
     using namespace gk;
-    __shared__ __attribute__((aligned(16))) char slices[BACK_SLICES][SLICE_BYTES];
+    __shared__ __attribute__((aligned(16))) char slices[NW][REFL_SLICE];
     const int wave = wave_id(), lane = lane_id();
     const int b = blockIdx.x * NW + wave;
-    const bool active = b < batch && info[b] <= N;
-    for (int r = 0; r < (NW + BACK_SLICES - 1) / BACK_SLICES; ++r) {
-        if (active && wave / BACK_SLICES == r) {
-            const size_t off = (size_t)b * N * N;
-            cx* sl = reinterpret_cast<cx*>(slices[wave % BACK_SLICES]);
-            cx v[N];
+    if (!(b < batch && info[b] <= N)) return;       // no workgroup barrier below
+    const size_t off = (size_t)b * N * N;
+    cx* sl = reinterpret_cast<cx*>(slices[wave]);
+    cx v[N];
 #pragma unroll
-            for (int i = 0; i < N; ++i) v[i] = cm((lane < N) ? Zw[off + lane * N + i] : real(0.0), 0.0);
-            stage_reflectors(sl, U32 + off);
-            wave_lds_sync();
-            apply_q_wave(sl, v);
-            store_columns(U32 + off, v);
-        }
-        __syncthreads();
-    }
+    for (int i = 0; i < N; ++i) v[i] = cm((lane < N) ? Zw[off + lane * N + i] : real(0.0), 0.0);
+    stage_reflectors(sl, U32 + off);
+    wave_lds_sync();
+    apply_q_wave(sl, v);
+    store_columns(U32 + off, v);
 }
 #endif  // GENEIG_FP32
 
@@ -1922,6 +1921,7 @@ namespace gk {
 typedef double d4 __attribute__((ext_vector_type(4)));
 constexpr int NB = (N + 15) / 16;          // 16-blocks per dimension
 constexpr int REF_NT = 64 * NB;            // one wave per column block
+constexpr int KEND = (N + 3) / 4 * 4;      // k-steps that touch rows/columns < N
 constexpr int LDR = N | 1;
 
 struct ctile {
@@ -1963,8 +1963,8 @@ __device__ __forceinline__ void cgemm(ctile (&acc)[NB], FA fa, FB fb, int cb) {
         acc[rt].re = d4{0.0, 0.0, 0.0, 0.0};
         acc[rt].im = d4{0.0, 0.0, 0.0, 0.0};
     }
-#pragma unroll 4
-    for (int k0 = 0; k0 < 16 * NB; k0 += 4) {
+#pragma unroll 2
+    for (int k0 = 0; k0 < KEND; k0 += 4) {
         const cx b = fb(k0 + k4, cb * 16 + c16);
 #pragma unroll
         for (int rt = 0; rt < NB; ++rt) cmfma(acc[rt], fa(rt * 16 + c16, k0 + k4), b);
@@ -1988,6 +1988,7 @@ __device__ __forceinline__ void cgemm_reg(ctile (&acc)[NB], FA fa, const ctile (
     for (int q = 0; q < NB; ++q) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
+            if (16 * q + 4 * v >= KEND) continue;    // k-steps entirely beyond N
             const cx b = tile_get(Bt[q], v);
             const int k = 16 * q + 4 * v + k4;
 #pragma unroll
@@ -2005,9 +2006,18 @@ __device__ __forceinline__ real col_sum(real v) {
 
 struct RefWork {
     cx U[N * LDR];        // eigenvector estimate V
-    cx B[N * LDR];        // H, then M = W H W^H
+    cx M[NPK];            // M = W H W^H, lower triangle packed by columns
     real lam[N];
 };
+
+// M(i, k) from its packed lower triangle (Hermitian), zero outside.
+__device__ __forceinline__ cx melem(const cx* P, int i, int k) {
+    const bool in = i < N && k < N;
+    const bool lo = i >= k;
+    const int ii = in ? (lo ? i : k) : 0, kk = in ? (lo ? k : i) : 0;
+    const cx v = P[pk(kk) + ii - kk];
+    return in ? (lo ? v : cm(v.x, -v.y)) : cm(0.0, 0.0);
+}
 
 // 2 x 2 Hermitian-definite pencil (T, G) (G close to I): eigenvalues
 // mu1 <= mu2 and G-orthonormal eigenvectors X = [x1 x2], by a Cholesky of G
@@ -2052,7 +2062,7 @@ __device__ __forceinline__ void eig2_pencil(real t11, real t22, cx t12, real g11
     X[0][1] = (y2a - cmul(cl21, X[1][1])) * (real(1.0) / l11);
 }
 
-// Rayleigh-Ritz on columns (a, b) of V for the Hermitian M (held in r.B), by
+// Rayleigh-Ritz on columns (a, b) of V for the Hermitian M (held in r.M), by
 // one wave: the 2 x 2 projected pencil (V^H M V, V^H V) of the pair, then
 // V(:, [a b]) <- V(:, [a b]) X and lam[a], lam[b] <- the Ritz values.
 __device__ __forceinline__ void ritz_pair(RefWork& r, int a, int b) {
@@ -2061,7 +2071,7 @@ __device__ __forceinline__ void ritz_pair(RefWork& r, int a, int b) {
     const bool in = lane < N;
     cx ha = cm(0.0, 0.0), hb = cm(0.0, 0.0);
     for (int k = 0; k < N; ++k) {
-        const cx mk = r.B[k * LDR + lr];
+        const cx mk = melem(r.M, lr, k);
         ha = ha + cmul(mk, r.U[a * LDR + k]);
         hb = hb + cmul(mk, r.U[b * LDR + k]);
     }
@@ -2103,7 +2113,7 @@ geneig_mx_refine(const cx* __restrict__ H, const cx* __restrict__ Wd, real* __re
                  cx* __restrict__ U, const int* __restrict__ info, const gk::cf* __restrict__ V32,
                  const float* __restrict__ W32, int batch, int iters, real delta_rel) {
     using namespace gk;
-    __shared__ RefWork r;                      // r.U: V; r.B: H, then M
+    __shared__ RefWork r;
     const int b = blockIdx.x;
     if (b >= batch || info[b] > N) return;     // Cholesky failures keep their outputs
     const int l = lane_id(), cb = wave_id();
@@ -2113,27 +2123,24 @@ geneig_mx_refine(const cx* __restrict__ H, const cx* __restrict__ Wd, real* __re
     const size_t off = (size_t)b * N * N;
     const cx* Hb = H + off;
     const cx* Wb = Wd + (size_t)b * NPK;
-    for (int e = threadIdx.x; e < N * N; e += REF_NT) {
-        r.U[(e / N) * LDR + e % N] = to_cx(V32[off + e]);
-        r.B[(e / N) * LDR + e % N] = Hb[e];
-    }
+    for (int e = threadIdx.x; e < N * N; e += REF_NT) r.U[(e / N) * LDR + e % N] = to_cx(V32[off + e]);
     if (threadIdx.x < N) r.lam[threadIdx.x] = (real)W32[(size_t)b * N + threadIdx.x];
-    __syncthreads();
-    auto fB = [&](int i, int k) { return elem(r.B, LDR, i, k); };
+    auto fM = [&](int i, int k) { return melem(r.M, i, k); };
     auto fV = [&](int i, int k) { return elem(r.U, LDR, i, k); };
     auto fVh = [&](int i, int k) { return cconj(elem(r.U, LDR, k, i)); };
     {
-        // M = W (H W^H), into B.
+        // M = W (H W^H), operands from global memory; its lower triangle to LDS
+        // (which makes it exactly Hermitian).
         ctile Z[NB], M[NB];
-        cgemm(Z, fB, [&](int k, int jj) { return cconj(pelem(Wb, jj, k)); }, cb);
+        cgemm(Z, [&](int i, int k) { return elem(Hb, N, i, k); },
+              [&](int k, int jj) { return cconj(pelem(Wb, jj, k)); }, cb);
         cgemm_reg(M, [&](int i, int k) { return pelem(Wb, i, k); }, Z);
-        __syncthreads();     // every wave is done reading H
 #pragma unroll
         for (int rt = 0; rt < NB; ++rt) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = rt * 16 + g4 + 4 * v;
-                if (i < N && jin) r.B[j * LDR + i] = tile_get(M[rt], v);
+                if (i < N && jin && i >= j) r.M[pk(j) + i - j] = tile_get(M[rt], v);
             }
         }
         __syncthreads();
@@ -2143,7 +2150,7 @@ geneig_mx_refine(const cx* __restrict__ H, const cx* __restrict__ Wd, real* __re
         ctile T[NB], G[NB];
         {
             ctile X[NB];
-            cgemm(X, fB, fV, cb);                // X = M V
+            cgemm(X, fM, fV, cb);                // X = M V
             cgemm_reg(T, fVh, X);                // T = V^H X
         }
         cgemm(G, fVh, fV, cb);                   // G = V^H V
