@@ -1,15 +1,21 @@
-"""Benchmark: batched generalized Hermitian eigensolve H u = lambda S u on one
-MI355X through libsunny_geneig.so's C ABI.
+"""Benchmark: batched generalized eigensolve H u = lambda S u on one MI355X
+through libsunny_geneig.so's C ABI.
 
-Workload (one "step" = one solve of the whole batch): N = 54 complex128,
-batch 65536, inputs H = Q diag(lambda) Q^H (lambda ~ U[-10, 10], 6 pairs split
-by 1e-8) and S = B B^H + 0.05 N I, generated on the device (synthetic), fused
-FP64 kernel. Inputs are resident in HBM before timing starts. The split mode
-is timed on the same batch and reported beside it. The CPU baseline is LAPACK
-zhegv over a bounded sample of the same matrices on the host cores.
+Workload (one "step" = one solve of the whole batch): N = 54 complex, batch
+65536, inputs H = Q diag(lambda) Q^H (lambda ~ U[-10, 10], 6 pairs split by
+1e-8) and S = B B^H + 0.05 N I, generated on the device (synthetic) and
+resident in HBM before timing starts.
 
-Prints one JSON line (rank 0). Usage: python bench.py [--gpus 1] [--steps K]
-[--warmup W] [--batch B] [--n N] [--no-cpu] [--no-split]
+The headline is the mixed-precision mode (FP64 Cholesky, FP32 reduction /
+tridiagonalisation / QL / eigenvectors, FP64 refinement on the matrix cores).
+The plain FP64 fused and split modes are timed on the same batch beside it,
+and the accuracy of every mode is checked against the 50 N eps bar (residual
+and S-orthogonality, 1-norms). The CPU baseline is LAPACK zhegv over a
+bounded sample of the same matrices on the host cores, which also gives the
+mixed mode's eigenvalue error against zhegv.
+
+Prints one JSON line. Usage: python bench.py [--gpus 1] [--steps K]
+[--warmup W] [--batch B] [--n N] [--no-cpu] [--no-fp64]
 """
 from __future__ import annotations
 
@@ -25,12 +31,20 @@ sys.path.insert(0, REPO)
 
 FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 (vector and matrix)
 HBM_PEAK_GBS = 8000.0
+EPS = 2.0 ** -52
 
 
 def flops_per_solve(n: int) -> float:
     """LAWN 41 count of the zhegv path: potrf 4/3, hegst 4, hetrd 16/3, Q 16/3,
     QL with vectors 12, back-trsm 4 (x N^3) = 32 N^3 real flops."""
     return 32.0 * n ** 3
+
+
+def refine_flops_per_solve(n: int, iters: int) -> float:
+    """Complex N^3 GEMMs of the refinement kernel, 8 real flops per complex
+    multiply-add: M = W (H W^H) (2), per step X = M V, T = V^H X, G = V^H V,
+    V E (4), and U = W^H V (1)."""
+    return 8.0 * (3 + 4 * iters) * n ** 3
 
 
 def bytes_per_solve(n: int) -> float:
@@ -53,7 +67,7 @@ def cpu_baseline(H, S, n, sample: int, threads: int):
     times = []
     for _ in range(5):
         t0 = time.perf_counter()
-        _, _, info = O.zhegv_batch(Hs, Ss, threads=threads)
+        W, _, info = O.zhegv_batch(Hs, Ss, threads=threads)
         times.append(time.perf_counter() - t0)
     med = statistics.median(times)
     one = min(256, sample)
@@ -67,7 +81,7 @@ def cpu_baseline(H, S, n, sample: int, threads: int):
                   f"median of 5 runs after 1 warm-up",
         "single_core_solves_per_s": one / t1,
         "info_nonzero": int((info != 0).sum()),
-    }
+    }, W
 
 
 def main():
@@ -79,8 +93,7 @@ def main():
     ap.add_argument("--n", type=int, default=54)
     ap.add_argument("--seed", type=lambda s: int(s, 0), default=0x5EED0002)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-split", action="store_true")
-    ap.add_argument("--bpb", type=int, default=0, help="matrices per workgroup (0 = solver default)")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 fused / split modes")
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="launch one device copy of this many MiB first (HBM counter calibration)")
@@ -130,21 +143,59 @@ def main():
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / steps   # ms per step
 
-    solver = sg.Solver(n, kernel="unified", bpb_request=a.bpb)
-    log(f"unified: wg={solver.block_dim_x} lds={solver.shared_mem_bytes} B cache={solver.cache_layer}")
-    t_wall = time.perf_counter()
-    ms = timed(solver, a.steps, a.warmup)
-    wall = time.perf_counter() - t_wall
-    res, orth = inputs.quality(H, S, W, U)
-    eps = 2.0 ** -52
-    bad_info = int((info != 0).sum().item())
-    value = B / (ms * 1e-3)
-    log(f"unified: {ms:.3f} ms/step, {value:.0f} solves/s; max resid {res.max().item()/(n*eps):.3f} N eps, "
-        f"max S-orth {orth.max().item()/(n*eps):.3f} N eps, info!=0: {bad_info}")
+    def stages(solver):
+        """Per-kernel durations of one more launch, from HIP events the library
+        records on the launch stream around each kernel."""
+        solver.set_stage_timing(True)
+        sg.run_kernel(solver, H, S, W, U, info, stream=stream)
+        st = solver.stage_times()
+        solver.set_stage_timing(False)
+        return st
 
-    achieved_tf = B * flops_per_solve(n) / (ms * 1e-3) / 1e12
+    def accuracy():
+        res, orth = inputs.quality(H, S, W, U)
+        return {"max_residual_over_N_eps": res.max().item() / (n * EPS),
+                "max_s_orth_over_N_eps": orth.max().item() / (n * EPS),
+                "bar_over_N_eps": 50.0, "info_nonzero": int((info != 0).sum().item())}
+
+    modes = {}
+    W_fp64 = None
+    if not a.no_fp64:
+        for kernel in ("unified", "hybrid"):
+            s = sg.Solver(n, kernel=kernel)
+            ms = timed(s, a.steps, a.warmup)
+            acc = accuracy()
+            st = stages(s)
+            if kernel == "unified":
+                W_fp64 = W.clone()
+            modes[kernel] = {"ms_per_step": ms, "solves_per_s": B / (ms * 1e-3), "accuracy": acc,
+                             "stages_ms": {k: v for k, v in st}, "block_dim_x": s.block_dim_x}
+            log(f"{kernel}: {ms:.3f} ms/step, {B / (ms * 1e-3):.0f} solves/s, "
+                f"resid {acc['max_residual_over_N_eps']:.2f} Ne, s_orth {acc['max_s_orth_over_N_eps']:.2f} Ne")
+            s.close()
+
+    mixed = sg.Solver(n, kernel="mixed")
+    t_wall = time.perf_counter()
+    ms = timed(mixed, a.steps, a.warmup)
+    wall = time.perf_counter() - t_wall
+    acc = accuracy()
+    st = stages(mixed)
+    iters = int(os.environ.get("SUNNY_GENEIG_REFINE_ITERS", "2"))
+    value = B / (ms * 1e-3)
+    log(f"mixed: {ms:.3f} ms/step, {value:.0f} solves/s, resid {acc['max_residual_over_N_eps']:.2f} Ne, "
+        f"s_orth {acc['max_s_orth_over_N_eps']:.2f} Ne; stages " +
+        ", ".join(f"{k} {v:.2f}" for k, v in st))
+    if W_fp64 is not None:
+        acc["max_eig_rel_diff_vs_fp64_fused"] = (
+            ((W - W_fp64).abs() / W_fp64.abs().clamp(min=1.0)).max().item())
+
+    # Roofline of the dominant kernel (the refinement, on the matrix cores):
+    # its own algorithmic GEMM flops per launch / its launch time.
+    dom_name, dom_ms = max(st, key=lambda kv: kv[1])
+    rf = refine_flops_per_solve(n, iters) if dom_name == "geneig_mx_refine" else flops_per_solve(n)
+    achieved = B * rf / (dom_ms * 1e-3) / 1e12
     traffic = None
-    tpath = os.path.join(REPO, "profiles", f"traffic_fused_n{n}.json")
+    tpath = os.path.join(REPO, "profiles", f"traffic_mx_refine_n{n}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
@@ -162,43 +213,49 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128",
+        "dtype": "c64+c128",
         "data": "synthetic",
-        "config": {"workload": f"fused FP64 zhegv, N={n} complex128, batch {B}, "
+        "config": {"workload": f"mixed-precision zhegv (FP64 Cholesky, FP32 tridiagonalisation and "
+                               f"eigenvectors, {iters} FP64 refinement steps), N={n} complex, batch {B}, "
                                f"H=Q diag(lambda) Q^H (6 pairs split 1e-8), S=BB^H+0.05N I",
-                   "n": n, "batch": B, "seed": hex(a.seed), "mode": "unified",
-                   "block_dim_x": solver.block_dim_x, "lds_bytes": solver.shared_mem_bytes},
-        "accuracy": {"max_residual_over_N_eps": res.max().item() / (n * eps),
-                     "max_s_orth_over_N_eps": orth.max().item() / (n * eps),
-                     "bar_over_N_eps": 50.0, "info_nonzero": bad_info},
-        "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved_tf / FP64_PEAK_TFLOPS,
-                     "traffic": traffic,
-                     "kernel": "geneig_fused",
-                     "note": "FP64 ceiling (vector and MFMA FP64 are both 78.6 TFLOP/s); "
-                             "algorithmic flops 32 N^3 per solve x batch / mean launch time"},
+                   "n": n, "batch": B, "seed": hex(a.seed), "mode": "mixed", "refine_iters": iters},
+        "accuracy": acc,
+        "stages_ms": {k: v for k, v in st},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": dom_name,
+                     "note": f"{dom_name}: {rf:.0f} flops per solve (complex N^3 GEMMs of the refinement, "
+                             f"8 real flops per complex multiply-add) x batch / its launch time, timed with "
+                             f"HIP events on the launch stream; FP64 MFMA peak"},
+        "path_fp64_equivalent": {
+            "achieved_TFLOPs": B * flops_per_solve(n) / (ms * 1e-3) / 1e12,
+            "frac_of_fp64_peak": B * flops_per_solve(n) / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+            "note": "32 N^3 algorithmic flops per solve (refinement not credited) / whole step"},
         "hbm": {"compulsory_GBs": B * bytes_per_solve(n) / (ms * 1e-3) / 1e9,
                 "frac_of_peak": B * bytes_per_solve(n) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "fp64_modes": modes,
         "wall_s_timed_region": wall,
     }
-    solver.close()
-
-    if not a.no_split:
-        split = sg.Solver(n, kernel="hybrid")
-        ms_s = timed(split, a.steps, a.warmup)
-        res_s, orth_s = inputs.quality(H, S, W, U)
-        out["split_mode"] = {"ms_per_step": ms_s, "solves_per_s": B / (ms_s * 1e-3),
-                             "max_residual_over_N_eps": res_s.max().item() / (n * eps),
-                             "max_s_orth_over_N_eps": orth_s.max().item() / (n * eps)}
-        log(f"hybrid: {ms_s:.3f} ms/step, {B / (ms_s * 1e-3):.0f} solves/s")
-        split.close()
+    if "unified" in modes:
+        out["mixed_vs_fp64_fused"] = value / modes["unified"]["solves_per_s"]
+        out["mixed_vs_fp64_split"] = value / modes["hybrid"]["solves_per_s"]
+    mixed.close()
 
     if not a.no_cpu:
         try:
             threads = min(16, len(os.sched_getaffinity(0)))
         except Exception:
             threads = min(16, os.cpu_count() or 1)
-        cb = cpu_baseline(H, S, n, min(a.cpu_sample, B), threads)
+        # The mixed result for the sample, recomputed (W holds the last mode's).
+        m2 = sg.Solver(n, kernel="mixed")
+        sg.run_kernel(m2, H, S, W, U, info, stream=stream)
+        torch.cuda.synchronize()
+        m2.close()
+        cb, Wl = cpu_baseline(H, S, n, min(a.cpu_sample, B), threads)
+        import numpy as np
+        Wg = W[:Wl.shape[0]].cpu().numpy()
+        out["accuracy"]["max_eig_rel_err_vs_zhegv_sample"] = float(
+            (np.abs(Wg - Wl) / np.maximum(np.abs(Wl), 1.0)).max())
         out["cpu_baseline"] = cb
         log(f"cpu: {cb['value']:.0f} solves/s on {threads} threads, "
             f"{cb['single_core_solves_per_s']:.0f} on 1")

@@ -130,6 +130,17 @@ const char* sunny_geneig_cache_layer(sunny_geneig_solver_t solver);
 /* Resolved on-disk cache directory (reference: unified_solver.cpp:403-420). */
 const char* sunny_geneig_cache_dir(void);
 
+/* Stage timing (no counterpart in the reference ABI; its timing is done by
+ * callers around whole launches, unified_solver.cpp:838-856). When enabled,
+ * HIP events bracket every kernel a launch issues, on the launch's stream.
+ * sunny_geneig_stage_times waits for the last launch, writes up to
+ * max_stages durations (ms) and, if names is not NULL, the kernel names, and
+ * returns the stage count (UNIFIED 1, HYBRID 3, MIXED 5) or 0 when there is
+ * no timed launch; -1 on NULL. Not for concurrent launches of one handle. */
+int sunny_geneig_set_stage_timing(sunny_geneig_solver_t solver, int enable);
+int sunny_geneig_stage_times(sunny_geneig_solver_t solver, double* ms, const char** names,
+                             int max_stages);
+
 /* ---- uneven batches (no counterpart in the reference ABI) ----------------
  * One call solves `count` pencils of different orders n[k] in [1, max_n].
  * Every matrix goes to the smallest bucket order >= n[k]; each bucket is one

@@ -46,6 +46,7 @@ EXPORTED_SYMBOLS = (
     "sunny_geneig_cache_layer", "sunny_geneig_cache_dir",
     "sunny_geneig_varlen_create", "sunny_geneig_varlen_destroy", "sunny_geneig_varlen_launch",
     "sunny_geneig_varlen_num_buckets", "sunny_geneig_varlen_bucket_order",
+    "sunny_geneig_set_stage_timing", "sunny_geneig_stage_times",
 )
 
 _lib = None
@@ -98,6 +99,11 @@ def lib() -> ctypes.CDLL:
     L.sunny_geneig_varlen_num_buckets.restype = ci
     L.sunny_geneig_varlen_bucket_order.argtypes = [vp, ci]
     L.sunny_geneig_varlen_bucket_order.restype = ci
+    L.sunny_geneig_set_stage_timing.argtypes = [vp, ci]
+    L.sunny_geneig_set_stage_timing.restype = ci
+    L.sunny_geneig_stage_times.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_char_p), ci]
+    L.sunny_geneig_stage_times.restype = ci
     _lib = L
     return L
 
@@ -179,6 +185,20 @@ class Solver:
         st = lib().sunny_geneig_launch(self.handle, d_H, d_S, d_W, d_U, d_info, batch, stream)
         if st != STATUS_OK:
             raise GeneigError(st, last_error())
+
+    def set_stage_timing(self, on: bool = True) -> None:
+        st = lib().sunny_geneig_set_stage_timing(self.handle, 1 if on else 0)
+        if st != STATUS_OK:
+            raise GeneigError(st, last_error())
+
+    def stage_times(self):
+        """[(kernel name, ms)] of the last launch (stage timing must be on)."""
+        ms = (ctypes.c_double * 8)()
+        names = (ctypes.c_char_p * 8)()
+        n = lib().sunny_geneig_stage_times(self.handle, ms, names, 8)
+        if n < 0:
+            raise GeneigError(STATUS_INVALID_ARG, last_error())
+        return [(names[i].decode(), ms[i]) for i in range(n)]
 
 
 def _stream_handle(stream) -> int:

@@ -181,6 +181,41 @@ const char* sunny_geneig_cache_dir(void) {
     return g_cache_dir.c_str();
 }
 
+int sunny_geneig_set_stage_timing(sunny_geneig_solver_t s, int enable) {
+    clear_error();
+    if (!s) {
+        set_error("sunny_geneig_set_stage_timing: solver is NULL");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    try {
+        s->impl->set_stage_timing(enable != 0);
+        return SUNNY_GENEIG_OK;
+    } catch (const std::exception& e) {
+        set_error(std::string("sunny_geneig_set_stage_timing: ") + e.what());
+        return SUNNY_GENEIG_DEVICE_ERROR;
+    }
+}
+
+int sunny_geneig_stage_times(sunny_geneig_solver_t s, double* ms, const char** names, int max_stages) {
+    clear_error();
+    if (!s) {
+        set_error("sunny_geneig_stage_times: solver is NULL");
+        return -1;
+    }
+    try {
+        double tmp[8];
+        const int n = s->impl->stage_times(tmp, 8);
+        for (int i = 0; i < n && i < max_stages; ++i) {
+            if (ms) ms[i] = tmp[i];
+            if (names) names[i] = sgeig::Solver::stage_name(s->impl->mode(), i);
+        }
+        return n;
+    } catch (const std::exception& e) {
+        set_error(std::string("sunny_geneig_stage_times: ") + e.what());
+        return -1;
+    }
+}
+
 sunny_geneig_varlen_t sunny_geneig_varlen_create(int max_n, int device_id, const int* bucket_orders,
                                                  int num_buckets, sunny_geneig_cache_mode_t cache_mode) {
     clear_error();

@@ -119,6 +119,7 @@ Solver::Solver(int n, int device_id, int bpb_request, Tuning tuning, CacheMode c
 }
 
 Solver::~Solver() {
+    for (auto& e : ev_) (void)hipEventDestroy(e);
     if (lbuf_ || mbuf_) {
         int cur = 0;
         (void)hipGetDevice(&cur);
@@ -250,6 +251,8 @@ void Solver::launch(const void* H, const void* S, double* W, void* U, int* info,
     int cur = 0;
     SG_HIP(hipGetDevice(&cur));
     if (cur != device_) SG_HIP(hipSetDevice(device_));
+    ev_used_ = 0;
+    mark_(stream);
     if (mode_ == Mode::Unified) launch_unified_(H, S, W, U, info, batch, stream);
     else if (mode_ == Mode::Hybrid) launch_hybrid_(H, S, W, U, info, batch, stream);
     else launch_mixed_(H, S, W, U, info, batch, stream);
@@ -262,6 +265,7 @@ void Solver::launch_unified_(const void* H, const void* S, double* W, void* U, i
     const unsigned grid = (unsigned)((batch + bpb_ - 1) / bpb_);
     SG_HIP(hipModuleLaunchKernel(fused_, grid, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream, args,
                                  nullptr));
+    mark_(stream);
 }
 
 void Solver::ensure_workspace_(size_t batch) {
@@ -287,13 +291,16 @@ void Solver::launch_hybrid_(const void* H, const void* S, double* W, void* U, in
     void* a1[] = {&H, &S, &W, &U, &info, &E, &batch};
     SG_HIP(hipModuleLaunchKernel(k_reduce_, (unsigned)batch, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream,
                                  a1, nullptr));
+    mark_(stream);
     const unsigned grid2 = (unsigned)((batch + bpb_ - 1) / bpb_);
     void* a2[] = {&W, &E, &Zw, &info, &batch};
     SG_HIP(hipModuleLaunchKernel(k_heev_, grid2, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream, a2,
                                  nullptr));
+    mark_(stream);
     void* a3[] = {&U, &Zw, &info, &batch};
     SG_HIP(hipModuleLaunchKernel(k_back_, grid2, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream, a3,
                                  nullptr));
+    mark_(stream);
 }
 
 void Solver::ensure_mixed_workspace_(size_t batch) {
@@ -329,18 +336,59 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
     const unsigned bd = (unsigned)block_dim_;
     void* a1[] = {&S, &W, &U, &info, &Wd, &batch};
     SG_HIP(hipModuleLaunchKernel(k_mx_chol_, grid_b, 1, 1, bd, 1, 1, 0, stream, a1, nullptr));
+    mark_(stream);
     void* a2[] = {&H, &Wd, &W32, &V32, &info, &E32, &batch};
     SG_HIP(hipModuleLaunchKernel(k_mx_front_, grid_b, 1, 1, bd, 1, 1, 0, stream, a2, nullptr));
+    mark_(stream);
     void* a3[] = {&W32, &E32, &Z32, &info, &batch};
     SG_HIP(hipModuleLaunchKernel(k_ql32_, grid_w, 1, 1, bd, 1, 1, 0, stream, a3, nullptr));
+    mark_(stream);
     void* a4[] = {&V32, &Z32, &info, &batch};
     SG_HIP(hipModuleLaunchKernel(k_back32_, grid_w, 1, 1, bd, 1, 1, 0, stream, a4, nullptr));
+    mark_(stream);
     int iters = refine_iters_;
     double delta = refine_delta_;
     void* a5[] = {&H, &Wd, &W, &U, &info, &V32, &W32, &batch, &iters, &delta};
     // The refinement runs one wave per 16-column block of the matrix.
     const unsigned bd_ref = (unsigned)(64 * ((n_ + 15) / 16));
     SG_HIP(hipModuleLaunchKernel(k_refine_, grid_b, 1, 1, bd_ref, 1, 1, 0, stream, a5, nullptr));
+    mark_(stream);
+}
+
+void Solver::set_stage_timing(bool on) {
+    timing_ = on;
+    if (on && ev_.empty()) {
+        ev_.resize(8);
+        for (auto& e : ev_) SG_HIP(hipEventCreate(&e));
+    }
+    ev_used_ = 0;
+}
+
+void Solver::mark_(hipStream_t stream) {
+    if (!timing_ || ev_used_ >= (int)ev_.size()) return;
+    SG_HIP(hipEventRecord(ev_[ev_used_++], stream));
+}
+
+int Solver::stage_times(double* ms, int n) {
+    if (!timing_ || ev_used_ < 2) return 0;
+    SG_HIP(hipEventSynchronize(ev_[ev_used_ - 1]));
+    const int stages = ev_used_ - 1;
+    for (int i = 0; i < stages && i < n; ++i) {
+        float t = 0.f;
+        SG_HIP(hipEventElapsedTime(&t, ev_[i], ev_[i + 1]));
+        ms[i] = t;
+    }
+    return stages;
+}
+
+const char* Solver::stage_name(Mode mode, int i) {
+    static const char* unified[] = {"geneig_fused"};
+    static const char* hybrid[] = {"geneig_split_front", "geneig_split_ql", "geneig_split_back"};
+    static const char* mixed[] = {"geneig_mx_chol", "geneig_mx_front", "geneig_split_ql",
+                                  "geneig_mx_back", "geneig_mx_refine"};
+    if (mode == Mode::Unified) return (i == 0) ? unified[0] : "";
+    if (mode == Mode::Hybrid) return (i >= 0 && i < 3) ? hybrid[i] : "";
+    return (i >= 0 && i < 5) ? mixed[i] : "";
 }
 
 }  // namespace sgeig

@@ -11,6 +11,7 @@
 
 #include <memory>
 #include <string>
+#include <vector>
 
 namespace sgeig {
 
@@ -84,9 +85,21 @@ private:
     int refine_iters_ = 2;
     double refine_delta_ = 1e-4;
 
+    // Stage timing: when enabled, events bracket every kernel of a launch on
+    // its stream; stage_times() reads the last launch's durations.
+    bool timing_ = false;
+    std::vector<hipEvent_t> ev_;
+    int ev_used_ = 0;
+    void mark_(hipStream_t stream);
+
 public:
     int refine_iters() const { return refine_iters_; }
     double refine_delta() const { return refine_delta_; }
+    void set_stage_timing(bool on);
+    // Fills ms[0 .. min(n, stages)) and returns the number of stages of the
+    // last timed launch (0 if none); waits for that launch to finish.
+    int stage_times(double* ms, int n);
+    static const char* stage_name(Mode mode, int i);
 };
 
 }  // namespace sgeig
