@@ -14,8 +14,11 @@ and S-orthogonality, 1-norms). The CPU baseline is LAPACK zhegv over a
 bounded sample of the same matrices on the host cores, which also gives the
 mixed mode's eigenvalue error against zhegv.
 
+A serving-loop measurement follows (continuous batching through the
+library's session; Poisson arrivals at 70% of the measured peak; p50/p99).
+
 Prints one JSON line. Usage: python bench.py [--gpus 1] [--steps K]
-[--warmup W] [--batch B] [--n N] [--no-cpu] [--no-fp64]
+[--warmup W] [--batch B] [--n N] [--no-cpu] [--no-fp64] [--no-serve]
 """
 from __future__ import annotations
 
@@ -84,6 +87,78 @@ def cpu_baseline(H, S, n, sample: int, threads: int):
     }, W
 
 
+def serving_bench(kernel: str, pools, peak_requests: int, poisson_requests: int, load: float,
+                  seed: int):
+    """Continuous batching through the library's serving session.
+
+    Requests are q-point batches: K pencils, K log-uniform in [16, 4096], one
+    order N per request drawn from `pools` (device-resident inputs of the
+    benchmark generator; a request uses the first K pencils of its order's
+    pool and gets its own output buffers). First a burst of `peak_requests`
+    gives the peak sustained rate; then `poisson_requests` arrive as a Poisson
+    process at `load` x that rate. Latency = submission to results ready, as
+    measured by the session."""
+    import numpy as np
+    import torch
+    import sunny_gen_eigensolver_amd as sg
+    rng = np.random.default_rng(seed)
+    orders = sorted(pools)
+
+    def draw(count):
+        ks = np.exp(rng.uniform(np.log(16), np.log(4096), size=count)).astype(int)
+        ns = rng.choice(orders, size=count)
+        return list(zip(ns.tolist(), ks.tolist()))
+
+    ss = sg.ServingSession(orders, kernel=kernel, max_count=4096, slots=4, hbm_budget=1 << 30)
+
+    def submit(n, k):
+        H, S = pools[n]
+        W = torch.empty((k, n), dtype=torch.float64, device=H.device)
+        U = torch.empty((k, n, n), dtype=torch.complex128, device=H.device)
+        info = torch.zeros((k,), dtype=torch.int32, device=H.device)
+        return ss.submit(H[:k], S[:k], W, U, info)
+
+    # Warm-up, then the peak rate from a burst.
+    for n in orders:
+        ss.wait(submit(n, 64))
+    burst = draw(peak_requests)
+    t0 = time.perf_counter()
+    ids = [submit(n, k) for n, k in burst]
+    for i in ids:
+        ss.wait(i)
+    peak_s = time.perf_counter() - t0
+    peak_rate = sum(k for _, k in burst) / peak_s                 # solves/s
+    # Poisson arrivals at load x peak.
+    reqs = draw(poisson_requests)
+    mean_k = float(np.mean([k for _, k in reqs]))
+    lam = load * peak_rate / mean_k                                # requests/s
+    gaps = rng.exponential(1.0 / lam, size=len(reqs))
+    ids = []
+    t0 = time.perf_counter()
+    t_next = t0
+    for (n, k), g in zip(reqs, gaps):
+        t_next += g
+        while time.perf_counter() < t_next:
+            pass
+        ids.append(submit(n, k))
+    lats = [ss.wait(i)[0] for i in ids]
+    span = time.perf_counter() - t0
+    st = ss.stats()
+    ss.close()
+    lats = np.array(lats)
+    return {
+        "mode": kernel, "orders": orders, "requests": len(reqs),
+        "peak_solves_per_s": peak_rate, "offered_load": load,
+        "offered_solves_per_s": lam * mean_k,
+        "achieved_solves_per_s": sum(k for _, k in reqs) / span,
+        "p50_ms": float(np.percentile(lats, 50)), "p99_ms": float(np.percentile(lats, 99)),
+        "max_ms": float(lats.max()), "max_concurrent": st["max_concurrent"],
+        "note": "K log-uniform in [16, 4096] per request, N uniform over the served orders; "
+                "device-resident inputs and outputs (no PCIe in the latency); "
+                "4 slots; latency from submission to results ready in HBM",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +170,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 fused / split modes")
     ap.add_argument("--cpu-sample", type=int, default=4096)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="launch one device copy of this many MiB first (HBM counter calibration)")
     a = ap.parse_args()
@@ -240,6 +317,22 @@ def main():
         out["mixed_vs_fp64_fused"] = value / modes["unified"]["solves_per_s"]
         out["mixed_vs_fp64_split"] = value / modes["hybrid"]["solves_per_s"]
     mixed.close()
+
+    if not a.no_serve:
+        # Serving: orders 24 and 54 of the request mix (the LDS-resident ones),
+        # inputs from the same generator, resident in HBM.
+        pools = {54: (H[:4096], S[:4096])}
+        H24, S24 = inputs.spectrum_batch(24, 4096, a.seed + 24, device=dev)
+        pools[24] = (H24, S24)
+        serving = {}
+        for kernel in ("mixed", "unified"):
+            r = serving_bench(kernel, pools, 200, a.serve_requests, 0.7, a.seed)
+            serving[kernel] = r
+            log(f"serving {kernel}: peak {r['peak_solves_per_s']:.0f} solves/s; at 70%: "
+                f"p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms, achieved {r['achieved_solves_per_s']:.0f}")
+        out["serving"] = serving
+        out["p99_ms"] = serving["mixed"]["p99_ms"]
+        del H24, S24, pools
 
     if not a.no_cpu:
         try:

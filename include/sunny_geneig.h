@@ -179,6 +179,52 @@ sunny_geneig_status_t sunny_geneig_varlen_launch(
 int sunny_geneig_varlen_num_buckets (sunny_geneig_varlen_t solver);
 int sunny_geneig_varlen_bucket_order(sunny_geneig_varlen_t solver, int i);
 
+/* ---- serving session (no counterpart in the reference ABI) ---------------
+ * A request loop over resident solvers: requests (count pencils of one order
+ * from `orders`) are queued on submission and admitted in arrival order when
+ * one of `num_slots` slots (a stream with its own solver handles, workspaces
+ * reserved for max_count pencils at creation) is free and the request's
+ * device footprint fits `hbm_budget_bytes`. Admitted requests run
+ * concurrently: copy in, solve, copy out. With on_device != 0 the five
+ * buffers are caller-owned device memory (no copies, no footprint); otherwise
+ * they are host memory (pinned memory gives asynchronous copies), staged
+ * through the session's device arena. Buffers must stay valid until the
+ * request is done. Per-pencil results equal those of sunny_geneig_launch. */
+typedef struct sunny_geneig_session* sunny_geneig_session_t;
+
+sunny_geneig_session_t sunny_geneig_session_create(int device_id, sunny_geneig_kernel_mode_t kernel_mode,
+                                                   const int* orders, int num_orders, int max_count,
+                                                   int num_slots, uint64_t hbm_budget_bytes,
+                                                   sunny_geneig_cache_mode_t cache_mode);
+
+/* Drains queued and running requests, then frees everything. NULL-safe. */
+void sunny_geneig_session_destroy(sunny_geneig_session_t session);
+
+/* Returns the request id (>= 0) or a negative status (INVALID_ARG for an
+ * order the session does not serve, a count outside [1, max_count], NULL
+ * buffers, or a footprint larger than the budget). */
+int64_t sunny_geneig_session_submit(sunny_geneig_session_t session, int n, int count,
+                                    uintptr_t H, uintptr_t S, uintptr_t W, uintptr_t U,
+                                    uintptr_t info, int on_device);
+
+/* 1 when the request is done, 0 while queued or running, a negative status
+ * for an unknown id. */
+int sunny_geneig_session_poll(sunny_geneig_session_t session, int64_t request_id);
+
+/* Blocks until the request is done and forgets it; latency_ms = submission
+ * to completion, service_ms = admission to completion (either may be NULL).
+ * Returns 0 or a negative status (LAUNCH_ERROR if the request failed). */
+int sunny_geneig_session_wait(sunny_geneig_session_t session, int64_t request_id,
+                              double* latency_ms, double* service_ms);
+
+/* Counters since creation: requests submitted / completed, the arena's
+ * high-water mark in bytes and the most requests running at once. */
+int sunny_geneig_session_stats(sunny_geneig_session_t session, int64_t* submitted, int64_t* completed,
+                               uint64_t* hbm_peak_bytes, int* max_concurrent);
+
+/* Device footprint of a host-buffer request: H, S, U, W and info. */
+uint64_t sunny_geneig_session_request_bytes(int n, int count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,9 @@ EXPORTED_SYMBOLS = (
     "sunny_geneig_varlen_create", "sunny_geneig_varlen_destroy", "sunny_geneig_varlen_launch",
     "sunny_geneig_varlen_num_buckets", "sunny_geneig_varlen_bucket_order",
     "sunny_geneig_set_stage_timing", "sunny_geneig_stage_times",
+    "sunny_geneig_session_create", "sunny_geneig_session_destroy", "sunny_geneig_session_submit",
+    "sunny_geneig_session_poll", "sunny_geneig_session_wait", "sunny_geneig_session_stats",
+    "sunny_geneig_session_request_bytes",
 )
 
 _lib = None
@@ -104,6 +107,23 @@ def lib() -> ctypes.CDLL:
     L.sunny_geneig_stage_times.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_char_p), ci]
     L.sunny_geneig_stage_times.restype = ci
+    i64, u64 = ctypes.c_int64, ctypes.c_uint64
+    pd = ctypes.POINTER(ctypes.c_double)
+    L.sunny_geneig_session_create.argtypes = [ci, ci, pi, ci, ci, ci, u64, ci]
+    L.sunny_geneig_session_create.restype = vp
+    L.sunny_geneig_session_destroy.argtypes = [vp]
+    L.sunny_geneig_session_destroy.restype = None
+    L.sunny_geneig_session_submit.argtypes = [vp, ci, ci, cu, cu, cu, cu, cu, ci]
+    L.sunny_geneig_session_submit.restype = i64
+    L.sunny_geneig_session_poll.argtypes = [vp, i64]
+    L.sunny_geneig_session_poll.restype = ci
+    L.sunny_geneig_session_wait.argtypes = [vp, i64, pd, pd]
+    L.sunny_geneig_session_wait.restype = ci
+    L.sunny_geneig_session_stats.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64),
+                                             ctypes.POINTER(u64), pi]
+    L.sunny_geneig_session_stats.restype = ci
+    L.sunny_geneig_session_request_bytes.argtypes = [ci, ci]
+    L.sunny_geneig_session_request_bytes.restype = u64
     _lib = L
     return L
 
@@ -355,3 +375,81 @@ def unpack_vectors(flat, orders):
         out.append(flat[o:o + n])
         o += n
     return out
+
+
+class ServingSession:
+    """Request loop over resident solvers (sunny_geneig_session_*).
+
+    ServingSession(orders=[24, 54], kernel="mixed", max_count=4096, slots=4,
+    hbm_budget=2 << 30). submit(H, S, W, U, info) queues one request: with
+    device tensors the buffers are used in place, with (pinned) CPU tensors the
+    session stages them through its device arena. Returns a request id;
+    wait(id) -> (latency_ms, service_ms)."""
+
+    def __init__(self, orders, kernel: str = "mixed", max_count: int = 4096, slots: int = 4,
+                 hbm_budget: int = 2 << 30, device_id: int = 0, cache: str = "auto"):
+        L = lib()
+        k = {"hybrid": KERNEL_HYBRID, "mixed": KERNEL_MIXED}.get(kernel, KERNEL_UNIFIED)
+        c = {"nodisk": CACHE_NODISK, "nocache": CACHE_NOCACHE}.get(cache, CACHE_AUTO)
+        arr = (ctypes.c_int * len(orders))(*orders)
+        h = L.sunny_geneig_session_create(device_id, k, arr, len(orders), max_count, slots,
+                                          hbm_budget, c)
+        if not h:
+            raise GeneigError(STATUS_INVALID_ARG, f"sunny_geneig_session_create failed: {last_error()}")
+        self.handle = h
+        self._keep = {}
+
+    def submit(self, H, S, W, U, info) -> int:
+        n = H.shape[-1]
+        count = H.shape[0]
+        on_dev = 1 if H.is_cuda else 0
+        rid = lib().sunny_geneig_session_submit(self.handle, n, count, H.data_ptr(), S.data_ptr(),
+                                                W.data_ptr(), U.data_ptr(), info.data_ptr(), on_dev)
+        if rid < 0:
+            raise GeneigError(int(rid), last_error())
+        self._keep[rid] = (H, S, W, U, info)     # buffers stay alive until wait()
+        return rid
+
+    def poll(self, rid: int) -> bool:
+        st = lib().sunny_geneig_session_poll(self.handle, rid)
+        if st < 0:
+            raise GeneigError(st, last_error())
+        return st == 1
+
+    def wait(self, rid: int):
+        lat, svc = ctypes.c_double(), ctypes.c_double()
+        st = lib().sunny_geneig_session_wait(self.handle, rid, ctypes.byref(lat), ctypes.byref(svc))
+        self._keep.pop(rid, None)
+        if st != STATUS_OK:
+            raise GeneigError(st, last_error())
+        return lat.value, svc.value
+
+    def stats(self):
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        c, d = ctypes.c_uint64(), ctypes.c_int()
+        lib().sunny_geneig_session_stats(self.handle, ctypes.byref(a), ctypes.byref(b),
+                                         ctypes.byref(c), ctypes.byref(d))
+        return {"submitted": a.value, "completed": b.value, "hbm_peak_bytes": c.value,
+                "max_concurrent": d.value}
+
+    @staticmethod
+    def request_bytes(n: int, count: int) -> int:
+        return lib().sunny_geneig_session_request_bytes(n, count)
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().sunny_geneig_session_destroy(self.handle)
+            self.handle = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

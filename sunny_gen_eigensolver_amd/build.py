@@ -62,9 +62,9 @@ def embed_kernel_source() -> str:
 def build_library() -> str:
     embed_kernel_source()
     os.makedirs(LIBDIR, exist_ok=True)
-    srcs = [os.path.join(CSRC, f) for f in ("codeobj.cpp", "solver.cpp", "varlen.cpp", "c_api.cpp")]
+    srcs = [os.path.join(CSRC, f) for f in ("codeobj.cpp", "solver.cpp", "varlen.cpp", "serving.cpp", "c_api.cpp")]
     tmp = LIB + ".tmp"
-    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall",
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-pthread",
           "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", f"-I{os.path.join(REPO, 'include')}",
           f"-I{CSRC}", *srcs, "-o", tmp, f"-L{ROCM}/lib", "-lamdhip64", "-lhiprtc",
           f"-Wl,-rpath,{ROCM}/lib"])

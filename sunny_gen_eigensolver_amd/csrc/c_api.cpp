@@ -4,6 +4,7 @@
 #include "sunny_geneig.h"
 
 #include "solver.hpp"
+#include "serving.hpp"
 #include "varlen.hpp"
 
 #include <exception>
@@ -19,6 +20,10 @@ struct sunny_geneig_solver {
 
 struct sunny_geneig_varlen_solver {
     std::unique_ptr<sgeig::VarlenSolver> impl;
+};
+
+struct sunny_geneig_session {
+    std::unique_ptr<sgeig::ServingSession> impl;
 };
 
 namespace {
@@ -275,6 +280,113 @@ sunny_geneig_status_t sunny_geneig_varlen_launch(sunny_geneig_varlen_t solver, u
         set_error("unknown C++ exception in sunny_geneig_varlen_launch");
     }
     return SUNNY_GENEIG_LAUNCH_ERROR;
+}
+
+sunny_geneig_session_t sunny_geneig_session_create(int device_id, sunny_geneig_kernel_mode_t kernel_mode,
+                                                   const int* orders, int num_orders, int max_count,
+                                                   int num_slots, uint64_t hbm_budget_bytes,
+                                                   sunny_geneig_cache_mode_t cache_mode) {
+    clear_error();
+    try {
+        if (!orders || num_orders < 1) throw std::invalid_argument("no matrix orders to serve");
+        if (kernel_mode != SUNNY_GENEIG_KERNEL_UNIFIED && kernel_mode != SUNNY_GENEIG_KERNEL_HYBRID &&
+            kernel_mode != SUNNY_GENEIG_KERNEL_MIXED)
+            throw std::invalid_argument("unknown kernel_mode");
+        const sgeig::Mode mode = kernel_mode == SUNNY_GENEIG_KERNEL_HYBRID  ? sgeig::Mode::Hybrid
+                                 : kernel_mode == SUNNY_GENEIG_KERNEL_MIXED ? sgeig::Mode::Mixed
+                                                                            : sgeig::Mode::Unified;
+        auto h = std::make_unique<sunny_geneig_session>();
+        h->impl = std::make_unique<sgeig::ServingSession>(
+            device_id, mode, std::vector<int>(orders, orders + num_orders), max_count, num_slots,
+            (size_t)hbm_budget_bytes, to_cache(cache_mode));
+        return h.release();
+    } catch (const std::bad_alloc&) {
+        set_error("out of memory in sunny_geneig_session_create");
+    } catch (const std::exception& e) {
+        set_error(std::string("sunny_geneig_session_create: ") + e.what());
+    } catch (...) {
+        set_error("unknown C++ exception in sunny_geneig_session_create");
+    }
+    return nullptr;
+}
+
+void sunny_geneig_session_destroy(sunny_geneig_session_t s) {
+    try {
+        delete s;
+    } catch (...) {
+    }
+}
+
+int64_t sunny_geneig_session_submit(sunny_geneig_session_t s, int n, int count, uintptr_t H, uintptr_t S,
+                                    uintptr_t W, uintptr_t U, uintptr_t info, int on_device) {
+    clear_error();
+    if (!s) {
+        set_error("sunny_geneig_session_submit: session is NULL");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    try {
+        return s->impl->submit(n, count, reinterpret_cast<const void*>(H), reinterpret_cast<const void*>(S),
+                               reinterpret_cast<double*>(W), reinterpret_cast<void*>(U),
+                               reinterpret_cast<int*>(info), on_device != 0);
+    } catch (const std::invalid_argument& e) {
+        set_error(std::string("sunny_geneig_session_submit: ") + e.what());
+        return SUNNY_GENEIG_INVALID_ARG;
+    } catch (const std::exception& e) {
+        set_error(std::string("sunny_geneig_session_submit: ") + e.what());
+        return SUNNY_GENEIG_LAUNCH_ERROR;
+    }
+}
+
+int sunny_geneig_session_poll(sunny_geneig_session_t s, int64_t id) {
+    clear_error();
+    if (!s) {
+        set_error("sunny_geneig_session_poll: session is NULL");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    try {
+        return s->impl->poll(id);
+    } catch (const std::exception& e) {
+        set_error(std::string("sunny_geneig_session_poll: ") + e.what());
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+}
+
+int sunny_geneig_session_wait(sunny_geneig_session_t s, int64_t id, double* latency_ms, double* service_ms) {
+    clear_error();
+    if (!s) {
+        set_error("sunny_geneig_session_wait: session is NULL");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    try {
+        s->impl->wait(id, latency_ms, service_ms);
+        return SUNNY_GENEIG_OK;
+    } catch (const std::invalid_argument& e) {
+        set_error(std::string("sunny_geneig_session_wait: ") + e.what());
+        return SUNNY_GENEIG_INVALID_ARG;
+    } catch (const std::exception& e) {
+        set_error(std::string("sunny_geneig_session_wait: ") + e.what());
+        return SUNNY_GENEIG_LAUNCH_ERROR;
+    }
+}
+
+int sunny_geneig_session_stats(sunny_geneig_session_t s, int64_t* submitted, int64_t* completed,
+                               uint64_t* hbm_peak, int* max_concurrent) {
+    clear_error();
+    if (!s) {
+        set_error("sunny_geneig_session_stats: session is NULL");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    const sgeig::ServingStats st = s->impl->stats();
+    if (submitted) *submitted = st.submitted;
+    if (completed) *completed = st.completed;
+    if (hbm_peak) *hbm_peak = st.hbm_peak;
+    if (max_concurrent) *max_concurrent = st.max_concurrent;
+    return SUNNY_GENEIG_OK;
+}
+
+uint64_t sunny_geneig_session_request_bytes(int n, int count) {
+    if (n < 1 || count < 1) return 0;
+    return sgeig::ServingSession::request_bytes(n, count);
 }
 
 int sunny_geneig_varlen_num_buckets(sunny_geneig_varlen_t s) {

@@ -277,7 +277,7 @@ void Solver::ensure_workspace_(size_t batch) {
         lbuf_batch_ = 0;
     }
     // Per matrix: the tridiagonal's off-diagonal (N doubles) and Z (N x N doubles).
-    SG_HIP(hipMalloc(&lbuf_, batch * ((size_t)n_ + (size_t)n_ * n_) * 8));
+    SG_HIP(hipMalloc(&lbuf_, workspace_bytes(Mode::Hybrid, n_, batch)));
     lbuf_batch_ = batch;
 }
 
@@ -311,10 +311,9 @@ void Solver::ensure_mixed_workspace_(size_t batch) {
         mbuf_ = nullptr;
         mbuf_batch_ = 0;
     }
-    const size_t n = (size_t)n_, npk = n * (n + 1) / 2;
     // complex128 W = L^-1 (packed), complex64 stash / V, float eigenvalues,
     // off-diagonals and Z.
-    SG_HIP(hipMalloc(&mbuf_, batch * (npk * 16 + n * n * 8 + n * 4 + n * 4 + n * n * 4)));
+    SG_HIP(hipMalloc(&mbuf_, workspace_bytes(Mode::Mixed, n_, batch)));
     mbuf_batch_ = batch;
 }
 
@@ -353,6 +352,23 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
     const unsigned bd_ref = (unsigned)(64 * ((n_ + 15) / 16));
     SG_HIP(hipModuleLaunchKernel(k_refine_, grid_b, 1, 1, bd_ref, 1, 1, 0, stream, a5, nullptr));
     mark_(stream);
+}
+
+size_t Solver::workspace_bytes(Mode mode, int n_, size_t batch) {
+    const size_t n = (size_t)n_, npk = n * (n + 1) / 2;
+    if (mode == Mode::Hybrid) return batch * (n + n * n) * 8;
+    if (mode == Mode::Mixed) return batch * (npk * 16 + n * n * 8 + n * 4 + n * 4 + n * n * 4);
+    return 0;
+}
+
+size_t Solver::reserve(size_t batch) {
+    int cur = 0;
+    SG_HIP(hipGetDevice(&cur));
+    if (cur != device_) SG_HIP(hipSetDevice(device_));
+    if (mode_ == Mode::Hybrid) ensure_workspace_(batch);
+    if (mode_ == Mode::Mixed) ensure_mixed_workspace_(batch);
+    if (cur != device_) SG_HIP(hipSetDevice(cur));
+    return workspace_bytes(mode_, n_, batch);
 }
 
 void Solver::set_stage_timing(bool on) {

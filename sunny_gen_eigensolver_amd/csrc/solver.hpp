@@ -95,6 +95,11 @@ private:
 public:
     int refine_iters() const { return refine_iters_; }
     double refine_delta() const { return refine_delta_; }
+    // Grow the mode's device workspace for `batch` matrices now (no-op for the
+    // fused mode), so later launches of up to `batch` allocate nothing; returns
+    // its size in bytes.
+    size_t reserve(size_t batch);
+    static size_t workspace_bytes(Mode mode, int n, size_t batch);
     void set_stage_timing(bool on);
     // Fills ms[0 .. min(n, stages)) and returns the number of stages of the
     // last timed launch (0 if none); waits for that launch to finish.

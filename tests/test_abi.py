@@ -122,3 +122,27 @@ def test_varlen_null_and_bad_arguments():
         h = L.sunny_geneig_varlen_create(max_n, 0, arr, len(buckets) if buckets else 0, 0)
         assert not h
         assert "sunny_geneig_varlen_create" in sg.last_error()
+
+
+def test_session_null_and_bad_arguments():
+    L = sg.lib()
+    L.sunny_geneig_session_destroy(None)
+    assert L.sunny_geneig_session_submit(None, 8, 1, 0, 0, 0, 0, 0, 1) == sg.STATUS_INVALID_ARG
+    assert L.sunny_geneig_session_poll(None, 0) == sg.STATUS_INVALID_ARG
+    assert L.sunny_geneig_session_wait(None, 0, None, None) == sg.STATUS_INVALID_ARG
+    assert L.sunny_geneig_session_stats(None, None, None, None, None) == sg.STATUS_INVALID_ARG
+    h = L.sunny_geneig_session_create(0, 0, None, 0, 16, 2, 1 << 20, 0)
+    assert not h and "orders" in sg.last_error()
+    orders = (ctypes.c_int * 1)(80)
+    h = L.sunny_geneig_session_create(0, 0, orders, 1, 16, 2, 1 << 20, 0)
+    assert not h and "outside [1, 64]" in sg.last_error()
+
+
+def test_session_request_bytes():
+    """H, S and U (16 n^2 each) plus W (8 n) and info (4), each 256-aligned."""
+    def al(x):
+        return (x + 255) // 256 * 256
+    for n, c in ((54, 1), (24, 100), (54, 4096)):
+        want = 3 * al(16 * n * n * c) + al(8 * n * c) + al(4 * c)
+        assert sg.ServingSession.request_bytes(n, c) == want
+    assert sg.ServingSession.request_bytes(0, 5) == 0
