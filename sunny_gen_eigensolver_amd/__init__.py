@@ -65,6 +65,14 @@ def lib() -> ctypes.CDLL:
                           "`python -m sunny_gen_eigensolver_amd.build` first")
     # The in-tree code-object cache ships warm; use it unless told otherwise.
     os.environ.setdefault("SUNNY_GENEIG_CACHE_DIR", DEFAULT_CACHE_DIR)
+    # torch ships its own HIP runtime and finds it by the unversioned name, this
+    # library by the versioned SONAME. Loaded first, torch's copy also serves
+    # this library (same SONAME); loaded second, it would be a second runtime
+    # in the process and torch's device initialisation fails.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, ci, cu = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64
     L.sunny_geneig_create.argtypes = [ci, ci, ci, ci, ci]
