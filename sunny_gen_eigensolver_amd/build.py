@@ -38,6 +38,7 @@ HIP_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 # (multiples of 4 up to 64) ship warm at their own workgroup size.
 WARM_ORDERS = [16, 24, 32, 48, 54, 64]
 BUCKET_ORDERS = list(range(4, 65, 4))
+LARGE_ORDERS = [96, 128, 150, 192]       # staged kernels (orders above 64)
 
 
 def _run(cmd, **kw):
@@ -48,10 +49,15 @@ def _run(cmd, **kw):
 def embed_kernel_source() -> str:
     os.makedirs(GEN, exist_ok=True)
     src = open(KERNEL).read()
+    # hipRTC sees one text: the staged kernels' file goes in place of its #include.
+    inc = '#include "geneig_large.hip"\n'
+    assert src.count(inc) == 1
+    src = src.replace(inc, open(os.path.join(os.path.dirname(KERNEL), "geneig_large.hip")).read())
     delim = "GENEIG_SRC"
     assert (")" + delim + '"') not in src
     out = os.path.join(GEN, "kernel_source.inc")
-    text = ("// Generated by sunny_gen_eigensolver_amd/build.py from kernels/geneig_kernels.hip.\n"
+    text = ("// Generated by sunny_gen_eigensolver_amd/build.py from kernels/geneig_kernels.hip\n"
+            "// (with kernels/geneig_large.hip inlined).\n"
             f'static const char kGeneigKernelSource[] = R"{delim}(' + src + f'){delim}";\n')
     old = open(out).read() if os.path.exists(out) else None
     if old != text:
@@ -80,6 +86,9 @@ def aot_check() -> None:
           "-o", os.path.join(BUILD, "geneig_kernels.o")])
     _run(["hipcc", "--offload-arch=gfx950", "-O3", "-std=c++17", *HIP_FLAGS, "-DGENEIG_FP32",
           "-Werror=double-promotion", "-c", KERNEL, "-o", os.path.join(BUILD, "geneig_kernels_fp32.o")])
+    # The staged kernels (orders above 64), FP32 build.
+    _run(["hipcc", "--offload-arch=gfx950", "-O3", "-std=c++17", *HIP_FLAGS, "-DGENEIG_FP32", "-DGENEIG_N=192",
+          "-Werror=double-promotion", "-c", KERNEL, "-o", os.path.join(BUILD, "geneig_large_fp32.o")])
 
 
 def _precompile(job):
@@ -91,7 +100,7 @@ def _precompile(job):
     return n, bd, rc, (lib.sunny_geneig_last_error().decode() if rc else "")
 
 
-def warm_cache(orders=WARM_ORDERS, buckets=BUCKET_ORDERS) -> None:
+def warm_cache(orders=WARM_ORDERS, buckets=BUCKET_ORDERS, large=LARGE_ORDERS) -> None:
     from concurrent.futures import ProcessPoolExecutor
     os.makedirs(CACHE, exist_ok=True)
     # Entries are keyed by a hash of source + options: drop stale ones.
@@ -101,6 +110,7 @@ def warm_cache(orders=WARM_ORDERS, buckets=BUCKET_ORDERS) -> None:
     os.environ["SUNNY_GENEIG_CACHE_DIR"] = CACHE
     jobs = [(n, bd) for n in orders for bd in (64, 128, 256)]
     jobs += [(n, 0) for n in buckets if n not in orders]
+    jobs = [(n, 0) for n in large] + jobs           # the longest compiles first
     workers = max(1, min(8, len(os.sched_getaffinity(0))))
     with ProcessPoolExecutor(workers) as ex:
         for n, bd, rc, msg in ex.map(_precompile, jobs):

@@ -159,9 +159,10 @@ sunny_geneig_kernel_mode_t sunny_geneig_kernel_mode(sunny_geneig_solver_t s) {
 int sunny_geneig_precompile(int n, int block_dim_x, const char* arch) {
     clear_error();
     try {
-        if (n < 1 || n > 64) throw std::invalid_argument("n out of range for the LDS-resident kernel");
-        sgeig::ModuleKey key{n, block_dim_x > 0 ? block_dim_x : sgeig::Solver::suggested_block_dim(n),
-                             arch && arch[0] ? arch : "gfx950"};
+        if (n < 1 || n > sgeig::Solver::kMaxOrder) throw std::invalid_argument("n out of range");
+        int bd = block_dim_x > 0 ? block_dim_x : sgeig::Solver::suggested_block_dim(n);
+        if (n > sgeig::Solver::kMaxLdsOrder) bd = 256;     // the staged kernels' only workgroup size
+        sgeig::ModuleKey key{n, bd, arch && arch[0] ? arch : "gfx950"};
         sgeig::write_disk(key, sgeig::compile_code_object(key));
         key.fp32 = true;
         sgeig::write_disk(key, sgeig::compile_code_object(key));
@@ -212,7 +213,7 @@ int sunny_geneig_stage_times(sunny_geneig_solver_t s, double* ms, const char** n
         const int n = s->impl->stage_times(tmp, 8);
         for (int i = 0; i < n && i < max_stages; ++i) {
             if (ms) ms[i] = tmp[i];
-            if (names) names[i] = sgeig::Solver::stage_name(s->impl->mode(), i);
+            if (names) names[i] = sgeig::Solver::stage_name(s->impl->mode(), i, s->impl->n());
         }
         return n;
     } catch (const std::exception& e) {

@@ -46,7 +46,7 @@ constexpr int CPW = (N + NW - 1) / NW;     // columns (or rows) owned per wave
 constexpr int QL_MAXIT = 30;               // sweeps per eigenvalue before giving up
 
 static_assert(NT % 64 == 0 && NW >= 1, "workgroup must be whole waves");
-static_assert(N >= 1 && N <= 64, "LDS-resident path holds one row per lane");
+static_assert(N >= 1 && N <= 192, "orders above 64 run the staged kernels, up to 192");
 
 // Module precision: FP64 by default; -DGENEIG_FP32 builds the FP32 variant
 // of the split-mode stages used by the mixed-precision path.
@@ -158,6 +158,87 @@ __device__ __forceinline__ T wave_max(T v) {
     v = fmax(v, dpp_mov<0x140>(v));
     return fmax(fmax(bcast(v, 0), bcast(v, 16)), fmax(bcast(v, 32), bcast(v, 48)));
 }
+
+// Lane `target` takes x, the others keep v.
+__device__ __forceinline__ real sel_lane(real v, real x, int target) {
+    return (lane_id() == target) ? x : v;
+}
+
+// 1/sqrt(t) for the QL rotations.
+#ifdef GENEIG_FP32
+__device__ __forceinline__ float rsqrt_nr(float t) {
+    // v_rsq_f32 is accurate to about one ulp: no refinement needed in FP32.
+    return __builtin_amdgcn_rsqf(t);
+}
+#else
+__device__ __forceinline__ double rsqrt_nr(double t) {
+    double y = __builtin_amdgcn_rsq(t);
+    const double h = 0.5 * t;
+    double r = fma(-(h * y), y, 0.5);
+    y = fma(y, r, y);
+    r = fma(-(h * y), y, 0.5);
+    y = fma(y, r, y);
+    return y;
+}
+#endif
+
+// Orders this wave's LDS writes before its later LDS reads.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// 2 x 2 Hermitian-definite pencil (T, G) (G close to I): eigenvalues
+// mu1 <= mu2 and G-orthonormal eigenvectors X = [x1 x2], by a Cholesky of G
+// and the closed-form eigensystem of C = L^-1 T L^-H. Arguments wave-uniform.
+__device__ __forceinline__ void eig2_pencil(real t11, real t22, cx t12, real g11, real g22, cx g12,
+                                            real& mu1, real& mu2, cx (&X)[2][2]) {
+    const real l11 = sqrt_r(g11);
+    const cx l21 = cm(g12.x / l11, -g12.y / l11);            // conj(g12) / l11
+    const real l22 = sqrt_r(g22 - cabs2(l21));
+    const cx m11 = cm(t11 / l11, 0.0), m12 = cm(t12.x / l11, t12.y / l11);
+    const cx m21 = (cm(t12.x, -t12.y) - cmul(l21, m11)) * (real(1.0) / l22);
+    const cx m22 = (cm(t22, 0.0) - cmul(l21, m12)) * (real(1.0) / l22);
+    const cx c21 = m21 * (real(1.0) / l11);
+    const real a = t11 / (l11 * l11);
+    const real c = (m22 - cmulc(c21, l21)).x / l22;
+    const cx bb = cm(c21.x, -c21.y);                          // C(1, 2)
+    const real d = real(0.5) * (c - a);
+    const real r = sqrt_r(d * d + cabs2(bb));
+    mu1 = real(0.5) * (a + c) - r;
+    mu2 = real(0.5) * (a + c) + r;
+    cx y1a, y1b;                                              // eigenvector of mu1
+    if (d >= real(0.0)) {
+        y1a = cm(d + r, 0.0);
+        y1b = cm(-bb.x, bb.y);
+    } else {
+        y1a = bb;
+        y1b = cm(d - r, 0.0);
+    }
+    const real nrm = sqrt_r(cabs2(y1a) + cabs2(y1b));
+    if (nrm > real(0.0)) {
+        y1a = y1a * (real(1.0) / nrm);
+        y1b = y1b * (real(1.0) / nrm);
+    } else {
+        y1a = cm(1.0, 0.0);
+        y1b = cm(0.0, 0.0);
+    }
+    const cx y2a = cm(-y1b.x, y1b.y), y2b = cm(y1a.x, -y1a.y);
+    const cx cl21 = cm(l21.x, -l21.y);
+    X[1][0] = y1b * (real(1.0) / l22);
+    X[0][0] = (y1a - cmul(cl21, X[1][0])) * (real(1.0) / l11);
+    X[1][1] = y2b * (real(1.0) / l22);
+    X[0][1] = (y2a - cmul(cl21, X[1][1])) * (real(1.0) / l11);
+}
+
+}  // namespace gk
+
+#if GENEIG_N > 64
+#include "geneig_large.hip"
+#else
+
+namespace gk {
 
 // Shared-memory work area of one matrix.
 struct Work {
@@ -422,27 +503,6 @@ __device__ __forceinline__ void stash_reflectors(Work& w) {
 // Wilkinson, "The QR and QL algorithms for symmetric matrices" (Handbook
 // II/3). Returns zheev-style INFO (off-diagonals left unconverged, 0 = ok).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ real sel_lane(real v, real x, int target) {
-    return (lane_id() == target) ? x : v;
-}
-
-#ifdef GENEIG_FP32
-__device__ __forceinline__ float rsqrt_nr(float t) {
-    // v_rsq_f32 is accurate to about one ulp: no refinement needed in FP32.
-    return __builtin_amdgcn_rsqf(t);
-}
-#else
-__device__ __forceinline__ double rsqrt_nr(double t) {
-    double y = __builtin_amdgcn_rsq(t);
-    const double h = 0.5 * t;
-    double r = fma(-(h * y), y, 0.5);
-    y = fma(y, r, y);
-    r = fma(-(h * y), y, 0.5);
-    y = fma(y, r, y);
-    return y;
-}
-#endif
-
 __device__ __forceinline__ int ql_wave(real& dreg, real& ereg, real (&z)[N]) {
     const int lane = lane_id();
     const real eps = REAL_HALF_ULP;
@@ -731,11 +791,6 @@ __device__ __forceinline__ void stage_lower(cx* sl, const cx* __restrict__ Ub) {
     }
 }
 
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // V <- Q V for this lane's column; reflectors from the staged slice.
 __device__ __forceinline__ void apply_q_wave(const cx* sl, cx (&b)[N]) {
@@ -2019,49 +2074,6 @@ __device__ __forceinline__ cx melem(const cx* P, int i, int k) {
     return in ? (lo ? v : cm(v.x, -v.y)) : cm(0.0, 0.0);
 }
 
-// 2 x 2 Hermitian-definite pencil (T, G) (G close to I): eigenvalues
-// mu1 <= mu2 and G-orthonormal eigenvectors X = [x1 x2], by a Cholesky of G
-// and the closed-form eigensystem of C = L^-1 T L^-H. Arguments wave-uniform.
-__device__ __forceinline__ void eig2_pencil(real t11, real t22, cx t12, real g11, real g22, cx g12,
-                                            real& mu1, real& mu2, cx (&X)[2][2]) {
-    const real l11 = sqrt_r(g11);
-    const cx l21 = cm(g12.x / l11, -g12.y / l11);            // conj(g12) / l11
-    const real l22 = sqrt_r(g22 - cabs2(l21));
-    const cx m11 = cm(t11 / l11, 0.0), m12 = cm(t12.x / l11, t12.y / l11);
-    const cx m21 = (cm(t12.x, -t12.y) - cmul(l21, m11)) * (real(1.0) / l22);
-    const cx m22 = (cm(t22, 0.0) - cmul(l21, m12)) * (real(1.0) / l22);
-    const cx c21 = m21 * (real(1.0) / l11);
-    const real a = t11 / (l11 * l11);
-    const real c = (m22 - cmulc(c21, l21)).x / l22;
-    const cx bb = cm(c21.x, -c21.y);                          // C(1, 2)
-    const real d = real(0.5) * (c - a);
-    const real r = sqrt_r(d * d + cabs2(bb));
-    mu1 = real(0.5) * (a + c) - r;
-    mu2 = real(0.5) * (a + c) + r;
-    cx y1a, y1b;                                              // eigenvector of mu1
-    if (d >= real(0.0)) {
-        y1a = cm(d + r, 0.0);
-        y1b = cm(-bb.x, bb.y);
-    } else {
-        y1a = bb;
-        y1b = cm(d - r, 0.0);
-    }
-    const real nrm = sqrt_r(cabs2(y1a) + cabs2(y1b));
-    if (nrm > real(0.0)) {
-        y1a = y1a * (real(1.0) / nrm);
-        y1b = y1b * (real(1.0) / nrm);
-    } else {
-        y1a = cm(1.0, 0.0);
-        y1b = cm(0.0, 0.0);
-    }
-    const cx y2a = cm(-y1b.x, y1b.y), y2b = cm(y1a.x, -y1a.y);
-    const cx cl21 = cm(l21.x, -l21.y);
-    X[1][0] = y1b * (real(1.0) / l22);
-    X[0][0] = (y1a - cmul(cl21, X[1][0])) * (real(1.0) / l11);
-    X[1][1] = y2b * (real(1.0) / l22);
-    X[0][1] = (y2a - cmul(cl21, X[1][1])) * (real(1.0) / l11);
-}
-
 // Rayleigh-Ritz on columns (a, b) of V for the Hermitian M (held in r.M), by
 // one wave: the 2 x 2 projected pencil (V^H M V, V^H V) of the pair, then
 // V(:, [a b]) <- V(:, [a b]) X and lam[a], lam[b] <- the Ritz values.
@@ -2258,3 +2270,5 @@ geneig_mx_refine(const cx* __restrict__ H, const cx* __restrict__ Wd, real* __re
     if (threadIdx.x < N) W[(size_t)b * N + threadIdx.x] = r.lam[threadIdx.x];
 }
 #endif  // !GENEIG_FP32
+
+#endif  // GENEIG_N > 64

@@ -1,0 +1,1062 @@
+// Orders 64 < N <= 192: the HBM-staged pipeline. Included by
+// geneig_kernels.hip when GENEIG_N > 64 (the LDS-resident kernels hold one
+// row per lane and a whole pair of matrices in LDS, which stops at N = 64).
+//
+// Above that an N x N complex128 matrix alone is 66 KB (N = 64) to 590 KB
+// (N = 192), so every stage is its own kernel, one workgroup of 256 threads
+// per matrix, and the stages hand over through a per-matrix workspace in
+// global memory (layout below):
+//   geneig_lg_chol    FP64  S = L L^H by 16-column panels (the update of a
+//                           panel by the columns left of it on the matrix
+//                           cores), then W = L^-1 in place by 16-row blocks.
+//   geneig_lg_formm   FP64  M = W H W^H on the matrix cores, stored as a full
+//                           (exactly Hermitian) matrix.
+//   geneig_lg_hetrd   P     Householder tridiagonalisation with the lower
+//                           triangle of M packed in LDS. In FP64 above N = 131
+//                           the triangle does not fit: the first K0 columns are
+//                           reduced in place in global memory, then the
+//                           trailing triangle moves to LDS.
+//   geneig_lg_ql      P     implicit QL: each wave runs the scalar recurrence
+//                           (d, e spread over its lanes), records the rotations
+//                           of a sweep in LDS and applies them to its own rows
+//                           of Z (in LDS, or in global memory for the rows that
+//                           do not fit), then sorts.
+//   geneig_lg_back    P     V = Q Z, blocks of 8 columns per wave in registers.
+//   geneig_lg_backtri FP64  U = W^H V on the matrix cores.
+// P is the module precision: FP64 for the FP64 modes; FP32 in the mixed mode,
+// which then refines (M, V) in FP64 (geneig_lg_refine) instead of
+// geneig_lg_backtri.
+
+namespace gk {
+
+constexpr int LNT = 256;                      // workgroup size of the staged kernels
+constexpr int LNW = LNT / 64;
+constexpr int ND = (N + 63) / 64;             // rows per lane when lanes index rows
+constexpr int NB = (N + 15) / 16;             // 16-blocks per dimension
+constexpr int KEND = (N + 3) / 4 * 4;         // k-steps that touch rows/columns < N
+constexpr size_t NN = (size_t)N * N;
+constexpr size_t LDS_CAP = 160 * 1024;
+
+// Per-matrix workspace (byte offsets). The same layout in the FP64 and FP32
+// modules: regions are sized for FP64 data.
+constexpr size_t WS_WI = 0;                   // L, then W = L^-1 (complex128, upper zero)
+constexpr size_t WS_M = 16 * NN;              // M = W H W^H (complex128, full)
+constexpr size_t WS_X = 32 * NN;              // scratch: W H; Z rows beyond LDS; refinement products
+constexpr size_t WS_V = 48 * NN;              // V (complex, module precision; FP64 in refinement)
+constexpr size_t WS_R = 64 * NN;              // reflectors: v_k(i) at (i, k), i >= k+2; tau_k at (k+1, k)
+constexpr size_t WS_Z = 80 * NN;              // Z, columns in ascending eigenvalue order (real)
+constexpr size_t WS_D = 88 * NN;              // d[N], e[N], ascending eigenvalues[N] (real)
+constexpr size_t WS_BYTES = (88 * NN + 24 * (size_t)N + 255) / 256 * 256;
+
+__device__ __forceinline__ char* ws_of(char* ws, long long wss, int b) { return ws + (size_t)b * (size_t)wss; }
+
+// Element of a distributed array (element i at lane i % 64, register i / 64)
+// at a wave-uniform, run-time index.
+__device__ __forceinline__ real dget(const real (&a)[ND], int idx) {
+    real v = a[0];
+#pragma unroll
+    for (int q = 1; q < ND; ++q)
+        if ((idx >> 6) == q) v = a[q];
+    return bcast(v, idx & 63);
+}
+__device__ __forceinline__ void dset(real (&a)[ND], int idx, real x) {
+#pragma unroll
+    for (int q = 0; q < ND; ++q)
+        if ((idx >> 6) == q) a[q] = sel_lane(a[q], x, idx & 63);
+}
+
+__device__ __forceinline__ cx wave_sum_cx(cx v) { return cm(wave_sum(v.x), wave_sum(v.y)); }
+
+// ---------------------------------------------------------------------------
+// Tridiagonalisation (module precision).
+// ---------------------------------------------------------------------------
+// Bytes beside the packed triangle: v (two steps), w, per-wave partial sums
+// of A v and the column sums of its upper part.
+constexpr size_t HT_EXTRA = (size_t)(2 + 1 + LNW + 1) * N * sizeof(cx) + 64;
+constexpr int ht_fit() {
+    int nl = N;
+    while (nl > 1 && (size_t)nl * (nl + 1) / 2 * sizeof(cx) + HT_EXTRA > LDS_CAP) --nl;
+    return nl;
+}
+constexpr int NL = ht_fit();                  // order of the LDS-resident trailing triangle
+constexpr int K0 = N - NL;                    // columns reduced in global memory first
+#ifdef GENEIG_FP32
+static_assert(K0 == 0, "the FP32 triangle fits LDS up to N = 192");
+#endif
+
+struct HtWork {
+    cx P[NL * (NL + 1) / 2];                  // lower triangle of rows/columns K0.., packed by columns
+    cx vb[2][N];
+    cx wb[N];
+    cx part[LNW][N];
+    cx pc[N];
+    cx tau[2];
+};
+
+__device__ __forceinline__ int pkl(int c) { return c * NL - (c * (c - 1)) / 2; }
+
+struct HtLds {
+    cx* P;
+    __device__ __forceinline__ cx& at(int i, int j) const { return P[pkl(j - K0) + i - j]; }
+};
+struct HtGlb {
+    cx* G;
+    __device__ __forceinline__ cx& at(int i, int j) const { return G[(size_t)j * N + i]; }
+};
+
+// Householder vector of column k, by the wave that owns column k (k % LNW):
+// v(k+1) = 1, tau and the real beta as in LAPACK's zlarfg; v to LDS and to
+// the reflector store, d_k, e_k = beta and tau to global memory.
+template <class ST>
+__device__ __forceinline__ void ht_house(HtWork& w, const ST& A, int k, cx* __restrict__ R,
+                                         real* __restrict__ dg, real* __restrict__ eg) {
+    const int lane = lane_id();
+    const int m0 = k + 1;
+    cx x[ND];
+    real xn2 = real(0.0);
+#pragma unroll
+    for (int q = 0; q < ND; ++q) {
+        const int r = 64 * q + lane;
+        x[q] = (r > m0 && r < N) ? A.at(r, k) : cm(0.0, 0.0);
+        xn2 += cabs2(x[q]);
+    }
+    xn2 = wave_sum(xn2);
+    const cx alpha = A.at(m0, k);
+    cx t = cm(0.0, 0.0), scal = cm(0.0, 0.0);
+    real beta = alpha.x;
+    if (!(xn2 == real(0.0) && alpha.y == real(0.0))) {
+        const real nrm = sqrt_r(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+        beta = (alpha.x >= real(0.0)) ? -nrm : nrm;
+        t = cm((beta - alpha.x) / beta, -alpha.y / beta);
+        const cx den = cm(alpha.x - beta, alpha.y);
+        const real dd = real(1.0) / cabs2(den);
+        scal = cm(den.x * dd, -den.y * dd);    // 1 / (alpha - beta)
+    }
+#pragma unroll
+    for (int q = 0; q < ND; ++q) {
+        const int r = 64 * q + lane;
+        const cx v = (r == m0) ? cm(1.0, 0.0) : cmul(x[q], scal);
+        if (r < N) w.vb[k & 1][r] = (r >= m0) ? v : cm(0.0, 0.0);
+        if (r > m0 && r < N) R[(size_t)k * N + r] = v;
+    }
+    if (lane == 0) {
+        R[(size_t)k * N + m0] = t;
+        dg[k] = A.at(k, k).x;
+        eg[k] = beta;
+        w.tau[k & 1] = t;
+    }
+}
+
+// Step k: A' -= v w^H + w v^H on the trailing rows/columns m0 = k+1.. with
+// w = p - (tau/2)(p^H v) v, p = tau A' v. Column j belongs to wave j % LNW
+// (lane = row, ND rows per lane). The product reads the lower triangle once:
+// A(r, j) v_j goes to row r (per-lane sums, combined across waves in LDS)
+// and conj(A(r, j)) v_r, r > j, to row j (a wave sum). The owner of column
+// k+1 forms the next reflector right after its updates.
+template <class ST>
+__device__ __forceinline__ void ht_step(HtWork& w, const ST& A, int k, cx* __restrict__ R,
+                                        real* __restrict__ dg, real* __restrict__ eg) {
+    const int lane = lane_id(), wave = wave_id();
+    const int m0 = k + 1, p = k & 1;
+    const cx t = w.tau[p];
+    const int jfirst = m0 + (((wave - m0) % LNW) + LNW) % LNW;
+    if (t.x != real(0.0) || t.y != real(0.0)) {
+        cx vr[ND], acc[ND];
+#pragma unroll
+        for (int q = 0; q < ND; ++q) {
+            const int r = 64 * q + lane;
+            vr[q] = (r < N) ? w.vb[p][r] : cm(0.0, 0.0);
+            acc[q] = cm(0.0, 0.0);
+        }
+        for (int j = jfirst; j < N; j += LNW) {
+            const cx vj = w.vb[p][j];
+            cx s = cm(0.0, 0.0);
+#pragma unroll
+            for (int q = 0; q < ND; ++q) {
+                const int r = 64 * q + lane;
+                if (r >= j && r < N) {
+                    const cx a = A.at(r, j);
+                    acc[q] = acc[q] + cmul(a, vj);
+                    if (r > j) s = s + cconjmul(a, vr[q]);
+                }
+            }
+            s = wave_sum_cx(s);
+            if (lane == 0) w.pc[j] = s;
+        }
+#pragma unroll
+        for (int q = 0; q < ND; ++q) {
+            const int r = 64 * q + lane;
+            if (r < N) w.part[wave][r] = acc[q];
+        }
+        __syncthreads();
+        cx wr[ND];
+        cx pv = cm(0.0, 0.0);
+#pragma unroll
+        for (int q = 0; q < ND; ++q) {
+            const int r = 64 * q + lane;
+            cx pr = cm(0.0, 0.0);
+            if (r >= m0 && r < N) {
+                pr = w.pc[r];
+#pragma unroll
+                for (int u = 0; u < LNW; ++u) pr = pr + w.part[u][r];
+                pr = cmul(t, pr);
+            }
+            pv = pv + cconjmul(pr, vr[q]);
+            wr[q] = pr;
+        }
+        pv = wave_sum_cx(pv);
+        const cx cc = cmul(t, pv) * real(-0.5);
+#pragma unroll
+        for (int q = 0; q < ND; ++q) {
+            wr[q] = wr[q] + cmul(cc, vr[q]);
+            const int r = 64 * q + lane;
+            if (r < N && (r % LNW) == wave) w.wb[r] = wr[q];   // entries of this wave's columns
+        }
+        wave_lds_sync();
+        for (int j = jfirst; j < N; j += LNW) {
+            const cx vj = w.vb[p][j], wj = w.wb[j];
+#pragma unroll
+            for (int q = 0; q < ND; ++q) {
+                const int r = 64 * q + lane;
+                if (r >= j && r < N) A.at(r, j) = A.at(r, j) - cmulc(vr[q], wj) - cmulc(wr[q], vj);
+            }
+        }
+    }
+    if (k + 1 < N - 1 && wave == (k + 1) % LNW) ht_house(w, A, k + 1, R, dg, eg);
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// QL (module precision).
+// ---------------------------------------------------------------------------
+constexpr int QNT = 64 * ND;                  // one row of Z per thread
+constexpr size_t QL_FIXED = (size_t)ND * 2 * N * sizeof(real) + (size_t)N * sizeof(int) + 64;
+constexpr int ql_lds_waves() {
+    int wv = ND;
+    while (wv > 0 && (size_t)(64 * wv < N ? 64 * wv : N) * N * sizeof(real) + QL_FIXED > LDS_CAP) --wv;
+    return wv;
+}
+constexpr int QW_LDS = ql_lds_waves();        // waves whose rows of Z live in LDS
+constexpr int ZR_LDS = (64 * QW_LDS < N) ? 64 * QW_LDS : N;   // rows of Z in LDS (= its leading dimension)
+
+// Apply the rotations (c_i, s_i), i = m-1 down to l, of one sweep to a row of
+// Z whose element c sits at z[c * LD]. Loads run four steps ahead.
+template <int LD, class ZP>
+__device__ __forceinline__ void ql_apply(ZP z, const real* rc, const real* rs, int l, int m) {
+    real zc = z[m * LD];
+    int i = m - 1;
+    for (; i - 3 >= l; i -= 4) {
+        const real z0 = z[i * LD], z1 = z[(i - 1) * LD], z2 = z[(i - 2) * LD], z3 = z[(i - 3) * LD];
+        real c = rc[i], s = rs[i];
+        z[(i + 1) * LD] = fma(s, z0, c * zc);
+        zc = fma(c, z0, -(s * zc));
+        c = rc[i - 1];
+        s = rs[i - 1];
+        z[i * LD] = fma(s, z1, c * zc);
+        zc = fma(c, z1, -(s * zc));
+        c = rc[i - 2];
+        s = rs[i - 2];
+        z[(i - 1) * LD] = fma(s, z2, c * zc);
+        zc = fma(c, z2, -(s * zc));
+        c = rc[i - 3];
+        s = rs[i - 3];
+        z[(i - 2) * LD] = fma(s, z3, c * zc);
+        zc = fma(c, z3, -(s * zc));
+    }
+    for (; i >= l; --i) {
+        const real zi = z[i * LD];
+        const real c = rc[i], s = rs[i];
+        z[(i + 1) * LD] = fma(s, zi, c * zc);
+        zc = fma(c, zi, -(s * zc));
+    }
+    z[l * LD] = zc;
+}
+
+// ---------------------------------------------------------------------------
+// Complex FP64 products on the matrix cores (FP64 module only).
+// v_mfma_f64_16x16x4f64 takes A(i, k) from lane i + 16 k and B(k, j) from
+// lane j + 16 k (i, j < 16, k < 4) and leaves D(i, j) in lane j + 16 (i % 4),
+// element i / 4. A 16 x 16 complex tile is a pair of such accumulators.
+// ---------------------------------------------------------------------------
+#ifndef GENEIG_FP32
+typedef double d4 __attribute__((ext_vector_type(4)));
+struct ctile {
+    d4 re, im;
+};
+__device__ __forceinline__ void cmfma(ctile& c, cx a, cx b) {
+    c.re = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, c.re, 0, 0, 0);
+    c.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, c.re, 0, 0, 0);
+    c.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, c.im, 0, 0, 0);
+    c.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, c.im, 0, 0, 0);
+}
+__device__ __forceinline__ cx tile_get(const ctile& t, int v) { return cm(t.re[v], t.im[v]); }
+
+// Element (i, k) of an N x N column-major matrix in global memory, zero
+// outside; the address is clamped so the load is branch-free.
+__device__ __forceinline__ cx gel(const cx* __restrict__ M, int i, int k) {
+    const bool in = i < N && k < N;
+    const cx v = M[(size_t)(in ? k : 0) * N + (in ? i : 0)];
+    return in ? v : cm(0.0, 0.0);
+}
+
+// acc[g] = sum over k in [kb, ke) of A(i, k) B(k, j) for row tile rt0 + g and
+// column tile ct; fa / fb return zero outside N. kb, ke multiples of 4.
+template <int RG, class FA, class FB>
+__device__ __forceinline__ void gemm_rg(ctile (&acc)[RG], FA fa, FB fb, int rt0, int ct, int kb, int ke) {
+    const int l = lane_id();
+    const int c16 = l & 15, k4 = l >> 4;
+#pragma unroll
+    for (int g = 0; g < RG; ++g) {
+        acc[g].re = d4{0.0, 0.0, 0.0, 0.0};
+        acc[g].im = d4{0.0, 0.0, 0.0, 0.0};
+    }
+    for (int k0 = kb; k0 < ke; k0 += 4) {
+        const cx b = fb(k0 + k4, ct * 16 + c16);
+#pragma unroll
+        for (int g = 0; g < RG; ++g) cmfma(acc[g], fa((rt0 + g) * 16 + c16, k0 + k4), b);
+    }
+}
+// Row of element v of a result tile held by this lane (column: ct * 16 + lane % 16).
+__device__ __forceinline__ int tile_row(int rt, int v) { return rt * 16 + 4 * v + (lane_id() >> 4); }
+
+struct CholWork {
+    cx pan[256 * NB];         // panel rows j0.. (row i - j0), 16 columns; trtri: Y, then X
+    cx lii[256];              // trtri: diagonal block of L
+    real rinv[16];
+    int fail;
+};
+#endif  // !GENEIG_FP32
+
+}  // namespace gk
+
+using gk::cx;
+using gk::real;
+
+#ifndef GENEIG_FP32
+// S = L L^H and W = L^-1 in the workspace (upper triangle zero). On a
+// non-positive pivot k (0-based) of a leading minor: info = N + k + 1, W NaN,
+// U zero (LAPACK zhegv's convention), and the later stages skip the matrix.
+extern "C" __global__ void __launch_bounds__(gk::LNT)
+geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ U, int* __restrict__ info,
+               char* __restrict__ ws, long long wss, int batch) {
+    using namespace gk;
+    __shared__ CholWork cw;
+    const int b = blockIdx.x;
+    if (b >= batch) return;
+    const int tid = threadIdx.x, wave = wave_id(), lane = lane_id();
+    cx* Lg = reinterpret_cast<cx*>(ws_of(ws, wss, b) + WS_WI);
+    const cx* Sb = S + (size_t)b * NN;
+    for (int e = tid; e < (int)NN; e += LNT)
+        if (e % N < e / N) Lg[e] = cm(0.0, 0.0);
+    if (tid == 0) cw.fail = 0;
+    __syncthreads();
+    auto fL = [&](int i, int k) { return gel(Lg, i, k); };
+    for (int p = 0; p < NB; ++p) {
+        const int j0 = 16 * p;
+        const int jw = (N - j0 < 16) ? N - j0 : 16;
+        // Panel rows j0.. of columns j0 .. j0+jw: S minus the columns left of it.
+        for (int rt = p + wave; rt < NB; rt += LNW) {
+            ctile acc[1];
+            gemm_rg<1>(acc, fL, [&](int k, int c) { return cconj(c < jw ? gel(Lg, j0 + c, k) : cm(0.0, 0.0)); },
+                       rt, 0, 0, j0);
+            const int c = lane & 15;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = tile_row(rt, v);
+                if (i < N && c < jw) cw.pan[c * N + (i - j0)] = Sb[(size_t)(j0 + c) * N + i] - tile_get(acc[0], v);
+            }
+        }
+        __syncthreads();
+        // Unblocked Cholesky of the panel in LDS, thread t = row j0 + t
+        // (the panel row t is only ever touched by thread t, except that
+        // step c reads the pivot column's rows c2 < jw after a barrier).
+        const int t = tid;
+        const bool tin = t < N - j0;
+        int fail = 0;
+        for (int c = 0; c < jw; ++c) {
+            if (t == c) {
+                const real piv = cw.pan[c * N + c].x;
+                if (!(piv > real(0.0))) {
+                    cw.fail = j0 + c + 1;
+                } else {
+                    const real sq = sqrt_r(piv);
+                    cw.rinv[c] = real(1.0) / sq;
+                    cw.pan[c * N + c] = cm(sq, 0.0);
+                }
+            }
+            __syncthreads();
+            fail = cw.fail;
+            if (fail) break;
+            if (t > c && tin) cw.pan[c * N + t] = cw.pan[c * N + t] * cw.rinv[c];
+            __syncthreads();
+            if (t > c && tin) {
+                const cx ltc = cw.pan[c * N + t];
+                for (int c2 = c + 1; c2 < jw && c2 <= t; ++c2)
+                    cw.pan[c2 * N + t] = cw.pan[c2 * N + t] - cmulc(ltc, cw.pan[c * N + c2]);
+            }
+        }
+        if (fail) {
+            real* Wb = W + (size_t)b * N;
+            cx* Ub = U + (size_t)b * NN;
+            for (int e = tid; e < (int)NN; e += LNT) Ub[e] = cm(0.0, 0.0);
+            for (int e = tid; e < N; e += LNT) Wb[e] = __builtin_nan("");
+            if (tid == 0) info[b] = N + fail;
+            return;
+        }
+        for (int c = 0; c < jw; ++c)
+            if (tin && t >= c) Lg[(size_t)(j0 + c) * N + j0 + t] = cw.pan[c * N + t];
+        __syncthreads();
+    }
+    // W = L^-1 in place, block row ib after block row ib-1: with X = W,
+    // X(ib, j) = L_ii^-1 (delta - sum_{k < r0} L(ib, k) X(k, j)).
+    for (int ib = 0; ib < NB; ++ib) {
+        const int r0 = 16 * ib;
+        const int rw = (N - r0 < 16) ? N - r0 : 16;
+        for (int e = tid; e < 256; e += LNT) {
+            const int rr = e & 15, c = e >> 4;
+            cw.lii[e] = (rr < rw && c < rw) ? Lg[(size_t)(r0 + c) * N + r0 + rr] : cm(0.0, 0.0);
+        }
+        for (int ct = wave; ct < ib; ct += LNW) {
+            ctile acc[1];
+            gemm_rg<1>(acc, fL, fL, ib, ct, 16 * ct, r0);
+            const int j = ct * 16 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = tile_row(ib, v);
+                if (i < N) cw.pan[j * 16 + (i - r0)] = tile_get(acc[0], v);
+            }
+        }
+        __syncthreads();
+        // Forward substitution with L_ii, thread j = column j, in place in
+        // its 16 entries of pan.
+        const int j = tid;
+        if (j < r0 + rw) {
+            cx* xj = cw.pan + j * 16;
+            for (int rr = 0; rr < rw; ++rr) {
+                cx s = (j < r0) ? cm(0.0, 0.0) - xj[rr] : cm((rr == j - r0) ? 1.0 : 0.0, 0.0);
+                for (int ss = 0; ss < rr; ++ss) s = s - cmul(cw.lii[ss * 16 + rr], xj[ss]);
+                xj[rr] = s * (real(1.0) / cw.lii[rr * 16 + rr].x);
+            }
+            for (int rr = 0; rr < rw; ++rr) Lg[(size_t)j * N + r0 + rr] = xj[rr];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) info[b] = 0;
+}
+
+// M = W H W^H: X = W H (W lower triangular: k <= row), then the lower tiles
+// of X W^H, mirrored so that M is exactly Hermitian.
+extern "C" __global__ void __launch_bounds__(gk::LNT)
+geneig_lg_formm(const cx* __restrict__ H, const int* __restrict__ info, char* __restrict__ ws, long long wss,
+                int batch) {
+    using namespace gk;
+    const int b = blockIdx.x;
+    if (b >= batch || info[b] > N) return;
+    const int wave = wave_id(), lane = lane_id();
+    char* wsb = ws_of(ws, wss, b);
+    const cx* Wi = reinterpret_cast<const cx*>(wsb + WS_WI);
+    cx* X = reinterpret_cast<cx*>(wsb + WS_X);
+    cx* M = reinterpret_cast<cx*>(wsb + WS_M);
+    const cx* Hb = H + (size_t)b * NN;
+    constexpr int NRG = (NB + 3) / 4;
+    for (int t = wave; t < NRG * NB; t += LNW) {
+        const int rt0 = 4 * (t / NB), ct = t % NB;
+        const int ke = (16 * (rt0 + 4) < KEND) ? 16 * (rt0 + 4) : KEND;
+        ctile acc[4];
+        gemm_rg<4>(acc, [&](int i, int k) { return gel(Wi, i, k); }, [&](int k, int j) { return gel(Hb, k, j); },
+                   rt0, ct, 0, ke);
+        const int j = ct * 16 + (lane & 15);
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = tile_row(rt0 + g, v);
+                if (i < N && j < N) X[(size_t)j * N + i] = tile_get(acc[g], v);
+            }
+    }
+    __syncthreads();
+    int t = 0;
+    for (int ct = 0; ct < NB; ++ct) {
+        const int ke = (16 * (ct + 1) < KEND) ? 16 * (ct + 1) : KEND;
+        for (int rt0 = ct; rt0 < NB; rt0 += 4, ++t) {
+            if (t % LNW != wave) continue;
+            ctile acc[4];
+            gemm_rg<4>(acc, [&](int i, int k) { return gel(X, i, k); },
+                       [&](int k, int jj) { return cconj(gel(Wi, jj, k)); }, rt0, ct, 0, ke);
+            const int j = ct * 16 + (lane & 15);
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = tile_row(rt0 + g, v);
+                    if (i < N && j < N && i >= j) {
+                        const cx m = tile_get(acc[g], v);
+                        if (i == j) {
+                            M[(size_t)j * N + i] = cm(m.x, 0.0);
+                        } else {
+                            M[(size_t)j * N + i] = m;
+                            M[(size_t)i * N + j] = cconj(m);
+                        }
+                    }
+                }
+        }
+    }
+}
+
+// U = W^H V (V in FP64 from the workspace); W^H is upper triangular, so row
+// tile rt needs k >= 16 rt only.
+extern "C" __global__ void __launch_bounds__(gk::LNT)
+geneig_lg_backtri(cx* __restrict__ U, const int* __restrict__ info, char* __restrict__ ws, long long wss,
+                  int batch) {
+    using namespace gk;
+    const int b = blockIdx.x;
+    if (b >= batch || info[b] > N) return;
+    const int wave = wave_id(), lane = lane_id();
+    char* wsb = ws_of(ws, wss, b);
+    const cx* Wi = reinterpret_cast<const cx*>(wsb + WS_WI);
+    const cx* V = reinterpret_cast<const cx*>(wsb + WS_V);
+    cx* Ub = U + (size_t)b * NN;
+    constexpr int NRG = (NB + 3) / 4;
+    for (int t = wave; t < NRG * NB; t += LNW) {
+        const int rt0 = 4 * (t / NB), ct = t % NB;
+        ctile acc[4];
+        gemm_rg<4>(acc, [&](int i, int k) { return cconj(gel(Wi, k, i)); }, [&](int k, int j) { return gel(V, k, j); },
+                   rt0, ct, 16 * rt0, KEND);
+        const int j = ct * 16 + (lane & 15);
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = tile_row(rt0 + g, v);
+                if (i < N && j < N) Ub[(size_t)j * N + i] = tile_get(acc[g], v);
+            }
+    }
+}
+#endif  // !GENEIG_FP32
+
+// Tridiagonalisation of M (read as FP64, reduced in the module precision):
+// d, e and the reflectors to the workspace.
+extern "C" __global__ void __launch_bounds__(gk::LNT)
+geneig_lg_hetrd(const int* __restrict__ info, char* __restrict__ ws, long long wss, int batch) {
+    using namespace gk;
+    __shared__ HtWork w;
+    const int b = blockIdx.x;
+    if (b >= batch || info[b] > N) return;
+    const int tid = threadIdx.x;
+    char* wsb = ws_of(ws, wss, b);
+    const cd* Mg = reinterpret_cast<const cd*>(wsb + WS_M);
+    cx* R = reinterpret_cast<cx*>(wsb + WS_R);
+    real* dg = reinterpret_cast<real*>(wsb + WS_D);
+    real* eg = dg + N;
+    const HtLds L{w.P};
+    auto load_trailing = [&]() {
+        for (int e = tid; e < (int)NN; e += LNT) {
+            const int i = e % N, j = e / N;
+            if (j >= K0 && i >= j) w.P[pkl(j - K0) + i - j] = to_cx(Mg[e]);
+        }
+    };
+#ifndef GENEIG_FP32
+    if constexpr (K0 > 0) {
+        // Columns 0 .. K0-1 in place in global memory (the FP64 M is not needed
+        // after this stage in the FP64 modes).
+        const HtGlb G{reinterpret_cast<cx*>(wsb + WS_M)};
+        if (wave_id() == 0) ht_house(w, G, 0, R, dg, eg);
+        __syncthreads();
+        for (int k = 0; k < K0; ++k) ht_step(w, G, k, R, dg, eg);
+        load_trailing();
+        __syncthreads();
+    } else
+#endif
+    {
+        load_trailing();
+        __syncthreads();
+        if (wave_id() == 0) ht_house(w, L, 0, R, dg, eg);
+        __syncthreads();
+    }
+    for (int k = K0; k < N - 1; ++k) ht_step(w, L, k, R, dg, eg);
+    if (tid == 0) {
+        dg[N - 1] = L.at(N - 1, N - 1).x;
+        eg[N - 1] = real(0.0);
+    }
+}
+
+// Implicit QL on (d, e) with the rotations accumulated into Z (one row per
+// thread), then ascending order: eigenvalues to wout (matrix b at
+// wout + b * wout_stride) and Z with its columns in that order to the
+// workspace. A sweep that has not converged after QL_MAXIT iterations sets
+// info to the number of off-diagonals left (LAPACK's convention).
+extern "C" __global__ void __launch_bounds__(gk::QNT)
+geneig_lg_ql(real* __restrict__ wout, long long wout_stride, int* __restrict__ info, char* __restrict__ ws,
+             long long wss, int batch) {
+    using namespace gk;
+    __shared__ real zs[ZR_LDS > 0 ? ZR_LDS * N : 1];
+    __shared__ real rot[ND][2][N];
+    __shared__ int rank_s[N];
+    const int b = blockIdx.x;
+    if (b >= batch || info[b] > N) return;
+    const int lane = lane_id(), wave = wave_id();
+    const int r = wave * 64 + lane;
+    const bool rin = r < N;
+    const bool in_lds = wave < QW_LDS;
+    char* wsb = ws_of(ws, wss, b);
+    const real* dsrc = reinterpret_cast<const real*>(wsb + WS_D);
+    const real* esrc = dsrc + N;
+    real* zg = reinterpret_cast<real*>(wsb + WS_X);      // rows of Z beyond the LDS
+    real* zout = reinterpret_cast<real*>(wsb + WS_Z);
+    real dr[ND], er[ND];
+#pragma unroll
+    for (int q = 0; q < ND; ++q) {
+        const int i = 64 * q + lane;
+        dr[q] = (i < N) ? dsrc[i] : real(0.0);
+        er[q] = (i < N - 1) ? esrc[i] : real(0.0);
+    }
+    if (rin) {
+        if (in_lds) {
+            for (int c = 0; c < N; ++c) zs[c * ZR_LDS + r] = (c == r) ? real(1.0) : real(0.0);
+        } else {
+            for (int c = 0; c < N; ++c) zg[(size_t)c * N + r] = (c == r) ? real(1.0) : real(0.0);
+        }
+    }
+    real* rc = rot[wave][0];
+    real* rs = rot[wave][1];
+    const real eps = REAL_HALF_ULP;
+    int l = 0, iter = 0, unconverged = 0;
+    while (l < N) {
+        int m = N - 1;
+#pragma unroll
+        for (int q = ND - 1; q >= 0; --q) {
+            real dn = __shfl_down(dr[q], 1, 64);
+            if (q + 1 < ND) {
+                const real nx = bcast(dr[q + 1 < ND ? q + 1 : q], 0);
+                if (lane == 63) dn = nx;
+            }
+            const int i = 64 * q + lane;
+            const bool small = fabs(er[q]) <= eps * (fabs(dr[q]) + fabs(dn));
+            const unsigned long long mask = __ballot(small && i >= l && i < N - 1);
+            if (mask) m = 64 * q + __ffsll((long long)mask) - 1;
+        }
+        if (m == l) {
+            ++l;
+            iter = 0;
+            continue;
+        }
+        if (iter++ == QL_MAXIT) {
+#pragma unroll
+            for (int q = 0; q < ND; ++q) {
+                const int i = 64 * q + lane;
+                unconverged += __popcll(__ballot(er[q] != real(0.0) && i >= l && i < N - 1));
+            }
+            break;
+        }
+        const real d_l = dget(dr, l);
+        const real e_l = dget(er, l);
+        real g = (dget(dr, l + 1) - d_l) / (real(2.0) * e_l);
+        real rr = sqrt_r(g * g + real(1.0));
+        const real d_m = dget(dr, m);
+        g = d_m - d_l + e_l / (g + copysign(rr, g));
+        real s = real(1.0), c = real(1.0), p = real(0.0);
+        real d_up = d_m;
+        real cr[ND], sr[ND];
+#pragma unroll
+        for (int q = 0; q < ND; ++q) {
+            cr[q] = real(1.0);
+            sr[q] = real(0.0);
+        }
+#pragma unroll
+        for (int i = N - 2; i >= 0; --i) {
+            if (i < m && i >= l) {
+                const real e_i = bcast(er[i >> 6], i & 63);
+                const real d_i = bcast(dr[i >> 6], i & 63);
+                const real f = s * e_i;
+                const real bb = c * e_i;
+                const real tt = fmax(fma(f, f, g * g), REAL_TINY);
+                const real y = rsqrt_nr(tt);
+                er[(i + 1) >> 6] = sel_lane(er[(i + 1) >> 6], tt * y, (i + 1) & 63);
+                s = f * y;
+                c = g * y;
+                g = d_up - p;
+                const real r2 = fma(d_i - g, s, real(2.0) * c * bb);
+                p = s * r2;
+                dr[(i + 1) >> 6] = sel_lane(dr[(i + 1) >> 6], g + p, (i + 1) & 63);
+                g = fma(c, r2, -bb);
+                cr[i >> 6] = sel_lane(cr[i >> 6], c, i & 63);
+                sr[i >> 6] = sel_lane(sr[i >> 6], s, i & 63);
+                d_up = d_i;
+            }
+        }
+        dset(dr, l, d_l - p);
+        dset(er, l, g);
+        dset(er, m, real(0.0));
+#pragma unroll
+        for (int q = 0; q < ND; ++q) {
+            const int i = 64 * q + lane;
+            if (i < N) {
+                rc[i] = cr[q];
+                rs[i] = sr[q];
+            }
+        }
+        wave_lds_sync();
+        if (rin) {
+            if (in_lds) ql_apply<ZR_LDS>(zs + r, rc, rs, l, m);
+            else ql_apply<N>(zg + r, rc, rs, l, m);
+        }
+        wave_lds_sync();
+    }
+    // Ascending ranks (ties by index): wave q ranks eigenvalues 64 q + lane.
+    {
+        const int cidx = 64 * wave + lane;
+        real dc = dr[0];
+#pragma unroll
+        for (int q = 1; q < ND; ++q)
+            if (wave == q) dc = dr[q];
+        int rank = 0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const real dk = bcast(dr[k >> 6], k & 63);
+            rank += (dk < dc || (dk == dc && k < cidx)) ? 1 : 0;
+        }
+        if (cidx < N) {
+            rank_s[cidx] = rank;
+            wout[(size_t)b * (size_t)wout_stride + rank] = dc;
+        }
+    }
+    if (threadIdx.x == 0 && unconverged) info[b] = unconverged;
+    __syncthreads();
+    if (rin) {
+        if (in_lds) {
+            for (int c = 0; c < N; ++c) zout[(size_t)rank_s[c] * N + r] = zs[c * ZR_LDS + r];
+        } else {
+            for (int c = 0; c < N; ++c) zout[(size_t)rank_s[c] * N + r] = zg[(size_t)c * N + r];
+        }
+    }
+}
+
+// V = Q Z = H_0 H_1 ... H_{N-2} Z (reflectors applied last first), eight
+// columns per wave at a time in registers (lane = row, ND rows per lane):
+// per reflector a wave sum per column and a rank-1 update.
+extern "C" __global__ void __launch_bounds__(gk::LNT)
+geneig_lg_back(const int* __restrict__ info, char* __restrict__ ws, long long wss, int batch) {
+    using namespace gk;
+    constexpr int CB = 8;
+    constexpr int NCB = (N + CB - 1) / CB;
+    const int b = blockIdx.x;
+    if (b >= batch || info[b] > N) return;
+    const int lane = lane_id(), wave = wave_id();
+    char* wsb = ws_of(ws, wss, b);
+    const cx* R = reinterpret_cast<const cx*>(wsb + WS_R);
+    const real* Z = reinterpret_cast<const real*>(wsb + WS_Z);
+    cx* V = reinterpret_cast<cx*>(wsb + WS_V);
+    for (int cbk = wave; cbk < NCB; cbk += LNW) {
+        const int c0 = cbk * CB;
+        cx x[ND][CB];
+#pragma unroll
+        for (int q = 0; q < ND; ++q)
+#pragma unroll
+            for (int c = 0; c < CB; ++c) {
+                const int r = 64 * q + lane, col = c0 + c;
+                x[q][c] = cm((r < N && col < N) ? Z[(size_t)col * N + r] : real(0.0), 0.0);
+            }
+        for (int k = N - 2; k >= 0; --k) {
+            const cx tau = R[(size_t)k * N + k + 1];
+            if (tau.x == real(0.0) && tau.y == real(0.0)) continue;
+            cx v[ND];
+#pragma unroll
+            for (int q = 0; q < ND; ++q) {
+                const int r = 64 * q + lane;
+                v[q] = (r == k + 1) ? cm(1.0, 0.0) : ((r > k + 1 && r < N) ? R[(size_t)k * N + r] : cm(0.0, 0.0));
+            }
+#pragma unroll
+            for (int c = 0; c < CB; ++c) {
+                cx s = cm(0.0, 0.0);
+#pragma unroll
+                for (int q = 0; q < ND; ++q) s = s + cconjmul(v[q], x[q][c]);
+                s = cmul(tau, wave_sum_cx(s));
+#pragma unroll
+                for (int q = 0; q < ND; ++q) x[q][c] = x[q][c] - cmul(v[q], s);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < ND; ++q)
+#pragma unroll
+            for (int c = 0; c < CB; ++c) {
+                const int r = 64 * q + lane, col = c0 + c;
+                if (r < N && col < N) V[(size_t)col * N + r] = x[q][c];
+            }
+    }
+}
+
+#ifndef GENEIG_FP32
+namespace gk {
+// acc1[g] = A B1 and acc2[g] = A B2 over k in [kb, ke) (shared A loads).
+template <int RG, class FA, class FB1, class FB2>
+__device__ __forceinline__ void gemm2_rg(ctile (&acc1)[RG], ctile (&acc2)[RG], FA fa, FB1 fb1, FB2 fb2, int rt0,
+                                         int ct, int kb, int ke) {
+    const int l = lane_id();
+    const int c16 = l & 15, k4 = l >> 4;
+#pragma unroll
+    for (int g = 0; g < RG; ++g) {
+        acc1[g].re = d4{0.0, 0.0, 0.0, 0.0};
+        acc1[g].im = d4{0.0, 0.0, 0.0, 0.0};
+        acc2[g].re = d4{0.0, 0.0, 0.0, 0.0};
+        acc2[g].im = d4{0.0, 0.0, 0.0, 0.0};
+    }
+    for (int k0 = kb; k0 < ke; k0 += 4) {
+        const cx b1 = fb1(k0 + k4, ct * 16 + c16);
+        const cx b2 = fb2(k0 + k4, ct * 16 + c16);
+#pragma unroll
+        for (int g = 0; g < RG; ++g) {
+            const cx a = fa((rt0 + g) * 16 + c16, k0 + k4);
+            cmfma(acc1[g], a, b1);
+            cmfma(acc2[g], a, b2);
+        }
+    }
+}
+
+// Rayleigh-Ritz on columns (a, b) of V (global memory) for the Hermitian M
+// (global), by one wave (lane = row): the 2 x 2 projected pencil
+// (V^H M V, V^H V) of the pair, then V(:, [a b]) <- V(:, [a b]) X and
+// lam[a], lam[b] <- the Ritz values. The columns' entries travel between
+// lanes by v_readlane, never as wave-uniform loads of memory this kernel
+// writes.
+__device__ __forceinline__ void ritz_pair_g(const cx* __restrict__ M, cx* V, real* lam, int a, int b) {
+    const int lane = lane_id();
+    cx ua[ND], ub[ND], ha[ND], hb[ND];
+#pragma unroll
+    for (int q = 0; q < ND; ++q) {
+        const int r = 64 * q + lane;
+        ua[q] = (r < N) ? V[(size_t)a * N + r] : cm(0.0, 0.0);
+        ub[q] = (r < N) ? V[(size_t)b * N + r] : cm(0.0, 0.0);
+        ha[q] = cm(0.0, 0.0);
+        hb[q] = cm(0.0, 0.0);
+    }
+#pragma unroll
+    for (int qk = 0; qk < ND; ++qk) {
+        for (int kk = 0; kk < 64 && 64 * qk + kk < N; ++kk) {
+            const int k = 64 * qk + kk;
+            const cx va = bcast(ua[qk], kk), vb = bcast(ub[qk], kk);
+#pragma unroll
+            for (int q = 0; q < ND; ++q) {
+                const int r = 64 * q + lane;
+                const cx m = (r < N) ? M[(size_t)k * N + r] : cm(0.0, 0.0);
+                ha[q] = ha[q] + cmul(m, va);
+                hb[q] = hb[q] + cmul(m, vb);
+            }
+        }
+    }
+    real t11 = 0.0, t22 = 0.0, g11 = 0.0, g22 = 0.0;
+    cx t12 = cm(0.0, 0.0), g12 = cm(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < ND; ++q) {
+        t11 += cconjmul(ua[q], ha[q]).x;
+        t22 += cconjmul(ub[q], hb[q]).x;
+        t12 = t12 + cconjmul(ua[q], hb[q]);
+        g11 += cabs2(ua[q]);
+        g22 += cabs2(ub[q]);
+        g12 = g12 + cconjmul(ua[q], ub[q]);
+    }
+    t11 = wave_sum(t11);
+    t22 = wave_sum(t22);
+    t12 = wave_sum_cx(t12);
+    g11 = wave_sum(g11);
+    g22 = wave_sum(g22);
+    g12 = wave_sum_cx(g12);
+    real mu1, mu2;
+    cx X[2][2];
+    eig2_pencil(t11, t22, t12, g11, g22, g12, mu1, mu2, X);
+#pragma unroll
+    for (int q = 0; q < ND; ++q) {
+        const int r = 64 * q + lane;
+        if (r < N) {
+            V[(size_t)a * N + r] = cmul(ua[q], X[0][0]) + cmul(ub[q], X[1][0]);
+            V[(size_t)b * N + r] = cmul(ua[q], X[0][1]) + cmul(ub[q], X[1][1]);
+        }
+    }
+    if (lane == 0) {
+        lam[a] = mu1;
+        lam[b] = mu2;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+}  // namespace gk
+
+// Refinement of the FP32 eigenpairs in the Cholesky frame (mixed mode): with
+// W = L^-1, M = W H W^H (both FP64, in the workspace) and V the FP32
+// eigenvectors of M, `iters` Ogita-Aishima steps on the matrix cores
+// (P = M V, lambda_j = v_j^H p_j / v_j^H v_j, T = V^H P, G = V^H V,
+// E = -(G - I)/2 + A with A_ij = (T_ij - (lambda_i + lambda_j) G_ij / 2) /
+// (lambda_j - lambda_i) for pairs whose gap is at least delta and 0
+// otherwise, V <- V + V E), Rayleigh-Ritz on every cluster (a run of
+// eigenvalues with adjacent gaps below delta: one exact 2 x 2 step for a
+// pair, Jacobi sweeps over its pairs otherwise), then U = W^H V. In step it
+// (from 0), delta = 2^it delta_rel max |lambda|. Buffers: V in the scratch
+// and V slots alternately, P in the reflector slot, E in the caller's U.
+extern "C" __global__ void __launch_bounds__(gk::LNT)
+geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, const int* __restrict__ info, char* __restrict__ ws,
+                 long long wss, int batch, int iters, real delta_rel) {
+    using namespace gk;
+    __shared__ real lam[N];
+    __shared__ real wmax[LNW];
+    const int b = blockIdx.x;
+    if (b >= batch || info[b] > N) return;
+    const int tid = threadIdx.x, wave = wave_id(), lane = lane_id();
+    char* wsb = ws_of(ws, wss, b);
+    const cx* Wi = reinterpret_cast<const cx*>(wsb + WS_WI);
+    const cx* M = reinterpret_cast<const cx*>(wsb + WS_M);
+    cx* Va = reinterpret_cast<cx*>(wsb + WS_X);
+    cx* Vb = reinterpret_cast<cx*>(wsb + WS_V);
+    cx* P = reinterpret_cast<cx*>(wsb + WS_R);
+    cx* Ub = U + (size_t)b * NN;
+    cx* E = Ub;
+    const cf* V32 = reinterpret_cast<const cf*>(wsb + WS_V);
+    const float* lam32 = reinterpret_cast<const float*>(wsb + WS_D) + 2 * N;
+    for (int e = tid; e < (int)NN; e += LNT) Va[e] = to_cx(V32[e]);
+    for (int e = tid; e < N; e += LNT) lam[e] = (real)lam32[e];
+    __syncthreads();
+    constexpr int NRG = (NB + 3) / 4;
+    real delta = 0.0;
+    for (int it = 0; it < iters; ++it) {
+        const cx* V = Va;
+        // P = M V.
+        for (int t = wave; t < NRG * NB; t += LNW) {
+            const int rt0 = 4 * (t / NB), ct = t % NB;
+            ctile acc[4];
+            gemm_rg<4>(acc, [&](int i, int k) { return gel(M, i, k); }, [&](int k, int j) { return gel(V, k, j); },
+                       rt0, ct, 0, KEND);
+            const int j = ct * 16 + (lane & 15);
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = tile_row(rt0 + g, v);
+                    if (i < N && j < N) P[(size_t)j * N + i] = tile_get(acc[g], v);
+                }
+        }
+        __syncthreads();
+        // lambda_j = v_j^H p_j / v_j^H v_j, column j by wave j % LNW.
+        real mx = 0.0;
+        for (int j = wave; j < N; j += LNW) {
+            real tj = 0.0, gj = 0.0;
+#pragma unroll
+            for (int q = 0; q < ND; ++q) {
+                const int r = 64 * q + lane;
+                if (r < N) {
+                    const cx v = V[(size_t)j * N + r];
+                    tj += cconjmul(v, P[(size_t)j * N + r]).x;
+                    gj += cabs2(v);
+                }
+            }
+            const real lj = wave_sum(tj) / wave_sum(gj);
+            if (lane == 0) lam[j] = lj;
+            mx = fmax(mx, fabs(lj));
+        }
+        if (lane == 0) wmax[wave] = mx;
+        __syncthreads();
+        mx = wmax[0];
+#pragma unroll
+        for (int u = 1; u < LNW; ++u) mx = fmax(mx, wmax[u]);
+        // The threshold doubles every step, so a pair treated as close once
+        // stays close and ends in Rayleigh-Ritz.
+        delta = delta_rel * (real)(1 << it) * mx;
+        // E from the tiles of T = V^H P and G = V^H V.
+        for (int t = wave; t < NRG * NB; t += LNW) {
+            const int rt0 = 4 * (t / NB), ct = t % NB;
+            ctile T[4], G[4];
+            gemm2_rg<4>(T, G, [&](int i, int k) { return cconj(gel(V, k, i)); },
+                        [&](int k, int j) { return gel(P, k, j); }, [&](int k, int j) { return gel(V, k, j); }, rt0,
+                        ct, 0, KEND);
+            const int j = ct * 16 + (lane & 15);
+            const real lj = lam[j < N ? j : 0];
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = tile_row(rt0 + g, v);
+                    if (i < N && j < N) {
+                        const real li = lam[i];
+                        const cx tv = tile_get(T[g], v), gv = tile_get(G[g], v);
+                        cx e = gv * real(-0.5);
+                        const real gap = lj - li;
+                        if (i == j) e = cm(real(0.5) * (real(1.0) - gv.x), 0.0);
+                        else if (!(fabs(gap) < delta)) e = e + (tv - gv * (real(0.5) * (li + lj))) * (real(1.0) / gap);
+                        E[(size_t)j * N + i] = e;
+                    }
+                }
+        }
+        __syncthreads();
+        // V <- V + V E, into the other buffer.
+        for (int t = wave; t < NRG * NB; t += LNW) {
+            const int rt0 = 4 * (t / NB), ct = t % NB;
+            ctile acc[4];
+            gemm_rg<4>(acc, [&](int i, int k) { return gel(V, i, k); }, [&](int k, int j) { return gel(E, k, j); },
+                       rt0, ct, 0, KEND);
+            const int j = ct * 16 + (lane & 15);
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = tile_row(rt0 + g, v);
+                    if (i < N && j < N) Vb[(size_t)j * N + i] = gel(V, i, j) + tile_get(acc[g], v);
+                }
+        }
+        __syncthreads();
+        cx* tmp = Va;
+        Va = Vb;
+        Vb = tmp;
+    }
+    // Clusters: runs [c0, c1] with adjacent gaps below delta (wave 0).
+    if (wave == 0 && iters > 0) {
+        int c0 = 0;
+        while (c0 < N - 1) {
+            int c1 = c0;
+            while (c1 + 1 < N && fabs(lam[c1 + 1] - lam[c1]) < delta) ++c1;
+            if (c1 > c0) {
+                const int sweeps = (c1 == c0 + 1) ? 1 : 4;
+                for (int sw = 0; sw < sweeps; ++sw)
+                    for (int a = c0; a < c1; ++a)
+                        for (int bb = a + 1; bb <= c1; ++bb) ritz_pair_g(M, Va, lam, a, bb);
+                // Keep the run ascending.
+                for (int a = c0 + 1; a <= c1; ++a)
+                    for (int q = a; q > c0 && lam[q] < lam[q - 1]; --q) {
+                        const real tl = lam[q];
+#pragma unroll
+                        for (int qq = 0; qq < ND; ++qq) {
+                            const int r = 64 * qq + lane;
+                            if (r < N) {
+                                const cx t0 = Va[(size_t)q * N + r];
+                                Va[(size_t)q * N + r] = Va[(size_t)(q - 1) * N + r];
+                                Va[(size_t)(q - 1) * N + r] = t0;
+                            }
+                        }
+                        wave_lds_sync();
+                        if (lane == 0) {
+                            lam[q] = lam[q - 1];
+                            lam[q - 1] = tl;
+                        }
+                        wave_lds_sync();
+                    }
+            }
+            c0 = c1 + 1;
+        }
+    }
+    __syncthreads();
+    // U = W^H V.
+    const cx* V = Va;
+    for (int t = wave; t < NRG * NB; t += LNW) {
+        const int rt0 = 4 * (t / NB), ct = t % NB;
+        ctile acc[4];
+        gemm_rg<4>(acc, [&](int i, int k) { return cconj(gel(Wi, k, i)); }, [&](int k, int j) { return gel(V, k, j); },
+                   rt0, ct, 16 * rt0, KEND);
+        const int j = ct * 16 + (lane & 15);
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = tile_row(rt0 + g, v);
+                if (i < N && j < N) Ub[(size_t)j * N + i] = tile_get(acc[g], v);
+            }
+    }
+    for (int e = tid; e < N; e += LNT) W[(size_t)b * N + e] = lam[e];
+}
+#endif  // !GENEIG_FP32

@@ -65,7 +65,9 @@ ServingSession::ServingSession(int device, Mode mode, const std::vector<int>& or
     : device_(device), mode_(mode), max_count_(max_count), orders_(orders) {
     if (orders_.empty()) throw std::invalid_argument("no matrix orders to serve");
     for (int n : orders_)
-        if (n < 1 || n > 64) throw std::invalid_argument("order " + std::to_string(n) + " outside [1, 64]");
+        if (n < 1 || n > Solver::kMaxOrder)
+            throw std::invalid_argument("order " + std::to_string(n) + " outside [1, " +
+                                        std::to_string(Solver::kMaxOrder) + "]");
     if (max_count < 1) throw std::invalid_argument("max_count must be >= 1");
     if (slots < 1 || slots > kMaxSlots)
         throw std::invalid_argument("slots must be in [1, " + std::to_string(kMaxSlots) + "]");

@@ -17,7 +17,8 @@ namespace sgeig {
 
 namespace {
 
-constexpr int kMaxLdsOrder = 64;   // LDS-resident kernels hold one row per lane
+// Workspace of one round of the staged kernels (orders above 64).
+constexpr size_t kLargeRoundBytes = size_t(16) << 30;
 
 // Persisted autotune winners: "<n> <arch> <block_dim>" lines in
 // <cache_dir>/autotune.tsv. The reference memoises per process only
@@ -95,10 +96,10 @@ int Solver::suggested_block_dim(int n) {
 Solver::Solver(int n, int device_id, int bpb_request, Tuning tuning, CacheMode cache, Mode mode)
     : n_(n), device_(device_id), mode_(mode) {
     if (n < 1) throw std::invalid_argument("matrix size must be >= 1");
-    if (n > kMaxLdsOrder)
-        throw std::invalid_argument("matrix size " + std::to_string(n) +
-                                    " exceeds the LDS-resident kernel limit of " +
-                                    std::to_string(kMaxLdsOrder));
+    if (n > kMaxOrder)
+        throw std::invalid_argument("matrix size " + std::to_string(n) + " exceeds the supported maximum of " +
+                                    std::to_string(kMaxOrder));
+    large_ = n > kMaxLdsOrder;
     if (bpb_request > 4)
         throw std::invalid_argument("batches_per_block must be <= 4 (one wave per matrix, "
                                     "LDS-resident work area)");
@@ -111,7 +112,9 @@ Solver::Solver(int n, int device_id, int bpb_request, Tuning tuning, CacheMode c
     // A workgroup of W waves solves W matrices (one QL chain per wave), so a
     // requested matrices-per-block count fixes the workgroup size.
     int bd = (bpb_request > 0) ? 64 * bpb_request : suggested_block_dim(n);
-    if (mode == Mode::Unified && tuning == Tuning::Autotune && bpb_request <= 0) {
+    // Staged kernels: one matrix per 256-thread workgroup, nothing to tune.
+    if (large_) bd = 256;
+    if (mode == Mode::Unified && tuning == Tuning::Autotune && bpb_request <= 0 && !large_) {
         bd = autotune_(cache);
         autotuned_ = true;
     }
@@ -120,12 +123,13 @@ Solver::Solver(int n, int device_id, int bpb_request, Tuning tuning, CacheMode c
 
 Solver::~Solver() {
     for (auto& e : ev_) (void)hipEventDestroy(e);
-    if (lbuf_ || mbuf_) {
+    if (lbuf_ || mbuf_ || gbuf_) {
         int cur = 0;
         (void)hipGetDevice(&cur);
         (void)hipSetDevice(device_);
         if (lbuf_) (void)hipFree(lbuf_);
         if (mbuf_) (void)hipFree(mbuf_);
+        if (gbuf_) (void)hipFree(gbuf_);
         (void)hipSetDevice(cur);
     }
 }
@@ -135,6 +139,36 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
     module_ = acquire_module(key, cache, &layer_, &acquire_ms_);
     block_dim_ = block_dim;
     bpb_ = block_dim / 64;
+    if (large_) {
+        bpb_ = 1;
+        k_lg_chol_ = module_->get("geneig_lg_chol");
+        k_lg_formm_ = module_->get("geneig_lg_formm");
+        k_lg_backtri_ = module_->get("geneig_lg_backtri");
+        hipFunction_t lds_probe = nullptr;
+        if (mode_ == Mode::Mixed) {
+            ModuleKey k32 = key;
+            k32.fp32 = true;
+            CacheLayer l32;
+            double ms32 = 0.0;
+            module32_ = acquire_module(k32, cache, &l32, &ms32);
+            acquire_ms_ += ms32;
+            k_lg_hetrd_ = module32_->get("geneig_lg_hetrd");
+            k_lg_ql_ = module32_->get("geneig_lg_ql");
+            k_lg_back_ = module32_->get("geneig_lg_back");
+            k_lg_refine_ = module_->get("geneig_lg_refine");
+            if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_ITERS")) refine_iters_ = std::max(0, std::atoi(v));
+            if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_DELTA")) refine_delta_ = std::atof(v);
+        } else {
+            k_lg_hetrd_ = module_->get("geneig_lg_hetrd");
+            k_lg_ql_ = module_->get("geneig_lg_ql");
+            k_lg_back_ = module_->get("geneig_lg_back");
+        }
+        lds_probe = k_lg_hetrd_;
+        int lds = 0;
+        SG_HIP(hipFuncGetAttribute(&lds, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, lds_probe));
+        lds_bytes_ = (unsigned)lds;
+        return;
+    }
     fused_ = module_->get("geneig_fused");
     k_reduce_ = module_->get("geneig_split_front");
     k_heev_ = module_->get("geneig_split_ql");
@@ -253,7 +287,8 @@ void Solver::launch(const void* H, const void* S, double* W, void* U, int* info,
     if (cur != device_) SG_HIP(hipSetDevice(device_));
     ev_used_ = 0;
     mark_(stream);
-    if (mode_ == Mode::Unified) launch_unified_(H, S, W, U, info, batch, stream);
+    if (large_) launch_large_(H, S, W, U, info, batch, stream);
+    else if (mode_ == Mode::Unified) launch_unified_(H, S, W, U, info, batch, stream);
     else if (mode_ == Mode::Hybrid) launch_hybrid_(H, S, W, U, info, batch, stream);
     else launch_mixed_(H, S, W, U, info, batch, stream);
     if (cur != device_) SG_HIP(hipSetDevice(cur));
@@ -354,7 +389,85 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
     mark_(stream);
 }
 
+size_t Solver::large_matrix_bytes(int n_) {
+    const size_t n = (size_t)n_;
+    return (88 * n * n + 24 * n + 255) / 256 * 256;
+}
+
+size_t Solver::large_chunk(int n) { return std::max<size_t>(1, kLargeRoundBytes / large_matrix_bytes(n)); }
+
+void Solver::ensure_large_workspace_(size_t batch) {
+    batch = std::min(batch, large_chunk(n_));
+    if (batch <= gbuf_batch_) return;
+    if (gbuf_) {
+        SG_HIP(hipDeviceSynchronize());
+        SG_HIP(hipFree(gbuf_));
+        gbuf_ = nullptr;
+        gbuf_batch_ = 0;
+    }
+    SG_HIP(hipMalloc(&gbuf_, batch * large_matrix_bytes(n_)));
+    gbuf_batch_ = batch;
+}
+
+// Orders above 64, every mode: the staged kernels, one workgroup per matrix,
+// in rounds of at most large_chunk(n) matrices. FP64 modes: Cholesky and
+// L^-1, M = W H W^H, tridiagonalisation, QL, V = Q Z and U = W^H V, all in
+// FP64. Mixed: the middle three in FP32, then FP64 refinement of (M, V).
+void Solver::launch_large_(const void* H, const void* S, double* W, void* U, int* info, int batch,
+                           hipStream_t stream) {
+    ensure_large_workspace_((size_t)batch);
+    const size_t n = (size_t)n_, nn = n * n;
+    const long long wss = (long long)large_matrix_bytes(n_);
+    const unsigned bd = 256, bd_ql = (unsigned)(64 * ((n_ + 63) / 64));
+    const int chunk = (int)gbuf_batch_;
+    char* ws = static_cast<char*>(gbuf_);
+    for (int b0 = 0; b0 < batch; b0 += chunk) {
+        int cnt = std::min(chunk, batch - b0);
+        const void* Hc = static_cast<const char*>(H) + (size_t)b0 * nn * 16;
+        const void* Sc = static_cast<const char*>(S) + (size_t)b0 * nn * 16;
+        double* Wc = W + (size_t)b0 * n;
+        void* Uc = static_cast<char*>(U) + (size_t)b0 * nn * 16;
+        int* Ic = info + b0;
+        const bool marks = (b0 == 0);
+        auto mark = [&]() {
+            if (marks) mark_(stream);
+        };
+        void* a1[] = {&Sc, &Wc, &Uc, &Ic, &ws, (void*)&wss, &cnt};
+        SG_HIP(hipModuleLaunchKernel(k_lg_chol_, (unsigned)cnt, 1, 1, bd, 1, 1, 0, stream, a1, nullptr));
+        mark();
+        void* a2[] = {&Hc, &Ic, &ws, (void*)&wss, &cnt};
+        SG_HIP(hipModuleLaunchKernel(k_lg_formm_, (unsigned)cnt, 1, 1, bd, 1, 1, 0, stream, a2, nullptr));
+        mark();
+        void* a3[] = {&Ic, &ws, (void*)&wss, &cnt};
+        SG_HIP(hipModuleLaunchKernel(k_lg_hetrd_, (unsigned)cnt, 1, 1, bd, 1, 1, 0, stream, a3, nullptr));
+        mark();
+        // QL eigenvalues: the caller's W (FP64), or the workspace's FP32 slot.
+        void* wout = Wc;
+        long long wstride = (long long)n;
+        if (mode_ == Mode::Mixed) {
+            wout = ws + 88 * nn + 8 * n;
+            wstride = wss / 4;
+        }
+        void* a4[] = {&wout, &wstride, &Ic, &ws, (void*)&wss, &cnt};
+        SG_HIP(hipModuleLaunchKernel(k_lg_ql_, (unsigned)cnt, 1, 1, bd_ql, 1, 1, 0, stream, a4, nullptr));
+        mark();
+        SG_HIP(hipModuleLaunchKernel(k_lg_back_, (unsigned)cnt, 1, 1, bd, 1, 1, 0, stream, a3, nullptr));
+        mark();
+        if (mode_ == Mode::Mixed) {
+            int iters = refine_iters_;
+            double delta = refine_delta_;
+            void* a6[] = {&Wc, &Uc, &Ic, &ws, (void*)&wss, &cnt, &iters, &delta};
+            SG_HIP(hipModuleLaunchKernel(k_lg_refine_, (unsigned)cnt, 1, 1, bd, 1, 1, 0, stream, a6, nullptr));
+        } else {
+            void* a6[] = {&Uc, &Ic, &ws, (void*)&wss, &cnt};
+            SG_HIP(hipModuleLaunchKernel(k_lg_backtri_, (unsigned)cnt, 1, 1, bd, 1, 1, 0, stream, a6, nullptr));
+        }
+        mark();
+    }
+}
+
 size_t Solver::workspace_bytes(Mode mode, int n_, size_t batch) {
+    if (n_ > kMaxLdsOrder) return std::min(batch, large_chunk(n_)) * large_matrix_bytes(n_);
     const size_t n = (size_t)n_, npk = n * (n + 1) / 2;
     if (mode == Mode::Hybrid) return batch * (n + n * n) * 8;
     if (mode == Mode::Mixed) return batch * (npk * 16 + n * n * 8 + n * 4 + n * 4 + n * n * 4);
@@ -365,8 +478,9 @@ size_t Solver::reserve(size_t batch) {
     int cur = 0;
     SG_HIP(hipGetDevice(&cur));
     if (cur != device_) SG_HIP(hipSetDevice(device_));
-    if (mode_ == Mode::Hybrid) ensure_workspace_(batch);
-    if (mode_ == Mode::Mixed) ensure_mixed_workspace_(batch);
+    if (large_) ensure_large_workspace_(batch);
+    else if (mode_ == Mode::Hybrid) ensure_workspace_(batch);
+    else if (mode_ == Mode::Mixed) ensure_mixed_workspace_(batch);
     if (cur != device_) SG_HIP(hipSetDevice(cur));
     return workspace_bytes(mode_, n_, batch);
 }
@@ -397,7 +511,15 @@ int Solver::stage_times(double* ms, int n) {
     return stages;
 }
 
-const char* Solver::stage_name(Mode mode, int i) {
+const char* Solver::stage_name(Mode mode, int i, int n) {
+    static const char* large64[] = {"geneig_lg_chol", "geneig_lg_formm", "geneig_lg_hetrd", "geneig_lg_ql",
+                                    "geneig_lg_back", "geneig_lg_backtri"};
+    static const char* large_mx[] = {"geneig_lg_chol", "geneig_lg_formm", "geneig_lg_hetrd", "geneig_lg_ql",
+                                     "geneig_lg_back", "geneig_lg_refine"};
+    if (n > kMaxLdsOrder) {
+        if (i < 0 || i >= 6) return "";
+        return (mode == Mode::Mixed) ? large_mx[i] : large64[i];
+    }
     static const char* unified[] = {"geneig_fused"};
     static const char* hybrid[] = {"geneig_split_front", "geneig_split_ql", "geneig_split_back"};
     static const char* mixed[] = {"geneig_mx_chol", "geneig_mx_front", "geneig_split_ql",

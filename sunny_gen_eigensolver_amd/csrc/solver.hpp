@@ -50,8 +50,11 @@ private:
                         hipStream_t stream);
     void launch_mixed_(const void* H, const void* S, double* W, void* U, int* info, int batch,
                        hipStream_t stream);
+    void launch_large_(const void* H, const void* S, double* W, void* U, int* info, int batch,
+                       hipStream_t stream);
     void ensure_workspace_(size_t batch);
     void ensure_mixed_workspace_(size_t batch);
+    void ensure_large_workspace_(size_t batch);
 
     int n_ = 0;
     int device_ = 0;
@@ -82,6 +85,13 @@ private:
     hipFunction_t k_refine_ = nullptr;
     void* mbuf_ = nullptr;
     size_t mbuf_batch_ = 0;
+    // Orders above 64 (the staged kernels, every mode): per-matrix workspace
+    // for up to large_chunk(n) matrices at a time.
+    bool large_ = false;
+    hipFunction_t k_lg_chol_ = nullptr, k_lg_formm_ = nullptr, k_lg_hetrd_ = nullptr, k_lg_ql_ = nullptr,
+                  k_lg_back_ = nullptr, k_lg_backtri_ = nullptr, k_lg_refine_ = nullptr;
+    void* gbuf_ = nullptr;
+    size_t gbuf_batch_ = 0;
     int refine_iters_ = 2;
     double refine_delta_ = 1e-4;
 
@@ -100,11 +110,18 @@ public:
     // its size in bytes.
     size_t reserve(size_t batch);
     static size_t workspace_bytes(Mode mode, int n, size_t batch);
+    // Orders above kMaxLdsOrder run the staged kernels; up to kMaxOrder.
+    static constexpr int kMaxLdsOrder = 64;
+    static constexpr int kMaxOrder = 192;
+    // Staged kernels: workspace bytes per matrix, and matrices per launch
+    // round (a larger batch is solved in rounds of this many).
+    static size_t large_matrix_bytes(int n);
+    static size_t large_chunk(int n);
     void set_stage_timing(bool on);
     // Fills ms[0 .. min(n, stages)) and returns the number of stages of the
     // last timed launch (0 if none); waits for that launch to finish.
     int stage_times(double* ms, int n);
-    static const char* stage_name(Mode mode, int i);
+    static const char* stage_name(Mode mode, int i, int n = 1);
 };
 
 }  // namespace sgeig

@@ -133,9 +133,9 @@ def test_session_null_and_bad_arguments():
     assert L.sunny_geneig_session_stats(None, None, None, None, None) == sg.STATUS_INVALID_ARG
     h = L.sunny_geneig_session_create(0, 0, None, 0, 16, 2, 1 << 20, 0)
     assert not h and "orders" in sg.last_error()
-    orders = (ctypes.c_int * 1)(80)
+    orders = (ctypes.c_int * 1)(200)
     h = L.sunny_geneig_session_create(0, 0, orders, 1, 16, 2, 1 << 20, 0)
-    assert not h and "outside [1, 64]" in sg.last_error()
+    assert not h and "outside [1, 192]" in sg.last_error()
 
 
 def test_session_request_bytes():
