@@ -87,6 +87,14 @@ def cpu_baseline(H, S, n, sample: int, threads: int):
     }, W
 
 
+def cpu_baseline_sample(Hs, Ss, threads: int):
+    """zhegv eigenvalues of a small host sample (the staged orders' accuracy
+    check against the oracle)."""
+    from oracle import oracle as O
+    W, _, info = O.zhegv_batch(Hs, Ss, threads=threads)
+    return info, W
+
+
 def serving_bench(kernel: str, pools, peak_requests: int, poisson_requests: int, load: float,
                   seed: int):
     """Continuous batching through the library's serving session.
@@ -171,6 +179,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 fused / split modes")
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the staged-kernel orders (N=96 batch 16384, N=192 batch 2048)")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="launch one device copy of this many MiB first (HBM counter calibration)")
@@ -318,21 +328,79 @@ def main():
         out["mixed_vs_fp64_split"] = value / modes["hybrid"]["solves_per_s"]
     mixed.close()
 
+    # Mixed precision at the staged orders (FP64 Cholesky / products /
+    # refinement around FP32 tridiagonalisation and eigenvectors) beside the
+    # FP64 staged pipeline on the same inputs.
+    large, large_samples = {}, {}
+    if not a.no_large:
+        for n2, B2 in ((96, 16384), (192, 2048)):
+            H2, S2 = inputs.spectrum_batch(n2, B2, a.seed + n2, device=dev)
+            W2 = torch.empty((B2, n2), dtype=torch.float64, device=dev)
+            U2 = torch.empty((B2, n2, n2), dtype=torch.complex128, device=dev)
+            I2 = torch.zeros((B2,), dtype=torch.int32, device=dev)
+            rec, W64 = {}, None
+            for kernel in ("unified", "mixed"):
+                s2 = sg.Solver(n2, kernel=kernel)
+                for _ in range(max(1, a.warmup)):
+                    sg.run_kernel(s2, H2, S2, W2, U2, I2, stream=stream)
+                torch.cuda.synchronize()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(a.steps):
+                    sg.run_kernel(s2, H2, S2, W2, U2, I2, stream=stream)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                ms2 = e0.elapsed_time(e1) / a.steps
+                res, orth = inputs.quality(H2[:512], S2[:512], W2[:512], U2[:512])
+                s2.set_stage_timing(True)
+                sg.run_kernel(s2, H2, S2, W2, U2, I2, stream=stream)
+                st2 = s2.stage_times()
+                s2.set_stage_timing(False)
+                s2.close()
+                r = {"ms_per_step": ms2, "solves_per_s": B2 / (ms2 * 1e-3),
+                     "max_residual_over_N_eps": res.max().item() / (n2 * EPS),
+                     "max_s_orth_over_N_eps": orth.max().item() / (n2 * EPS),
+                     "info_nonzero": int((I2 != 0).sum().item()),
+                     "frac_of_fp64_peak": B2 * flops_per_solve(n2) / (ms2 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                     "stages_ms": {k: v for k, v in st2}}
+                if kernel == "unified":
+                    W64 = W2.clone()
+                else:
+                    r["max_eig_rel_diff_vs_fp64_mode"] = (
+                        ((W2 - W64).abs() / W64.abs().clamp(min=1.0)).max().item())
+                large_samples[(n2, kernel)] = W2[:32].cpu().numpy()
+                rec["fp64" if kernel == "unified" else "mixed"] = r
+                log(f"N={n2} B={B2} {kernel}: {ms2:.2f} ms/step, {B2 / (ms2 * 1e-3):.0f} solves/s, resid "
+                    f"{r['max_residual_over_N_eps']:.2f} Ne, s_orth {r['max_s_orth_over_N_eps']:.2f} Ne")
+            rec["mixed_vs_fp64"] = rec["mixed"]["solves_per_s"] / rec["fp64"]["solves_per_s"]
+            rec["batch"] = B2
+            large_samples[(n2, "inputs")] = (H2[:32].cpu().numpy(), S2[:32].cpu().numpy())
+            large[str(n2)] = rec
+            del H2, S2, W2, U2, I2, W64
+            torch.cuda.empty_cache()
+        out["large_orders"] = large
+
     if not a.no_serve:
         # Serving: orders 24 and 54 of the request mix (the LDS-resident ones),
         # inputs from the same generator, resident in HBM.
+        # Serving: one order per request from {24, 54, 96, 150} (the last two
+        # on the staged kernels), inputs from the same generator, resident in
+        # HBM; then the LDS-resident orders alone.
         pools = {54: (H[:4096], S[:4096])}
-        H24, S24 = inputs.spectrum_batch(24, 4096, a.seed + 24, device=dev)
-        pools[24] = (H24, S24)
+        for n3 in (24, 96, 150):
+            pools[n3] = inputs.spectrum_batch(n3, 4096, a.seed + n3, device=dev)
         serving = {}
-        for kernel in ("mixed", "unified"):
-            r = serving_bench(kernel, pools, 200, a.serve_requests, 0.7, a.seed)
-            serving[kernel] = r
-            log(f"serving {kernel}: peak {r['peak_solves_per_s']:.0f} solves/s; at 70%: "
+        runs = [("mixed", (24, 54, 96, 150)), ("unified", (24, 54, 96, 150)), ("mixed", (24, 54))]
+        for kernel, orders in runs:
+            r = serving_bench(kernel, {o: pools[o] for o in orders}, 200, a.serve_requests, 0.7, a.seed)
+            key = kernel if len(orders) == 4 else f"{kernel}_orders_24_54"
+            serving[key] = r
+            log(f"serving {key}: peak {r['peak_solves_per_s']:.0f} solves/s; at 70%: "
                 f"p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms, achieved {r['achieved_solves_per_s']:.0f}")
         out["serving"] = serving
         out["p99_ms"] = serving["mixed"]["p99_ms"]
-        del H24, S24, pools
+        del pools
 
     if not a.no_cpu:
         try:
@@ -350,6 +418,15 @@ def main():
         out["accuracy"]["max_eig_rel_err_vs_zhegv_sample"] = float(
             (np.abs(Wg - Wl) / np.maximum(np.abs(Wl), 1.0)).max())
         out["cpu_baseline"] = cb
+        # The staged orders' samples against zhegv as well.
+        for key, rec in large.items():
+            n2 = int(key)
+            Hs, Ss = large_samples[(n2, "inputs")]
+            _, Wl2 = cpu_baseline_sample(Hs, Ss, threads)
+            for kernel, tag in (("unified", "fp64"), ("mixed", "mixed")):
+                Wg2 = large_samples[(n2, kernel)]
+                rec[tag]["max_eig_rel_err_vs_zhegv_sample"] = float(
+                    (np.abs(Wg2 - Wl2) / np.maximum(np.abs(Wl2), 1.0)).max())
         log(f"cpu: {cb['value']:.0f} solves/s on {threads} threads, "
             f"{cb['single_core_solves_per_s']:.0f} on 1")
 

@@ -143,16 +143,19 @@ int sunny_geneig_stage_times(sunny_geneig_solver_t solver, double* ms, const cha
 
 /* ---- uneven batches (no counterpart in the reference ABI) ----------------
  * One call solves `count` pencils of different orders n[k] in [1, max_n].
- * Every matrix goes to the smallest bucket order >= n[k]; each bucket is one
- * grid of a kernel compiled for that order (the n x n pencil is embedded as
- * (H (+) pad I, S (+) I), which leaves its eigenpairs unchanged).
+ * Every matrix goes to the smallest bucket order >= n[k] and is embedded in
+ * that order as (H (+) pad I, S (+) I), which leaves its eigenpairs
+ * unchanged. Buckets up to 64 are one grid of an LDS-resident kernel each;
+ * buckets above 64 run the staged FP64 kernels of sunny_geneig_launch on a
+ * padded copy (gathered and scattered on the device, in rounds of 4096).
  *
  * Packed buffers, matrix k column-major at element offset sum_{j<k} n[j]^2:
  *   d_H, d_S, d_U : complex128, sum n[k]^2 elements;
  *   d_W           : float64, sum n[k] elements (matrix k at sum_{j<k} n[j]);
  *   d_info        : int32 [count], same codes as sunny_geneig_launch.
- * bucket_orders: strictly ascending orders in [1, 64], the last >= max_n; NULL
- * (or num_buckets == 0) = multiples of 4 below max_n, then max_n. */
+ * bucket_orders: strictly ascending orders in [1, 192], the last >= max_n; NULL
+ * (or num_buckets == 0) = multiples of 4 below min(max_n, 64), of 8 below
+ * min(max_n, 128), of 16 below max_n, then max_n. */
 typedef struct sunny_geneig_varlen_solver* sunny_geneig_varlen_t;
 
 sunny_geneig_varlen_t sunny_geneig_varlen_create(int max_n, int device_id,

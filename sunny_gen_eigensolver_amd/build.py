@@ -38,7 +38,7 @@ HIP_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 # (multiples of 4 up to 64) ship warm at their own workgroup size.
 WARM_ORDERS = [16, 24, 32, 48, 54, 64]
 BUCKET_ORDERS = list(range(4, 65, 4))
-LARGE_ORDERS = [96, 128, 150, 192]       # staged kernels (orders above 64)
+LARGE_ORDERS = list(range(72, 129, 8)) + [150, 192]   # staged kernels (orders above 64)
 
 
 def _run(cmd, **kw):

@@ -1060,3 +1060,57 @@ geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, const int* __restrict
     for (int e = tid; e < N; e += LNT) W[(size_t)b * N + e] = lam[e];
 }
 #endif  // !GENEIG_FP32
+
+#ifndef GENEIG_FP32
+// Uneven batches, buckets above 64: the bucket's matrices (work list ids,
+// packed back to back at element offsets moff, orders nn) are embedded as
+// (H (+) pad I, S (+) I) into a contiguous batch of order N for the staged
+// pipeline; pad is far above any eigenvalue of the pencils, so the padded
+// problem decouples exactly and its n smallest eigenpairs are the original
+// ones. The scatter returns the leading n eigenvalues, the leading n x n
+// block of U and zhegv's info for order n (a failing leading minor k lies in
+// the original block: N + k becomes n + k).
+namespace gk {
+constexpr double LG_PAD = 0x1p600;
+}
+
+extern "C" __global__ void __launch_bounds__(gk::LNT)
+geneig_lg_gather(const cx* __restrict__ H, const cx* __restrict__ S, const int* __restrict__ ids,
+                 const long long* __restrict__ moff, const int* __restrict__ nn, int count, cx* __restrict__ Hp,
+                 cx* __restrict__ Sp) {
+    using namespace gk;
+    const int b = blockIdx.x;
+    if (b >= count) return;
+    const int id = ids[b];
+    const int n = nn[id];
+    const long long mo = moff[id];
+    cx* hb = Hp + (size_t)b * NN;
+    cx* sb = Sp + (size_t)b * NN;
+    for (int e = threadIdx.x; e < (int)NN; e += LNT) {
+        const int i = e % N, j = e / N;
+        const bool in = i < n && j < n;
+        sb[e] = in ? S[mo + (long long)j * n + i] : cm(i == j ? 1.0 : 0.0, 0.0);
+        hb[e] = in ? H[mo + (long long)j * n + i] : cm(i == j ? LG_PAD : 0.0, 0.0);
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(gk::LNT)
+geneig_lg_scatter(const real* __restrict__ Wp, const cx* __restrict__ Up, const int* __restrict__ Ip,
+                  const int* __restrict__ ids, const long long* __restrict__ moff,
+                  const long long* __restrict__ woff, const int* __restrict__ nn, int count, real* __restrict__ W,
+                  cx* __restrict__ U, int* __restrict__ info) {
+    using namespace gk;
+    const int b = blockIdx.x;
+    if (b >= count) return;
+    const int id = ids[b];
+    const int n = nn[id];
+    const long long mo = moff[id], wo = woff[id];
+    const cx* ub = Up + (size_t)b * NN;
+    for (int e = threadIdx.x; e < n * n; e += LNT) U[mo + e] = ub[(size_t)(e / n) * N + e % n];
+    for (int e = threadIdx.x; e < n; e += LNT) W[wo + e] = Wp[(size_t)b * N + e];
+    if (threadIdx.x == 0) {
+        const int ip = Ip[b];
+        info[id] = (ip > N) ? n + (ip - N) : ip;
+    }
+}
+#endif  // !GENEIG_FP32

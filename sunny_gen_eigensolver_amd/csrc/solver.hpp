@@ -104,6 +104,9 @@ private:
 
 public:
     int refine_iters() const { return refine_iters_; }
+    // A kernel of this handle's FP64 module (the staged orders' uneven-batch
+    // gather / scatter live there).
+    hipFunction_t function(const char* name) const { return module_->get(name); }
     double refine_delta() const { return refine_delta_; }
     // Grow the mode's device workspace for `batch` matrices now (no-op for the
     // fused mode), so later launches of up to `batch` allocate nothing; returns

@@ -10,7 +10,8 @@ namespace sgeig {
 
 namespace {
 
-constexpr int kMaxVarlenOrder = 64;
+constexpr int kMaxVarlenOrder = Solver::kMaxOrder;
+size_t al256(size_t x) { return (x + 255) / 256 * 256; }
 
 }  // namespace
 
@@ -18,16 +19,16 @@ VarlenSolver::VarlenSolver(int max_n, int device_id, const std::vector<int>& buc
                            CacheMode cache)
     : max_n_(max_n), device_(device_id) {
     if (max_n < 1 || max_n > kMaxVarlenOrder)
-        throw std::invalid_argument("max_n must be in [1, " + std::to_string(kMaxVarlenOrder) +
-                                    "] for the LDS-resident uneven-batch kernel");
+        throw std::invalid_argument("max_n must be in [1, " + std::to_string(kMaxVarlenOrder) + "]");
     if (buckets.empty()) {
-        for (int o = 4; o < max_n; o += 4) orders_.push_back(o);
+        for (int o = 4; o < max_n; o += (o < 64 ? 4 : o < 128 ? 8 : 16)) orders_.push_back(o);
         orders_.push_back(max_n);
     } else {
         orders_ = buckets;
         for (size_t i = 0; i < orders_.size(); ++i) {
             if (orders_[i] < 1 || orders_[i] > kMaxVarlenOrder)
-                throw std::invalid_argument("bucket order out of range [1, 64]");
+                throw std::invalid_argument("bucket order out of range [1, " +
+                                            std::to_string(kMaxVarlenOrder) + "]");
             if (i && orders_[i] <= orders_[i - 1])
                 throw std::invalid_argument("bucket orders must be strictly ascending");
         }
@@ -45,10 +46,17 @@ VarlenSolver::VarlenSolver(int max_n, int device_id, const std::vector<int>& buc
         b.order = o;
         b.block_dim = Solver::suggested_block_dim(o);   // same warm code objects
         b.nw = b.block_dim / 64;
-        CacheLayer layer;
-        double ms = 0.0;
-        b.module = acquire_module(ModuleKey{o, b.block_dim, arch}, cache, &layer, &ms);
-        b.fn = b.module->get("geneig_varlen");
+        if (o > Solver::kMaxLdsOrder) {
+            b.nw = 1;
+            b.large = std::make_unique<Solver>(o, device_id, 0, Tuning::Suggested, cache, Mode::Unified);
+            b.gather = b.large->function("geneig_lg_gather");
+            b.scatter = b.large->function("geneig_lg_scatter");
+        } else {
+            CacheLayer layer;
+            double ms = 0.0;
+            b.module = acquire_module(ModuleKey{o, b.block_dim, arch}, cache, &layer, &ms);
+            b.fn = b.module->get("geneig_varlen");
+        }
         buckets_.push_back(std::move(b));
     }
     SG_HIP(hipEventCreateWithFlags(&staged_, hipEventDisableTiming));
@@ -65,6 +73,8 @@ VarlenSolver::~VarlenSolver() {
     if (host_desc_) (void)hipHostFree(host_desc_);
     if (dev_desc_) (void)hipFree(dev_desc_);
     if (scratch_) (void)hipFree(scratch_);
+    if (pad_) (void)hipFree(pad_);
+    buckets_.clear();
     (void)hipSetDevice(cur);
 }
 
@@ -103,12 +113,27 @@ void VarlenSolver::launch(const void* H, const void* S, double* W, void* U, int*
     if (cur != device_) SG_HIP(hipSetDevice(device_));
 
     ensure_capacity_(count);
-    size_t scratch_need = 0;
+    size_t scratch_need = 0, pad_need = 0;
     for (int b = 0; b < nb; ++b) {
         if (!per_bucket[b]) continue;
         const Bucket& B = buckets_[b];
+        const size_t no = (size_t)B.order;
+        if (B.large) {
+            const size_t r = std::min<size_t>(per_bucket[b], kLargeRound);
+            pad_need = std::max(pad_need, 3 * al256(r * no * no * 16) + al256(r * no * 8) + al256(r * 4));
+            B.large->reserve(r);
+            continue;
+        }
         const size_t wgs = std::min<size_t>((per_bucket[b] + B.nw - 1) / B.nw, kMaxGrid);
-        scratch_need = std::max(scratch_need, wgs * B.nw * (size_t)B.order * B.order * 16);
+        scratch_need = std::max(scratch_need, wgs * B.nw * no * no * 16);
+    }
+    if (pad_need > pad_bytes_) {
+        SG_HIP(hipEventSynchronize(done_));
+        if (pad_) SG_HIP(hipFree(pad_));
+        pad_ = nullptr;
+        pad_bytes_ = 0;
+        SG_HIP(hipMalloc(&pad_, pad_need));
+        pad_bytes_ = pad_need;
     }
     if (scratch_need > scratch_bytes_) {
         SG_HIP(hipEventSynchronize(done_));
@@ -156,6 +181,26 @@ void VarlenSolver::launch(const void* H, const void* S, double* W, void* U, int*
     // in behind them.
     for (int b = nb - 1; b >= 0; --b) {
         const Bucket& B = buckets_[b];
+        if (B.large) {
+            const size_t no = (size_t)B.order;
+            for (int off = 0; off < per_bucket[b]; off += kLargeRound) {
+                int cnt = std::min(per_bucket[b] - off, kLargeRound);
+                const size_t r = (size_t)cnt;
+                char* base = static_cast<char*>(pad_);
+                void* Hp = base;
+                void* Sp = base + al256(r * no * no * 16);
+                void* Up = base + 2 * al256(r * no * no * 16);
+                double* Wp = reinterpret_cast<double*>(base + 3 * al256(r * no * no * 16));
+                int* Ip = reinterpret_cast<int*>(base + 3 * al256(r * no * no * 16) + al256(r * no * 8));
+                const int* list = d_ids + start[b] + off;
+                void* ga[] = {&H, &S, &list, &d_moff, &d_nn, &cnt, &Hp, &Sp};
+                SG_HIP(hipModuleLaunchKernel(B.gather, (unsigned)cnt, 1, 1, 256, 1, 1, 0, stream, ga, nullptr));
+                B.large->launch(Hp, Sp, Wp, Up, Ip, cnt, stream);
+                void* sa[] = {&Wp, &Up, &Ip, &list, &d_moff, &d_woff, &d_nn, &cnt, &W, &U, &info};
+                SG_HIP(hipModuleLaunchKernel(B.scatter, (unsigned)cnt, 1, 1, 256, 1, 1, 0, stream, sa, nullptr));
+            }
+            continue;
+        }
         for (int off = 0; off < per_bucket[b]; off += kMaxGrid * B.nw) {
             int cnt = std::min(per_bucket[b] - off, kMaxGrid * B.nw);
             const int* list = d_ids + start[b] + off;

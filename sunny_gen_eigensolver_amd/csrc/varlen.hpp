@@ -1,14 +1,17 @@
 // Uneven batches: matrices of different orders solved by one call. Every
-// matrix goes to the smallest bucket order >= its own; each bucket is one
-// grid of the geneig_varlen kernel compiled for that order, which embeds an
-// n x n pencil in the bucket's order as (H (+) pad I, S (+) I). The host sorts
-// the work list by bucket, so each grid reads a contiguous run of it.
+// matrix goes to the smallest bucket order >= its own and is embedded in the
+// bucket's order as (H (+) pad I, S (+) I). The host sorts the work list by
+// bucket, so each bucket reads a contiguous run of it. Buckets up to 64 are
+// one grid of the LDS-resident geneig_varlen kernel each; a bucket above 64
+// gathers its pencils into a padded contiguous batch, runs the staged FP64
+// kernels on it and scatters the results back (in rounds of kLargeRound).
 //
 // No counterpart in the reference (its solvers take one order per handle,
 // src/unified/unified_solver.hpp:81-217).
 #pragma once
 
 #include "codeobj.hpp"
+#include "solver.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -20,8 +23,8 @@ namespace sgeig {
 
 class VarlenSolver {
 public:
-    // Buckets: ascending orders in [1, 64]; empty = multiples of 4 up to max_n
-    // (plus max_n itself).
+    // Buckets: ascending orders in [1, 192]; empty = multiples of 4 up to 64,
+    // then of 8 up to 128, then of 16, ending with max_n itself.
     VarlenSolver(int max_n, int device_id, const std::vector<int>& buckets, CacheMode cache);
     ~VarlenSolver();
     VarlenSolver(const VarlenSolver&) = delete;
@@ -42,6 +45,8 @@ public:
     // Workgroups per grid launch; longer runs are split into several grids
     // that reuse one scratch area.
     static constexpr int kMaxGrid = 4096;
+    // Matrices per round of a bucket above 64.
+    static constexpr int kLargeRound = 4096;
 
 private:
     struct Bucket {
@@ -50,6 +55,9 @@ private:
         int nw = 0;
         std::shared_ptr<LoadedModule> module;
         hipFunction_t fn = nullptr;
+        // Buckets above 64: the staged FP64 solver and its gather / scatter.
+        std::unique_ptr<Solver> large;
+        hipFunction_t gather = nullptr, scatter = nullptr;
     };
     void ensure_capacity_(int count);
 
@@ -68,6 +76,9 @@ private:
     // Per-workgroup factor stash, sized for the largest bucket's grid.
     void* scratch_ = nullptr;
     size_t scratch_bytes_ = 0;
+    // Padded batch of a large bucket's round: H, S, U, W, info.
+    void* pad_ = nullptr;
+    size_t pad_bytes_ = 0;
 };
 
 }  // namespace sgeig

@@ -175,3 +175,61 @@ def test_fused_at_bucket_orders(n):
     for kernel in ("unified", "hybrid", "mixed"):
         W, U, info = T.run_gpu(H, S, kernel=kernel)
         T.assert_parity(H, S, W, U, info, Wl, Ul, tag=f"{kernel} n={n}")
+
+
+# ---------------------------------------------------------------------------
+# Buckets above 64 (the staged kernels on a padded copy of the bucket).
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def vs128():
+    import sunny_gen_eigensolver_amd as sg
+    s = sg.VarlenSolver(128)
+    yield s
+    s.close()
+
+
+def test_default_buckets_128(vs128):
+    assert vs128.buckets == list(range(4, 65, 4)) + list(range(72, 129, 8))
+
+
+def test_log_uniform_12_128_vs_lapack(vs128):
+    """The uneven-batch workload's order law: log-uniform over 12..128, so
+    about 30% of the pencils fall in the staged buckets."""
+    rng = np.random.default_rng(13)
+    orders = [int(x) for x in np.exp(rng.uniform(np.log(12), np.log(128.999), size=120))]
+    orders += [65, 72, 73, 100, 127, 128]
+    rng.shuffle(orders)
+    Hs, Ss = _gen_mix(orders, 21000)
+    W, U, info = _solve(vs128, Hs, Ss, orders)
+    _check_groups(Hs, Ss, W, U, info, orders, "varlen 12..128")
+
+
+def test_large_bucket_alone_equals_mixed(vs128):
+    orders = [100, 7, 128, 54, 65, 100, 90]
+    Hs, Ss = _gen_mix(orders, 31000)
+    W, U, info = _solve(vs128, Hs, Ss, orders)
+    for k in (0, 2, 4, 6):
+        Wa, Ua, ia = _solve(vs128, [Hs[k]], [Ss[k]], [orders[k]])
+        assert ia[0] == info[k] == 0
+        assert np.array_equal(Wa[0], W[k]), f"matrix {k}: eigenvalues differ alone vs mixed"
+        assert np.array_equal(Ua[0], U[k]), f"matrix {k}: eigenvectors differ alone vs mixed"
+
+
+def test_large_bucket_non_hpd(vs128):
+    """INFO = n + k in a staged bucket (the padded order's code mapped back to
+    the pencil's own order), W NaN and U zero; neighbours unaffected."""
+    orders = [90, 90, 100, 77]
+    Hs, Ss = _gen_mix(orders, 41000)
+    Ss = [s.copy() for s in Ss]
+    bad = {1: 0, 2: 70}
+    for k, p in bad.items():
+        Sm = np.eye(orders[k], dtype=np.complex128)
+        Sm[p, p] = -1.0
+        Ss[k] = Sm.T.copy()
+    W, U, info = _solve(vs128, Hs, Ss, orders)
+    for k, p in bad.items():
+        assert int(info[k]) == orders[k] + p + 1, (k, info[k])
+        assert np.isnan(W[k]).all() and (U[k] == 0).all()
+    good = [0, 3]
+    _check_groups([Hs[k] for k in good], [Ss[k] for k in good], [W[k] for k in good], [U[k] for k in good],
+                  info[good], [orders[k] for k in good], "varlen beside non-HPD")
