@@ -87,6 +87,12 @@ def cpu_baseline(H, S, n, sample: int, threads: int):
     }, W
 
 
+def O_zhegv(Hs, Ss, threads: int):
+    """zhegv eigenpairs of a host sample (W, U, info)."""
+    from oracle import oracle as O
+    return O.zhegv_batch(Hs, Ss, threads=threads)
+
+
 def cpu_baseline_sample(Hs, Ss, threads: int):
     """zhegv eigenvalues of a small host sample (the staged orders' accuracy
     check against the oracle)."""
@@ -179,6 +185,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 fused / split modes")
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="skip the uneven batch (200000 pencils, N log-uniform over 12..128)")
+    ap.add_argument("--uneven-count", type=int, default=200000)
     ap.add_argument("--no-large", action="store_true",
                     help="skip the staged-kernel orders (N=96 batch 16384, N=192 batch 2048)")
     ap.add_argument("--serve-requests", type=int, default=1000)
@@ -331,7 +340,7 @@ def main():
     # Mixed precision at the staged orders (FP64 Cholesky / products /
     # refinement around FP32 tridiagonalisation and eigenvectors) beside the
     # FP64 staged pipeline on the same inputs.
-    large, large_samples = {}, {}
+    large, large_samples, uneven_samples = {}, {}, {}
     if not a.no_large:
         for n2, B2 in ((96, 16384), (192, 2048)):
             H2, S2 = inputs.spectrum_batch(n2, B2, a.seed + n2, device=dev)
@@ -381,6 +390,95 @@ def main():
             torch.cuda.empty_cache()
         out["large_orders"] = large
 
+    # Uneven batch: pencils of orders drawn log-uniform over the integers
+    # 12..128 with cond(S) log-uniform in [1e1, 1e4], packed back to back in
+    # draw order, solved by one call of the uneven-batch path (LDS-resident
+    # buckets up to 64, staged buckets above).
+    if not a.no_uneven:
+        import ctypes
+        import numpy as np
+        rng = np.random.default_rng(a.seed)
+        cnt = a.uneven_count
+        orders = np.floor(np.exp(rng.uniform(np.log(12), np.log(129), size=cnt))).astype(np.int64)
+        orders = np.clip(orders, 12, 128)
+        conds = np.exp(rng.uniform(np.log(1e1), np.log(1e4), size=cnt))
+        nsq = orders * orders
+        moff = np.concatenate([[0], np.cumsum(nsq)[:-1]])
+        tot = int(nsq.sum())
+        t0 = time.perf_counter()
+        Hv = torch.empty((tot,), dtype=torch.complex128, device=dev)
+        Sv = torch.empty((tot,), dtype=torch.complex128, device=dev)
+        for n3 in np.unique(orders):
+            idx = np.nonzero(orders == n3)[0]
+            Hn, Sn = inputs.spectrum_batch(int(n3), len(idx), a.seed + 1000 + int(n3), device=dev,
+                                           cond_s=torch.tensor(conds[idx], device=dev))
+            dst = (torch.tensor(moff[idx], device=dev)[:, None]
+                   + torch.arange(int(n3) * int(n3), device=dev)[None, :]).reshape(-1)
+            Hv[dst] = Hn.reshape(-1)
+            Sv[dst] = Sn.reshape(-1)
+            del Hn, Sn, dst
+        Wv = torch.empty((int(orders.sum()),), dtype=torch.float64, device=dev)
+        Uv = torch.empty((tot,), dtype=torch.complex128, device=dev)
+        Iv = torch.zeros((cnt,), dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
+        log(f"uneven: generated {cnt} pencils (sum N^2 = {tot}) in {time.perf_counter() - t0:.1f}s")
+        vs = sg.VarlenSolver(128)
+        arr = (ctypes.c_int * cnt)(*orders.tolist())
+        L = sg.lib()
+        sh = sg._stream_handle(stream)
+
+        def vlaunch():
+            st_ = L.sunny_geneig_varlen_launch(vs.handle, Hv.data_ptr(), Sv.data_ptr(), Wv.data_ptr(),
+                                               Uv.data_ptr(), Iv.data_ptr(), arr, cnt, sh)
+            if st_ != sg.STATUS_OK:
+                raise RuntimeError(sg.last_error())
+        for _ in range(max(1, a.warmup)):
+            vlaunch()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(a.steps):
+            vlaunch()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        msv = e0.elapsed_time(e1) / a.steps
+        # Residual and S-orthogonality per order on a sample of each order.
+        woff = np.concatenate([[0], np.cumsum(orders)[:-1]])
+        worst_r, worst_o = 0.0, 0.0
+        uneven_samples = {}
+        for n3 in (12, 33, 54, 64, 65, 96, 128):
+            idx = np.nonzero(orders == n3)[0][:64]
+            if len(idx) == 0:
+                continue
+            gi = (torch.tensor(moff[idx], device=dev)[:, None]
+                  + torch.arange(n3 * n3, device=dev)[None, :])
+            wi = (torch.tensor(woff[idx], device=dev)[:, None] + torch.arange(n3, device=dev)[None, :])
+            res, orth = inputs.quality(Hv[gi].view(-1, n3, n3), Sv[gi].view(-1, n3, n3), Wv[wi],
+                                       Uv[gi].view(-1, n3, n3))
+            worst_r = max(worst_r, res.max().item() / (n3 * EPS))
+            worst_o = max(worst_o, orth.max().item() / (n3 * EPS))
+            uneven_samples[n3] = (Hv[gi].view(-1, n3, n3).cpu().numpy(), Sv[gi].view(-1, n3, n3).cpu().numpy(),
+                                  res.max().item() / (n3 * EPS), orth.max().item() / (n3 * EPS))
+        flops = float(sum(flops_per_solve(int(x)) for x in orders))
+        out["uneven"] = {
+            "count": cnt, "orders": "log-uniform integers 12..128", "cond_s": "log-uniform [1e1, 1e4]",
+            "ms_per_step": msv, "solves_per_s": cnt / (msv * 1e-3),
+            "frac_of_fp64_peak": flops / (msv * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+            "compulsory_GBs": float((48 * nsq + 8 * orders + 4).sum()) / (msv * 1e-3) / 1e9,
+            "share_above_64": float((orders > 64).mean()),
+            "info_nonzero": int((Iv != 0).sum().item()),
+            "max_residual_over_N_eps_sample": worst_r, "max_s_orth_over_N_eps_sample": worst_o,
+            "buckets": vs.buckets,
+            "note": "one call per step: host bucket sort and descriptor upload included; sample = up "
+                    "to 64 pencils of each of the orders 12, 33, 54, 64, 65, 96, 128",
+        }
+        log(f"uneven: {msv:.1f} ms/step, {cnt / (msv * 1e-3):.0f} solves/s, resid {worst_r:.2f} Ne, "
+            f"s_orth {worst_o:.2f} Ne, info!=0 {out['uneven']['info_nonzero']}")
+        vs.close()
+        del Hv, Sv, Wv, Uv, Iv
+        torch.cuda.empty_cache()
+
     if not a.no_serve:
         # Serving: orders 24 and 54 of the request mix (the LDS-resident ones),
         # inputs from the same generator, resident in HBM.
@@ -418,6 +516,21 @@ def main():
         out["accuracy"]["max_eig_rel_err_vs_zhegv_sample"] = float(
             (np.abs(Wg - Wl) / np.maximum(np.abs(Wl), 1.0)).max())
         out["cpu_baseline"] = cb
+        # The uneven batch's sample: zhegv's own residual and S-orthogonality
+        # on the same pencils (cond(S) up to 1e4 lifts both above 50 N eps).
+        if uneven_samples and "uneven" in out:
+            zr, zo, per = 0.0, 0.0, {}
+            for n3, (Hs, Ss, r_ours, o_ours) in sorted(uneven_samples.items()):
+                Wz, Uz, _ = O_zhegv(Hs, Ss, threads)
+                rz, oz = inputs.quality(torch.from_numpy(Hs).to(dev), torch.from_numpy(Ss).to(dev),
+                                        torch.from_numpy(Wz).to(dev), torch.from_numpy(Uz).to(dev))
+                rz, oz = rz.max().item() / (n3 * EPS), oz.max().item() / (n3 * EPS)
+                zr, zo = max(zr, rz), max(zo, oz)
+                per[str(n3)] = {"ours_resid": r_ours, "ours_s_orth": o_ours, "zhegv_resid": rz,
+                                "zhegv_s_orth": oz}
+            out["uneven"]["zhegv_max_residual_over_N_eps_sample"] = zr
+            out["uneven"]["zhegv_max_s_orth_over_N_eps_sample"] = zo
+            out["uneven"]["per_order_over_N_eps"] = per
         # The staged orders' samples against zhegv as well.
         for key, rec in large.items():
             n2 = int(key)

@@ -46,8 +46,10 @@ def _hermitian_from(Q: torch.Tensor, w: torch.Tensor) -> torch.Tensor:
 
 @torch.no_grad()
 def spectrum_batch(n: int, batch: int, seed: int, device="cuda", n_pairs: int = 6,
-                   pair_gap: float = 1e-8, cond_s: float = 0.0, chunk: int = 8192):
-    """Generate (H, S) on `device`; deterministic for a given seed and device."""
+                   pair_gap: float = 1e-8, cond_s=0.0, chunk: int = 8192):
+    """Generate (H, S) on `device`; deterministic for a given seed and device.
+    cond_s > 0 (a number, or one value per matrix) replaces S by
+    Q_S diag(sigma) Q_S^H with sigma geometric from 1 to cond_s."""
     dev = torch.device(device)
     gen = torch.Generator(device=dev)
     gen.manual_seed(int(seed) & 0x7FFFFFFFFFFFFFFF)
@@ -61,10 +63,11 @@ def spectrum_batch(n: int, batch: int, seed: int, device="cuda", n_pairs: int = 
         for k in range(min(n_pairs, n // 2)):
             lam[:, 2 * k + 1] = lam[:, 2 * k] + pair_gap
         H[s:s + b] = _colmajor(_hermitian_from(Q, lam))
-        if cond_s > 0:
+        if torch.is_tensor(cond_s) or cond_s > 0:
             QS = _haar_unitary(b, n, gen, dev)
-            sig = torch.logspace(0.0, float(torch.log10(torch.tensor(cond_s))), n,
-                                 dtype=torch.float64, device=dev).expand(b, n)
+            cs = torch.as_tensor(cond_s, dtype=torch.float64, device=dev).expand(batch)[s:s + b]
+            frac = torch.arange(n, dtype=torch.float64, device=dev) / max(n - 1, 1)
+            sig = torch.exp(torch.log(cs)[:, None] * frac[None, :])
             S[s:s + b] = _colmajor(_hermitian_from(QS, sig))
         else:
             Bm = torch.randn((b, n, n), dtype=torch.complex128, generator=gen, device=dev)
