@@ -232,11 +232,30 @@ __device__ __forceinline__ void eig2_pencil(real t11, real t22, cx t12, real g11
     X[0][1] = (y2a - cmul(cl21, X[1][1])) * (real(1.0) / l11);
 }
 
+#ifndef GENEIG_FP32
+// A 16 x 16 complex FP64 tile on the matrix cores: v_mfma_f64_16x16x4f64
+// takes A(i, k) from lane i + 16 k and B(k, j) from lane j + 16 k (i, j < 16,
+// k < 4) and leaves D(i, j) in lane j + 16 (i % 4), element i / 4; a complex
+// product is four real MFMAs into a (re, im) pair of accumulators.
+typedef double d4 __attribute__((ext_vector_type(4)));
+struct ctile {
+    d4 re, im;
+};
+__device__ __forceinline__ void cmfma(ctile& c, cx a, cx b) {
+    c.re = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, c.re, 0, 0, 0);
+    c.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, c.re, 0, 0, 0);
+    c.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, c.im, 0, 0, 0);
+    c.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, c.im, 0, 0, 0);
+}
+__device__ __forceinline__ cx tile_get(const ctile& t, int v) { return cm(t.re[v], t.im[v]); }
+#endif
+
 }  // namespace gk
 
-#if GENEIG_N > 64
+// The staged kernels (orders above 64; their Cholesky for every order).
 #include "geneig_large.hip"
-#else
+
+#if GENEIG_N <= 64
 
 namespace gk {
 
@@ -1881,41 +1900,19 @@ geneig_split_back(cx* __restrict__ U, const real* __restrict__ Zw, const int* __
 // ---------------------------------------------------------------------------
 // Mixed precision (FP32 tridiagonalisation and eigenvectors, FP64 Cholesky
 // and refinement). Per matrix:
-//   geneig_mx_chol    (FP64 module): S = L L^H in FP64; L rounded to complex64
-//                     (packed lower triangle) into L32. A non-positive pivot
-//                     ends the matrix here with zhegv's INFO, as in the other
-//                     modes.
-//   geneig_mx_front   (FP32 module): W = L^-1, M = W H W^H (H read as complex128
-//                     and rounded), Householder tridiagonalisation; the stash
-//                     (W, reflectors, tau) goes to U32, T to W32 / E32.
+//   geneig_lg_chol    (FP64 module): S = L L^H and W = L^-1 in FP64 by 16-column
+//                     panels with the products on the matrix cores; W (full,
+//                     upper zero) to Wd. A non-positive pivot ends the matrix
+//                     here with zhegv's INFO, as in the other modes.
+//   geneig_mx_front   (FP32 module): M = W H W^H (W and H read as complex128
+//                     and rounded), Householder tridiagonalisation; reflectors
+//                     and tau to U32, T to W32 / E32.
 //   geneig_split_ql   (FP32 module): QL with FP32 unit roundoff, Z to Z32.
-//   geneig_split_back (FP32 module): U32 = W^H Q Z.
-//   geneig_mx_refine  (FP64 module): Ogita-Aishima refinement of (W32, U32)
-//                     on the original pencil (H, S) in FP64, then exact 2 x 2
-//                     Rayleigh-Ritz on pairs of close eigenvalues; W, U out.
+//   geneig_mx_back    (FP32 module): V = Q Z into U32.
+//   geneig_mx_refine  (FP64 module): Ogita-Aishima refinement of (M, V) in
+//                     the Cholesky frame on the matrix cores, Rayleigh-Ritz on
+//                     clusters, then U = W^H V; W, U out.
 // ---------------------------------------------------------------------------
-#ifndef GENEIG_FP32
-extern "C" __global__ void __launch_bounds__(GENEIG_NT, 2)
-geneig_mx_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ U,
-               int* __restrict__ info, cx* __restrict__ Wd, int batch) {
-    using namespace gk;
-    __shared__ WorkP w;
-    const int b = blockIdx.x;
-    if (b >= batch) return;
-    const size_t off = (size_t)b * N * N;
-    const int pinfo = potrf_reg(w, S + off);
-    if (pinfo) {
-        fail_store(N + pinfo, W + (size_t)b * N, U + off, info + b);
-        return;
-    }
-    __syncthreads();
-    trtri_packed(w);
-    __syncthreads();
-    cx* Wb = Wd + (size_t)b * NPK;
-    for (int e = threadIdx.x; e < NPK; e += NT) Wb[e] = w.Wi[e];
-    if (threadIdx.x == 0) info[b] = 0;
-}
-#endif  // !GENEIG_FP32
 
 #ifdef GENEIG_FP32
 extern "C" __global__ void __launch_bounds__(GENEIG_NT, 2)
@@ -1926,8 +1923,11 @@ geneig_mx_front(const gk::cd* __restrict__ H, const gk::cd* __restrict__ Wd, rea
     const int b = blockIdx.x;
     if (b >= batch || info[b] != 0) return;
     const size_t off = (size_t)b * N * N;
-    const cd* Wb = Wd + (size_t)b * NPK;
-    for (int e = threadIdx.x; e < NPK; e += NT) w.Wi[e] = to_cx(Wb[e]);
+    const cd* Wb = Wd + (size_t)b * N * N;          // W = L^-1, full (upper zero)
+    for (int e = threadIdx.x; e < N * N; e += NT) {
+        const int i = e % N, j = e / N;
+        if (i >= j) w.Wi[pk(j) + i - j] = to_cx(Wb[e]);
+    }
     __syncthreads();
     cx m[CPW];
     form_m_reg(w, H + off, m);
@@ -1973,37 +1973,16 @@ namespace gk {
 // element i / 4. A 16 x 16 complex tile is a pair of such accumulators; a wave
 // owns one 16-column block of the result (all row tiles), so reductions over
 // a column stay inside the wave.
-typedef double d4 __attribute__((ext_vector_type(4)));
 constexpr int NB = (N + 15) / 16;          // 16-blocks per dimension
 constexpr int REF_NT = 64 * NB;            // one wave per column block
 constexpr int KEND = (N + 3) / 4 * 4;      // k-steps that touch rows/columns < N
 constexpr int LDR = N | 1;
-
-struct ctile {
-    d4 re, im;
-};
-
-__device__ __forceinline__ void cmfma(ctile& c, cx a, cx b) {
-    c.re = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, c.re, 0, 0, 0);
-    c.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, c.re, 0, 0, 0);
-    c.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, c.im, 0, 0, 0);
-    c.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, c.im, 0, 0, 0);
-}
-__device__ __forceinline__ cx tile_get(const ctile& t, int v) { return cm(t.re[v], t.im[v]); }
 
 // Element (i, k) of an N x N column-major matrix (leading dimension ld), zero
 // outside; the address is clamped so the load is branch-free.
 __device__ __forceinline__ cx elem(const cx* M, int ld, int i, int k) {
     const bool in = i < N && k < N;
     const cx v = M[(in ? k : 0) * ld + (in ? i : 0)];
-    return in ? v : cm(0.0, 0.0);
-}
-
-// Element (i, k) of a lower-triangular matrix stored packed by columns in
-// global memory (zero above the diagonal and outside).
-__device__ __forceinline__ cx pelem(const cx* P, int i, int k) {
-    const bool in = i < N && k < N && i >= k;
-    const cx v = P[in ? pk(k) + i - k : 0];
     return in ? v : cm(0.0, 0.0);
 }
 
@@ -2134,7 +2113,7 @@ geneig_mx_refine(const cx* __restrict__ H, const cx* __restrict__ Wd, real* __re
     const bool jin = j < N;
     const size_t off = (size_t)b * N * N;
     const cx* Hb = H + off;
-    const cx* Wb = Wd + (size_t)b * NPK;
+    const cx* Wb = Wd + (size_t)b * N * N;          // W = L^-1, full (upper zero)
     for (int e = threadIdx.x; e < N * N; e += REF_NT) r.U[(e / N) * LDR + e % N] = to_cx(V32[off + e]);
     if (threadIdx.x < N) r.lam[threadIdx.x] = (real)W32[(size_t)b * N + threadIdx.x];
     auto fM = [&](int i, int k) { return melem(r.M, i, k); };
@@ -2145,8 +2124,8 @@ geneig_mx_refine(const cx* __restrict__ H, const cx* __restrict__ Wd, real* __re
         // (which makes it exactly Hermitian).
         ctile Z[NB], M[NB];
         cgemm(Z, [&](int i, int k) { return elem(Hb, N, i, k); },
-              [&](int k, int jj) { return cconj(pelem(Wb, jj, k)); }, cb);
-        cgemm_reg(M, [&](int i, int k) { return pelem(Wb, i, k); }, Z);
+              [&](int k, int jj) { return cconj(elem(Wb, N, jj, k)); }, cb);
+        cgemm_reg(M, [&](int i, int k) { return elem(Wb, N, i, k); }, Z);
 #pragma unroll
         for (int rt = 0; rt < NB; ++rt) {
 #pragma unroll
@@ -2257,7 +2236,7 @@ geneig_mx_refine(const cx* __restrict__ H, const cx* __restrict__ Wd, real* __re
     __syncthreads();
     // U = W^H V.
     ctile Uo[NB];
-    cgemm(Uo, [&](int i, int k) { return cconj(pelem(Wb, k, i)); }, fV, cb);
+    cgemm(Uo, [&](int i, int k) { return cconj(elem(Wb, N, k, i)); }, fV, cb);
     cx* Ub = U + off;
 #pragma unroll
     for (int rt = 0; rt < NB; ++rt) {
@@ -2271,4 +2250,4 @@ geneig_mx_refine(const cx* __restrict__ H, const cx* __restrict__ Wd, real* __re
 }
 #endif  // !GENEIG_FP32
 
-#endif  // GENEIG_N > 64
+#endif  // GENEIG_N <= 64

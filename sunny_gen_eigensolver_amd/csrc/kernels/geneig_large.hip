@@ -1,6 +1,8 @@
 // Orders 64 < N <= 192: the HBM-staged pipeline. Included by
-// geneig_kernels.hip when GENEIG_N > 64 (the LDS-resident kernels hold one
-// row per lane and a whole pair of matrices in LDS, which stops at N = 64).
+// geneig_kernels.hip for every order: the LDS-resident kernels (one row per
+// lane, a whole pair of matrices in LDS) stop at N = 64, so above that every
+// kernel below is built; at N <= 64 only geneig_lg_chol is, for the mixed
+// mode's FP64 Cholesky and W = L^-1.
 //
 // Above that an N x N complex128 matrix alone is 66 KB (N = 64) to 590 KB
 // (N = 192), so every stage is its own kernel, one workgroup of 256 threads
@@ -32,8 +34,8 @@ namespace gk {
 constexpr int LNT = 256;                      // workgroup size of the staged kernels
 constexpr int LNW = LNT / 64;
 constexpr int ND = (N + 63) / 64;             // rows per lane when lanes index rows
-constexpr int NB = (N + 15) / 16;             // 16-blocks per dimension
-constexpr int KEND = (N + 3) / 4 * 4;         // k-steps that touch rows/columns < N
+constexpr int LNB = (N + 15) / 16;             // 16-blocks per dimension
+constexpr int LKEND = (N + 3) / 4 * 4;         // k-steps that touch rows/columns < N
 constexpr size_t NN = (size_t)N * N;
 constexpr size_t LDS_CAP = 160 * 1024;
 
@@ -273,23 +275,10 @@ __device__ __forceinline__ void ql_apply(ZP z, const real* rc, const real* rs, i
 }
 
 // ---------------------------------------------------------------------------
-// Complex FP64 products on the matrix cores (FP64 module only).
-// v_mfma_f64_16x16x4f64 takes A(i, k) from lane i + 16 k and B(k, j) from
-// lane j + 16 k (i, j < 16, k < 4) and leaves D(i, j) in lane j + 16 (i % 4),
-// element i / 4. A 16 x 16 complex tile is a pair of such accumulators.
+// Complex FP64 products on the matrix cores (FP64 module only), in 16 x 16
+// complex tiles (ctile / cmfma).
 // ---------------------------------------------------------------------------
 #ifndef GENEIG_FP32
-typedef double d4 __attribute__((ext_vector_type(4)));
-struct ctile {
-    d4 re, im;
-};
-__device__ __forceinline__ void cmfma(ctile& c, cx a, cx b) {
-    c.re = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, c.re, 0, 0, 0);
-    c.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, c.re, 0, 0, 0);
-    c.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, c.im, 0, 0, 0);
-    c.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, c.im, 0, 0, 0);
-}
-__device__ __forceinline__ cx tile_get(const ctile& t, int v) { return cm(t.re[v], t.im[v]); }
 
 // Element (i, k) of an N x N column-major matrix in global memory, zero
 // outside; the address is clamped so the load is branch-free.
@@ -320,7 +309,7 @@ __device__ __forceinline__ void gemm_rg(ctile (&acc)[RG], FA fa, FB fb, int rt0,
 __device__ __forceinline__ int tile_row(int rt, int v) { return rt * 16 + 4 * v + (lane_id() >> 4); }
 
 struct CholWork {
-    cx pan[256 * NB];         // panel rows j0.. (row i - j0), 16 columns; trtri: Y, then X
+    cx pan[256 * LNB];         // panel rows j0.. (row i - j0), 16 columns; trtri: Y, then X
     cx lii[256];              // trtri: diagonal block of L
     real rinv[16];
     int fail;
@@ -351,11 +340,11 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
     if (tid == 0) cw.fail = 0;
     __syncthreads();
     auto fL = [&](int i, int k) { return gel(Lg, i, k); };
-    for (int p = 0; p < NB; ++p) {
+    for (int p = 0; p < LNB; ++p) {
         const int j0 = 16 * p;
         const int jw = (N - j0 < 16) ? N - j0 : 16;
         // Panel rows j0.. of columns j0 .. j0+jw: S minus the columns left of it.
-        for (int rt = p + wave; rt < NB; rt += LNW) {
+        for (int rt = p + wave; rt < LNB; rt += LNW) {
             ctile acc[1];
             gemm_rg<1>(acc, fL, [&](int k, int c) { return cconj(c < jw ? gel(Lg, j0 + c, k) : cm(0.0, 0.0)); },
                        rt, 0, 0, j0);
@@ -409,7 +398,7 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
     }
     // W = L^-1 in place, block row ib after block row ib-1: with X = W,
     // X(ib, j) = L_ii^-1 (delta - sum_{k < r0} L(ib, k) X(k, j)).
-    for (int ib = 0; ib < NB; ++ib) {
+    for (int ib = 0; ib < LNB; ++ib) {
         const int r0 = 16 * ib;
         const int rw = (N - r0 < 16) ? N - r0 : 16;
         for (int e = tid; e < 256; e += LNT) {
@@ -444,6 +433,7 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
     if (tid == 0) info[b] = 0;
 }
 
+#if GENEIG_N > 64
 // M = W H W^H: X = W H (W lower triangular: k <= row), then the lower tiles
 // of X W^H, mirrored so that M is exactly Hermitian.
 extern "C" __global__ void __launch_bounds__(gk::LNT)
@@ -458,10 +448,10 @@ geneig_lg_formm(const cx* __restrict__ H, const int* __restrict__ info, char* __
     cx* X = reinterpret_cast<cx*>(wsb + WS_X);
     cx* M = reinterpret_cast<cx*>(wsb + WS_M);
     const cx* Hb = H + (size_t)b * NN;
-    constexpr int NRG = (NB + 3) / 4;
-    for (int t = wave; t < NRG * NB; t += LNW) {
-        const int rt0 = 4 * (t / NB), ct = t % NB;
-        const int ke = (16 * (rt0 + 4) < KEND) ? 16 * (rt0 + 4) : KEND;
+    constexpr int NRG = (LNB + 3) / 4;
+    for (int t = wave; t < NRG * LNB; t += LNW) {
+        const int rt0 = 4 * (t / LNB), ct = t % LNB;
+        const int ke = (16 * (rt0 + 4) < LKEND) ? 16 * (rt0 + 4) : LKEND;
         ctile acc[4];
         gemm_rg<4>(acc, [&](int i, int k) { return gel(Wi, i, k); }, [&](int k, int j) { return gel(Hb, k, j); },
                    rt0, ct, 0, ke);
@@ -476,9 +466,9 @@ geneig_lg_formm(const cx* __restrict__ H, const int* __restrict__ info, char* __
     }
     __syncthreads();
     int t = 0;
-    for (int ct = 0; ct < NB; ++ct) {
-        const int ke = (16 * (ct + 1) < KEND) ? 16 * (ct + 1) : KEND;
-        for (int rt0 = ct; rt0 < NB; rt0 += 4, ++t) {
+    for (int ct = 0; ct < LNB; ++ct) {
+        const int ke = (16 * (ct + 1) < LKEND) ? 16 * (ct + 1) : LKEND;
+        for (int rt0 = ct; rt0 < LNB; rt0 += 4, ++t) {
             if (t % LNW != wave) continue;
             ctile acc[4];
             gemm_rg<4>(acc, [&](int i, int k) { return gel(X, i, k); },
@@ -516,12 +506,12 @@ geneig_lg_backtri(cx* __restrict__ U, const int* __restrict__ info, char* __rest
     const cx* Wi = reinterpret_cast<const cx*>(wsb + WS_WI);
     const cx* V = reinterpret_cast<const cx*>(wsb + WS_V);
     cx* Ub = U + (size_t)b * NN;
-    constexpr int NRG = (NB + 3) / 4;
-    for (int t = wave; t < NRG * NB; t += LNW) {
-        const int rt0 = 4 * (t / NB), ct = t % NB;
+    constexpr int NRG = (LNB + 3) / 4;
+    for (int t = wave; t < NRG * LNB; t += LNW) {
+        const int rt0 = 4 * (t / LNB), ct = t % LNB;
         ctile acc[4];
         gemm_rg<4>(acc, [&](int i, int k) { return cconj(gel(Wi, k, i)); }, [&](int k, int j) { return gel(V, k, j); },
-                   rt0, ct, 16 * rt0, KEND);
+                   rt0, ct, 16 * rt0, LKEND);
         const int j = ct * 16 + (lane & 15);
 #pragma unroll
         for (int g = 0; g < 4; ++g)
@@ -532,8 +522,10 @@ geneig_lg_backtri(cx* __restrict__ U, const int* __restrict__ info, char* __rest
             }
     }
 }
+#endif  // GENEIG_N > 64
 #endif  // !GENEIG_FP32
 
+#if GENEIG_N > 64
 // Tridiagonalisation of M (read as FP64, reduced in the module precision):
 // d, e and the reflectors to the workspace.
 extern "C" __global__ void __launch_bounds__(gk::LNT)
@@ -914,16 +906,16 @@ geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, const int* __restrict
     for (int e = tid; e < (int)NN; e += LNT) Va[e] = to_cx(V32[e]);
     for (int e = tid; e < N; e += LNT) lam[e] = (real)lam32[e];
     __syncthreads();
-    constexpr int NRG = (NB + 3) / 4;
+    constexpr int NRG = (LNB + 3) / 4;
     real delta = 0.0;
     for (int it = 0; it < iters; ++it) {
         const cx* V = Va;
         // P = M V.
-        for (int t = wave; t < NRG * NB; t += LNW) {
-            const int rt0 = 4 * (t / NB), ct = t % NB;
+        for (int t = wave; t < NRG * LNB; t += LNW) {
+            const int rt0 = 4 * (t / LNB), ct = t % LNB;
             ctile acc[4];
             gemm_rg<4>(acc, [&](int i, int k) { return gel(M, i, k); }, [&](int k, int j) { return gel(V, k, j); },
-                       rt0, ct, 0, KEND);
+                       rt0, ct, 0, LKEND);
             const int j = ct * 16 + (lane & 15);
 #pragma unroll
             for (int g = 0; g < 4; ++g)
@@ -960,12 +952,12 @@ geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, const int* __restrict
         // stays close and ends in Rayleigh-Ritz.
         delta = delta_rel * (real)(1 << it) * mx;
         // E from the tiles of T = V^H P and G = V^H V.
-        for (int t = wave; t < NRG * NB; t += LNW) {
-            const int rt0 = 4 * (t / NB), ct = t % NB;
+        for (int t = wave; t < NRG * LNB; t += LNW) {
+            const int rt0 = 4 * (t / LNB), ct = t % LNB;
             ctile T[4], G[4];
             gemm2_rg<4>(T, G, [&](int i, int k) { return cconj(gel(V, k, i)); },
                         [&](int k, int j) { return gel(P, k, j); }, [&](int k, int j) { return gel(V, k, j); }, rt0,
-                        ct, 0, KEND);
+                        ct, 0, LKEND);
             const int j = ct * 16 + (lane & 15);
             const real lj = lam[j < N ? j : 0];
 #pragma unroll
@@ -986,11 +978,11 @@ geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, const int* __restrict
         }
         __syncthreads();
         // V <- V + V E, into the other buffer.
-        for (int t = wave; t < NRG * NB; t += LNW) {
-            const int rt0 = 4 * (t / NB), ct = t % NB;
+        for (int t = wave; t < NRG * LNB; t += LNW) {
+            const int rt0 = 4 * (t / LNB), ct = t % LNB;
             ctile acc[4];
             gemm_rg<4>(acc, [&](int i, int k) { return gel(V, i, k); }, [&](int k, int j) { return gel(E, k, j); },
-                       rt0, ct, 0, KEND);
+                       rt0, ct, 0, LKEND);
             const int j = ct * 16 + (lane & 15);
 #pragma unroll
             for (int g = 0; g < 4; ++g)
@@ -1043,11 +1035,11 @@ geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, const int* __restrict
     __syncthreads();
     // U = W^H V.
     const cx* V = Va;
-    for (int t = wave; t < NRG * NB; t += LNW) {
-        const int rt0 = 4 * (t / NB), ct = t % NB;
+    for (int t = wave; t < NRG * LNB; t += LNW) {
+        const int rt0 = 4 * (t / LNB), ct = t % LNB;
         ctile acc[4];
         gemm_rg<4>(acc, [&](int i, int k) { return cconj(gel(Wi, k, i)); }, [&](int k, int j) { return gel(V, k, j); },
-                   rt0, ct, 16 * rt0, KEND);
+                   rt0, ct, 16 * rt0, LKEND);
         const int j = ct * 16 + (lane & 15);
 #pragma unroll
         for (int g = 0; g < 4; ++g)
@@ -1114,3 +1106,4 @@ geneig_lg_scatter(const real* __restrict__ Wp, const cx* __restrict__ Up, const 
     }
 }
 #endif  // !GENEIG_FP32
+#endif  // GENEIG_N > 64

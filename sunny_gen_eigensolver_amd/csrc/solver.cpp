@@ -174,7 +174,7 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
     k_heev_ = module_->get("geneig_split_ql");
     k_back_ = module_->get("geneig_split_back");
     if (mode_ == Mode::Mixed) {
-        k_mx_chol_ = module_->get("geneig_mx_chol");
+        k_mx_chol_ = module_->get("geneig_lg_chol");
         k_refine_ = module_->get("geneig_mx_refine");
         ModuleKey k32 = key;
         k32.fp32 = true;
@@ -346,7 +346,7 @@ void Solver::ensure_mixed_workspace_(size_t batch) {
         mbuf_ = nullptr;
         mbuf_batch_ = 0;
     }
-    // complex128 W = L^-1 (packed), complex64 stash / V, float eigenvalues,
+    // complex128 W = L^-1 (full), complex64 stash / V, float eigenvalues,
     // off-diagonals and Z.
     SG_HIP(hipMalloc(&mbuf_, workspace_bytes(Mode::Mixed, n_, batch)));
     mbuf_batch_ = batch;
@@ -358,18 +358,21 @@ void Solver::ensure_mixed_workspace_(size_t batch) {
 void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int* info, int batch,
                            hipStream_t stream) {
     ensure_mixed_workspace_((size_t)batch);
-    const size_t n = (size_t)n_, npk = n * (n + 1) / 2, B = (size_t)batch;
+    const size_t n = (size_t)n_, B = (size_t)batch;
     char* base = static_cast<char*>(mbuf_);
     void* Wd = base;
-    void* V32 = base + B * npk * 16;
-    void* W32 = base + B * (npk * 16 + n * n * 8);
-    void* E32 = base + B * (npk * 16 + n * n * 8 + n * 4);
-    void* Z32 = base + B * (npk * 16 + n * n * 8 + n * 8);
+    void* V32 = base + B * n * n * 16;
+    void* W32 = base + B * (n * n * 16 + n * n * 8);
+    void* E32 = base + B * (n * n * 16 + n * n * 8 + n * 4);
+    void* Z32 = base + B * (n * n * 16 + n * n * 8 + n * 8);
     const unsigned grid_b = (unsigned)batch;
     const unsigned grid_w = (unsigned)((batch + bpb_ - 1) / bpb_);
     const unsigned bd = (unsigned)block_dim_;
-    void* a1[] = {&S, &W, &U, &info, &Wd, &batch};
-    SG_HIP(hipModuleLaunchKernel(k_mx_chol_, grid_b, 1, 1, bd, 1, 1, 0, stream, a1, nullptr));
+    // FP64 Cholesky and W = L^-1 (full, upper zero): the staged kernels'
+    // panel factorisation, one 256-thread workgroup per matrix.
+    long long wstride = (long long)(n * n * 16);
+    void* a1[] = {&S, &W, &U, &info, &Wd, &wstride, &batch};
+    SG_HIP(hipModuleLaunchKernel(k_mx_chol_, grid_b, 1, 1, 256, 1, 1, 0, stream, a1, nullptr));
     mark_(stream);
     void* a2[] = {&H, &Wd, &W32, &V32, &info, &E32, &batch};
     SG_HIP(hipModuleLaunchKernel(k_mx_front_, grid_b, 1, 1, bd, 1, 1, 0, stream, a2, nullptr));
@@ -468,9 +471,9 @@ void Solver::launch_large_(const void* H, const void* S, double* W, void* U, int
 
 size_t Solver::workspace_bytes(Mode mode, int n_, size_t batch) {
     if (n_ > kMaxLdsOrder) return std::min(batch, large_chunk(n_)) * large_matrix_bytes(n_);
-    const size_t n = (size_t)n_, npk = n * (n + 1) / 2;
+    const size_t n = (size_t)n_;
     if (mode == Mode::Hybrid) return batch * (n + n * n) * 8;
-    if (mode == Mode::Mixed) return batch * (npk * 16 + n * n * 8 + n * 4 + n * 4 + n * n * 4);
+    if (mode == Mode::Mixed) return batch * (n * n * 16 + n * n * 8 + n * 4 + n * 4 + n * n * 4);
     return 0;
 }
 
@@ -522,7 +525,7 @@ const char* Solver::stage_name(Mode mode, int i, int n) {
     }
     static const char* unified[] = {"geneig_fused"};
     static const char* hybrid[] = {"geneig_split_front", "geneig_split_ql", "geneig_split_back"};
-    static const char* mixed[] = {"geneig_mx_chol", "geneig_mx_front", "geneig_split_ql",
+    static const char* mixed[] = {"geneig_lg_chol", "geneig_mx_front", "geneig_split_ql",
                                   "geneig_mx_back", "geneig_mx_refine"};
     if (mode == Mode::Unified) return (i == 0) ? unified[0] : "";
     if (mode == Mode::Hybrid) return (i >= 0 && i < 3) ? hybrid[i] : "";
