@@ -311,8 +311,6 @@ __device__ __forceinline__ int tile_row(int rt, int v) { return rt * 16 + 4 * v 
 struct CholWork {
     cx pan[256 * LNB];         // panel rows j0.. (row i - j0), 16 columns; trtri: Y, then X
     cx lii[256];              // trtri: diagonal block of L
-    real rinv[16];
-    int fail;
 };
 #endif  // !GENEIG_FP32
 
@@ -325,7 +323,7 @@ using gk::real;
 // S = L L^H and W = L^-1 in the workspace (upper triangle zero). On a
 // non-positive pivot k (0-based) of a leading minor: info = N + k + 1, W NaN,
 // U zero (LAPACK zhegv's convention), and the later stages skip the matrix.
-extern "C" __global__ void __launch_bounds__(gk::LNT)
+extern "C" __global__ void __launch_bounds__(gk::LNT, 3)
 geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ U, int* __restrict__ info,
                char* __restrict__ ws, long long wss, int batch) {
     using namespace gk;
@@ -337,7 +335,6 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
     const cx* Sb = S + (size_t)b * NN;
     for (int e = tid; e < (int)NN; e += LNT)
         if (e % N < e / N) Lg[e] = cm(0.0, 0.0);
-    if (tid == 0) cw.fail = 0;
     __syncthreads();
     auto fL = [&](int i, int k) { return gel(Lg, i, k); };
     for (int p = 0; p < LNB; ++p) {
@@ -356,33 +353,29 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
             }
         }
         __syncthreads();
-        // Unblocked Cholesky of the panel in LDS, thread t = row j0 + t
-        // (the panel row t is only ever touched by thread t, except that
-        // step c reads the pivot column's rows c2 < jw after a barrier).
+        // Unblocked Cholesky of the panel in LDS, thread t = row j0 + t, one
+        // barrier per column: every thread derives the pivot of column c from
+        // LDS, reads column c unscaled (scaling each entry itself) and writes
+        // its scaled entry of column c straight to L in global memory, so
+        // column c is never rewritten in LDS while others read it.
         const int t = tid;
         const bool tin = t < N - j0;
         int fail = 0;
         for (int c = 0; c < jw; ++c) {
-            if (t == c) {
-                const real piv = cw.pan[c * N + c].x;
-                if (!(piv > real(0.0))) {
-                    cw.fail = j0 + c + 1;
-                } else {
-                    const real sq = sqrt_r(piv);
-                    cw.rinv[c] = real(1.0) / sq;
-                    cw.pan[c * N + c] = cm(sq, 0.0);
-                }
+            const real piv = cw.pan[c * N + c].x;
+            if (!(piv > real(0.0))) {
+                fail = j0 + c + 1;
+                break;
             }
-            __syncthreads();
-            fail = cw.fail;
-            if (fail) break;
-            if (t > c && tin) cw.pan[c * N + t] = cw.pan[c * N + t] * cw.rinv[c];
-            __syncthreads();
-            if (t > c && tin) {
-                const cx ltc = cw.pan[c * N + t];
+            const real sq = sqrt_r(piv);
+            const real ri = real(1.0) / sq;
+            if (t >= c && tin) {
+                const cx ltc = (t == c) ? cm(sq, 0.0) : cw.pan[c * N + t] * ri;
+                Lg[(size_t)(j0 + c) * N + j0 + t] = ltc;
                 for (int c2 = c + 1; c2 < jw && c2 <= t; ++c2)
-                    cw.pan[c2 * N + t] = cw.pan[c2 * N + t] - cmulc(ltc, cw.pan[c * N + c2]);
+                    cw.pan[c2 * N + t] = cw.pan[c2 * N + t] - cmulc(ltc, cw.pan[c * N + c2] * ri);
             }
+            __syncthreads();
         }
         if (fail) {
             real* Wb = W + (size_t)b * N;
@@ -392,8 +385,6 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
             if (tid == 0) info[b] = N + fail;
             return;
         }
-        for (int c = 0; c < jw; ++c)
-            if (tin && t >= c) Lg[(size_t)(j0 + c) * N + j0 + t] = cw.pan[c * N + t];
         __syncthreads();
     }
     // W = L^-1 in place, block row ib after block row ib-1: with X = W,
