@@ -44,10 +44,11 @@ def flops_per_solve(n: int) -> float:
 
 
 def refine_flops_per_solve(n: int, iters: int) -> float:
-    """Complex N^3 GEMMs of the refinement kernel, 8 real flops per complex
-    multiply-add: M = W (H W^H) (2), per step X = M V, T = V^H X, G = V^H V,
-    V E (4), and U = W^H V (1)."""
-    return 8.0 * (3 + 4 * iters) * n ** 3
+    """Algorithmic complex multiply-adds of the refinement kernel, 8 real
+    flops each: with W = L^-1 lower triangular, Z = H W^H (N^3 / 2), the lower
+    triangle of M = W Z (N^3 / 3) and U = W^H V (N^3 / 2); per step X = M V,
+    T = V^H X, G = V^H V and V E (4 N^3)."""
+    return 8.0 * (4.0 / 3.0 + 4 * iters) * n ** 3
 
 
 def bytes_per_solve(n: int) -> float:
@@ -320,9 +321,10 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": dom_name,
-                     "note": f"{dom_name}: {rf:.0f} flops per solve (complex N^3 GEMMs of the refinement, "
-                             f"8 real flops per complex multiply-add) x batch / its launch time, timed with "
-                             f"HIP events on the launch stream; FP64 MFMA peak"},
+                     "note": f"{dom_name}: {rf:.0f} algorithmic flops per solve (the refinement's complex "
+                             f"products, triangular ones at their true size, 8 real flops per complex "
+                             f"multiply-add) x batch / its launch time, timed with HIP events on the launch "
+                             f"stream; FP64 MFMA peak"},
         "path_fp64_equivalent": {
             "achieved_TFLOPs": B * flops_per_solve(n) / (ms * 1e-3) / 1e12,
             "frac_of_fp64_peak": B * flops_per_solve(n) / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,

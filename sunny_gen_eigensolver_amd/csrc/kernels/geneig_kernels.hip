@@ -1987,9 +1987,10 @@ __device__ __forceinline__ cx elem(const cx* M, int ld, int i, int k) {
 }
 
 // acc[rt] (row tile rt, column block cb) = A B, A(i, k) = fa(i, k),
-// B(k, j) = fb(k, j), k < 16 NB (the fetchers return zero outside N).
-template <class FA, class FB>
-__device__ __forceinline__ void cgemm(ctile (&acc)[NB], FA fa, FB fb, int cb) {
+// B(k, j) = fb(k, j), k < ke (the fetchers return zero outside N). With
+// UPPER_A, A is upper triangular: row tile rt only takes k >= 16 rt.
+template <bool UPPER_A = false, class FA, class FB>
+__device__ __forceinline__ void cgemm(ctile (&acc)[NB], FA fa, FB fb, int cb, int ke = KEND) {
     const int l = lane_id();
     const int c16 = l & 15, k4 = l >> 4;
 #pragma unroll
@@ -1998,10 +1999,11 @@ __device__ __forceinline__ void cgemm(ctile (&acc)[NB], FA fa, FB fb, int cb) {
         acc[rt].im = d4{0.0, 0.0, 0.0, 0.0};
     }
 #pragma unroll 2
-    for (int k0 = 0; k0 < KEND; k0 += 4) {
+    for (int k0 = 0; k0 < ke; k0 += 4) {
         const cx b = fb(k0 + k4, cb * 16 + c16);
 #pragma unroll
-        for (int rt = 0; rt < NB; ++rt) cmfma(acc[rt], fa(rt * 16 + c16, k0 + k4), b);
+        for (int rt = 0; rt < NB; ++rt)
+            if (!UPPER_A || k0 >= 16 * rt) cmfma(acc[rt], fa(rt * 16 + c16, k0 + k4), b);
     }
 }
 
@@ -2009,8 +2011,10 @@ __device__ __forceinline__ void cgemm(ctile (&acc)[NB], FA fa, FB fb, int cb) {
 // held in registers in the result layout: the result layout of row tile q is
 // the B-operand layout of the k-steps 16 q .. 16 q + 15 (element v of tile q
 // = rows 16 q + 4 v .. 16 q + 4 v + 3), so no LDS round trip is needed.
-template <class FA>
-__device__ __forceinline__ void cgemm_reg(ctile (&acc)[NB], FA fa, const ctile (&Bt)[NB]) {
+// With LOWER_A, A is lower triangular (row tile rt only takes k-tiles q <= rt)
+// and only the row tiles rt >= rt_min are wanted.
+template <bool LOWER_A = false, class FA>
+__device__ __forceinline__ void cgemm_reg(ctile (&acc)[NB], FA fa, const ctile (&Bt)[NB], int rt_min = 0) {
     const int l = lane_id();
     const int c16 = l & 15, k4 = l >> 4;
 #pragma unroll
@@ -2026,7 +2030,10 @@ __device__ __forceinline__ void cgemm_reg(ctile (&acc)[NB], FA fa, const ctile (
             const cx b = tile_get(Bt[q], v);
             const int k = 16 * q + 4 * v + k4;
 #pragma unroll
-            for (int rt = 0; rt < NB; ++rt) cmfma(acc[rt], fa(rt * 16 + c16, k), b);
+            for (int rt = 0; rt < NB; ++rt) {
+                if (LOWER_A && (q > rt || rt < rt_min)) continue;
+                cmfma(acc[rt], fa(rt * 16 + c16, k), b);
+            }
         }
     }
 }
@@ -2123,9 +2130,12 @@ geneig_mx_refine(const cx* __restrict__ H, const cx* __restrict__ Wd, real* __re
         // M = W (H W^H), operands from global memory; its lower triangle to LDS
         // (which makes it exactly Hermitian).
         ctile Z[NB], M[NB];
+        // W is lower triangular: Z(:, j) needs k <= j; M(i, j) needs k <= i,
+        // and only the tiles on and below the diagonal are kept.
         cgemm(Z, [&](int i, int k) { return elem(Hb, N, i, k); },
-              [&](int k, int jj) { return cconj(elem(Wb, N, jj, k)); }, cb);
-        cgemm_reg(M, [&](int i, int k) { return elem(Wb, N, i, k); }, Z);
+              [&](int k, int jj) { return cconj(elem(Wb, N, jj, k)); }, cb,
+              16 * (cb + 1) < KEND ? 16 * (cb + 1) : KEND);
+        cgemm_reg<true>(M, [&](int i, int k) { return elem(Wb, N, i, k); }, Z, cb);
 #pragma unroll
         for (int rt = 0; rt < NB; ++rt) {
 #pragma unroll
@@ -2236,7 +2246,7 @@ geneig_mx_refine(const cx* __restrict__ H, const cx* __restrict__ Wd, real* __re
     __syncthreads();
     // U = W^H V.
     ctile Uo[NB];
-    cgemm(Uo, [&](int i, int k) { return cconj(elem(Wb, N, k, i)); }, fV, cb);
+    cgemm<true>(Uo, [&](int i, int k) { return cconj(elem(Wb, N, k, i)); }, fV, cb);
     cx* Ub = U + off;
 #pragma unroll
     for (int rt = 0; rt < NB; ++rt) {
