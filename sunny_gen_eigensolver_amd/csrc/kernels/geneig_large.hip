@@ -48,7 +48,18 @@ constexpr size_t WS_V = 48 * NN;              // V (complex, module precision; F
 constexpr size_t WS_R = 64 * NN;              // reflectors: v_k(i) at (i, k), i >= k+2; tau_k at (k+1, k)
 constexpr size_t WS_Z = 80 * NN;              // Z, columns in ascending eigenvalue order (real)
 constexpr size_t WS_D = 88 * NN;              // d[N], e[N], ascending eigenvalues[N] (real)
-constexpr size_t WS_BYTES = (88 * NN + 24 * (size_t)N + 255) / 256 * 256;
+constexpr size_t WS_Q = (88 * NN + 24 * (size_t)N + 255) / 256 * 256;   // QL rotation log
+// The log: header (sweeps, overflow flag), the window (l, m) of every sweep,
+// the ascending rank of every eigenvalue, then the rotations (c_i, s_i) of all
+// sweeps back to back. A matrix whose QL needs more sweeps or rotations than
+// the log holds is redone by the applier with the rotations applied inline.
+constexpr int QL_SW_CAP = 8 * N;
+constexpr int QL_ROT_CAP = 2 * N * N;
+constexpr size_t Q_SW = 16;
+constexpr size_t Q_RANK = Q_SW + 8 * (size_t)QL_SW_CAP;
+constexpr size_t Q_ROT = (Q_RANK + 4 * (size_t)N + 15) / 16 * 16;
+constexpr size_t Q_BYTES = Q_ROT + 16 * (size_t)QL_ROT_CAP;
+constexpr size_t WS_BYTES = (WS_Q + Q_BYTES + 255) / 256 * 256;
 
 __device__ __forceinline__ char* ws_of(char* ws, long long wss, int b) { return ws + (size_t)b * (size_t)wss; }
 
@@ -272,6 +283,154 @@ __device__ __forceinline__ void ql_apply(ZP z, const real* rc, const real* rs, i
         zc = fma(c, zi, -(s * zc));
     }
     z[l * LD] = zc;
+}
+
+// Rotation pair of the log.
+struct rpair {
+    real c, s;
+};
+
+// Apply one sweep's logged rotations (rotation l + j held by lane j % 64 in
+// cur[j / 64], fetched by the wave beforehand with coalesced loads) to a row of
+// Z whose element c sits at z[c * LD], for i = m-1 down to l: each rotation
+// travels to the lanes by v_readlane, so the only dependence between steps is
+// the carried element.
+template <int LD, class ZP>
+__device__ __forceinline__ void ql_apply_reg(ZP z, const rpair (&cur)[ND], int l, int m) {
+    real zc = z[m * LD];
+#pragma unroll
+    for (int q = ND - 1; q >= 0; --q) {
+        const int jtop = m - 1 - l - 64 * q;          // highest rotation of this chunk
+        if (jtop < 0) continue;
+        int jj = jtop > 63 ? 63 : jtop;
+        const int i0 = l + 64 * q;
+        for (; jj >= 3; jj -= 4) {
+            const int i = i0 + jj;
+            const real z0 = z[i * LD], z1 = z[(i - 1) * LD], z2 = z[(i - 2) * LD], z3 = z[(i - 3) * LD];
+            real c = bcast(cur[q].c, jj), s = bcast(cur[q].s, jj);
+            z[(i + 1) * LD] = fma(s, z0, c * zc);
+            zc = fma(c, z0, -(s * zc));
+            c = bcast(cur[q].c, jj - 1);
+            s = bcast(cur[q].s, jj - 1);
+            z[i * LD] = fma(s, z1, c * zc);
+            zc = fma(c, z1, -(s * zc));
+            c = bcast(cur[q].c, jj - 2);
+            s = bcast(cur[q].s, jj - 2);
+            z[(i - 1) * LD] = fma(s, z2, c * zc);
+            zc = fma(c, z2, -(s * zc));
+            c = bcast(cur[q].c, jj - 3);
+            s = bcast(cur[q].s, jj - 3);
+            z[(i - 2) * LD] = fma(s, z3, c * zc);
+            zc = fma(c, z3, -(s * zc));
+        }
+        for (; jj >= 0; --jj) {
+            const int i = i0 + jj;
+            const real zi = z[i * LD];
+            const real c = bcast(cur[q].c, jj), s = bcast(cur[q].s, jj);
+            z[(i + 1) * LD] = fma(s, zi, c * zc);
+            zc = fma(c, zi, -(s * zc));
+        }
+    }
+    z[l * LD] = zc;
+}
+
+// Lane share of one sweep's rotations: rotation l + j (j < cnt) to lane j % 64
+// of register j / 64.
+__device__ __forceinline__ void ql_fetch(rpair (&r)[ND], const rpair* __restrict__ base, int cnt) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int q = 0; q < ND; ++q) {
+        const int j = 64 * q + lane;
+        r[q] = (j < cnt) ? base[j] : rpair{real(1.0), real(0.0)};
+    }
+}
+
+// First m >= l with a negligible off-diagonal e_m (m = N - 1 if none).
+__device__ __forceinline__ int ql_find_m(const real (&dr)[ND], const real (&er)[ND], int l) {
+    const int lane = lane_id();
+    int m = N - 1;
+#pragma unroll
+    for (int q = ND - 1; q >= 0; --q) {
+        real dn = __shfl_down(dr[q], 1, 64);
+        if (q + 1 < ND) {
+            const real nx = bcast(dr[q + 1 < ND ? q + 1 : q], 0);
+            if (lane == 63) dn = nx;
+        }
+        const int i = 64 * q + lane;
+        const bool small = fabs(er[q]) <= REAL_HALF_ULP * (fabs(dr[q]) + fabs(dn));
+        const unsigned long long mask = __ballot(small && i >= l && i < N - 1);
+        if (mask) m = 64 * q + __ffsll((long long)mask) - 1;
+    }
+    return m;
+}
+
+__device__ __forceinline__ int ql_unconverged(const real (&er)[ND], int l) {
+    const int lane = lane_id();
+    int n = 0;
+#pragma unroll
+    for (int q = 0; q < ND; ++q) {
+        const int i = 64 * q + lane;
+        n += __popcll(__ballot(er[q] != real(0.0) && i >= l && i < N - 1));
+    }
+    return n;
+}
+
+// One implicit QL sweep on the window [l, m) with a Wilkinson shift
+// (Bowdler, Martin, Reinsch and Wilkinson, Handbook II/3): updates (d, e)
+// and leaves the sweep's rotation i in element i of (cr, sr).
+__device__ __forceinline__ void ql_sweep(real (&dr)[ND], real (&er)[ND], int l, int m, real (&cr)[ND],
+                                         real (&sr)[ND]) {
+    const real d_l = dget(dr, l);
+    const real e_l = dget(er, l);
+    real g = (dget(dr, l + 1) - d_l) / (real(2.0) * e_l);
+    const real rr = sqrt_r(g * g + real(1.0));
+    const real d_m = dget(dr, m);
+    g = d_m - d_l + e_l / (g + copysign(rr, g));
+    real s = real(1.0), c = real(1.0), p = real(0.0);
+    real d_up = d_m;
+#pragma unroll
+    for (int q = 0; q < ND; ++q) {
+        cr[q] = real(1.0);
+        sr[q] = real(0.0);
+    }
+#pragma unroll
+    for (int i = N - 2; i >= 0; --i) {
+        if (i < m && i >= l) {
+            const real e_i = bcast(er[i >> 6], i & 63);
+            const real d_i = bcast(dr[i >> 6], i & 63);
+            const real f = s * e_i;
+            const real bb = c * e_i;
+            // f = s e_i != 0 inside the window (every e_i there is above the
+            // deflation threshold); the floor keeps t finite near underflow.
+            const real tt = fmax(fma(f, f, g * g), REAL_TINY);
+            const real y = rsqrt_nr(tt);
+            er[(i + 1) >> 6] = sel_lane(er[(i + 1) >> 6], tt * y, (i + 1) & 63);
+            s = f * y;
+            c = g * y;
+            g = d_up - p;
+            const real r2 = fma(d_i - g, s, real(2.0) * c * bb);
+            p = s * r2;
+            dr[(i + 1) >> 6] = sel_lane(dr[(i + 1) >> 6], g + p, (i + 1) & 63);
+            g = fma(c, r2, -bb);
+            cr[i >> 6] = sel_lane(cr[i >> 6], c, i & 63);
+            sr[i >> 6] = sel_lane(sr[i >> 6], s, i & 63);
+            d_up = d_i;
+        }
+    }
+    dset(dr, l, d_l - p);
+    dset(er, l, g);
+    dset(er, m, real(0.0));
+}
+
+// Ascending rank of eigenvalue cidx (ties by index).
+__device__ __forceinline__ int ql_rank(const real (&dr)[ND], real dc, int cidx) {
+    int rank = 0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const real dk = bcast(dr[k >> 6], k & 63);
+        rank += (dk < dc || (dk == dc && k < cidx)) ? 1 : 0;
+    }
+    return rank;
 }
 
 // ---------------------------------------------------------------------------
@@ -517,6 +676,9 @@ geneig_lg_backtri(cx* __restrict__ U, const int* __restrict__ info, char* __rest
 #endif  // !GENEIG_FP32
 
 #if GENEIG_N > 64
+// Bytes of one matrix's workspace, for the host's consistency check.
+extern "C" __device__ const unsigned long long geneig_lg_ws_bytes = gk::WS_BYTES;
+
 // Tridiagonalisation of M (read as FP64, reduced in the module precision):
 // d, e and the reflectors to the workspace.
 extern "C" __global__ void __launch_bounds__(gk::LNT)
@@ -564,24 +726,21 @@ geneig_lg_hetrd(const int* __restrict__ info, char* __restrict__ ws, long long w
 }
 
 // Implicit QL on (d, e) with the rotations accumulated into Z (one row per
-// thread), then ascending order: eigenvalues to wout (matrix b at
-// wout + b * wout_stride) and Z with its columns in that order to the
-// workspace. A sweep that has not converged after QL_MAXIT iterations sets
-// info to the number of off-diagonals left (LAPACK's convention).
-extern "C" __global__ void __launch_bounds__(gk::QNT)
-geneig_lg_ql(real* __restrict__ wout, long long wout_stride, int* __restrict__ info, char* __restrict__ ws,
-             long long wss, int batch) {
-    using namespace gk;
-    __shared__ real zs[ZR_LDS > 0 ? ZR_LDS * N : 1];
-    __shared__ real rot[ND][2][N];
-    __shared__ int rank_s[N];
-    const int b = blockIdx.x;
-    if (b >= batch || info[b] > N) return;
+// thread, QNT threads), then ascending order: eigenvalues to wout (matrix b
+// at wout + b * wout_stride) and Z with its columns in that order to the
+// workspace. Every wave runs the scalar recurrence itself and applies each
+// sweep's rotations to its own rows at once. A sweep that has not converged
+// after QL_MAXIT iterations sets info to the number of off-diagonals left
+// (LAPACK's convention). The inline form, for matrices whose rotations
+// overflow the log.
+namespace gk {
+__device__ __forceinline__ void ql_inline(int b, real* __restrict__ wout, long long wout_stride,
+                                          int* __restrict__ info, char* wsb, real* zs, real (*rot)[2][N],
+                                          int* rank_s) {
     const int lane = lane_id(), wave = wave_id();
     const int r = wave * 64 + lane;
     const bool rin = r < N;
     const bool in_lds = wave < QW_LDS;
-    char* wsb = ws_of(ws, wss, b);
     const real* dsrc = reinterpret_cast<const real*>(wsb + WS_D);
     const real* esrc = dsrc + N;
     real* zg = reinterpret_cast<real*>(wsb + WS_X);      // rows of Z beyond the LDS
@@ -602,74 +761,20 @@ geneig_lg_ql(real* __restrict__ wout, long long wout_stride, int* __restrict__ i
     }
     real* rc = rot[wave][0];
     real* rs = rot[wave][1];
-    const real eps = REAL_HALF_ULP;
     int l = 0, iter = 0, unconverged = 0;
     while (l < N) {
-        int m = N - 1;
-#pragma unroll
-        for (int q = ND - 1; q >= 0; --q) {
-            real dn = __shfl_down(dr[q], 1, 64);
-            if (q + 1 < ND) {
-                const real nx = bcast(dr[q + 1 < ND ? q + 1 : q], 0);
-                if (lane == 63) dn = nx;
-            }
-            const int i = 64 * q + lane;
-            const bool small = fabs(er[q]) <= eps * (fabs(dr[q]) + fabs(dn));
-            const unsigned long long mask = __ballot(small && i >= l && i < N - 1);
-            if (mask) m = 64 * q + __ffsll((long long)mask) - 1;
-        }
+        const int m = ql_find_m(dr, er, l);
         if (m == l) {
             ++l;
             iter = 0;
             continue;
         }
         if (iter++ == QL_MAXIT) {
-#pragma unroll
-            for (int q = 0; q < ND; ++q) {
-                const int i = 64 * q + lane;
-                unconverged += __popcll(__ballot(er[q] != real(0.0) && i >= l && i < N - 1));
-            }
+            unconverged = ql_unconverged(er, l);
             break;
         }
-        const real d_l = dget(dr, l);
-        const real e_l = dget(er, l);
-        real g = (dget(dr, l + 1) - d_l) / (real(2.0) * e_l);
-        real rr = sqrt_r(g * g + real(1.0));
-        const real d_m = dget(dr, m);
-        g = d_m - d_l + e_l / (g + copysign(rr, g));
-        real s = real(1.0), c = real(1.0), p = real(0.0);
-        real d_up = d_m;
         real cr[ND], sr[ND];
-#pragma unroll
-        for (int q = 0; q < ND; ++q) {
-            cr[q] = real(1.0);
-            sr[q] = real(0.0);
-        }
-#pragma unroll
-        for (int i = N - 2; i >= 0; --i) {
-            if (i < m && i >= l) {
-                const real e_i = bcast(er[i >> 6], i & 63);
-                const real d_i = bcast(dr[i >> 6], i & 63);
-                const real f = s * e_i;
-                const real bb = c * e_i;
-                const real tt = fmax(fma(f, f, g * g), REAL_TINY);
-                const real y = rsqrt_nr(tt);
-                er[(i + 1) >> 6] = sel_lane(er[(i + 1) >> 6], tt * y, (i + 1) & 63);
-                s = f * y;
-                c = g * y;
-                g = d_up - p;
-                const real r2 = fma(d_i - g, s, real(2.0) * c * bb);
-                p = s * r2;
-                dr[(i + 1) >> 6] = sel_lane(dr[(i + 1) >> 6], g + p, (i + 1) & 63);
-                g = fma(c, r2, -bb);
-                cr[i >> 6] = sel_lane(cr[i >> 6], c, i & 63);
-                sr[i >> 6] = sel_lane(sr[i >> 6], s, i & 63);
-                d_up = d_i;
-            }
-        }
-        dset(dr, l, d_l - p);
-        dset(er, l, g);
-        dset(er, m, real(0.0));
+        ql_sweep(dr, er, l, m, cr, sr);
 #pragma unroll
         for (int q = 0; q < ND; ++q) {
             const int i = 64 * q + lane;
@@ -685,19 +790,14 @@ geneig_lg_ql(real* __restrict__ wout, long long wout_stride, int* __restrict__ i
         }
         wave_lds_sync();
     }
-    // Ascending ranks (ties by index): wave q ranks eigenvalues 64 q + lane.
+    // Wave q ranks eigenvalues 64 q + lane.
     {
         const int cidx = 64 * wave + lane;
         real dc = dr[0];
 #pragma unroll
         for (int q = 1; q < ND; ++q)
             if (wave == q) dc = dr[q];
-        int rank = 0;
-#pragma unroll
-        for (int k = 0; k < N; ++k) {
-            const real dk = bcast(dr[k >> 6], k & 63);
-            rank += (dk < dc || (dk == dc && k < cidx)) ? 1 : 0;
-        }
+        const int rank = ql_rank(dr, dc, cidx);
         if (cidx < N) {
             rank_s[cidx] = rank;
             wout[(size_t)b * (size_t)wout_stride + rank] = dc;
@@ -710,6 +810,155 @@ geneig_lg_ql(real* __restrict__ wout, long long wout_stride, int* __restrict__ i
             for (int c = 0; c < N; ++c) zout[(size_t)rank_s[c] * N + r] = zs[c * ZR_LDS + r];
         } else {
             for (int c = 0; c < N; ++c) zout[(size_t)rank_s[c] * N + r] = zg[(size_t)c * N + r];
+        }
+    }
+}
+}  // namespace gk
+
+// QL, first half: one wave per matrix runs the scalar recurrence alone
+// (registers only, so many matrices share a CU and hide each other's
+// latency) and logs every sweep's window and rotations, then the ranks, and
+// writes the ascending eigenvalues to wout and info as above.
+extern "C" __global__ void __launch_bounds__(64)
+geneig_lg_qlr(real* __restrict__ wout, long long wout_stride, int* __restrict__ info, char* __restrict__ ws,
+              long long wss, int batch) {
+    using namespace gk;
+    const int b = blockIdx.x;
+    if (b >= batch || info[b] > N) return;
+    const int lane = lane_id();
+    char* wsb = ws_of(ws, wss, b);
+    const real* dsrc = reinterpret_cast<const real*>(wsb + WS_D);
+    const real* esrc = dsrc + N;
+    char* qb = wsb + WS_Q;
+    int* hdr = reinterpret_cast<int*>(qb);
+    int* sw = reinterpret_cast<int*>(qb + Q_SW);
+    int* rk = reinterpret_cast<int*>(qb + Q_RANK);
+    rpair* rot = reinterpret_cast<rpair*>(qb + Q_ROT);
+    real dr[ND], er[ND];
+#pragma unroll
+    for (int q = 0; q < ND; ++q) {
+        const int i = 64 * q + lane;
+        dr[q] = (i < N) ? dsrc[i] : real(0.0);
+        er[q] = (i < N - 1) ? esrc[i] : real(0.0);
+    }
+    int l = 0, iter = 0, nsw = 0, nrot = 0, unconverged = 0;
+    bool overflow = false;
+    while (l < N) {
+        const int m = ql_find_m(dr, er, l);
+        if (m == l) {
+            ++l;
+            iter = 0;
+            continue;
+        }
+        if (iter++ == QL_MAXIT) {
+            unconverged = ql_unconverged(er, l);
+            break;
+        }
+        if (nsw == QL_SW_CAP || nrot + (m - l) > QL_ROT_CAP) {
+            overflow = true;
+            break;
+        }
+        real cr[ND], sr[ND];
+        ql_sweep(dr, er, l, m, cr, sr);
+#pragma unroll
+        for (int q = 0; q < ND; ++q) {
+            const int i = 64 * q + lane;
+            if (i >= l && i < m) rot[nrot + i - l] = rpair{cr[q], sr[q]};
+        }
+        if (lane == 0) {
+            sw[2 * nsw] = l;
+            sw[2 * nsw + 1] = m;
+        }
+        ++nsw;
+        nrot += m - l;
+    }
+    if (overflow) {
+        if (lane == 0) {
+            hdr[0] = 0;
+            hdr[1] = 1;
+        }
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < ND; ++q) {
+        const int cidx = 64 * q + lane;
+        const int rank = ql_rank(dr, dr[q], cidx);
+        if (cidx < N) {
+            rk[cidx] = rank;
+            wout[(size_t)b * (size_t)wout_stride + rank] = dr[q];
+        }
+    }
+    if (lane == 0) {
+        hdr[0] = nsw;
+        hdr[1] = 0;
+        if (unconverged) info[b] = unconverged;
+    }
+}
+
+// QL, second half: Z = product of the logged rotations, one row per thread
+// (rows in LDS, or in global memory for the rows that do not fit), then Z with
+// its columns in ascending order to the workspace. A matrix whose log
+// overflowed is solved here inline instead.
+extern "C" __global__ void __launch_bounds__(gk::QNT)
+geneig_lg_qlz(real* __restrict__ wout, long long wout_stride, int* __restrict__ info, char* __restrict__ ws,
+              long long wss, int batch) {
+    using namespace gk;
+    __shared__ real zs[ZR_LDS > 0 ? ZR_LDS * N : 1];
+    __shared__ real rotb[ND][2][N];
+    __shared__ int rank_s[N];
+    const int b = blockIdx.x;
+    if (b >= batch || info[b] > N) return;
+    char* wsb = ws_of(ws, wss, b);
+    const char* qb = wsb + WS_Q;
+    const int* hdr = reinterpret_cast<const int*>(qb);
+    if (hdr[1]) {
+        gk::ql_inline(b, wout, wout_stride, info, wsb, zs, rotb, rank_s);
+        return;
+    }
+    const int* sw = reinterpret_cast<const int*>(qb + Q_SW);
+    const int* rk = reinterpret_cast<const int*>(qb + Q_RANK);
+    const rpair* rot = reinterpret_cast<const rpair*>(qb + Q_ROT);
+    const int lane = lane_id(), wave = wave_id();
+    const int r = wave * 64 + lane;
+    const bool rin = r < N;
+    const bool in_lds = wave < QW_LDS;
+    real* zg = reinterpret_cast<real*>(wsb + WS_X);
+    real* zout = reinterpret_cast<real*>(wsb + WS_Z);
+    if (rin) {
+        if (in_lds) {
+            for (int c = 0; c < N; ++c) zs[c * ZR_LDS + r] = (c == r) ? real(1.0) : real(0.0);
+        } else {
+            for (int c = 0; c < N; ++c) zg[(size_t)c * N + r] = (c == r) ? real(1.0) : real(0.0);
+        }
+    }
+    const int nsw = hdr[0];
+    // The next sweep's window and rotations are fetched while the current one
+    // is applied.
+    int off = 0;
+    int l = nsw > 0 ? sw[0] : 0, m = nsw > 0 ? sw[1] : 0;
+    rpair cur[ND];
+    ql_fetch(cur, rot, m - l);
+    for (int t = 0; t < nsw; ++t) {
+        const int off_n = off + (m - l);
+        const bool last = t + 1 == nsw;
+        const int ln = last ? 0 : sw[2 * t + 2], mn = last ? 0 : sw[2 * t + 3];
+        rpair nxt[ND];
+        ql_fetch(nxt, rot + off_n, mn - ln);
+        if (rin) {
+            if (in_lds) ql_apply_reg<ZR_LDS>(zs + r, cur, l, m);
+            else ql_apply_reg<N>(zg + r, cur, l, m);
+        }
+#pragma unroll
+        for (int q = 0; q < ND; ++q) cur[q] = nxt[q];
+        off = off_n;
+        l = ln;
+        m = mn;
+    }
+    if (rin) {
+        if (in_lds) {
+            for (int c = 0; c < N; ++c) zout[(size_t)rk[c] * N + r] = zs[c * ZR_LDS + r];
+        } else {
+            for (int c = 0; c < N; ++c) zout[(size_t)rk[c] * N + r] = zg[(size_t)c * N + r];
         }
     }
 }

@@ -153,15 +153,27 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
             module32_ = acquire_module(k32, cache, &l32, &ms32);
             acquire_ms_ += ms32;
             k_lg_hetrd_ = module32_->get("geneig_lg_hetrd");
-            k_lg_ql_ = module32_->get("geneig_lg_ql");
+            k_lg_ql_ = module32_->get("geneig_lg_qlr");
+            k_lg_qlz_ = module32_->get("geneig_lg_qlz");
             k_lg_back_ = module32_->get("geneig_lg_back");
             k_lg_refine_ = module_->get("geneig_lg_refine");
             if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_ITERS")) refine_iters_ = std::max(0, std::atoi(v));
             if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_DELTA")) refine_delta_ = std::atof(v);
         } else {
             k_lg_hetrd_ = module_->get("geneig_lg_hetrd");
-            k_lg_ql_ = module_->get("geneig_lg_ql");
+            k_lg_ql_ = module_->get("geneig_lg_qlr");
+            k_lg_qlz_ = module_->get("geneig_lg_qlz");
             k_lg_back_ = module_->get("geneig_lg_back");
+        }
+        {
+            hipDeviceptr_t sym = nullptr;
+            size_t sym_bytes = 0;
+            unsigned long long dev_ws = 0;
+            SG_HIP(hipModuleGetGlobal(&sym, &sym_bytes, module_->module, "geneig_lg_ws_bytes"));
+            SG_HIP(hipMemcpyDtoH(&dev_ws, sym, sizeof(dev_ws)));
+            if (dev_ws != large_matrix_bytes(n_))
+                throw std::runtime_error("staged workspace layout mismatch: module " + std::to_string(dev_ws) +
+                                         " B, host " + std::to_string(large_matrix_bytes(n_)) + " B");
         }
         lds_probe = k_lg_hetrd_;
         int lds = 0;
@@ -393,8 +405,15 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
 }
 
 size_t Solver::large_matrix_bytes(int n_) {
+    // The staged kernels' per-matrix workspace (geneig_large.hip: WS_* and the
+    // QL rotation log Q_*); bind_module_ checks it against the module's own.
     const size_t n = (size_t)n_;
-    return (88 * n * n + 24 * n + 255) / 256 * 256;
+    auto al = [](size_t x, size_t a) { return (x + a - 1) / a * a; };
+    const size_t wsq = al(88 * n * n + 24 * n, 256);
+    const size_t q_rank = 16 + 8 * (8 * n);
+    const size_t q_rot = al(q_rank + 4 * n, 16);
+    const size_t q_bytes = q_rot + 16 * (2 * n * n);
+    return al(wsq + q_bytes, 256);
 }
 
 size_t Solver::large_chunk(int n) { return std::max<size_t>(1, kLargeRoundBytes / large_matrix_bytes(n)); }
@@ -452,7 +471,9 @@ void Solver::launch_large_(const void* H, const void* S, double* W, void* U, int
             wstride = wss / 4;
         }
         void* a4[] = {&wout, &wstride, &Ic, &ws, (void*)&wss, &cnt};
-        SG_HIP(hipModuleLaunchKernel(k_lg_ql_, (unsigned)cnt, 1, 1, bd_ql, 1, 1, 0, stream, a4, nullptr));
+        SG_HIP(hipModuleLaunchKernel(k_lg_ql_, (unsigned)cnt, 1, 1, 64, 1, 1, 0, stream, a4, nullptr));
+        mark();
+        SG_HIP(hipModuleLaunchKernel(k_lg_qlz_, (unsigned)cnt, 1, 1, bd_ql, 1, 1, 0, stream, a4, nullptr));
         mark();
         SG_HIP(hipModuleLaunchKernel(k_lg_back_, (unsigned)cnt, 1, 1, bd, 1, 1, 0, stream, a3, nullptr));
         mark();
@@ -515,12 +536,12 @@ int Solver::stage_times(double* ms, int n) {
 }
 
 const char* Solver::stage_name(Mode mode, int i, int n) {
-    static const char* large64[] = {"geneig_lg_chol", "geneig_lg_formm", "geneig_lg_hetrd", "geneig_lg_ql",
-                                    "geneig_lg_back", "geneig_lg_backtri"};
-    static const char* large_mx[] = {"geneig_lg_chol", "geneig_lg_formm", "geneig_lg_hetrd", "geneig_lg_ql",
-                                     "geneig_lg_back", "geneig_lg_refine"};
+    static const char* large64[] = {"geneig_lg_chol", "geneig_lg_formm", "geneig_lg_hetrd", "geneig_lg_qlr",
+                                    "geneig_lg_qlz",  "geneig_lg_back",  "geneig_lg_backtri"};
+    static const char* large_mx[] = {"geneig_lg_chol", "geneig_lg_formm", "geneig_lg_hetrd", "geneig_lg_qlr",
+                                     "geneig_lg_qlz",  "geneig_lg_back",  "geneig_lg_refine"};
     if (n > kMaxLdsOrder) {
-        if (i < 0 || i >= 6) return "";
+        if (i < 0 || i >= 7) return "";
         return (mode == Mode::Mixed) ? large_mx[i] : large64[i];
     }
     static const char* unified[] = {"geneig_fused"};

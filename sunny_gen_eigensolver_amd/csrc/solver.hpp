@@ -89,7 +89,7 @@ private:
     // for up to large_chunk(n) matrices at a time.
     bool large_ = false;
     hipFunction_t k_lg_chol_ = nullptr, k_lg_formm_ = nullptr, k_lg_hetrd_ = nullptr, k_lg_ql_ = nullptr,
-                  k_lg_back_ = nullptr, k_lg_backtri_ = nullptr, k_lg_refine_ = nullptr;
+                  k_lg_qlz_ = nullptr, k_lg_back_ = nullptr, k_lg_backtri_ = nullptr, k_lg_refine_ = nullptr;
     void* gbuf_ = nullptr;
     size_t gbuf_batch_ = 0;
     int refine_iters_ = 2;
