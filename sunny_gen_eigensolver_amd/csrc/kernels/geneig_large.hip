@@ -458,6 +458,7 @@ __device__ __forceinline__ void gemm_rg(ctile (&acc)[RG], FA fa, FB fb, int rt0,
         acc[g].re = d4{0.0, 0.0, 0.0, 0.0};
         acc[g].im = d4{0.0, 0.0, 0.0, 0.0};
     }
+#pragma unroll 2
     for (int k0 = kb; k0 < ke; k0 += 4) {
         const cx b = fb(k0 + k4, ct * 16 + c16);
 #pragma unroll
@@ -1019,6 +1020,10 @@ geneig_lg_back(const int* __restrict__ info, char* __restrict__ ws, long long ws
 
 #ifndef GENEIG_FP32
 namespace gk {
+// Row tiles per task in the refinement's products (two keep its registers
+// low enough for two workgroups per CU).
+constexpr int RRG = 2;
+
 // acc1[g] = A B1 and acc2[g] = A B2 over k in [kb, ke) (shared A loads).
 template <int RG, class FA, class FB1, class FB2>
 __device__ __forceinline__ void gemm2_rg(ctile (&acc1)[RG], ctile (&acc2)[RG], FA fa, FB1 fb1, FB2 fb2, int rt0,
@@ -1032,6 +1037,7 @@ __device__ __forceinline__ void gemm2_rg(ctile (&acc1)[RG], ctile (&acc2)[RG], F
         acc2[g].re = d4{0.0, 0.0, 0.0, 0.0};
         acc2[g].im = d4{0.0, 0.0, 0.0, 0.0};
     }
+#pragma unroll 2
     for (int k0 = kb; k0 < ke; k0 += 4) {
         const cx b1 = fb1(k0 + k4, ct * 16 + c16);
         const cx b2 = fb2(k0 + k4, ct * 16 + c16);
@@ -1063,6 +1069,7 @@ __device__ __forceinline__ void ritz_pair_g(const cx* __restrict__ M, cx* V, rea
     }
 #pragma unroll
     for (int qk = 0; qk < ND; ++qk) {
+#pragma unroll 1
         for (int kk = 0; kk < 64 && 64 * qk + kk < N; ++kk) {
             const int k = 64 * qk + kk;
             const cx va = bcast(ua[qk], kk), vb = bcast(ub[qk], kk);
@@ -1146,19 +1153,19 @@ geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, const int* __restrict
     for (int e = tid; e < (int)NN; e += LNT) Va[e] = to_cx(V32[e]);
     for (int e = tid; e < N; e += LNT) lam[e] = (real)lam32[e];
     __syncthreads();
-    constexpr int NRG = (LNB + 3) / 4;
+    constexpr int NRG = (LNB + RRG - 1) / RRG;
     real delta = 0.0;
     for (int it = 0; it < iters; ++it) {
         const cx* V = Va;
         // P = M V.
         for (int t = wave; t < NRG * LNB; t += LNW) {
-            const int rt0 = 4 * (t / LNB), ct = t % LNB;
-            ctile acc[4];
-            gemm_rg<4>(acc, [&](int i, int k) { return gel(M, i, k); }, [&](int k, int j) { return gel(V, k, j); },
+            const int rt0 = RRG * (t / LNB), ct = t % LNB;
+            ctile acc[RRG];
+            gemm_rg<RRG>(acc, [&](int i, int k) { return gel(M, i, k); }, [&](int k, int j) { return gel(V, k, j); },
                        rt0, ct, 0, LKEND);
             const int j = ct * 16 + (lane & 15);
 #pragma unroll
-            for (int g = 0; g < 4; ++g)
+            for (int g = 0; g < RRG; ++g)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int i = tile_row(rt0 + g, v);
@@ -1193,15 +1200,15 @@ geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, const int* __restrict
         delta = delta_rel * (real)(1 << it) * mx;
         // E from the tiles of T = V^H P and G = V^H V.
         for (int t = wave; t < NRG * LNB; t += LNW) {
-            const int rt0 = 4 * (t / LNB), ct = t % LNB;
-            ctile T[4], G[4];
-            gemm2_rg<4>(T, G, [&](int i, int k) { return cconj(gel(V, k, i)); },
+            const int rt0 = RRG * (t / LNB), ct = t % LNB;
+            ctile T[RRG], G[RRG];
+            gemm2_rg<RRG>(T, G, [&](int i, int k) { return cconj(gel(V, k, i)); },
                         [&](int k, int j) { return gel(P, k, j); }, [&](int k, int j) { return gel(V, k, j); }, rt0,
                         ct, 0, LKEND);
             const int j = ct * 16 + (lane & 15);
             const real lj = lam[j < N ? j : 0];
 #pragma unroll
-            for (int g = 0; g < 4; ++g)
+            for (int g = 0; g < RRG; ++g)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int i = tile_row(rt0 + g, v);
@@ -1219,13 +1226,13 @@ geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, const int* __restrict
         __syncthreads();
         // V <- V + V E, into the other buffer.
         for (int t = wave; t < NRG * LNB; t += LNW) {
-            const int rt0 = 4 * (t / LNB), ct = t % LNB;
-            ctile acc[4];
-            gemm_rg<4>(acc, [&](int i, int k) { return gel(V, i, k); }, [&](int k, int j) { return gel(E, k, j); },
+            const int rt0 = RRG * (t / LNB), ct = t % LNB;
+            ctile acc[RRG];
+            gemm_rg<RRG>(acc, [&](int i, int k) { return gel(V, i, k); }, [&](int k, int j) { return gel(E, k, j); },
                        rt0, ct, 0, LKEND);
             const int j = ct * 16 + (lane & 15);
 #pragma unroll
-            for (int g = 0; g < 4; ++g)
+            for (int g = 0; g < RRG; ++g)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int i = tile_row(rt0 + g, v);
@@ -1276,13 +1283,13 @@ geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, const int* __restrict
     // U = W^H V.
     const cx* V = Va;
     for (int t = wave; t < NRG * LNB; t += LNW) {
-        const int rt0 = 4 * (t / LNB), ct = t % LNB;
-        ctile acc[4];
-        gemm_rg<4>(acc, [&](int i, int k) { return cconj(gel(Wi, k, i)); }, [&](int k, int j) { return gel(V, k, j); },
+        const int rt0 = RRG * (t / LNB), ct = t % LNB;
+        ctile acc[RRG];
+        gemm_rg<RRG>(acc, [&](int i, int k) { return cconj(gel(Wi, k, i)); }, [&](int k, int j) { return gel(V, k, j); },
                    rt0, ct, 16 * rt0, LKEND);
         const int j = ct * 16 + (lane & 15);
 #pragma unroll
-        for (int g = 0; g < 4; ++g)
+        for (int g = 0; g < RRG; ++g)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = tile_row(rt0 + g, v);
