@@ -468,8 +468,13 @@ __device__ __forceinline__ void gemm_rg(ctile (&acc)[RG], FA fa, FB fb, int rt0,
 // Row of element v of a result tile held by this lane (column: ct * 16 + lane % 16).
 __device__ __forceinline__ int tile_row(int rt, int v) { return rt * 16 + 4 * v + (lane_id() >> 4); }
 
+// Panel column stride (odd, so the 16 columns of a tile row fall in distinct
+// LDS banks) and the stride of one column of a block row in the inverse's
+// staging (17: consecutive threads' columns in distinct banks).
+constexpr int PLD = N | 1;
+constexpr int YLD = 17;
 struct CholWork {
-    cx pan[256 * LNB];         // panel rows j0.. (row i - j0), 16 columns; trtri: Y, then X
+    cx pan[(16 * PLD > YLD * 16 * LNB) ? 16 * PLD : YLD * 16 * LNB];   // panel; trtri: Y, then X
     cx lii[256];              // trtri: diagonal block of L
 };
 #endif  // !GENEIG_FP32
@@ -509,7 +514,7 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = tile_row(rt, v);
-                if (i < N && c < jw) cw.pan[c * N + (i - j0)] = Sb[(size_t)(j0 + c) * N + i] - tile_get(acc[0], v);
+                if (i < N && c < jw) cw.pan[c * PLD + (i - j0)] = Sb[(size_t)(j0 + c) * N + i] - tile_get(acc[0], v);
             }
         }
         __syncthreads();
@@ -522,7 +527,7 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
         const bool tin = t < N - j0;
         int fail = 0;
         for (int c = 0; c < jw; ++c) {
-            const real piv = cw.pan[c * N + c].x;
+            const real piv = cw.pan[c * PLD + c].x;
             if (!(piv > real(0.0))) {
                 fail = j0 + c + 1;
                 break;
@@ -530,10 +535,10 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
             const real sq = sqrt_r(piv);
             const real ri = real(1.0) / sq;
             if (t >= c && tin) {
-                const cx ltc = (t == c) ? cm(sq, 0.0) : cw.pan[c * N + t] * ri;
+                const cx ltc = (t == c) ? cm(sq, 0.0) : cw.pan[c * PLD + t] * ri;
                 Lg[(size_t)(j0 + c) * N + j0 + t] = ltc;
                 for (int c2 = c + 1; c2 < jw && c2 <= t; ++c2)
-                    cw.pan[c2 * N + t] = cw.pan[c2 * N + t] - cmulc(ltc, cw.pan[c * N + c2] * ri);
+                    cw.pan[c2 * PLD + t] = cw.pan[c2 * PLD + t] - cmulc(ltc, cw.pan[c * PLD + c2] * ri);
             }
             __syncthreads();
         }
@@ -563,7 +568,7 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = tile_row(ib, v);
-                if (i < N) cw.pan[j * 16 + (i - r0)] = tile_get(acc[0], v);
+                if (i < N) cw.pan[j * YLD + (i - r0)] = tile_get(acc[0], v);
             }
         }
         __syncthreads();
@@ -571,7 +576,7 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
         // its 16 entries of pan.
         const int j = tid;
         if (j < r0 + rw) {
-            cx* xj = cw.pan + j * 16;
+            cx* xj = cw.pan + j * YLD;
             for (int rr = 0; rr < rw; ++rr) {
                 cx s = (j < r0) ? cm(0.0, 0.0) - xj[rr] : cm((rr == j - r0) ? 1.0 : 0.0, 0.0);
                 for (int ss = 0; ss < rr; ++ss) s = s - cmul(cw.lii[ss * 16 + rr], xj[ss]);
