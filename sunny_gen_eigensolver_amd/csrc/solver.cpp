@@ -18,7 +18,7 @@ namespace sgeig {
 namespace {
 
 // Workspace of one round of the staged kernels (orders above 64).
-constexpr size_t kLargeRoundBytes = size_t(16) << 30;
+constexpr size_t kLargeRoundBytes = size_t(32) << 30;
 
 // Persisted autotune winners: "<n> <arch> <block_dim>" lines in
 // <cache_dir>/autotune.tsv. The reference memoises per process only
