@@ -141,6 +141,17 @@ int sunny_geneig_set_stage_timing(sunny_geneig_solver_t solver, int enable);
 int sunny_geneig_stage_times(sunny_geneig_solver_t solver, double* ms, const char** names,
                              int max_stages);
 
+/* hipGraph replay (no counterpart in the reference ABI). The split, mixed and
+ * staged launches issue several kernels; a handle captures them once per
+ * (d_H, d_S, d_W, d_U, d_info, batch_size) into a hipGraph and replays it on
+ * the caller's stream when those come back (up to 8 cached; none while stage
+ * timing is on; the environment variable SUNNY_GENEIG_GRAPH set to 0 at
+ * creation turns it off).
+ * set_graphs(h, 0) launches the kernels directly; graph_stats reports
+ * captures and replays so far. */
+int sunny_geneig_set_graphs(sunny_geneig_solver_t solver, int enable);
+int sunny_geneig_graph_stats(sunny_geneig_solver_t solver, long long* captures, long long* replays);
+
 /* ---- uneven batches (no counterpart in the reference ABI) ----------------
  * One call solves `count` pencils of different orders n[k] in [1, max_n].
  * Every matrix goes to the smallest bucket order >= n[k] and is embedded in

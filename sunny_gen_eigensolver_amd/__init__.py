@@ -47,6 +47,7 @@ EXPORTED_SYMBOLS = (
     "sunny_geneig_varlen_create", "sunny_geneig_varlen_destroy", "sunny_geneig_varlen_launch",
     "sunny_geneig_varlen_num_buckets", "sunny_geneig_varlen_bucket_order",
     "sunny_geneig_set_stage_timing", "sunny_geneig_stage_times",
+    "sunny_geneig_set_graphs", "sunny_geneig_graph_stats",
     "sunny_geneig_session_create", "sunny_geneig_session_destroy", "sunny_geneig_session_submit",
     "sunny_geneig_session_poll", "sunny_geneig_session_wait", "sunny_geneig_session_stats",
     "sunny_geneig_session_request_bytes",
@@ -112,6 +113,10 @@ def lib() -> ctypes.CDLL:
     L.sunny_geneig_varlen_bucket_order.restype = ci
     L.sunny_geneig_set_stage_timing.argtypes = [vp, ci]
     L.sunny_geneig_set_stage_timing.restype = ci
+    L.sunny_geneig_set_graphs.argtypes = [vp, ci]
+    L.sunny_geneig_set_graphs.restype = ci
+    L.sunny_geneig_graph_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
+    L.sunny_geneig_graph_stats.restype = ci
     L.sunny_geneig_stage_times.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_char_p), ci]
     L.sunny_geneig_stage_times.restype = ci
@@ -213,6 +218,20 @@ class Solver:
         st = lib().sunny_geneig_launch(self.handle, d_H, d_S, d_W, d_U, d_info, batch, stream)
         if st != STATUS_OK:
             raise GeneigError(st, last_error())
+
+    def set_graphs(self, on: bool = True) -> None:
+        """Replay captured hipGraphs for repeated multi-kernel launches (default on)."""
+        st = lib().sunny_geneig_set_graphs(self.handle, 1 if on else 0)
+        if st != STATUS_OK:
+            raise GeneigError(st, last_error())
+
+    def graph_stats(self):
+        """(captures, replays) of this handle's hipGraph cache."""
+        c, r = ctypes.c_longlong(), ctypes.c_longlong()
+        st = lib().sunny_geneig_graph_stats(self.handle, ctypes.byref(c), ctypes.byref(r))
+        if st != STATUS_OK:
+            raise GeneigError(st, last_error())
+        return c.value, r.value
 
     def set_stage_timing(self, on: bool = True) -> None:
         st = lib().sunny_geneig_set_stage_timing(self.handle, 1 if on else 0)

@@ -222,6 +222,27 @@ int sunny_geneig_stage_times(sunny_geneig_solver_t s, double* ms, const char** n
     }
 }
 
+int sunny_geneig_set_graphs(sunny_geneig_solver_t s, int enable) {
+    clear_error();
+    if (!s) {
+        set_error("sunny_geneig_set_graphs: solver is NULL");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    s->impl->set_graphs(enable != 0);
+    return SUNNY_GENEIG_OK;
+}
+
+int sunny_geneig_graph_stats(sunny_geneig_solver_t s, long long* captures, long long* replays) {
+    clear_error();
+    if (!s) {
+        set_error("sunny_geneig_graph_stats: solver is NULL");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    if (captures) *captures = s->impl->graph_captures();
+    if (replays) *replays = s->impl->graph_replays();
+    return SUNNY_GENEIG_OK;
+}
+
 sunny_geneig_varlen_t sunny_geneig_varlen_create(int max_n, int device_id, const int* bucket_orders,
                                                  int num_buckets, sunny_geneig_cache_mode_t cache_mode) {
     clear_error();

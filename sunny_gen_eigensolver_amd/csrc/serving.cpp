@@ -81,6 +81,7 @@ ServingSession::ServingSession(int device, Mode mode, const std::vector<int>& or
         SG_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
         for (int n : orders_) {
             auto sol = std::make_unique<Solver>(n, device, 0, Tuning::Suggested, cache, mode);
+            sol->set_graphs(false);              // buffers differ per request: nothing to replay
             sol->reserve((size_t)max_count);     // launches then allocate nothing
             s.solvers[n] = std::move(sol);
         }

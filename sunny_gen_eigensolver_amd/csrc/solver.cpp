@@ -100,6 +100,7 @@ Solver::Solver(int n, int device_id, int bpb_request, Tuning tuning, CacheMode c
         throw std::invalid_argument("matrix size " + std::to_string(n) + " exceeds the supported maximum of " +
                                     std::to_string(kMaxOrder));
     large_ = n > kMaxLdsOrder;
+    if (const char* v = std::getenv("SUNNY_GENEIG_GRAPH")) graphs_ = std::atoi(v) != 0;
     if (bpb_request > 4)
         throw std::invalid_argument("batches_per_block must be <= 4 (one wave per matrix, "
                                     "LDS-resident work area)");
@@ -123,6 +124,8 @@ Solver::Solver(int n, int device_id, int bpb_request, Tuning tuning, CacheMode c
 
 Solver::~Solver() {
     for (auto& e : ev_) (void)hipEventDestroy(e);
+    clear_graphs_();
+    if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
     if (lbuf_ || mbuf_ || gbuf_) {
         int cur = 0;
         (void)hipGetDevice(&cur);
@@ -298,12 +301,61 @@ void Solver::launch(const void* H, const void* S, double* W, void* U, int* info,
     SG_HIP(hipGetDevice(&cur));
     if (cur != device_) SG_HIP(hipSetDevice(device_));
     ev_used_ = 0;
-    mark_(stream);
+    const bool multi = large_ || mode_ != Mode::Unified;
+    if (graphs_ && multi && !timing_) {
+        reserve((size_t)batch);                    // any allocation happens outside capture
+        GraphEntry* hit = nullptr;
+        for (auto& e : graph_cache_)
+            if (e.H == H && e.S == S && e.W == W && e.U == U && e.info == info && e.batch == batch) hit = &e;
+        if (!hit) {
+            if (!cap_stream_) SG_HIP(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking));
+            SG_HIP(hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal));
+            try {
+                dispatch_(H, S, W, U, info, batch, cap_stream_);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(cap_stream_, &g);
+                if (g) (void)hipGraphDestroy(g);
+                if (cur != device_) (void)hipSetDevice(cur);
+                throw;
+            }
+            hipGraph_t g = nullptr;
+            SG_HIP(hipStreamEndCapture(cap_stream_, &g));
+            hipGraphExec_t ex = nullptr;
+            const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            SG_HIP(ie);
+            if (graph_cache_.size() >= 8) clear_graphs_();
+            graph_cache_.push_back({H, S, W, U, info, batch, ex});
+            hit = &graph_cache_.back();
+            ++graph_captures_;
+        } else {
+            ++graph_replays_;
+        }
+        SG_HIP(hipGraphLaunch(hit->exec, stream));
+    } else {
+        mark_(stream);
+        dispatch_(H, S, W, U, info, batch, stream);
+    }
+    if (cur != device_) SG_HIP(hipSetDevice(cur));
+}
+
+void Solver::dispatch_(const void* H, const void* S, double* W, void* U, int* info, int batch,
+                       hipStream_t stream) {
     if (large_) launch_large_(H, S, W, U, info, batch, stream);
     else if (mode_ == Mode::Unified) launch_unified_(H, S, W, U, info, batch, stream);
     else if (mode_ == Mode::Hybrid) launch_hybrid_(H, S, W, U, info, batch, stream);
     else launch_mixed_(H, S, W, U, info, batch, stream);
-    if (cur != device_) SG_HIP(hipSetDevice(cur));
+}
+
+void Solver::clear_graphs_() {
+    for (auto& e : graph_cache_) (void)hipGraphExecDestroy(e.exec);
+    graph_cache_.clear();
+}
+
+void Solver::set_graphs(bool on) {
+    graphs_ = on;
+    if (!on) clear_graphs_();
 }
 
 void Solver::launch_unified_(const void* H, const void* S, double* W, void* U, int* info, int batch,
@@ -317,6 +369,7 @@ void Solver::launch_unified_(const void* H, const void* S, double* W, void* U, i
 
 void Solver::ensure_workspace_(size_t batch) {
     if (batch <= lbuf_batch_) return;
+    clear_graphs_();              // captured launches hold the old workspace
     if (lbuf_) {
         SG_HIP(hipDeviceSynchronize());
         SG_HIP(hipFree(lbuf_));
@@ -352,6 +405,7 @@ void Solver::launch_hybrid_(const void* H, const void* S, double* W, void* U, in
 
 void Solver::ensure_mixed_workspace_(size_t batch) {
     if (batch <= mbuf_batch_) return;
+    clear_graphs_();
     if (mbuf_) {
         SG_HIP(hipDeviceSynchronize());
         SG_HIP(hipFree(mbuf_));
@@ -421,6 +475,7 @@ size_t Solver::large_chunk(int n) { return std::max<size_t>(1, kLargeRoundBytes 
 void Solver::ensure_large_workspace_(size_t batch) {
     batch = std::min(batch, large_chunk(n_));
     if (batch <= gbuf_batch_) return;
+    clear_graphs_();
     if (gbuf_) {
         SG_HIP(hipDeviceSynchronize());
         SG_HIP(hipFree(gbuf_));

@@ -95,6 +95,26 @@ private:
     int refine_iters_ = 2;
     double refine_delta_ = 1e-4;
 
+    // Multi-kernel launches (split, mixed, staged) are captured once per
+    // (buffers, batch) into a hipGraph on an internal stream and replayed on
+    // the caller's stream afterwards; SUNNY_GENEIG_GRAPH=0 or set_graphs(false)
+    // launches the kernels directly.
+    struct GraphEntry {
+        const void* H;
+        const void* S;
+        double* W;
+        void* U;
+        int* info;
+        int batch;
+        hipGraphExec_t exec;
+    };
+    bool graphs_ = true;
+    hipStream_t cap_stream_ = nullptr;
+    std::vector<GraphEntry> graph_cache_;
+    long long graph_captures_ = 0, graph_replays_ = 0;
+    void clear_graphs_();
+    void dispatch_(const void* H, const void* S, double* W, void* U, int* info, int batch, hipStream_t stream);
+
     // Stage timing: when enabled, events bracket every kernel of a launch on
     // its stream; stage_times() reads the last launch's durations.
     bool timing_ = false;
@@ -121,6 +141,9 @@ public:
     static size_t large_matrix_bytes(int n);
     static size_t large_chunk(int n);
     void set_stage_timing(bool on);
+    void set_graphs(bool on);
+    long long graph_captures() const { return graph_captures_; }
+    long long graph_replays() const { return graph_replays_; }
     // Fills ms[0 .. min(n, stages)) and returns the number of stages of the
     // last timed launch (0 if none); waits for that launch to finish.
     int stage_times(double* ms, int n);

@@ -62,6 +62,8 @@ def test_null_handle_accessors():
     assert L.sunny_geneig_cache_layer(None) == b""
     assert L.sunny_geneig_set_stage_timing(None, 1) == sg.STATUS_INVALID_ARG
     assert L.sunny_geneig_stage_times(None, None, None, 0) == -1
+    assert L.sunny_geneig_set_graphs(None, 1) == sg.STATUS_INVALID_ARG
+    assert L.sunny_geneig_graph_stats(None, None, None) == sg.STATUS_INVALID_ARG
     L.sunny_geneig_destroy(None)
 
 

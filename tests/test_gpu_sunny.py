@@ -1,0 +1,55 @@
+"""The Sunny.jl call pattern (julia/SunnyGeneigBench/scripts/setup_sw08.jl:6-18):
+H = Sigma = diag(I_L, -I_L), the paraunitary metric of Colpa's method, which is
+indefinite, and S = D, the dynamical matrix, Hermitian positive definite with
+the Bogoliubov block form D = [[A, B], [conj(B), conj(A)]] (A Hermitian, B
+symmetric). Sigma D then has a spectrum of +-omega pairs, so the pencil
+(Sigma, D) has eigenvalues in exact +- pairs: lambda_{2L-1-k} = -lambda_k.
+
+Checked through the C ABI in every mode, at LDS-resident and staged orders:
+the pairing (to 1e-12 relative), and parity with FP64 LAPACK zhegv on the
+same pencils with the usual bars."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import helpers as T
+
+pytestmark = pytest.mark.gpu
+
+
+def bogoliubov_pencils(L: int, batch: int, seed: int):
+    """(Sigma, D) pairs in the library layout (matrix b, element (i, j) at
+    [b, j, i]); D = [[A, B], [B*, A*]] + shift I with A Hermitian, B symmetric,
+    shifted to be positive definite with a condition number around 1e2."""
+    rng = np.random.default_rng(seed)
+    n = 2 * L
+    H = np.zeros((batch, n, n), dtype=np.complex128)
+    S = np.zeros((batch, n, n), dtype=np.complex128)
+    sig = np.concatenate([np.ones(L), -np.ones(L)])
+    for b in range(batch):
+        X = rng.standard_normal((L, L)) + 1j * rng.standard_normal((L, L))
+        A = X @ X.conj().T / L
+        Y = rng.standard_normal((L, L)) + 1j * rng.standard_normal((L, L))
+        Bs = 0.3 * (Y + Y.T) / 2
+        D = np.block([[A, Bs], [Bs.conj(), A.conj()]])
+        D = 0.5 * (D + D.conj().T)
+        w = np.linalg.eigvalsh(D)
+        D = D + (max(0.0, -w[0]) + 0.01 * (w[-1] - w[0]) + 1e-3) * np.eye(n)
+        H[b] = np.diag(sig).T
+        S[b] = D.T
+    return H, S
+
+
+@pytest.mark.parametrize("kernel", ["unified", "hybrid", "mixed"])
+@pytest.mark.parametrize("L", [12, 27, 48])
+def test_paraunitary_metric_pencil(kernel, L):
+    H, S = bogoliubov_pencils(L, 8, 1000 + L)
+    W, U, info = T.run_gpu(H, S, kernel=kernel)
+    assert (info == 0).all()
+    # +- pairing of the ascending eigenvalues.
+    scale = np.abs(W).max(axis=1, keepdims=True)
+    pair_err = np.abs(W + W[:, ::-1]) / scale
+    assert pair_err.max() < 1e-12, f"pairing {pair_err.max():.2e}"
+    Wl, Ul, il = O.zhegv_batch(H, S, threads=8)
+    assert (il == 0).all()
+    T.assert_parity(H, S, W, U, info, Wl, Ul, tag=f"{kernel} Sigma/D L={L}")
