@@ -144,3 +144,34 @@ def test_rejects_what_it_cannot_serve():
         with pytest.raises(sg.GeneigError) as e3:
             ss.submit(h, h, W, U, info)                  # footprint above the budget
         assert e3.value.status == sg.STATUS_INVALID_ARG
+
+
+def test_staged_orders_in_the_mix():
+    """Orders above 64 (the staged kernels) served beside an LDS-resident one:
+    bit-identical alone and in the mix, and zhegv parity."""
+    import sunny_gen_eigensolver_amd as sg
+    reqs = [(96, 9), (24, 5), (150, 3), (96, 2), (150, 1)]
+    data = [O.gen_reference(n, c, 500 + k, "random") for k, (n, c) in enumerate(reqs)]
+    with sg.ServingSession([24, 96, 150], kernel="mixed", max_count=16, slots=3, hbm_budget=64 << 20) as ss:
+        alone = []
+        for (H, S), (n, c) in zip(data, reqs):
+            dH, dS = _dev((H, S))
+            W, U, info = _outputs(n, c, "cuda")
+            ss.wait(ss.submit(dH, dS, W, U, info))
+            alone.append((W.cpu().numpy(), U.cpu().numpy(), info.cpu().numpy()))
+        ids, outs = [], []
+        for (H, S), (n, c) in zip(data, reqs):
+            dH, dS = _dev((H, S))
+            W, U, info = _outputs(n, c, "cuda")
+            ids.append(ss.submit(dH, dS, W, U, info))
+            outs.append((W, U, info))
+        for i in ids:
+            ss.wait(i)
+    for k, ((W, U, info), (Wa, Ua, ia)) in enumerate(zip(outs, alone)):
+        assert np.array_equal(W.cpu().numpy(), Wa) and np.array_equal(U.cpu().numpy(), Ua), k
+        assert (ia == 0).all()
+    for k in (0, 2):
+        H, S = data[k]
+        Wl, Ul, il = O.zhegv_batch(H, S, threads=8)
+        T.assert_parity(H, S, alone[k][0], alone[k][1], alone[k][2], Wl, Ul, tag=f"serving staged request {k}")
+
