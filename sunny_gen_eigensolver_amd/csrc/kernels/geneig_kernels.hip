@@ -1834,24 +1834,211 @@ geneig_split_front(const cx* __restrict__ H, const cx* __restrict__ S, real* __r
 }
 #endif  // !GENEIG_FP32
 
+// ---------------------------------------------------------------------------
+// QL in two kernels. The scalar recurrence of a sweep is the same for every
+// row of Z, so instead of every lane of a wave evaluating it for one matrix,
+// geneig_ql_record runs it for 64 matrices at once, one per lane (its d and e
+// in LDS, element i of lane l at [i][l], so a lane always hits its own bank
+// whatever index it is at), as a per-lane state machine that advances every
+// lane by one step per iteration: a deflation scan of four off-diagonals, or
+// one rotation. Each matrix's sweeps (l, m), rotations, ascending ranks and
+// eigenvalues go to its log; geneig_split_ql then rotates the register rows
+// of Z with them. A matrix whose QL outgrows its log is redone inline there.
+// ---------------------------------------------------------------------------
+namespace gk {
+constexpr int QS_CAP = 8 * N;                       // sweeps per matrix
+constexpr int QR_CAP = 2 * N * N;                   // rotations per matrix
+constexpr size_t QG_SW = 16;                        // int[2 QS_CAP]: (l, m) per sweep
+constexpr size_t QG_RANK = QG_SW + 8 * (size_t)QS_CAP;                     // int[N]
+constexpr size_t QG_LAM = (QG_RANK + 4 * (size_t)N + 15) / 16 * 16;        // real[N]
+constexpr size_t QG_ROT = (QG_LAM + 8 * (size_t)N + 15) / 16 * 16;         // (c, s) pairs
+constexpr size_t QG_BYTES = (QG_ROT + 16 * (size_t)QR_CAP + 255) / 256 * 256;
+}  // namespace gk
+
+// Bytes of one matrix's QL log, for the host's consistency check.
+extern "C" __device__ const unsigned long long geneig_ql_log_bytes = gk::QG_BYTES;
+
+extern "C" __global__ void __launch_bounds__(64)
+geneig_ql_record(const real* __restrict__ W, const real* __restrict__ E, const int* __restrict__ info,
+                 char* __restrict__ qlog, long long log_stride, int batch) {
+    using namespace gk;
+    __shared__ real ds[N][64];
+    __shared__ real es[N][64];
+    const int lane = lane_id();
+    const int b = blockIdx.x * 64 + lane;
+    const bool mine = b < batch && info[b] == 0;
+    for (int i = 0; i < N; ++i) {
+        ds[i][lane] = mine ? W[(size_t)b * N + i] : real(0.0);
+        es[i][lane] = (mine && i < N - 1) ? E[(size_t)b * N + i] : real(0.0);
+    }
+    char* lg = qlog + (size_t)(mine ? b : 0) * (size_t)log_stride;
+    int* hdr = reinterpret_cast<int*>(lg);
+    int* sw = reinterpret_cast<int*>(lg + QG_SW);
+    int* rk = reinterpret_cast<int*>(lg + QG_RANK);
+    real* lam = reinterpret_cast<real*>(lg + QG_LAM);
+    real* rot = reinterpret_cast<real*>(lg + QG_ROT);
+    const real eps = REAL_HALF_ULP;
+    enum { SCAN = 0, SWEEP = 1, DONE = 2 };
+    int phase = mine ? SCAN : DONE;
+    int l = 0, sc = 0, m = 0, i = 0, iter = 0, nsw = 0, nrot = 0, unconverged = 0, overflow = 0;
+    real s = real(1.0), c = real(1.0), p = real(0.0), g = real(0.0), d_up = real(0.0), d_l = real(0.0);
+    while (__ballot(phase != DONE)) {
+        if (phase == SCAN) {
+            if (l >= N) {
+                phase = DONE;
+            } else {
+                // First small off-diagonal at or after sc (up to four tested).
+                int hit = -1;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int k = sc + u;
+                    if (hit < 0 && k < N) {
+                        const bool small = (k >= N - 1) ||
+                                           fabs(es[k][lane]) <= eps * (fabs(ds[k][lane]) + fabs(ds[k + 1 < N ? k + 1 : k][lane]));
+                        if (small) hit = k;
+                    }
+                }
+                if (hit < 0) {
+                    sc += 4;
+                } else if (hit == l) {
+                    ++l;
+                    iter = 0;
+                    sc = l;
+                } else if (iter == QL_MAXIT) {
+                    for (int k = l; k < N - 1; ++k) unconverged += (es[k][lane] != real(0.0)) ? 1 : 0;
+                    phase = DONE;
+                } else if (nsw == QS_CAP || nrot + (hit - l) > QR_CAP) {
+                    overflow = 1;
+                    phase = DONE;
+                } else {
+                    ++iter;
+                    m = hit;
+                    d_l = ds[l][lane];
+                    const real e_l = es[l][lane];
+                    g = (ds[l + 1][lane] - d_l) / (real(2.0) * e_l);
+                    const real r = sqrt_r(g * g + real(1.0));
+                    d_up = ds[m][lane];
+                    g = d_up - d_l + e_l / (g + copysign(r, g));
+                    s = real(1.0);
+                    c = real(1.0);
+                    p = real(0.0);
+                    i = m - 1;
+                    sw[2 * nsw] = l;
+                    sw[2 * nsw + 1] = m;
+                    ++nsw;
+                    phase = SWEEP;
+                }
+            }
+        } else if (phase == SWEEP) {
+            const real e_i = es[i][lane], d_i = ds[i][lane];
+            const real f = s * e_i;
+            const real bb = c * e_i;
+            // f = s e_i != 0 inside the window; the floor keeps t finite near underflow.
+            const real tt = fmax(fma(f, f, g * g), REAL_TINY);
+            const real y = rsqrt_nr(tt);
+            es[i + 1][lane] = tt * y;
+            s = f * y;
+            c = g * y;
+            g = d_up - p;
+            const real r2 = fma(d_i - g, s, real(2.0) * c * bb);
+            p = s * r2;
+            ds[i + 1][lane] = g + p;
+            g = fma(c, r2, -bb);
+            rot[2 * (nrot + i - l)] = c;
+            rot[2 * (nrot + i - l) + 1] = s;
+            d_up = d_i;
+            if (--i < l) {
+                ds[l][lane] = d_l - p;
+                es[l][lane] = g;
+                es[m][lane] = real(0.0);
+                nrot += m - l;
+                sc = l;
+                phase = SCAN;
+            }
+        }
+    }
+    if (!mine) return;
+    if (!overflow) {
+        // Ascending ranks (ties by index) and the eigenvalues.
+        for (int k = 0; k < N; ++k) {
+            const real dk = ds[k][lane];
+            int rank = 0;
+            for (int j = 0; j < N; ++j) {
+                const real dj = ds[j][lane];
+                rank += (dj < dk || (dj == dk && j < k)) ? 1 : 0;
+            }
+            rk[k] = rank;
+            lam[rank] = dk;
+        }
+    }
+    hdr[0] = nsw;
+    hdr[1] = overflow;
+    hdr[2] = unconverged;
+}
+
+// QL, second half: one wave per matrix (lane r = row r of Z, in registers)
+// applies the logged rotations sweep by sweep (each sweep's rotations fetched
+// by the lanes ahead of time, then broadcast by v_readlane), writes the sorted
+// eigenvalues to W and Z with its columns in ascending order to Zw, and info
+// (the number of unconverged off-diagonals, LAPACK's convention).
 extern "C" __global__ void __launch_bounds__(GENEIG_NT)
 geneig_split_ql(real* __restrict__ W, const real* __restrict__ E, real* __restrict__ Zw,
-                int* __restrict__ info, int batch) {
+                int* __restrict__ info, const char* __restrict__ qlog, long long log_stride, int batch) {
     using namespace gk;
     const int b = blockIdx.x * NW + wave_id();
     if (b >= batch) return;
     if (info[b] != 0) return;
     const int lane = lane_id();
     real* Wb = W + (size_t)b * N;
-    const real* Eb = E + (size_t)b * N;
-    real dreg = (lane < N) ? Wb[lane] : 0.0;
-    real ereg = (lane < N - 1) ? Eb[lane] : 0.0;
+    const char* lg = qlog + (size_t)b * (size_t)log_stride;
+    const int* hdr = reinterpret_cast<const int*>(lg);
     real z[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) z[j] = (lane == j) ? 1.0 : 0.0;
-    const int qi = ql_wave(dreg, ereg, z);
-    const int rank = ascending_rank(lane < N ? dreg : __builtin_inf());
-    if (lane < N) Wb[rank] = dreg;
+    int rank, qi;
+    if (hdr[1]) {
+        // The log overflowed: the recurrence inline, every lane redundantly.
+        const real* Eb = E + (size_t)b * N;
+        real dreg = (lane < N) ? Wb[lane] : 0.0;
+        real ereg = (lane < N - 1) ? Eb[lane] : 0.0;
+        qi = ql_wave(dreg, ereg, z);
+        rank = ascending_rank(lane < N ? dreg : __builtin_inf());
+        if (lane < N) Wb[rank] = dreg;
+    } else {
+        const int* sw = reinterpret_cast<const int*>(lg + QG_SW);
+        const real* rot = reinterpret_cast<const real*>(lg + QG_ROT);
+        const int nsw = hdr[0];
+        int off = 0;
+        int l = nsw > 0 ? sw[0] : 0, m = nsw > 0 ? sw[1] : 0;
+        real cr = (lane < m - l) ? rot[2 * lane] : real(1.0);
+        real sr = (lane < m - l) ? rot[2 * lane + 1] : real(0.0);
+        for (int t = 0; t < nsw; ++t) {
+            const int off_n = off + (m - l);
+            const bool last = t + 1 == nsw;
+            const int ln = last ? 0 : sw[2 * t + 2], mn = last ? 0 : sw[2 * t + 3];
+            const real cn = (lane < mn - ln) ? rot[2 * (off_n + lane)] : real(1.0);
+            const real sn = (lane < mn - ln) ? rot[2 * (off_n + lane) + 1] : real(0.0);
+#pragma unroll
+            for (int i = N - 2; i >= 0; --i) {
+                if (i < m && i >= l) {
+                    const real c = bcast(cr, i - l), s = bcast(sr, i - l);
+                    const real zi = z[i], zi1 = z[i + 1];
+                    z[i + 1] = fma(s, zi, c * zi1);
+                    z[i] = fma(c, zi, -(s * zi1));
+                }
+            }
+            cr = cn;
+            sr = sn;
+            off = off_n;
+            l = ln;
+            m = mn;
+        }
+        const int* rk = reinterpret_cast<const int*>(lg + QG_RANK);
+        const real* lam = reinterpret_cast<const real*>(lg + QG_LAM);
+        rank = (lane < N) ? rk[lane] : N;
+        if (lane < N) Wb[lane] = lam[lane];
+        qi = hdr[2];
+    }
     real* Zb = Zw + (size_t)b * N * N;
 #pragma unroll
     for (int j = 0; j < N; ++j) {

@@ -126,13 +126,14 @@ Solver::~Solver() {
     for (auto& e : ev_) (void)hipEventDestroy(e);
     clear_graphs_();
     if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
-    if (lbuf_ || mbuf_ || gbuf_) {
+    if (lbuf_ || mbuf_ || gbuf_ || qlog_) {
         int cur = 0;
         (void)hipGetDevice(&cur);
         (void)hipSetDevice(device_);
         if (lbuf_) (void)hipFree(lbuf_);
         if (mbuf_) (void)hipFree(mbuf_);
         if (gbuf_) (void)hipFree(gbuf_);
+        if (qlog_) (void)hipFree(qlog_);
         (void)hipSetDevice(cur);
     }
 }
@@ -188,6 +189,17 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
     k_reduce_ = module_->get("geneig_split_front");
     k_heev_ = module_->get("geneig_split_ql");
     k_back_ = module_->get("geneig_split_back");
+    k_qlrec_ = module_->get("geneig_ql_record");
+    {
+        hipDeviceptr_t sym = nullptr;
+        size_t sym_bytes = 0;
+        unsigned long long dev_q = 0;
+        SG_HIP(hipModuleGetGlobal(&sym, &sym_bytes, module_->module, "geneig_ql_log_bytes"));
+        SG_HIP(hipMemcpyDtoH(&dev_q, sym, sizeof(dev_q)));
+        if (dev_q != ql_log_bytes(n_))
+            throw std::runtime_error("QL log layout mismatch: module " + std::to_string(dev_q) + " B, host " +
+                                     std::to_string(ql_log_bytes(n_)) + " B");
+    }
     if (mode_ == Mode::Mixed) {
         k_mx_chol_ = module_->get("geneig_lg_chol");
         k_refine_ = module_->get("geneig_mx_refine");
@@ -199,6 +211,7 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
         acquire_ms_ += ms32;
         k_mx_front_ = module32_->get("geneig_mx_front");
         k_ql32_ = module32_->get("geneig_split_ql");
+        k_qlrec_ = module32_->get("geneig_ql_record");
         k_back32_ = module32_->get("geneig_mx_back");
         if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_ITERS")) refine_iters_ = std::max(0, std::atoi(v));
         if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_DELTA")) refine_delta_ = std::atof(v);
@@ -393,7 +406,14 @@ void Solver::launch_hybrid_(const void* H, const void* S, double* W, void* U, in
                                  a1, nullptr));
     mark_(stream);
     const unsigned grid2 = (unsigned)((batch + bpb_ - 1) / bpb_);
-    void* a2[] = {&W, &E, &Zw, &info, &batch};
+    ensure_qlog_((size_t)batch);
+    char* qlog = static_cast<char*>(qlog_);
+    long long qstride = (long long)ql_log_bytes(n_);
+    void* ar[] = {&W, &E, &info, &qlog, &qstride, &batch};
+    SG_HIP(hipModuleLaunchKernel(k_qlrec_, (unsigned)((batch + 63) / 64), 1, 1, 64, 1, 1, 0, stream, ar,
+                                 nullptr));
+    mark_(stream);
+    void* a2[] = {&W, &E, &Zw, &info, &qlog, &qstride, &batch};
     SG_HIP(hipModuleLaunchKernel(k_heev_, grid2, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream, a2,
                                  nullptr));
     mark_(stream);
@@ -443,7 +463,14 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
     void* a2[] = {&H, &Wd, &W32, &V32, &info, &E32, &batch};
     SG_HIP(hipModuleLaunchKernel(k_mx_front_, grid_b, 1, 1, bd, 1, 1, 0, stream, a2, nullptr));
     mark_(stream);
-    void* a3[] = {&W32, &E32, &Z32, &info, &batch};
+    ensure_qlog_((size_t)batch);
+    char* qlog = static_cast<char*>(qlog_);
+    long long qstride = (long long)ql_log_bytes(n_);
+    void* ar[] = {&W32, &E32, &info, &qlog, &qstride, &batch};
+    SG_HIP(hipModuleLaunchKernel(k_qlrec_, (unsigned)((batch + 63) / 64), 1, 1, 64, 1, 1, 0, stream, ar,
+                                 nullptr));
+    mark_(stream);
+    void* a3[] = {&W32, &E32, &Z32, &info, &qlog, &qstride, &batch};
     SG_HIP(hipModuleLaunchKernel(k_ql32_, grid_w, 1, 1, bd, 1, 1, 0, stream, a3, nullptr));
     mark_(stream);
     void* a4[] = {&V32, &Z32, &info, &batch};
@@ -468,6 +495,29 @@ size_t Solver::large_matrix_bytes(int n_) {
     const size_t q_rot = al(q_rank + 4 * n, 16);
     const size_t q_bytes = q_rot + 16 * (2 * n * n);
     return al(wsq + q_bytes, 256);
+}
+
+size_t Solver::ql_log_bytes(int n_) {
+    // geneig_kernels.hip: QG_* (sweep windows, ranks, eigenvalues, rotations).
+    const size_t n = (size_t)n_;
+    auto al = [](size_t x, size_t a) { return (x + a - 1) / a * a; };
+    const size_t rank = 16 + 8 * (8 * n);
+    const size_t lam = al(rank + 4 * n, 16);
+    const size_t rot = al(lam + 8 * n, 16);
+    return al(rot + 16 * (2 * n * n), 256);
+}
+
+void Solver::ensure_qlog_(size_t batch) {
+    if (batch <= qlog_batch_) return;
+    clear_graphs_();
+    if (qlog_) {
+        SG_HIP(hipDeviceSynchronize());
+        SG_HIP(hipFree(qlog_));
+        qlog_ = nullptr;
+        qlog_batch_ = 0;
+    }
+    SG_HIP(hipMalloc(&qlog_, batch * ql_log_bytes(n_)));
+    qlog_batch_ = batch;
 }
 
 size_t Solver::large_chunk(int n) { return std::max<size_t>(1, kLargeRoundBytes / large_matrix_bytes(n)); }
@@ -548,8 +598,9 @@ void Solver::launch_large_(const void* H, const void* S, double* W, void* U, int
 size_t Solver::workspace_bytes(Mode mode, int n_, size_t batch) {
     if (n_ > kMaxLdsOrder) return std::min(batch, large_chunk(n_)) * large_matrix_bytes(n_);
     const size_t n = (size_t)n_;
-    if (mode == Mode::Hybrid) return batch * (n + n * n) * 8;
-    if (mode == Mode::Mixed) return batch * (n * n * 16 + n * n * 8 + n * 4 + n * 4 + n * n * 4);
+    if (mode == Mode::Hybrid) return batch * ((n + n * n) * 8 + ql_log_bytes(n_));
+    if (mode == Mode::Mixed)
+        return batch * (n * n * 16 + n * n * 8 + n * 4 + n * 4 + n * n * 4 + ql_log_bytes(n_));
     return 0;
 }
 
@@ -560,6 +611,7 @@ size_t Solver::reserve(size_t batch) {
     if (large_) ensure_large_workspace_(batch);
     else if (mode_ == Mode::Hybrid) ensure_workspace_(batch);
     else if (mode_ == Mode::Mixed) ensure_mixed_workspace_(batch);
+    if (!large_ && mode_ != Mode::Unified) ensure_qlog_(batch);
     if (cur != device_) SG_HIP(hipSetDevice(cur));
     return workspace_bytes(mode_, n_, batch);
 }
@@ -600,12 +652,13 @@ const char* Solver::stage_name(Mode mode, int i, int n) {
         return (mode == Mode::Mixed) ? large_mx[i] : large64[i];
     }
     static const char* unified[] = {"geneig_fused"};
-    static const char* hybrid[] = {"geneig_split_front", "geneig_split_ql", "geneig_split_back"};
-    static const char* mixed[] = {"geneig_lg_chol", "geneig_mx_front", "geneig_split_ql",
+    static const char* hybrid[] = {"geneig_split_front", "geneig_ql_record", "geneig_split_ql",
+                                   "geneig_split_back"};
+    static const char* mixed[] = {"geneig_lg_chol", "geneig_mx_front", "geneig_ql_record", "geneig_split_ql",
                                   "geneig_mx_back", "geneig_mx_refine"};
     if (mode == Mode::Unified) return (i == 0) ? unified[0] : "";
-    if (mode == Mode::Hybrid) return (i >= 0 && i < 3) ? hybrid[i] : "";
-    return (i >= 0 && i < 5) ? mixed[i] : "";
+    if (mode == Mode::Hybrid) return (i >= 0 && i < 4) ? hybrid[i] : "";
+    return (i >= 0 && i < 6) ? mixed[i] : "";
 }
 
 }  // namespace sgeig

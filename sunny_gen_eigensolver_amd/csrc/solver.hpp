@@ -55,6 +55,7 @@ private:
     void ensure_workspace_(size_t batch);
     void ensure_mixed_workspace_(size_t batch);
     void ensure_large_workspace_(size_t batch);
+    void ensure_qlog_(size_t batch);
 
     int n_ = 0;
     int device_ = 0;
@@ -71,6 +72,11 @@ private:
     hipFunction_t k_reduce_ = nullptr;
     hipFunction_t k_heev_ = nullptr;
     hipFunction_t k_back_ = nullptr;
+    // QL recurrence recorder (64 matrices per wave) of the split QL and its
+    // per-matrix rotation logs (split and mixed modes, orders up to 64).
+    hipFunction_t k_qlrec_ = nullptr;
+    void* qlog_ = nullptr;
+    size_t qlog_batch_ = 0;
     // Split-mode workspace: tridiagonal off-diagonals and Z, one per matrix.
     void* lbuf_ = nullptr;
     size_t lbuf_batch_ = 0;
@@ -140,6 +146,8 @@ public:
     // round (a larger batch is solved in rounds of this many).
     static size_t large_matrix_bytes(int n);
     static size_t large_chunk(int n);
+    // Bytes of one matrix's QL rotation log (orders up to 64).
+    static size_t ql_log_bytes(int n);
     void set_stage_timing(bool on);
     void set_graphs(bool on);
     long long graph_captures() const { return graph_captures_; }
