@@ -994,24 +994,36 @@ geneig_lg_back(const int* __restrict__ info, char* __restrict__ ws, long long ws
                 const int r = 64 * q + lane, col = c0 + c;
                 x[q][c] = cm((r < N && col < N) ? Z[(size_t)col * N + r] : real(0.0), 0.0);
             }
-        for (int k = N - 2; k >= 0; --k) {
-            const cx tau = R[(size_t)k * N + k + 1];
-            if (tau.x == real(0.0) && tau.y == real(0.0)) continue;
-            cx v[ND];
+        // Reflector k is fetched while reflector k+1 is applied.
+        auto fetch = [&](int k, cx (&v)[ND], cx& tau) {
+            tau = R[(size_t)k * N + k + 1];
 #pragma unroll
             for (int q = 0; q < ND; ++q) {
                 const int r = 64 * q + lane;
                 v[q] = (r == k + 1) ? cm(1.0, 0.0) : ((r > k + 1 && r < N) ? R[(size_t)k * N + r] : cm(0.0, 0.0));
             }
+        };
+        cx v[ND], tau;
+        fetch(N - 2, v, tau);
+        for (int k = N - 2; k >= 0; --k) {
+            cx vn[ND], taun = cm(0.0, 0.0);
 #pragma unroll
-            for (int c = 0; c < CB; ++c) {
-                cx s = cm(0.0, 0.0);
+            for (int q = 0; q < ND; ++q) vn[q] = cm(0.0, 0.0);
+            if (k > 0) fetch(k - 1, vn, taun);
+            if (tau.x != real(0.0) || tau.y != real(0.0)) {
 #pragma unroll
-                for (int q = 0; q < ND; ++q) s = s + cconjmul(v[q], x[q][c]);
-                s = cmul(tau, wave_sum_cx(s));
+                for (int c = 0; c < CB; ++c) {
+                    cx s = cm(0.0, 0.0);
 #pragma unroll
-                for (int q = 0; q < ND; ++q) x[q][c] = x[q][c] - cmul(v[q], s);
+                    for (int q = 0; q < ND; ++q) s = s + cconjmul(v[q], x[q][c]);
+                    s = cmul(tau, wave_sum_cx(s));
+#pragma unroll
+                    for (int q = 0; q < ND; ++q) x[q][c] = x[q][c] - cmul(v[q], s);
+                }
             }
+#pragma unroll
+            for (int q = 0; q < ND; ++q) v[q] = vn[q];
+            tau = taun;
         }
 #pragma unroll
         for (int q = 0; q < ND; ++q)
