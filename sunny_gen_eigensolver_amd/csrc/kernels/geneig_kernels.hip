@@ -248,6 +248,30 @@ __device__ __forceinline__ void cmfma(ctile& c, cx a, cx b) {
     c.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, c.im, 0, 0, 0);
 }
 __device__ __forceinline__ cx tile_get(const ctile& t, int v) { return cm(t.re[v], t.im[v]); }
+
+// The same product in three real MFMAs (Gauss's 3M form): p += ar br,
+// q += ai bi, s += (ar + ai)(br + bi); then re = p - q, im = s - p - q. The
+// rounding error of im is bounded by eps (|ar| + |ai|)(|br| + |bi|) per term,
+// at most twice the four-product form's. bs = br + bi, formed once per B.
+struct ctile3 {
+    d4 p, q, s;
+};
+__device__ __forceinline__ void c3_zero(ctile3& c) {
+    c.p = d4{0.0, 0.0, 0.0, 0.0};
+    c.q = d4{0.0, 0.0, 0.0, 0.0};
+    c.s = d4{0.0, 0.0, 0.0, 0.0};
+}
+__device__ __forceinline__ void cmfma3(ctile3& c, cx a, cx b, double bs) {
+    c.p = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, c.p, 0, 0, 0);
+    c.q = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, c.q, 0, 0, 0);
+    c.s = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x + a.y, bs, c.s, 0, 0, 0);
+}
+__device__ __forceinline__ ctile c3_final(const ctile3& c) {
+    ctile t;
+    t.re = c.p - c.q;
+    t.im = c.s - c.p - c.q;
+    return t;
+}
 #endif
 
 }  // namespace gk
@@ -2164,6 +2188,7 @@ constexpr int NB = (N + 15) / 16;          // 16-blocks per dimension
 constexpr int REF_NT = 64 * NB;            // one wave per column block
 constexpr int KEND = (N + 3) / 4 * 4;      // k-steps that touch rows/columns < N
 constexpr int LDR = N | 1;
+constexpr int RGR = NB < 2 ? NB : 2;     // row tiles per accumulation group
 
 // Element (i, k) of an N x N column-major matrix (leading dimension ld), zero
 // outside; the address is clamped so the load is branch-free.
@@ -2180,17 +2205,25 @@ template <bool UPPER_A = false, class FA, class FB>
 __device__ __forceinline__ void cgemm(ctile (&acc)[NB], FA fa, FB fb, int cb, int ke = KEND) {
     const int l = lane_id();
     const int c16 = l & 15, k4 = l >> 4;
+    // Row tiles in groups of RGR, so that only RGR 3M accumulators are live.
 #pragma unroll
-    for (int rt = 0; rt < NB; ++rt) {
-        acc[rt].re = d4{0.0, 0.0, 0.0, 0.0};
-        acc[rt].im = d4{0.0, 0.0, 0.0, 0.0};
-    }
+    for (int g0 = 0; g0 < NB; g0 += RGR) {
+        ctile3 a3[RGR];
+#pragma unroll
+        for (int g = 0; g < RGR; ++g) c3_zero(a3[g]);
 #pragma unroll 2
-    for (int k0 = 0; k0 < ke; k0 += 4) {
-        const cx b = fb(k0 + k4, cb * 16 + c16);
+        for (int k0 = 0; k0 < ke; k0 += 4) {
+            const cx b = fb(k0 + k4, cb * 16 + c16);
+            const real bs = b.x + b.y;
 #pragma unroll
-        for (int rt = 0; rt < NB; ++rt)
-            if (!UPPER_A || k0 >= 16 * rt) cmfma(acc[rt], fa(rt * 16 + c16, k0 + k4), b);
+            for (int g = 0; g < RGR; ++g) {
+                const int rt = g0 + g;
+                if (rt < NB && (!UPPER_A || k0 >= 16 * rt)) cmfma3(a3[g], fa(rt * 16 + c16, k0 + k4), b, bs);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < RGR; ++g)
+            if (g0 + g < NB) acc[g0 + g] = c3_final(a3[g]);
     }
 }
 
@@ -2205,23 +2238,30 @@ __device__ __forceinline__ void cgemm_reg(ctile (&acc)[NB], FA fa, const ctile (
     const int l = lane_id();
     const int c16 = l & 15, k4 = l >> 4;
 #pragma unroll
-    for (int rt = 0; rt < NB; ++rt) {
-        acc[rt].re = d4{0.0, 0.0, 0.0, 0.0};
-        acc[rt].im = d4{0.0, 0.0, 0.0, 0.0};
-    }
+    for (int g0 = 0; g0 < NB; g0 += RGR) {
+        ctile3 a3[RGR];
 #pragma unroll
-    for (int q = 0; q < NB; ++q) {
+        for (int g = 0; g < RGR; ++g) c3_zero(a3[g]);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            if (16 * q + 4 * v >= KEND) continue;    // k-steps entirely beyond N
-            const cx b = tile_get(Bt[q], v);
-            const int k = 16 * q + 4 * v + k4;
+        for (int q = 0; q < NB; ++q) {
 #pragma unroll
-            for (int rt = 0; rt < NB; ++rt) {
-                if (LOWER_A && (q > rt || rt < rt_min)) continue;
-                cmfma(acc[rt], fa(rt * 16 + c16, k), b);
+            for (int v = 0; v < 4; ++v) {
+                if (16 * q + 4 * v >= KEND) continue;    // k-steps entirely beyond N
+                const cx b = tile_get(Bt[q], v);
+                const real bs = b.x + b.y;
+                const int k = 16 * q + 4 * v + k4;
+#pragma unroll
+                for (int g = 0; g < RGR; ++g) {
+                    const int rt = g0 + g;
+                    if (rt >= NB || (LOWER_A && (q > rt || rt < rt_min))) continue;
+                    cmfma3(a3[g], fa(rt * 16 + c16, k), b, bs);
+                }
             }
+            __builtin_amdgcn_sched_barrier(0);      // bounds the operand loads in flight
         }
+#pragma unroll
+        for (int g = 0; g < RGR; ++g)
+            if (g0 + g < NB) acc[g0 + g] = c3_final(a3[g]);
     }
 }
 
@@ -2247,6 +2287,23 @@ __device__ __forceinline__ cx melem(const cx* P, int i, int k) {
     return in ? (lo ? v : cm(v.x, -v.y)) : cm(0.0, 0.0);
 }
 
+// L(i, k) of a lower-triangular L from its packed lower triangle, zero above
+// the diagonal and outside.
+__device__ __forceinline__ cx lelem(const cx* P, int i, int k) {
+    const bool in = i < N && k < N && i >= k;
+    const cx v = P[in ? pk(k) + i - k : 0];
+    return in ? v : cm(0.0, 0.0);
+}
+
+// Lower triangle of the column-major N x N matrix A (global) into P, packed
+// by columns (cooperative, coalesced).
+__device__ __forceinline__ void stage_lower(cx* P, const cx* A, int tid, int nt) {
+    for (int e = tid; e < N * N; e += nt) {
+        const int i = e % N, j = e / N;
+        if (i >= j) P[pk(j) + i - j] = A[e];
+    }
+}
+
 // Rayleigh-Ritz on columns (a, b) of V for the Hermitian M (held in r.M), by
 // one wave: the 2 x 2 projected pencil (V^H M V, V^H V) of the pair, then
 // V(:, [a b]) <- V(:, [a b]) X and lam[a], lam[b] <- the Ritz values.
@@ -2255,6 +2312,7 @@ __device__ __forceinline__ void ritz_pair(RefWork& r, int a, int b) {
     const int lr = lane < N ? lane : N - 1;
     const bool in = lane < N;
     cx ha = cm(0.0, 0.0), hb = cm(0.0, 0.0);
+#pragma unroll 2
     for (int k = 0; k < N; ++k) {
         const cx mk = melem(r.M, lr, k);
         ha = ha + cmul(mk, r.U[a * LDR + k]);
@@ -2293,7 +2351,7 @@ __device__ __forceinline__ void ritz_pair(RefWork& r, int a, int b) {
 // pair, Jacobi sweeps over all its pairs otherwise), then U = W^H V. In step
 // it (from 0), delta = 2^it delta_rel max |lambda|. U^H S U = V^H V up to the
 // accuracy of W, as in the FP64 modes.
-extern "C" __global__ void __launch_bounds__(gk::REF_NT)
+extern "C" __global__ void __launch_bounds__(gk::REF_NT, 2)
 geneig_mx_refine(const cx* __restrict__ H, const cx* __restrict__ Wd, real* __restrict__ W,
                  cx* __restrict__ U, const int* __restrict__ info, const gk::cf* __restrict__ V32,
                  const float* __restrict__ W32, int batch, int iters, real delta_rel) {
@@ -2308,21 +2366,25 @@ geneig_mx_refine(const cx* __restrict__ H, const cx* __restrict__ Wd, real* __re
     const size_t off = (size_t)b * N * N;
     const cx* Hb = H + off;
     const cx* Wb = Wd + (size_t)b * N * N;          // W = L^-1, full (upper zero)
-    for (int e = threadIdx.x; e < N * N; e += REF_NT) r.U[(e / N) * LDR + e % N] = to_cx(V32[off + e]);
-    if (threadIdx.x < N) r.lam[threadIdx.x] = (real)W32[(size_t)b * N + threadIdx.x];
     auto fM = [&](int i, int k) { return melem(r.M, i, k); };
     auto fV = [&](int i, int k) { return elem(r.U, LDR, i, k); };
     auto fVh = [&](int i, int k) { return cconj(elem(r.U, LDR, k, i)); };
+    auto fW = [&](int i, int k) { return lelem(r.M, i, k); };
     {
-        // M = W (H W^H), operands from global memory; its lower triangle to LDS
-        // (which makes it exactly Hermitian).
+        // M = W (H W^H) with H staged in the V area and W's lower triangle in
+        // the M area (coalesced loads; the products then read LDS only), then
+        // M's lower triangle over W (which makes M exactly Hermitian).
+        for (int e = threadIdx.x; e < N * N; e += REF_NT) r.U[(e / N) * LDR + e % N] = Hb[e];
+        stage_lower(r.M, Wb, threadIdx.x, REF_NT);
+        __syncthreads();
         ctile Z[NB], M[NB];
         // W is lower triangular: Z(:, j) needs k <= j; M(i, j) needs k <= i,
         // and only the tiles on and below the diagonal are kept.
-        cgemm(Z, [&](int i, int k) { return elem(Hb, N, i, k); },
-              [&](int k, int jj) { return cconj(elem(Wb, N, jj, k)); }, cb,
+        cgemm(Z, [&](int i, int k) { return elem(r.U, LDR, i, k); },
+              [&](int k, int jj) { return cconj(fW(jj, k)); }, cb,
               16 * (cb + 1) < KEND ? 16 * (cb + 1) : KEND);
-        cgemm_reg<true>(M, [&](int i, int k) { return elem(Wb, N, i, k); }, Z, cb);
+        cgemm_reg<true>(M, fW, Z, cb);
+        __syncthreads();     // every wave is done reading H and W
 #pragma unroll
         for (int rt = 0; rt < NB; ++rt) {
 #pragma unroll
@@ -2331,6 +2393,8 @@ geneig_mx_refine(const cx* __restrict__ H, const cx* __restrict__ Wd, real* __re
                 if (i < N && jin && i >= j) r.M[pk(j) + i - j] = tile_get(M[rt], v);
             }
         }
+        for (int e = threadIdx.x; e < N * N; e += REF_NT) r.U[(e / N) * LDR + e % N] = to_cx(V32[off + e]);
+        if (threadIdx.x < N) r.lam[threadIdx.x] = (real)W32[(size_t)b * N + threadIdx.x];
         __syncthreads();
     }
     real delta = real(0.0);
@@ -2431,9 +2495,11 @@ geneig_mx_refine(const cx* __restrict__ H, const cx* __restrict__ Wd, real* __re
         }
     }
     __syncthreads();
-    // U = W^H V.
+    // U = W^H V, W's lower triangle staged again over M (no longer needed).
+    stage_lower(r.M, Wb, threadIdx.x, REF_NT);
+    __syncthreads();
     ctile Uo[NB];
-    cgemm<true>(Uo, [&](int i, int k) { return cconj(elem(Wb, N, k, i)); }, fV, cb);
+    cgemm<true>(Uo, [&](int i, int k) { return cconj(fW(k, i)); }, fV, cb);
     cx* Ub = U + off;
 #pragma unroll
     for (int rt = 0; rt < NB; ++rt) {
