@@ -240,12 +240,12 @@ class Solver:
 
     def stage_times(self):
         """[(kernel name, ms)] of the last launch (stage timing must be on)."""
-        ms = (ctypes.c_double * 8)()
-        names = (ctypes.c_char_p * 8)()
-        n = lib().sunny_geneig_stage_times(self.handle, ms, names, 8)
+        ms = (ctypes.c_double * 12)()
+        names = (ctypes.c_char_p * 12)()
+        n = lib().sunny_geneig_stage_times(self.handle, ms, names, 12)
         if n < 0:
             raise GeneigError(STATUS_INVALID_ARG, last_error())
-        return [(names[i].decode(), ms[i]) for i in range(n)]
+        return [(names[i].decode(), ms[i]) for i in range(min(n, 12))]
 
 
 def _stream_handle(stream) -> int:

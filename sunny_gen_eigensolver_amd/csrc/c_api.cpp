@@ -209,8 +209,8 @@ int sunny_geneig_stage_times(sunny_geneig_solver_t s, double* ms, const char** n
         return -1;
     }
     try {
-        double tmp[8];
-        const int n = s->impl->stage_times(tmp, 8);
+        double tmp[12];
+        const int n = s->impl->stage_times(tmp, 12);
         for (int i = 0; i < n && i < max_stages; ++i) {
             if (ms) ms[i] = tmp[i];
             if (names) names[i] = sgeig::Solver::stage_name(s->impl->mode(), i, s->impl->n());

@@ -1867,7 +1867,8 @@ geneig_split_front(const cx* __restrict__ H, const cx* __restrict__ S, real* __r
 // lane by one step per iteration: a deflation scan of four off-diagonals, or
 // one rotation. Each matrix's sweeps (l, m), rotations, ascending ranks and
 // eigenvalues go to its log; geneig_split_ql then rotates the register rows
-// of Z with them. A matrix whose QL outgrows its log is redone inline there.
+// of Z with them. A matrix whose QL outgrows its log (or the caller's sweep
+// cap, at most the log's) is redone by geneig_ql_full instead.
 // ---------------------------------------------------------------------------
 namespace gk {
 constexpr int QS_CAP = 8 * N;                       // sweeps per matrix
@@ -1884,8 +1885,9 @@ extern "C" __device__ const unsigned long long geneig_ql_log_bytes = gk::QG_BYTE
 
 extern "C" __global__ void __launch_bounds__(64)
 geneig_ql_record(const real* __restrict__ W, const real* __restrict__ E, const int* __restrict__ info,
-                 char* __restrict__ qlog, long long log_stride, int batch) {
+                 char* __restrict__ qlog, long long log_stride, int batch, int sweep_cap) {
     using namespace gk;
+    const int cap = sweep_cap < QS_CAP ? sweep_cap : QS_CAP;
     __shared__ real ds[N][64];
     __shared__ real es[N][64];
     const int lane = lane_id();
@@ -1931,7 +1933,7 @@ geneig_ql_record(const real* __restrict__ W, const real* __restrict__ E, const i
                 } else if (iter == QL_MAXIT) {
                     for (int k = l; k < N - 1; ++k) unconverged += (es[k][lane] != real(0.0)) ? 1 : 0;
                     phase = DONE;
-                } else if (nsw == QS_CAP || nrot + (hit - l) > QR_CAP) {
+                } else if (nsw >= cap || nrot + (hit - l) > QR_CAP) {
                     overflow = 1;
                     phase = DONE;
                 } else {
@@ -2004,7 +2006,21 @@ geneig_ql_record(const real* __restrict__ W, const real* __restrict__ E, const i
 // applies the logged rotations sweep by sweep (each sweep's rotations fetched
 // by the lanes ahead of time, then broadcast by v_readlane), writes the sorted
 // eigenvalues to W and Z with its columns in ascending order to Zw, and info
-// (the number of unconverged off-diagonals, LAPACK's convention).
+// (the number of unconverged off-diagonals, LAPACK's convention). Matrices
+// whose log overflowed are left to geneig_ql_full (info stays 0 for them).
+namespace gk {
+// Z (columns in ascending order of rank) to Zb and info, one wave per matrix.
+__device__ __forceinline__ void store_z(real* Zb, const real (&z)[N], int rank, int* info, int b, int qi) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const int pos = __builtin_amdgcn_readlane(rank, j);
+        if (lane < N) Zb[pos * N + lane] = z[j];
+    }
+    if (lane == 0) info[b] = qi;
+}
+}  // namespace gk
+
 extern "C" __global__ void __launch_bounds__(GENEIG_NT)
 geneig_split_ql(real* __restrict__ W, const real* __restrict__ E, real* __restrict__ Zw,
                 int* __restrict__ info, const char* __restrict__ qlog, long long log_stride, int batch) {
@@ -2016,60 +2032,69 @@ geneig_split_ql(real* __restrict__ W, const real* __restrict__ E, real* __restri
     real* Wb = W + (size_t)b * N;
     const char* lg = qlog + (size_t)b * (size_t)log_stride;
     const int* hdr = reinterpret_cast<const int*>(lg);
+    if (hdr[1]) return;
     real z[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) z[j] = (lane == j) ? 1.0 : 0.0;
-    int rank, qi;
-    if (hdr[1]) {
-        // The log overflowed: the recurrence inline, every lane redundantly.
-        const real* Eb = E + (size_t)b * N;
-        real dreg = (lane < N) ? Wb[lane] : 0.0;
-        real ereg = (lane < N - 1) ? Eb[lane] : 0.0;
-        qi = ql_wave(dreg, ereg, z);
-        rank = ascending_rank(lane < N ? dreg : __builtin_inf());
-        if (lane < N) Wb[rank] = dreg;
-    } else {
-        const int* sw = reinterpret_cast<const int*>(lg + QG_SW);
-        const real* rot = reinterpret_cast<const real*>(lg + QG_ROT);
-        const int nsw = hdr[0];
-        int off = 0;
-        int l = nsw > 0 ? sw[0] : 0, m = nsw > 0 ? sw[1] : 0;
-        real cr = (lane < m - l) ? rot[2 * lane] : real(1.0);
-        real sr = (lane < m - l) ? rot[2 * lane + 1] : real(0.0);
-        for (int t = 0; t < nsw; ++t) {
-            const int off_n = off + (m - l);
-            const bool last = t + 1 == nsw;
-            const int ln = last ? 0 : sw[2 * t + 2], mn = last ? 0 : sw[2 * t + 3];
-            const real cn = (lane < mn - ln) ? rot[2 * (off_n + lane)] : real(1.0);
-            const real sn = (lane < mn - ln) ? rot[2 * (off_n + lane) + 1] : real(0.0);
+    const int* sw = reinterpret_cast<const int*>(lg + QG_SW);
+    const real* rot = reinterpret_cast<const real*>(lg + QG_ROT);
+    const int nsw = hdr[0];
+    int off = 0;
+    int l = nsw > 0 ? sw[0] : 0, m = nsw > 0 ? sw[1] : 0;
+    real cr = (lane < m - l) ? rot[2 * lane] : real(1.0);
+    real sr = (lane < m - l) ? rot[2 * lane + 1] : real(0.0);
+    for (int t = 0; t < nsw; ++t) {
+        const int off_n = off + (m - l);
+        const bool last = t + 1 == nsw;
+        const int ln = last ? 0 : sw[2 * t + 2], mn = last ? 0 : sw[2 * t + 3];
+        const real cn = (lane < mn - ln) ? rot[2 * (off_n + lane)] : real(1.0);
+        const real sn = (lane < mn - ln) ? rot[2 * (off_n + lane) + 1] : real(0.0);
 #pragma unroll
-            for (int i = N - 2; i >= 0; --i) {
-                if (i < m && i >= l) {
-                    const real c = bcast(cr, i - l), s = bcast(sr, i - l);
-                    const real zi = z[i], zi1 = z[i + 1];
-                    z[i + 1] = fma(s, zi, c * zi1);
-                    z[i] = fma(c, zi, -(s * zi1));
-                }
+        for (int i = N - 2; i >= 0; --i) {
+            if (i < m && i >= l) {
+                const real c = bcast(cr, i - l), s = bcast(sr, i - l);
+                const real zi = z[i], zi1 = z[i + 1];
+                z[i + 1] = fma(s, zi, c * zi1);
+                z[i] = fma(c, zi, -(s * zi1));
             }
-            cr = cn;
-            sr = sn;
-            off = off_n;
-            l = ln;
-            m = mn;
         }
-        const int* rk = reinterpret_cast<const int*>(lg + QG_RANK);
-        const real* lam = reinterpret_cast<const real*>(lg + QG_LAM);
-        rank = (lane < N) ? rk[lane] : N;
-        if (lane < N) Wb[lane] = lam[lane];
-        qi = hdr[2];
+        cr = cn;
+        sr = sn;
+        off = off_n;
+        l = ln;
+        m = mn;
     }
-    real* Zb = Zw + (size_t)b * N * N;
+    const int* rk = reinterpret_cast<const int*>(lg + QG_RANK);
+    const real* lam = reinterpret_cast<const real*>(lg + QG_LAM);
+    const int rank = (lane < N) ? rk[lane] : N;
+    if (lane < N) Wb[lane] = lam[lane];
+    store_z(Zw + (size_t)b * N * N, z, rank, info, b, hdr[2]);
+}
+
+// The matrices whose QL log overflowed: the recurrence inline, every lane
+// redundantly, with the rows of Z in registers (a kernel of its own, so that
+// the common path above keeps its small register footprint).
+extern "C" __global__ void __launch_bounds__(GENEIG_NT)
+geneig_ql_full(real* __restrict__ W, const real* __restrict__ E, real* __restrict__ Zw,
+               int* __restrict__ info, const char* __restrict__ qlog, long long log_stride, int batch) {
+    using namespace gk;
+    const int b = blockIdx.x * NW + wave_id();
+    if (b >= batch) return;
+    if (info[b] != 0) return;
+    const int* hdr = reinterpret_cast<const int*>(qlog + (size_t)b * (size_t)log_stride);
+    if (!hdr[1]) return;
+    const int lane = lane_id();
+    real* Wb = W + (size_t)b * N;
+    const real* Eb = E + (size_t)b * N;
+    real z[N];
 #pragma unroll
-    for (int j = 0; j < N; ++j) {
-        const int pos = __builtin_amdgcn_readlane(rank, j);
-        if (lane < N) Zb[pos * N + lane] = z[j];
-    }
-    if (lane == 0) info[b] = qi;
+    for (int j = 0; j < N; ++j) z[j] = (lane == j) ? 1.0 : 0.0;
+    real dreg = (lane < N) ? Wb[lane] : 0.0;
+    real ereg = (lane < N - 1) ? Eb[lane] : 0.0;
+    const int qi = ql_wave(dreg, ereg, z);
+    const int rank = ascending_rank(lane < N ? dreg : __builtin_inf());
+    if (lane < N) Wb[rank] = dreg;
+    store_z(Zw + (size_t)b * N * N, z, rank, info, b, qi);
 }
 
 namespace gk {

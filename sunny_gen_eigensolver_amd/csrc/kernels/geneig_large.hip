@@ -824,11 +824,13 @@ __device__ __forceinline__ void ql_inline(int b, real* __restrict__ wout, long l
 // QL, first half: one wave per matrix runs the scalar recurrence alone
 // (registers only, so many matrices share a CU and hide each other's
 // latency) and logs every sweep's window and rotations, then the ranks, and
-// writes the ascending eigenvalues to wout and info as above.
+// writes the ascending eigenvalues to wout and info as above. More sweeps than
+// the log holds, or than the caller's sweep_cap, mark the log as overflowed.
 extern "C" __global__ void __launch_bounds__(64)
 geneig_lg_qlr(real* __restrict__ wout, long long wout_stride, int* __restrict__ info, char* __restrict__ ws,
-              long long wss, int batch) {
+              long long wss, int batch, int sweep_cap) {
     using namespace gk;
+    const int cap = sweep_cap < QL_SW_CAP ? sweep_cap : QL_SW_CAP;
     const int b = blockIdx.x;
     if (b >= batch || info[b] > N) return;
     const int lane = lane_id();
@@ -860,7 +862,7 @@ geneig_lg_qlr(real* __restrict__ wout, long long wout_stride, int* __restrict__ 
             unconverged = ql_unconverged(er, l);
             break;
         }
-        if (nsw == QL_SW_CAP || nrot + (m - l) > QL_ROT_CAP) {
+        if (nsw >= cap || nrot + (m - l) > QL_ROT_CAP) {
             overflow = true;
             break;
         }

@@ -101,6 +101,7 @@ Solver::Solver(int n, int device_id, int bpb_request, Tuning tuning, CacheMode c
                                     std::to_string(kMaxOrder));
     large_ = n > kMaxLdsOrder;
     if (const char* v = std::getenv("SUNNY_GENEIG_GRAPH")) graphs_ = std::atoi(v) != 0;
+    if (const char* v = std::getenv("SUNNY_GENEIG_QL_SWEEP_CAP")) ql_sweep_cap_ = std::max(0, std::atoi(v));
     if (bpb_request > 4)
         throw std::invalid_argument("batches_per_block must be <= 4 (one wave per matrix, "
                                     "LDS-resident work area)");
@@ -188,6 +189,7 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
     fused_ = module_->get("geneig_fused");
     k_reduce_ = module_->get("geneig_split_front");
     k_heev_ = module_->get("geneig_split_ql");
+    k_qlfull_ = module_->get("geneig_ql_full");
     k_back_ = module_->get("geneig_split_back");
     k_qlrec_ = module_->get("geneig_ql_record");
     {
@@ -211,6 +213,7 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
         acquire_ms_ += ms32;
         k_mx_front_ = module32_->get("geneig_mx_front");
         k_ql32_ = module32_->get("geneig_split_ql");
+        k_qlfull32_ = module32_->get("geneig_ql_full");
         k_qlrec_ = module32_->get("geneig_ql_record");
         k_back32_ = module32_->get("geneig_mx_back");
         if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_ITERS")) refine_iters_ = std::max(0, std::atoi(v));
@@ -409,12 +412,16 @@ void Solver::launch_hybrid_(const void* H, const void* S, double* W, void* U, in
     ensure_qlog_((size_t)batch);
     char* qlog = static_cast<char*>(qlog_);
     long long qstride = (long long)ql_log_bytes(n_);
-    void* ar[] = {&W, &E, &info, &qlog, &qstride, &batch};
+    int cap = ql_sweep_cap_;
+    void* ar[] = {&W, &E, &info, &qlog, &qstride, &batch, &cap};
     SG_HIP(hipModuleLaunchKernel(k_qlrec_, (unsigned)((batch + 63) / 64), 1, 1, 64, 1, 1, 0, stream, ar,
                                  nullptr));
     mark_(stream);
     void* a2[] = {&W, &E, &Zw, &info, &qlog, &qstride, &batch};
     SG_HIP(hipModuleLaunchKernel(k_heev_, grid2, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream, a2,
+                                 nullptr));
+    mark_(stream);
+    SG_HIP(hipModuleLaunchKernel(k_qlfull_, grid2, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream, a2,
                                  nullptr));
     mark_(stream);
     void* a3[] = {&U, &Zw, &info, &batch};
@@ -466,12 +473,15 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
     ensure_qlog_((size_t)batch);
     char* qlog = static_cast<char*>(qlog_);
     long long qstride = (long long)ql_log_bytes(n_);
-    void* ar[] = {&W32, &E32, &info, &qlog, &qstride, &batch};
+    int cap = ql_sweep_cap_;
+    void* ar[] = {&W32, &E32, &info, &qlog, &qstride, &batch, &cap};
     SG_HIP(hipModuleLaunchKernel(k_qlrec_, (unsigned)((batch + 63) / 64), 1, 1, 64, 1, 1, 0, stream, ar,
                                  nullptr));
     mark_(stream);
     void* a3[] = {&W32, &E32, &Z32, &info, &qlog, &qstride, &batch};
     SG_HIP(hipModuleLaunchKernel(k_ql32_, grid_w, 1, 1, bd, 1, 1, 0, stream, a3, nullptr));
+    mark_(stream);
+    SG_HIP(hipModuleLaunchKernel(k_qlfull32_, grid_w, 1, 1, bd, 1, 1, 0, stream, a3, nullptr));
     mark_(stream);
     void* a4[] = {&V32, &Z32, &info, &batch};
     SG_HIP(hipModuleLaunchKernel(k_back32_, grid_w, 1, 1, bd, 1, 1, 0, stream, a4, nullptr));
@@ -575,7 +585,8 @@ void Solver::launch_large_(const void* H, const void* S, double* W, void* U, int
             wout = ws + 88 * nn + 8 * n;
             wstride = wss / 4;
         }
-        void* a4[] = {&wout, &wstride, &Ic, &ws, (void*)&wss, &cnt};
+        int cap = ql_sweep_cap_;
+        void* a4[] = {&wout, &wstride, &Ic, &ws, (void*)&wss, &cnt, &cap};
         SG_HIP(hipModuleLaunchKernel(k_lg_ql_, (unsigned)cnt, 1, 1, 64, 1, 1, 0, stream, a4, nullptr));
         mark();
         SG_HIP(hipModuleLaunchKernel(k_lg_qlz_, (unsigned)cnt, 1, 1, bd_ql, 1, 1, 0, stream, a4, nullptr));
@@ -619,7 +630,7 @@ size_t Solver::reserve(size_t batch) {
 void Solver::set_stage_timing(bool on) {
     timing_ = on;
     if (on && ev_.empty()) {
-        ev_.resize(8);
+        ev_.resize(12);
         for (auto& e : ev_) SG_HIP(hipEventCreate(&e));
     }
     ev_used_ = 0;
@@ -652,13 +663,13 @@ const char* Solver::stage_name(Mode mode, int i, int n) {
         return (mode == Mode::Mixed) ? large_mx[i] : large64[i];
     }
     static const char* unified[] = {"geneig_fused"};
-    static const char* hybrid[] = {"geneig_split_front", "geneig_ql_record", "geneig_split_ql",
+    static const char* hybrid[] = {"geneig_split_front", "geneig_ql_record", "geneig_split_ql", "geneig_ql_full",
                                    "geneig_split_back"};
     static const char* mixed[] = {"geneig_lg_chol", "geneig_mx_front", "geneig_ql_record", "geneig_split_ql",
-                                  "geneig_mx_back", "geneig_mx_refine"};
+                                  "geneig_ql_full", "geneig_mx_back",   "geneig_mx_refine"};
     if (mode == Mode::Unified) return (i == 0) ? unified[0] : "";
-    if (mode == Mode::Hybrid) return (i >= 0 && i < 4) ? hybrid[i] : "";
-    return (i >= 0 && i < 6) ? mixed[i] : "";
+    if (mode == Mode::Hybrid) return (i >= 0 && i < 5) ? hybrid[i] : "";
+    return (i >= 0 && i < 7) ? mixed[i] : "";
 }
 
 }  // namespace sgeig

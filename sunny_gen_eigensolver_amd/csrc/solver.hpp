@@ -71,6 +71,7 @@ private:
     hipFunction_t fused_ = nullptr;
     hipFunction_t k_reduce_ = nullptr;
     hipFunction_t k_heev_ = nullptr;
+    hipFunction_t k_qlfull_ = nullptr;     // QL of the matrices whose log overflowed
     hipFunction_t k_back_ = nullptr;
     // QL recurrence recorder (64 matrices per wave) of the split QL and its
     // per-matrix rotation logs (split and mixed modes, orders up to 64).
@@ -87,6 +88,7 @@ private:
     hipFunction_t k_mx_chol_ = nullptr;
     hipFunction_t k_mx_front_ = nullptr;
     hipFunction_t k_ql32_ = nullptr;
+    hipFunction_t k_qlfull32_ = nullptr;
     hipFunction_t k_back32_ = nullptr;
     hipFunction_t k_refine_ = nullptr;
     void* mbuf_ = nullptr;
@@ -100,6 +102,10 @@ private:
     size_t gbuf_batch_ = 0;
     int refine_iters_ = 2;
     double refine_delta_ = 1e-4;
+    // Sweeps a QL log may hold before the matrix is redone without a log (the
+    // kernels clamp it to the log's capacity); SUNNY_GENEIG_QL_SWEEP_CAP lowers
+    // it, which tests use to run the no-log path.
+    int ql_sweep_cap_ = 1 << 30;
 
     // Multi-kernel launches (split, mixed, staged) are captured once per
     // (buffers, batch) into a hipGraph on an internal stream and replayed on
