@@ -449,21 +449,23 @@ __device__ __forceinline__ cx gel(const cx* __restrict__ M, int i, int k) {
 
 // acc[g] = sum over k in [kb, ke) of A(i, k) B(k, j) for row tile rt0 + g and
 // column tile ct; fa / fb return zero outside N. kb, ke multiples of 4.
+// Accumulated in the 3M form (three real MFMAs per complex product).
 template <int RG, class FA, class FB>
 __device__ __forceinline__ void gemm_rg(ctile (&acc)[RG], FA fa, FB fb, int rt0, int ct, int kb, int ke) {
     const int l = lane_id();
     const int c16 = l & 15, k4 = l >> 4;
+    ctile3 a3[RG];
 #pragma unroll
-    for (int g = 0; g < RG; ++g) {
-        acc[g].re = d4{0.0, 0.0, 0.0, 0.0};
-        acc[g].im = d4{0.0, 0.0, 0.0, 0.0};
-    }
+    for (int g = 0; g < RG; ++g) c3_zero(a3[g]);
 #pragma unroll 2
     for (int k0 = kb; k0 < ke; k0 += 4) {
         const cx b = fb(k0 + k4, ct * 16 + c16);
+        const real bs = b.x + b.y;
 #pragma unroll
-        for (int g = 0; g < RG; ++g) cmfma(acc[g], fa((rt0 + g) * 16 + c16, k0 + k4), b);
+        for (int g = 0; g < RG; ++g) cmfma3(a3[g], fa((rt0 + g) * 16 + c16, k0 + k4), b, bs);
     }
+#pragma unroll
+    for (int g = 0; g < RG; ++g) acc[g] = c3_final(a3[g]);
 }
 // Row of element v of a result tile held by this lane (column: ct * 16 + lane % 16).
 __device__ __forceinline__ int tile_row(int rt, int v) { return rt * 16 + 4 * v + (lane_id() >> 4); }
@@ -1043,29 +1045,35 @@ namespace gk {
 // low enough for two workgroups per CU).
 constexpr int RRG = 2;
 
-// acc1[g] = A B1 and acc2[g] = A B2 over k in [kb, ke) (shared A loads).
+// acc1[g] = A B1 and acc2[g] = A B2 over k in [kb, ke) (shared A loads),
+// in the 3M form.
 template <int RG, class FA, class FB1, class FB2>
 __device__ __forceinline__ void gemm2_rg(ctile (&acc1)[RG], ctile (&acc2)[RG], FA fa, FB1 fb1, FB2 fb2, int rt0,
                                          int ct, int kb, int ke) {
     const int l = lane_id();
     const int c16 = l & 15, k4 = l >> 4;
+    ctile3 a1[RG], a2[RG];
 #pragma unroll
     for (int g = 0; g < RG; ++g) {
-        acc1[g].re = d4{0.0, 0.0, 0.0, 0.0};
-        acc1[g].im = d4{0.0, 0.0, 0.0, 0.0};
-        acc2[g].re = d4{0.0, 0.0, 0.0, 0.0};
-        acc2[g].im = d4{0.0, 0.0, 0.0, 0.0};
+        c3_zero(a1[g]);
+        c3_zero(a2[g]);
     }
 #pragma unroll 2
     for (int k0 = kb; k0 < ke; k0 += 4) {
         const cx b1 = fb1(k0 + k4, ct * 16 + c16);
         const cx b2 = fb2(k0 + k4, ct * 16 + c16);
+        const real bs1 = b1.x + b1.y, bs2 = b2.x + b2.y;
 #pragma unroll
         for (int g = 0; g < RG; ++g) {
             const cx a = fa((rt0 + g) * 16 + c16, k0 + k4);
-            cmfma(acc1[g], a, b1);
-            cmfma(acc2[g], a, b2);
+            cmfma3(a1[g], a, b1, bs1);
+            cmfma3(a2[g], a, b2, bs2);
         }
+    }
+#pragma unroll
+    for (int g = 0; g < RG; ++g) {
+        acc1[g] = c3_final(a1[g]);
+        acc2[g] = c3_final(a2[g]);
     }
 }
 
@@ -1150,7 +1158,7 @@ __device__ __forceinline__ void ritz_pair_g(const cx* __restrict__ M, cx* V, rea
 // pair, Jacobi sweeps over its pairs otherwise), then U = W^H V. In step it
 // (from 0), delta = 2^it delta_rel max |lambda|. Buffers: V in the scratch
 // and V slots alternately, P in the reflector slot, E in the caller's U.
-extern "C" __global__ void __launch_bounds__(gk::LNT)
+extern "C" __global__ void __launch_bounds__(gk::LNT, 2)
 geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, const int* __restrict__ info, char* __restrict__ ws,
                  long long wss, int batch, int iters, real delta_rel) {
     using namespace gk;
