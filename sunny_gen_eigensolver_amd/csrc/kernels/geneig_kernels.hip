@@ -1436,7 +1436,8 @@ __device__ __forceinline__ void form_m_reg(WorkP& w, const HT* __restrict__ Hb, 
 }
 
 // Householder vector for column k held in registers (slot c) of its owner.
-__device__ __forceinline__ void house_reg(WorkP& w, int k, cx& col) {
+template <class WK>
+__device__ __forceinline__ void house_reg(WK& w, int k, cx& col) {
     const int lane = lane_id();
     const int m0 = k + 1;
     const cx x = (lane >= m0 && lane < N) ? col : cm(0.0, 0.0);
@@ -1469,7 +1470,8 @@ __device__ __forceinline__ void house_reg(WorkP& w, int k, cx& col) {
 
 // Tridiagonalisation with the full Hermitian M in the owning waves'
 // registers (schedule as in hetrd: two barriers per step).
-__device__ __forceinline__ void hetrd_reg(WorkP& w, cx (&m)[CPW]) {
+template <class WK>
+__device__ __forceinline__ void hetrd_reg(WK& w, cx (&m)[CPW]) {
     const int lane = lane_id(), wave = wave_id();
     const int lr = lane < N ? lane : N - 1;
     if (wave == 0) house_reg(w, 0, m[0]);
@@ -2140,35 +2142,67 @@ geneig_split_back(cx* __restrict__ U, const real* __restrict__ Zw, const int* __
 //                     panels with the products on the matrix cores; W (full,
 //                     upper zero) to Wd. A non-positive pivot ends the matrix
 //                     here with zhegv's INFO, as in the other modes.
-//   geneig_mx_front   (FP32 module): M = W H W^H (W and H read as complex128
-//                     and rounded), Householder tridiagonalisation; reflectors
-//                     and tau to U32, T to W32 / E32.
+//   geneig_mx_formm   (FP64 module): M = W H W^H on the matrix cores; its
+//                     lower triangle packed to Mp (complex128) and, made
+//                     Hermitian from that triangle, rounded in full to M32
+//                     (complex64).
+//   geneig_mx_front   (FP32 module): Householder tridiagonalisation of M32;
+//                     reflectors and tau to U32, T to W32 / E32.
 //   geneig_split_ql   (FP32 module): QL with FP32 unit roundoff, Z to Z32.
 //   geneig_mx_back    (FP32 module): V = Q Z into U32.
 //   geneig_mx_refine  (FP64 module): Ogita-Aishima refinement of (M, V) in
-//                     the Cholesky frame on the matrix cores, Rayleigh-Ritz on
-//                     clusters, then U = W^H V; W, U out.
+//                     the Cholesky frame on the matrix cores (M from Mp),
+//                     Rayleigh-Ritz on clusters, then U = W^H V; W, U out.
 // ---------------------------------------------------------------------------
 
 #ifdef GENEIG_FP32
+namespace gk {
+// Shared memory of the FP32 front half: the tridiagonalisation's buffers
+// only (M arrives formed, straight into the owning waves' registers).
+struct WorkF {
+    union {
+        struct {
+            cx vb[2][N];
+            cx wb[NW][N];
+            cx part[NW * N];
+        } h;
+    } u;
+    cx tau[N];
+    real d[N];
+    real e[N];
+};
+
+// Stash for the FP32 back half: tau on the superdiagonal and the reflector
+// entries from the owning waves' registers (column k, rows >= k+2, go to
+// (k, i)); the lower triangle is left alone (nothing reads it).
+__device__ __forceinline__ void stash_refl_reg(const WorkF& w, const cx (&m)[CPW], cx* __restrict__ Ub) {
+    const int lane = lane_id(), wave = wave_id();
+    for (int i = threadIdx.x; i < N - 1; i += NT) Ub[(size_t)(i + 1) * N + i] = w.tau[i];
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+        const int k = wave + NW * c;
+        if (k < N && lane >= k + 2 && lane < N) Ub[(size_t)lane * N + k] = m[c];
+    }
+}
+}  // namespace gk
+
 extern "C" __global__ void __launch_bounds__(GENEIG_NT, 2)
-geneig_mx_front(const gk::cd* __restrict__ H, const gk::cd* __restrict__ Wd, real* __restrict__ W32,
-                cx* __restrict__ U32, const int* __restrict__ info, real* __restrict__ E32, int batch) {
+geneig_mx_front(const gk::cf* __restrict__ M32, real* __restrict__ W32, cx* __restrict__ U32,
+                const int* __restrict__ info, real* __restrict__ E32, int batch) {
     using namespace gk;
-    __shared__ WorkP w;
+    __shared__ WorkF w;
     const int b = blockIdx.x;
     if (b >= batch || info[b] != 0) return;
     const size_t off = (size_t)b * N * N;
-    const cd* Wb = Wd + (size_t)b * N * N;          // W = L^-1, full (upper zero)
-    for (int e = threadIdx.x; e < N * N; e += NT) {
-        const int i = e % N, j = e / N;
-        if (i >= j) w.Wi[pk(j) + i - j] = to_cx(Wb[e]);
-    }
-    __syncthreads();
+    const int lane = lane_id(), wave = wave_id();
     cx m[CPW];
-    form_m_reg(w, H + off, m);
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+        const int j = wave + NW * c;
+        m[c] = (lane < N && j < N) ? to_cx(M32[off + (size_t)j * N + lane]) : cm(0.0f, 0.0f);
+    }
     hetrd_reg(w, m);
-    stash_reg(w, m, U32 + off);
+    stash_refl_reg(w, m, U32 + off);
     if (threadIdx.x < N) {
         W32[(size_t)b * N + threadIdx.x] = w.d[threadIdx.x];
         E32[(size_t)b * N + threadIdx.x] = w.e[threadIdx.x];
@@ -2322,7 +2356,7 @@ __device__ __forceinline__ cx lelem(const cx* P, int i, int k) {
 
 // Lower triangle of the column-major N x N matrix A (global) into P, packed
 // by columns (cooperative, coalesced).
-__device__ __forceinline__ void stage_lower(cx* P, const cx* A, int tid, int nt) {
+__device__ __forceinline__ void stage_lower_packed(cx* P, const cx* A, int tid, int nt) {
     for (int e = tid; e < N * N; e += nt) {
         const int i = e % N, j = e / N;
         if (i >= j) P[pk(j) + i - j] = A[e];
@@ -2368,6 +2402,73 @@ __device__ __forceinline__ void ritz_pair(RefWork& r, int a, int b) {
 }
 }  // namespace gk
 
+namespace gk {
+// geneig_mx_formm stages H and W in LDS when two workgroups still fit a CU.
+constexpr bool FORMM_LDS = ((size_t)N * LDR + NPK) * sizeof(cx) <= 80 * 1024;
+}  // namespace gk
+
+// M = W H W^H of the mixed mode, once, on the matrix cores (W = L^-1 lower
+// triangular: Z(:, j) = H W^H(:, j) needs k <= j, M(i, j) = W(i, :) Z(:, j)
+// needs k <= i). Its lower triangle goes packed by columns to Mp (complex128,
+// for the refinement) and, made Hermitian from that triangle (real
+// diagonal), rounded to complex64 in full to M32 (for the FP32 front half).
+extern "C" __global__ void __launch_bounds__(gk::REF_NT, 2)
+geneig_mx_formm(const cx* __restrict__ H, const cx* __restrict__ Wd, const int* __restrict__ info,
+                cx* __restrict__ Mp, gk::cf* __restrict__ M32, int batch) {
+    using namespace gk;
+    const int b = blockIdx.x;
+    if (b >= batch || info[b] != 0) return;
+    const int l = lane_id(), cb = wave_id();
+    const int g4 = l >> 4;
+    const int j = cb * 16 + (l & 15);          // this lane's column
+    const bool jin = j < N;
+    const size_t off = (size_t)b * N * N;
+    ctile Z[NB], M[NB];
+    const int kz = 16 * (cb + 1) < KEND ? 16 * (cb + 1) : KEND;
+    if constexpr (FORMM_LDS) {
+        // H and W's lower triangle staged in LDS (coalesced loads; the
+        // products then read LDS only).
+        __shared__ cx Hs[FORMM_LDS ? N * LDR : 1];
+        __shared__ cx Ws[FORMM_LDS ? NPK : 1];
+        const cx* Hb = H + off;
+        for (int e = threadIdx.x; e < N * N; e += REF_NT) Hs[(e / N) * LDR + e % N] = Hb[e];
+        stage_lower_packed(Ws, Wd + off, threadIdx.x, REF_NT);
+        __syncthreads();
+        auto fW = [&](int i, int k) { return lelem(Ws, i, k); };
+        cgemm(Z, [&](int i, int k) { return elem(Hs, LDR, i, k); },
+              [&](int k, int jj) { return cconj(fW(jj, k)); }, cb, kz);
+        cgemm_reg<true>(M, fW, Z, cb);
+    } else {
+        // Operands straight from global memory (no LDS, so more workgroups
+        // per CU hide the latency).
+        const cx* Hb = H + off;
+        const cx* Wb = Wd + off;
+        cgemm(Z, [&](int i, int k) { return elem(Hb, N, i, k); },
+              [&](int k, int jj) { return cconj(elem(Wb, N, jj, k)); }, cb, kz);
+        cgemm_reg<true>(M, [&](int i, int k) { return elem(Wb, N, i, k); }, Z, cb);
+    }
+    cx* Mb = Mp + (size_t)b * NPK;
+    cf* M32b = M32 + off;
+#pragma unroll
+    for (int rt = 0; rt < NB; ++rt) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = rt * 16 + g4 + 4 * v;
+            if (i < N && jin && i >= j) {
+                const cx mv = tile_get(M[rt], v);
+                Mb[pk(j) + i - j] = mv;
+                cf lo, up;
+                lo.x = (float)mv.x;
+                lo.y = (i == j) ? 0.0f : (float)mv.y;
+                up.x = lo.x;
+                up.y = -lo.y;
+                M32b[(size_t)j * N + i] = lo;
+                if (i != j) M32b[(size_t)i * N + j] = up;
+            }
+        }
+    }
+}
+
 // Refinement in the Cholesky frame: with W = L^-1 (FP64), M = W H W^H and the
 // FP32 eigenvectors V of M, iters Ogita-Aishima steps on the matrix cores
 // (T = V^H M V, G = V^H V, lambda_j = T_jj / G_jj, V <- V + V E with E as
@@ -2377,7 +2478,7 @@ __device__ __forceinline__ void ritz_pair(RefWork& r, int a, int b) {
 // it (from 0), delta = 2^it delta_rel max |lambda|. U^H S U = V^H V up to the
 // accuracy of W, as in the FP64 modes.
 extern "C" __global__ void __launch_bounds__(gk::REF_NT, 2)
-geneig_mx_refine(const cx* __restrict__ H, const cx* __restrict__ Wd, real* __restrict__ W,
+geneig_mx_refine(const cx* __restrict__ Mp, const cx* __restrict__ Wd, real* __restrict__ W,
                  cx* __restrict__ U, const int* __restrict__ info, const gk::cf* __restrict__ V32,
                  const float* __restrict__ W32, int batch, int iters, real delta_rel) {
     using namespace gk;
@@ -2389,35 +2490,14 @@ geneig_mx_refine(const cx* __restrict__ H, const cx* __restrict__ Wd, real* __re
     const int j = cb * 16 + (l & 15);          // this lane's column
     const bool jin = j < N;
     const size_t off = (size_t)b * N * N;
-    const cx* Hb = H + off;
     const cx* Wb = Wd + (size_t)b * N * N;          // W = L^-1, full (upper zero)
     auto fM = [&](int i, int k) { return melem(r.M, i, k); };
     auto fV = [&](int i, int k) { return elem(r.U, LDR, i, k); };
     auto fVh = [&](int i, int k) { return cconj(elem(r.U, LDR, k, i)); };
     auto fW = [&](int i, int k) { return lelem(r.M, i, k); };
     {
-        // M = W (H W^H) with H staged in the V area and W's lower triangle in
-        // the M area (coalesced loads; the products then read LDS only), then
-        // M's lower triangle over W (which makes M exactly Hermitian).
-        for (int e = threadIdx.x; e < N * N; e += REF_NT) r.U[(e / N) * LDR + e % N] = Hb[e];
-        stage_lower(r.M, Wb, threadIdx.x, REF_NT);
-        __syncthreads();
-        ctile Z[NB], M[NB];
-        // W is lower triangular: Z(:, j) needs k <= j; M(i, j) needs k <= i,
-        // and only the tiles on and below the diagonal are kept.
-        cgemm(Z, [&](int i, int k) { return elem(r.U, LDR, i, k); },
-              [&](int k, int jj) { return cconj(fW(jj, k)); }, cb,
-              16 * (cb + 1) < KEND ? 16 * (cb + 1) : KEND);
-        cgemm_reg<true>(M, fW, Z, cb);
-        __syncthreads();     // every wave is done reading H and W
-#pragma unroll
-        for (int rt = 0; rt < NB; ++rt) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = rt * 16 + g4 + 4 * v;
-                if (i < N && jin && i >= j) r.M[pk(j) + i - j] = tile_get(M[rt], v);
-            }
-        }
+        const cx* Mb = Mp + (size_t)b * NPK;
+        for (int e = threadIdx.x; e < NPK; e += REF_NT) r.M[e] = Mb[e];
         for (int e = threadIdx.x; e < N * N; e += REF_NT) r.U[(e / N) * LDR + e % N] = to_cx(V32[off + e]);
         if (threadIdx.x < N) r.lam[threadIdx.x] = (real)W32[(size_t)b * N + threadIdx.x];
         __syncthreads();
@@ -2521,7 +2601,7 @@ geneig_mx_refine(const cx* __restrict__ H, const cx* __restrict__ Wd, real* __re
     }
     __syncthreads();
     // U = W^H V, W's lower triangle staged again over M (no longer needed).
-    stage_lower(r.M, Wb, threadIdx.x, REF_NT);
+    stage_lower_packed(r.M, Wb, threadIdx.x, REF_NT);
     __syncthreads();
     ctile Uo[NB];
     cgemm<true>(Uo, [&](int i, int k) { return cconj(fW(k, i)); }, fV, cb);

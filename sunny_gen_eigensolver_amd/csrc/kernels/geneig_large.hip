@@ -490,27 +490,52 @@ using gk::real;
 // S = L L^H and W = L^-1 in the workspace (upper triangle zero). On a
 // non-positive pivot k (0-based) of a leading minor: info = N + k + 1, W NaN,
 // U zero (LAPACK zhegv's convention), and the later stages skip the matrix.
+// For small orders (while three workgroups still fit a CU), L and then W live
+// packed in LDS (the products read LDS), and W goes out in full at the end;
+// otherwise they live in the workspace.
+namespace gk {
+constexpr bool CHOL_LDS = N <= 64 && sizeof(CholWork) + (size_t)N * (N + 1) / 2 * sizeof(cx) <= 52 * 1024;
+constexpr int LPK = N * (N + 1) / 2;
+__device__ __forceinline__ int lpk(int j) { return j * N - (j * (j - 1)) / 2; }
+}  // namespace gk
+
 extern "C" __global__ void __launch_bounds__(gk::LNT, 3)
 geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ U, int* __restrict__ info,
                char* __restrict__ ws, long long wss, int batch) {
     using namespace gk;
     __shared__ CholWork cw;
+    __shared__ cx Lp[CHOL_LDS ? LPK : 1];
     const int b = blockIdx.x;
     if (b >= batch) return;
     const int tid = threadIdx.x, wave = wave_id(), lane = lane_id();
     cx* Lg = reinterpret_cast<cx*>(ws_of(ws, wss, b) + WS_WI);
     const cx* Sb = S + (size_t)b * NN;
-    for (int e = tid; e < (int)NN; e += LNT)
-        if (e % N < e / N) Lg[e] = cm(0.0, 0.0);
-    __syncthreads();
-    auto fL = [&](int i, int k) { return gel(Lg, i, k); };
+    if constexpr (!CHOL_LDS) {
+        for (int e = tid; e < (int)NN; e += LNT)
+            if (e % N < e / N) Lg[e] = cm(0.0, 0.0);
+        __syncthreads();
+    }
+    // L(i, k) (zero above the diagonal and outside), and its store (i >= k).
+    auto fL = [&](int i, int k) -> cx {
+        if constexpr (CHOL_LDS) {
+            const bool in = i < N && k < N && i >= k;
+            const cx v = Lp[in ? lpk(k) + i - k : 0];
+            return in ? v : cm(0.0, 0.0);
+        } else {
+            return gel(Lg, i, k);
+        }
+    };
+    auto setL = [&](int i, int k, cx v) {
+        if constexpr (CHOL_LDS) Lp[lpk(k) + i - k] = v;
+        else Lg[(size_t)k * N + i] = v;
+    };
     for (int p = 0; p < LNB; ++p) {
         const int j0 = 16 * p;
         const int jw = (N - j0 < 16) ? N - j0 : 16;
         // Panel rows j0.. of columns j0 .. j0+jw: S minus the columns left of it.
         for (int rt = p + wave; rt < LNB; rt += LNW) {
             ctile acc[1];
-            gemm_rg<1>(acc, fL, [&](int k, int c) { return cconj(c < jw ? gel(Lg, j0 + c, k) : cm(0.0, 0.0)); },
+            gemm_rg<1>(acc, fL, [&](int k, int c) { return cconj(c < jw ? fL(j0 + c, k) : cm(0.0, 0.0)); },
                        rt, 0, 0, j0);
             const int c = lane & 15;
 #pragma unroll
@@ -538,7 +563,7 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
             const real ri = real(1.0) / sq;
             if (t >= c && tin) {
                 const cx ltc = (t == c) ? cm(sq, 0.0) : cw.pan[c * PLD + t] * ri;
-                Lg[(size_t)(j0 + c) * N + j0 + t] = ltc;
+                setL(j0 + t, j0 + c, ltc);
                 for (int c2 = c + 1; c2 < jw && c2 <= t; ++c2)
                     cw.pan[c2 * PLD + t] = cw.pan[c2 * PLD + t] - cmulc(ltc, cw.pan[c * PLD + c2] * ri);
             }
@@ -561,7 +586,7 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
         const int rw = (N - r0 < 16) ? N - r0 : 16;
         for (int e = tid; e < 256; e += LNT) {
             const int rr = e & 15, c = e >> 4;
-            cw.lii[e] = (rr < rw && c < rw) ? Lg[(size_t)(r0 + c) * N + r0 + rr] : cm(0.0, 0.0);
+            cw.lii[e] = (rr < rw && c < rw) ? fL(r0 + rr, r0 + c) : cm(0.0, 0.0);
         }
         for (int ct = wave; ct < ib; ct += LNW) {
             ctile acc[1];
@@ -584,9 +609,16 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
                 for (int ss = 0; ss < rr; ++ss) s = s - cmul(cw.lii[ss * 16 + rr], xj[ss]);
                 xj[rr] = s * (real(1.0) / cw.lii[rr * 16 + rr].x);
             }
-            for (int rr = 0; rr < rw; ++rr) Lg[(size_t)j * N + r0 + rr] = xj[rr];
+            for (int rr = 0; rr < rw; ++rr)
+                if (CHOL_LDS ? r0 + rr >= j : true) setL(r0 + rr, j, xj[rr]);
         }
         __syncthreads();
+    }
+    if constexpr (CHOL_LDS) {
+        for (int e = tid; e < (int)NN; e += LNT) {
+            const int i = e % N, j = e / N;
+            Lg[e] = (i >= j) ? Lp[lpk(j) + i - j] : cm(0.0, 0.0);
+        }
     }
     if (tid == 0) info[b] = 0;
 }

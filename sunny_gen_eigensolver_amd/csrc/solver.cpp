@@ -93,6 +93,29 @@ int Solver::suggested_block_dim(int n) {
     return 256;
 }
 
+namespace {
+// Mixed-mode workspace for a batch of B matrices of order n (byte offsets):
+// W = L^-1 (complex128, full), the complex64 stash / V, the FP32 eigenvalues
+// and off-diagonals, Z (FP32), M's lower triangle packed (complex128) and M
+// in full (complex64).
+struct MixedLayout {
+    size_t wd, v32, w32, e32, z32, mp, m32, total;
+};
+MixedLayout mixed_layout(size_t n, size_t B) {
+    auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+    MixedLayout m;
+    m.wd = 0;
+    m.v32 = al(m.wd + B * n * n * 16);
+    m.w32 = al(m.v32 + B * n * n * 8);
+    m.e32 = al(m.w32 + B * n * 4);
+    m.z32 = al(m.e32 + B * n * 4);
+    m.mp = al(m.z32 + B * n * n * 4);
+    m.m32 = al(m.mp + B * n * (n + 1) / 2 * 16);
+    m.total = al(m.m32 + B * n * n * 8);
+    return m;
+}
+}  // namespace
+
 Solver::Solver(int n, int device_id, int bpb_request, Tuning tuning, CacheMode cache, Mode mode)
     : n_(n), device_(device_id), mode_(mode) {
     if (n < 1) throw std::invalid_argument("matrix size must be >= 1");
@@ -205,6 +228,7 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
     if (mode_ == Mode::Mixed) {
         k_mx_chol_ = module_->get("geneig_lg_chol");
         k_refine_ = module_->get("geneig_mx_refine");
+        k_mx_formm_ = module_->get("geneig_mx_formm");
         ModuleKey k32 = key;
         k32.fp32 = true;
         CacheLayer l32;
@@ -440,8 +464,8 @@ void Solver::ensure_mixed_workspace_(size_t batch) {
         mbuf_batch_ = 0;
     }
     // complex128 W = L^-1 (full), complex64 stash / V, float eigenvalues,
-    // off-diagonals and Z.
-    SG_HIP(hipMalloc(&mbuf_, workspace_bytes(Mode::Mixed, n_, batch)));
+    // off-diagonals and Z, M (packed complex128 and full complex64).
+    SG_HIP(hipMalloc(&mbuf_, mixed_layout((size_t)n_, batch).total));
     mbuf_batch_ = batch;
 }
 
@@ -452,12 +476,16 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
                            hipStream_t stream) {
     ensure_mixed_workspace_((size_t)batch);
     const size_t n = (size_t)n_, B = (size_t)batch;
+    // Offsets for this batch (the buffer holds at least this many matrices).
+    const MixedLayout ml = mixed_layout(n, B);
     char* base = static_cast<char*>(mbuf_);
-    void* Wd = base;
-    void* V32 = base + B * n * n * 16;
-    void* W32 = base + B * (n * n * 16 + n * n * 8);
-    void* E32 = base + B * (n * n * 16 + n * n * 8 + n * 4);
-    void* Z32 = base + B * (n * n * 16 + n * n * 8 + n * 8);
+    void* Wd = base + ml.wd;
+    void* V32 = base + ml.v32;
+    void* W32 = base + ml.w32;
+    void* E32 = base + ml.e32;
+    void* Z32 = base + ml.z32;
+    void* Mp = base + ml.mp;
+    void* M32 = base + ml.m32;
     const unsigned grid_b = (unsigned)batch;
     const unsigned grid_w = (unsigned)((batch + bpb_ - 1) / bpb_);
     const unsigned bd = (unsigned)block_dim_;
@@ -467,7 +495,12 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
     void* a1[] = {&S, &W, &U, &info, &Wd, &wstride, &batch};
     SG_HIP(hipModuleLaunchKernel(k_mx_chol_, grid_b, 1, 1, 256, 1, 1, 0, stream, a1, nullptr));
     mark_(stream);
-    void* a2[] = {&H, &Wd, &W32, &V32, &info, &E32, &batch};
+    // M = W H W^H once in FP64 on the matrix cores (one wave per 16-column block).
+    const unsigned bd_ref = (unsigned)(64 * ((n_ + 15) / 16));
+    void* af[] = {&H, &Wd, &info, &Mp, &M32, &batch};
+    SG_HIP(hipModuleLaunchKernel(k_mx_formm_, grid_b, 1, 1, bd_ref, 1, 1, 0, stream, af, nullptr));
+    mark_(stream);
+    void* a2[] = {&M32, &W32, &V32, &info, &E32, &batch};
     SG_HIP(hipModuleLaunchKernel(k_mx_front_, grid_b, 1, 1, bd, 1, 1, 0, stream, a2, nullptr));
     mark_(stream);
     ensure_qlog_((size_t)batch);
@@ -488,9 +521,8 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
     mark_(stream);
     int iters = refine_iters_;
     double delta = refine_delta_;
-    void* a5[] = {&H, &Wd, &W, &U, &info, &V32, &W32, &batch, &iters, &delta};
+    void* a5[] = {&Mp, &Wd, &W, &U, &info, &V32, &W32, &batch, &iters, &delta};
     // The refinement runs one wave per 16-column block of the matrix.
-    const unsigned bd_ref = (unsigned)(64 * ((n_ + 15) / 16));
     SG_HIP(hipModuleLaunchKernel(k_refine_, grid_b, 1, 1, bd_ref, 1, 1, 0, stream, a5, nullptr));
     mark_(stream);
 }
@@ -606,12 +638,12 @@ void Solver::launch_large_(const void* H, const void* S, double* W, void* U, int
     }
 }
 
+
 size_t Solver::workspace_bytes(Mode mode, int n_, size_t batch) {
     if (n_ > kMaxLdsOrder) return std::min(batch, large_chunk(n_)) * large_matrix_bytes(n_);
     const size_t n = (size_t)n_;
     if (mode == Mode::Hybrid) return batch * ((n + n * n) * 8 + ql_log_bytes(n_));
-    if (mode == Mode::Mixed)
-        return batch * (n * n * 16 + n * n * 8 + n * 4 + n * 4 + n * n * 4 + ql_log_bytes(n_));
+    if (mode == Mode::Mixed) return mixed_layout(n, batch).total + batch * ql_log_bytes(n_);
     return 0;
 }
 
@@ -665,11 +697,11 @@ const char* Solver::stage_name(Mode mode, int i, int n) {
     static const char* unified[] = {"geneig_fused"};
     static const char* hybrid[] = {"geneig_split_front", "geneig_ql_record", "geneig_split_ql", "geneig_ql_full",
                                    "geneig_split_back"};
-    static const char* mixed[] = {"geneig_lg_chol", "geneig_mx_front", "geneig_ql_record", "geneig_split_ql",
-                                  "geneig_ql_full", "geneig_mx_back",   "geneig_mx_refine"};
+    static const char* mixed[] = {"geneig_lg_chol",   "geneig_mx_formm", "geneig_mx_front", "geneig_ql_record",
+                                  "geneig_split_ql",  "geneig_ql_full",  "geneig_mx_back",  "geneig_mx_refine"};
     if (mode == Mode::Unified) return (i == 0) ? unified[0] : "";
     if (mode == Mode::Hybrid) return (i >= 0 && i < 5) ? hybrid[i] : "";
-    return (i >= 0 && i < 7) ? mixed[i] : "";
+    return (i >= 0 && i < 8) ? mixed[i] : "";
 }
 
 }  // namespace sgeig

@@ -89,6 +89,7 @@ private:
     hipFunction_t k_mx_front_ = nullptr;
     hipFunction_t k_ql32_ = nullptr;
     hipFunction_t k_qlfull32_ = nullptr;
+    hipFunction_t k_mx_formm_ = nullptr;
     hipFunction_t k_back32_ = nullptr;
     hipFunction_t k_refine_ = nullptr;
     void* mbuf_ = nullptr;
