@@ -173,6 +173,17 @@ sunny_geneig_varlen_t sunny_geneig_varlen_create(int max_n, int device_id,
                                                  const int* bucket_orders, int num_buckets,
                                                  sunny_geneig_cache_mode_t cache_mode);
 
+/* As sunny_geneig_varlen_create, with a tuning mode: AUTOTUNE gives every
+ * bucket up to 64 the workgroup size (matrices per workgroup) that the timed
+ * sweep of sunny_geneig_create_ex's AUTOTUNE picks for that order (run once
+ * per order and architecture, then read from the cache's table); SUGGESTED
+ * is sunny_geneig_varlen_create. Buckets above 64 have one size. Counterpart:
+ * the reference tunes one order per handle only (unified_solver.cpp:649-925). */
+sunny_geneig_varlen_t sunny_geneig_varlen_create_ex(int max_n, int device_id,
+                                                    const int* bucket_orders, int num_buckets,
+                                                    sunny_geneig_cache_mode_t cache_mode,
+                                                    sunny_geneig_tuning_mode_t tuning_mode);
+
 void sunny_geneig_varlen_destroy(sunny_geneig_varlen_t solver);
 
 /* Asynchronous on `stream`; n_host (host memory, `count` orders) is read
@@ -189,9 +200,11 @@ sunny_geneig_status_t sunny_geneig_varlen_launch(
     int                   count,
     uintptr_t             stream);
 
-/* Number of buckets and the order of bucket i (-1 on NULL / out of range). */
+/* Number of buckets, the order of bucket i and its matrices per workgroup
+ * (-1 on NULL / out of range). */
 int sunny_geneig_varlen_num_buckets (sunny_geneig_varlen_t solver);
 int sunny_geneig_varlen_bucket_order(sunny_geneig_varlen_t solver, int i);
+int sunny_geneig_varlen_bucket_waves(sunny_geneig_varlen_t solver, int i);
 
 /* ---- serving session (no counterpart in the reference ABI) ---------------
  * A request loop over resident solvers: requests (count pencils of one order

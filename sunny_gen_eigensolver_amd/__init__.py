@@ -46,6 +46,7 @@ EXPORTED_SYMBOLS = (
     "sunny_geneig_cache_layer", "sunny_geneig_cache_dir",
     "sunny_geneig_varlen_create", "sunny_geneig_varlen_destroy", "sunny_geneig_varlen_launch",
     "sunny_geneig_varlen_num_buckets", "sunny_geneig_varlen_bucket_order",
+    "sunny_geneig_varlen_create_ex", "sunny_geneig_varlen_bucket_waves",
     "sunny_geneig_set_stage_timing", "sunny_geneig_stage_times",
     "sunny_geneig_set_graphs", "sunny_geneig_graph_stats",
     "sunny_geneig_session_create", "sunny_geneig_session_destroy", "sunny_geneig_session_submit",
@@ -103,6 +104,10 @@ def lib() -> ctypes.CDLL:
     pi = ctypes.POINTER(ctypes.c_int)
     L.sunny_geneig_varlen_create.argtypes = [ci, ci, pi, ci, ci]
     L.sunny_geneig_varlen_create.restype = vp
+    L.sunny_geneig_varlen_create_ex.argtypes = [ci, ci, pi, ci, ci, ci]
+    L.sunny_geneig_varlen_create_ex.restype = vp
+    L.sunny_geneig_varlen_bucket_waves.argtypes = [vp, ci]
+    L.sunny_geneig_varlen_bucket_waves.restype = ci
     L.sunny_geneig_varlen_destroy.argtypes = [vp]
     L.sunny_geneig_varlen_destroy.restype = None
     L.sunny_geneig_varlen_launch.argtypes = [vp, cu, cu, cu, cu, cu, pi, ci, cu]
@@ -318,14 +323,20 @@ class VarlenSolver:
     sum_{j<k} n_j^2; eigenvalues at sum_{j<k} n_j); `pack` / `unpack` convert
     from and to lists of (n_k, n_k) tensors in the (B, n, n) convention above."""
 
-    def __init__(self, max_n: int = 64, device_id: int = 0, buckets=None, cache: str = "auto"):
+    def __init__(self, max_n: int = 64, device_id: int = 0, buckets=None, cache: str = "auto",
+                 tuning: str = "suggested"):
+        """tuning: "suggested" (fixed workgroup size per order) or "autotune"
+        (per bucket up to 64, the fused solver's timed sweep for that order)."""
         L = lib()
         c = {"nodisk": CACHE_NODISK, "nocache": CACHE_NOCACHE}.get(cache, CACHE_AUTO)
+        if tuning not in ("suggested", "autotune"):
+            raise ValueError("tuning must be 'suggested' or 'autotune'")
+        t = TUNING_AUTOTUNE if tuning == "autotune" else TUNING_SUGGESTED
         if buckets:
             arr = (ctypes.c_int * len(buckets))(*buckets)
-            h = L.sunny_geneig_varlen_create(max_n, device_id, arr, len(buckets), c)
+            h = L.sunny_geneig_varlen_create_ex(max_n, device_id, arr, len(buckets), c, t)
         else:
-            h = L.sunny_geneig_varlen_create(max_n, device_id, None, 0, c)
+            h = L.sunny_geneig_varlen_create_ex(max_n, device_id, None, 0, c, t)
         if not h:
             raise GeneigError(STATUS_INVALID_ARG, f"sunny_geneig_varlen_create failed: {last_error()}")
         self.handle = h
@@ -335,6 +346,13 @@ class VarlenSolver:
     def buckets(self):
         L = lib()
         return [L.sunny_geneig_varlen_bucket_order(self.handle, i)
+                for i in range(L.sunny_geneig_varlen_num_buckets(self.handle))]
+
+    @property
+    def bucket_waves(self):
+        """Matrices per workgroup of each bucket's grid."""
+        L = lib()
+        return [L.sunny_geneig_varlen_bucket_waves(self.handle, i)
                 for i in range(L.sunny_geneig_varlen_num_buckets(self.handle))]
 
     def close(self) -> None:

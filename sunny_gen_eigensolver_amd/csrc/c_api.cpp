@@ -245,13 +245,24 @@ int sunny_geneig_graph_stats(sunny_geneig_solver_t s, long long* captures, long 
 
 sunny_geneig_varlen_t sunny_geneig_varlen_create(int max_n, int device_id, const int* bucket_orders,
                                                  int num_buckets, sunny_geneig_cache_mode_t cache_mode) {
+    return sunny_geneig_varlen_create_ex(max_n, device_id, bucket_orders, num_buckets, cache_mode,
+                                         SUNNY_GENEIG_TUNING_SUGGESTED);
+}
+
+sunny_geneig_varlen_t sunny_geneig_varlen_create_ex(int max_n, int device_id, const int* bucket_orders,
+                                                    int num_buckets, sunny_geneig_cache_mode_t cache_mode,
+                                                    sunny_geneig_tuning_mode_t tuning_mode) {
     clear_error();
     try {
         if (num_buckets < 0) throw std::invalid_argument("num_buckets < 0");
+        if (tuning_mode != SUNNY_GENEIG_TUNING_SUGGESTED && tuning_mode != SUNNY_GENEIG_TUNING_AUTOTUNE)
+            throw std::invalid_argument("unknown tuning mode");
         std::vector<int> buckets;
         if (bucket_orders && num_buckets > 0) buckets.assign(bucket_orders, bucket_orders + num_buckets);
         auto h = std::make_unique<sunny_geneig_varlen_solver>();
-        h->impl = std::make_unique<sgeig::VarlenSolver>(max_n, device_id, buckets, to_cache(cache_mode));
+        h->impl = std::make_unique<sgeig::VarlenSolver>(
+            max_n, device_id, buckets, to_cache(cache_mode),
+            tuning_mode == SUNNY_GENEIG_TUNING_AUTOTUNE ? sgeig::Tuning::Autotune : sgeig::Tuning::Suggested);
         return h.release();
     } catch (const std::bad_alloc&) {
         set_error("out of memory in sunny_geneig_varlen_create");
@@ -418,6 +429,11 @@ int sunny_geneig_varlen_num_buckets(sunny_geneig_varlen_t s) {
 int sunny_geneig_varlen_bucket_order(sunny_geneig_varlen_t s, int i) {
     if (!s || i < 0 || i >= (int)s->impl->buckets().size()) return -1;
     return s->impl->buckets()[i];
+}
+
+int sunny_geneig_varlen_bucket_waves(sunny_geneig_varlen_t s, int i) {
+    if (!s || i < 0 || i >= (int)s->impl->buckets().size()) return -1;
+    return s->impl->bucket_waves(i);
 }
 
 }  // extern "C"

@@ -16,7 +16,7 @@ size_t al256(size_t x) { return (x + 255) / 256 * 256; }
 }  // namespace
 
 VarlenSolver::VarlenSolver(int max_n, int device_id, const std::vector<int>& buckets,
-                           CacheMode cache)
+                           CacheMode cache, Tuning tuning)
     : max_n_(max_n), device_(device_id) {
     if (max_n < 1 || max_n > kMaxVarlenOrder)
         throw std::invalid_argument("max_n must be in [1, " + std::to_string(kMaxVarlenOrder) + "]");
@@ -45,6 +45,10 @@ VarlenSolver::VarlenSolver(int max_n, int device_id, const std::vector<int>& buc
         Bucket b;
         b.order = o;
         b.block_dim = Solver::suggested_block_dim(o);   // same warm code objects
+        if (tuning == Tuning::Autotune && o <= Solver::kMaxLdsOrder) {
+            Solver probe(o, device_id, 0, Tuning::Autotune, cache, Mode::Unified);
+            b.block_dim = probe.block_dim_x();
+        }
         b.nw = b.block_dim / 64;
         if (o > Solver::kMaxLdsOrder) {
             b.nw = 1;

@@ -24,8 +24,13 @@ namespace sgeig {
 class VarlenSolver {
 public:
     // Buckets: ascending orders in [1, 192]; empty = multiples of 4 up to 64,
-    // then of 8 up to 128, then of 16, ending with max_n itself.
-    VarlenSolver(int max_n, int device_id, const std::vector<int>& buckets, CacheMode cache);
+    // then of 8 up to 128, then of 16, ending with max_n itself. With
+    // Tuning::Autotune every bucket up to 64 takes the workgroup size (and so
+    // the matrices per workgroup) that the fused solver's timed sweep picks for
+    // its order (persisted per order and architecture); Suggested takes the
+    // fixed rule. Buckets above 64 run the staged kernels (one size).
+    VarlenSolver(int max_n, int device_id, const std::vector<int>& buckets, CacheMode cache,
+                 Tuning tuning = Tuning::Suggested);
     ~VarlenSolver();
     VarlenSolver(const VarlenSolver&) = delete;
     VarlenSolver& operator=(const VarlenSolver&) = delete;

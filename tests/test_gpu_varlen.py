@@ -233,3 +233,23 @@ def test_large_bucket_non_hpd(vs128):
     good = [0, 3]
     _check_groups([Hs[k] for k in good], [Ss[k] for k in good], [W[k] for k in good], [U[k] for k in good],
                   info[good], [orders[k] for k in good], "varlen beside non-HPD")
+
+
+def test_autotuned_buckets_vs_lapack():
+    """tuning="autotune": every bucket up to 64 takes the workgroup size that
+    the fused solver's timed sweep picks for its order (the same choice as
+    Solver(n, tuning="autotune"), read back from the persisted table), and
+    the results stay within the parity bars."""
+    import sunny_gen_eigensolver_amd as sg
+    buckets = [16, 24, 32, 48]
+    with sg.VarlenSolver(48, buckets=buckets, tuning="autotune") as vs:
+        assert vs.buckets == buckets
+        for o, waves in zip(buckets, vs.bucket_waves):
+            s = sg.Solver(o, tuning="autotune")
+            assert waves == s.block_dim_x // 64, (o, waves, s.block_dim_x)
+            s.close()
+        rng = np.random.default_rng(23)
+        orders = [int(x) for x in rng.integers(9, 49, size=40)]
+        Hs, Ss = _gen_mix(orders, 8100)
+        W, U, info = _solve(vs, Hs, Ss, orders)
+    _check_groups(Hs, Ss, W, U, info, orders, "varlen autotuned")
