@@ -195,6 +195,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 fused / split modes")
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--uneven-tuning", choices=("suggested", "autotune"), default="suggested",
+                    help="workgroup size per uneven-batch bucket: fixed rule or the timed sweep")
     ap.add_argument("--no-uneven", action="store_true",
                     help="skip the uneven batch (200000 pencils, N log-uniform over 12..128)")
     ap.add_argument("--uneven-count", type=int, default=200000)
@@ -433,7 +435,10 @@ def main():
         Iv = torch.zeros((cnt,), dtype=torch.int32, device=dev)
         torch.cuda.synchronize()
         log(f"uneven: generated {cnt} pencils (sum N^2 = {tot}) in {time.perf_counter() - t0:.1f}s")
-        vs = sg.VarlenSolver(128)
+        t_vs = time.perf_counter()
+        vs = sg.VarlenSolver(128, tuning=a.uneven_tuning)
+        log(f"uneven: {a.uneven_tuning} buckets ready in {time.perf_counter() - t_vs:.1f}s, "
+            f"matrices per workgroup {vs.bucket_waves}")
         arr = (ctypes.c_int * cnt)(*orders.tolist())
         L = sg.lib()
         sh = sg._stream_handle(stream)
@@ -480,7 +485,7 @@ def main():
             "share_above_64": float((orders > 64).mean()),
             "info_nonzero": int((Iv != 0).sum().item()),
             "max_residual_over_N_eps_sample": worst_r, "max_s_orth_over_N_eps_sample": worst_o,
-            "buckets": vs.buckets,
+            "buckets": vs.buckets, "tuning": a.uneven_tuning, "bucket_waves": vs.bucket_waves,
             "note": "one call per step: host bucket sort and descriptor upload included; sample = up "
                     "to 64 pencils of each of the orders 12, 33, 54, 64, 65, 96, 128",
         }

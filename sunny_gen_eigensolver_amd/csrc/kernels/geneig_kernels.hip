@@ -2568,13 +2568,19 @@ geneig_mx_refine(const cx* __restrict__ Mp, const cx* __restrict__ Wd, real* __r
         }
         __syncthreads();
     }
-    // Clusters: runs [c0, c1] with adjacent gaps below delta (wave 0).
-    if (cb == 0 && iters > 0) {
-        int c0 = 0;
+    // Clusters: runs [c0, c1] with adjacent gaps below delta. Their bounds are
+    // fixed first (bit i: gap i to i + 1 is close; every wave computes the same
+    // mask before any wave changes lam), then cluster number nc goes to wave
+    // nc mod NB (clusters are disjoint).
+    unsigned long long close = 0;
+    if (iters > 0) close = __ballot(l < N - 1 && fabs(r.lam[l < N - 1 ? l + 1 : 0] - r.lam[l < N - 1 ? l : 0]) < delta);
+    __syncthreads();
+    if (iters > 0) {
+        int c0 = 0, nc = 0;
         while (c0 < N - 1) {
             int c1 = c0;
-            while (c1 + 1 < N && fabs(r.lam[c1 + 1] - r.lam[c1]) < delta) ++c1;
-            if (c1 > c0) {
+            while (c1 + 1 < N && ((close >> c1) & 1ull)) ++c1;
+            if (c1 > c0 && (nc++ % NB) == cb) {
                 const int sweeps = (c1 == c0 + 1) ? 1 : 4;
                 for (int sw = 0; sw < sweeps; ++sw)
                     for (int a = c0; a < c1; ++a)
