@@ -213,7 +213,8 @@ int sunny_geneig_stage_times(sunny_geneig_solver_t s, double* ms, const char** n
         const int n = s->impl->stage_times(tmp, 12);
         for (int i = 0; i < n && i < max_stages; ++i) {
             if (ms) ms[i] = tmp[i];
-            if (names) names[i] = sgeig::Solver::stage_name(s->impl->mode(), i, s->impl->n());
+            if (names)
+                names[i] = sgeig::Solver::stage_name(s->impl->mode(), i, s->impl->n(), s->impl->last_ql_log());
         }
         return n;
     } catch (const std::exception& e) {

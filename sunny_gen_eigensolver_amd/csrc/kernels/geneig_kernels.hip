@@ -2073,18 +2073,20 @@ geneig_split_ql(real* __restrict__ W, const real* __restrict__ E, real* __restri
     store_z(Zw + (size_t)b * N * N, z, rank, info, b, hdr[2]);
 }
 
-// The matrices whose QL log overflowed: the recurrence inline, every lane
-// redundantly, with the rows of Z in registers (a kernel of its own, so that
-// the common path above keeps its small register footprint).
+// QL without a log: the recurrence inline, every lane redundantly, with the
+// rows of Z in registers (a kernel of its own, so that the log-driven kernel
+// above keeps its small register footprint). With all == 0 it solves the
+// matrices whose log overflowed; with all != 0 every matrix (the host's choice
+// for small batches, where the recorder's one-wave latency would dominate;
+// the log is then not read).
 extern "C" __global__ void __launch_bounds__(GENEIG_NT)
 geneig_ql_full(real* __restrict__ W, const real* __restrict__ E, real* __restrict__ Zw,
-               int* __restrict__ info, const char* __restrict__ qlog, long long log_stride, int batch) {
+               int* __restrict__ info, const char* __restrict__ qlog, long long log_stride, int batch, int all) {
     using namespace gk;
     const int b = blockIdx.x * NW + wave_id();
     if (b >= batch) return;
     if (info[b] != 0) return;
-    const int* hdr = reinterpret_cast<const int*>(qlog + (size_t)b * (size_t)log_stride);
-    if (!hdr[1]) return;
+    if (!all && !reinterpret_cast<const int*>(qlog + (size_t)b * (size_t)log_stride)[1]) return;
     const int lane = lane_id();
     real* Wb = W + (size_t)b * N;
     const real* Eb = E + (size_t)b * N;

@@ -125,6 +125,7 @@ Solver::Solver(int n, int device_id, int bpb_request, Tuning tuning, CacheMode c
     large_ = n > kMaxLdsOrder;
     if (const char* v = std::getenv("SUNNY_GENEIG_GRAPH")) graphs_ = std::atoi(v) != 0;
     if (const char* v = std::getenv("SUNNY_GENEIG_QL_SWEEP_CAP")) ql_sweep_cap_ = std::max(0, std::atoi(v));
+    if (const char* v = std::getenv("SUNNY_GENEIG_QL_LOG_MIN_BATCH")) ql_log_min_batch_ = std::max(0, std::atoi(v));
     if (bpb_request > 4)
         throw std::invalid_argument("batches_per_block must be <= 4 (one wave per matrix, "
                                     "LDS-resident work area)");
@@ -437,15 +438,20 @@ void Solver::launch_hybrid_(const void* H, const void* S, double* W, void* U, in
     char* qlog = static_cast<char*>(qlog_);
     long long qstride = (long long)ql_log_bytes(n_);
     int cap = ql_sweep_cap_;
-    void* ar[] = {&W, &E, &info, &qlog, &qstride, &batch, &cap};
-    SG_HIP(hipModuleLaunchKernel(k_qlrec_, (unsigned)((batch + 63) / 64), 1, 1, 64, 1, 1, 0, stream, ar,
-                                 nullptr));
-    mark_(stream);
-    void* a2[] = {&W, &E, &Zw, &info, &qlog, &qstride, &batch};
-    SG_HIP(hipModuleLaunchKernel(k_heev_, grid2, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream, a2,
-                                 nullptr));
-    mark_(stream);
-    SG_HIP(hipModuleLaunchKernel(k_qlfull_, grid2, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream, a2,
+    int all = batch < ql_log_min_batch_ ? 1 : 0;
+    last_ql_log_ = !all;
+    if (!all) {
+        void* ar[] = {&W, &E, &info, &qlog, &qstride, &batch, &cap};
+        SG_HIP(hipModuleLaunchKernel(k_qlrec_, (unsigned)((batch + 63) / 64), 1, 1, 64, 1, 1, 0, stream, ar,
+                                     nullptr));
+        mark_(stream);
+        void* a2[] = {&W, &E, &Zw, &info, &qlog, &qstride, &batch};
+        SG_HIP(hipModuleLaunchKernel(k_heev_, grid2, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream, a2,
+                                     nullptr));
+        mark_(stream);
+    }
+    void* af[] = {&W, &E, &Zw, &info, &qlog, &qstride, &batch, &all};
+    SG_HIP(hipModuleLaunchKernel(k_qlfull_, grid2, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream, af,
                                  nullptr));
     mark_(stream);
     void* a3[] = {&U, &Zw, &info, &batch};
@@ -507,14 +513,19 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
     char* qlog = static_cast<char*>(qlog_);
     long long qstride = (long long)ql_log_bytes(n_);
     int cap = ql_sweep_cap_;
-    void* ar[] = {&W32, &E32, &info, &qlog, &qstride, &batch, &cap};
-    SG_HIP(hipModuleLaunchKernel(k_qlrec_, (unsigned)((batch + 63) / 64), 1, 1, 64, 1, 1, 0, stream, ar,
-                                 nullptr));
-    mark_(stream);
-    void* a3[] = {&W32, &E32, &Z32, &info, &qlog, &qstride, &batch};
-    SG_HIP(hipModuleLaunchKernel(k_ql32_, grid_w, 1, 1, bd, 1, 1, 0, stream, a3, nullptr));
-    mark_(stream);
-    SG_HIP(hipModuleLaunchKernel(k_qlfull32_, grid_w, 1, 1, bd, 1, 1, 0, stream, a3, nullptr));
+    int all = batch < ql_log_min_batch_ ? 1 : 0;
+    last_ql_log_ = !all;
+    if (!all) {
+        void* ar[] = {&W32, &E32, &info, &qlog, &qstride, &batch, &cap};
+        SG_HIP(hipModuleLaunchKernel(k_qlrec_, (unsigned)((batch + 63) / 64), 1, 1, 64, 1, 1, 0, stream, ar,
+                                     nullptr));
+        mark_(stream);
+        void* a3[] = {&W32, &E32, &Z32, &info, &qlog, &qstride, &batch};
+        SG_HIP(hipModuleLaunchKernel(k_ql32_, grid_w, 1, 1, bd, 1, 1, 0, stream, a3, nullptr));
+        mark_(stream);
+    }
+    void* aq[] = {&W32, &E32, &Z32, &info, &qlog, &qstride, &batch, &all};
+    SG_HIP(hipModuleLaunchKernel(k_qlfull32_, grid_w, 1, 1, bd, 1, 1, 0, stream, aq, nullptr));
     mark_(stream);
     void* a4[] = {&V32, &Z32, &info, &batch};
     SG_HIP(hipModuleLaunchKernel(k_back32_, grid_w, 1, 1, bd, 1, 1, 0, stream, a4, nullptr));
@@ -685,7 +696,7 @@ int Solver::stage_times(double* ms, int n) {
     return stages;
 }
 
-const char* Solver::stage_name(Mode mode, int i, int n) {
+const char* Solver::stage_name(Mode mode, int i, int n, bool ql_log) {
     static const char* large64[] = {"geneig_lg_chol", "geneig_lg_formm", "geneig_lg_hetrd", "geneig_lg_qlr",
                                     "geneig_lg_qlz",  "geneig_lg_back",  "geneig_lg_backtri"};
     static const char* large_mx[] = {"geneig_lg_chol", "geneig_lg_formm", "geneig_lg_hetrd", "geneig_lg_qlr",
@@ -699,9 +710,14 @@ const char* Solver::stage_name(Mode mode, int i, int n) {
                                    "geneig_split_back"};
     static const char* mixed[] = {"geneig_lg_chol",   "geneig_mx_formm", "geneig_mx_front", "geneig_ql_record",
                                   "geneig_split_ql",  "geneig_ql_full",  "geneig_mx_back",  "geneig_mx_refine"};
+    // Without the QL log: the no-log kernel alone in place of recorder + applier.
+    static const char* hybrid_nolog[] = {"geneig_split_front", "geneig_ql_full", "geneig_split_back"};
+    static const char* mixed_nolog[] = {"geneig_lg_chol", "geneig_mx_formm", "geneig_mx_front",
+                                        "geneig_ql_full", "geneig_mx_back",  "geneig_mx_refine"};
     if (mode == Mode::Unified) return (i == 0) ? unified[0] : "";
-    if (mode == Mode::Hybrid) return (i >= 0 && i < 5) ? hybrid[i] : "";
-    return (i >= 0 && i < 8) ? mixed[i] : "";
+    if (mode == Mode::Hybrid)
+        return ql_log ? ((i >= 0 && i < 5) ? hybrid[i] : "") : ((i >= 0 && i < 3) ? hybrid_nolog[i] : "");
+    return ql_log ? ((i >= 0 && i < 8) ? mixed[i] : "") : ((i >= 0 && i < 6) ? mixed_nolog[i] : "");
 }
 
 }  // namespace sgeig

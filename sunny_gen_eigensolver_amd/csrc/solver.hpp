@@ -107,6 +107,14 @@ private:
     // kernels clamp it to the log's capacity); SUNNY_GENEIG_QL_SWEEP_CAP lowers
     // it, which tests use to run the no-log path.
     int ql_sweep_cap_ = 1 << 30;
+    // Batches from this size on run QL through the log (recorder + applier);
+    // smaller ones run the no-log kernel for every matrix, since the recorder
+    // costs one wave's whole recurrence whatever the batch (2.7 ms at N = 54).
+    // Measured crossover about 40k matrices at N = 24 and 54;
+    // SUNNY_GENEIG_QL_LOG_MIN_BATCH overrides it.
+    static constexpr int kQlLogMinBatch = 45056;
+    int ql_log_min_batch_ = kQlLogMinBatch;
+    bool last_ql_log_ = true;               // QL path of the last split / mixed launch
 
     // Multi-kernel launches (split, mixed, staged) are captured once per
     // (buffers, batch) into a hipGraph on an internal stream and replayed on
@@ -162,7 +170,10 @@ public:
     // Fills ms[0 .. min(n, stages)) and returns the number of stages of the
     // last timed launch (0 if none); waits for that launch to finish.
     int stage_times(double* ms, int n);
-    static const char* stage_name(Mode mode, int i, int n = 1);
+    // Kernel of stage i of a launch; ql_log: whether that launch ran QL through
+    // the log (see last_ql_log()).
+    static const char* stage_name(Mode mode, int i, int n = 1, bool ql_log = true);
+    bool last_ql_log() const { return last_ql_log_; }
 };
 
 }  // namespace sgeig
