@@ -478,6 +478,7 @@ constexpr int YLD = 17;
 struct CholWork {
     cx pan[(16 * PLD > YLD * 16 * LNB) ? 16 * PLD : YLD * 16 * LNB];   // panel; trtri: Y, then X
     cx lii[256];              // trtri: diagonal block of L
+    int fail;                 // pivot failure of the panel (orders up to 64)
 };
 #endif  // !GENEIG_FP32
 
@@ -545,29 +546,65 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
             }
         }
         __syncthreads();
-        // Unblocked Cholesky of the panel in LDS, thread t = row j0 + t, one
-        // barrier per column: every thread derives the pivot of column c from
-        // LDS, reads column c unscaled (scaling each entry itself) and writes
-        // its scaled entry of column c straight to L in global memory, so
-        // column c is never rewritten in LDS while others read it.
-        const int t = tid;
-        const bool tin = t < N - j0;
         int fail = 0;
-        for (int c = 0; c < jw; ++c) {
-            const real piv = cw.pan[c * PLD + c].x;
-            if (!(piv > real(0.0))) {
-                fail = j0 + c + 1;
-                break;
-            }
-            const real sq = sqrt_r(piv);
-            const real ri = real(1.0) / sq;
-            if (t >= c && tin) {
-                const cx ltc = (t == c) ? cm(sq, 0.0) : cw.pan[c * PLD + t] * ri;
-                setL(j0 + t, j0 + c, ltc);
-                for (int c2 = c + 1; c2 < jw && c2 <= t; ++c2)
-                    cw.pan[c2 * PLD + t] = cw.pan[c2 * PLD + t] - cmulc(ltc, cw.pan[c * PLD + c2] * ri);
+        if constexpr (N <= 64) {
+            // Every panel row is a lane of wave 0: the unblocked Cholesky of the
+            // panel runs there with row t in registers, L(c2, c) broadcast from
+            // lane c2 by v_readlane, no barrier per column.
+            if (wave == 0) {
+                const int t = lane;
+                const bool tin = t < N - j0;
+                cx r[16];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) r[c] = (tin && c < jw) ? cw.pan[c * PLD + t] : cm(0.0, 0.0);
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+                    if (c < jw && !fail) {
+                        const real piv = bcast(r[c].x, c);
+                        if (!(piv > real(0.0))) {
+                            fail = j0 + c + 1;
+                        } else {
+                            const real sq = sqrt_r(piv);
+                            const real ri = real(1.0) / sq;
+                            const cx lc = (t == c) ? cm(sq, 0.0) : r[c] * ri;
+                            r[c] = lc;
+#pragma unroll
+                            for (int c2 = c + 1; c2 < 16; ++c2)
+                                if (c2 < jw) r[c2] = r[c2] - cmulc(lc, bcast(lc, c2));
+                        }
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < 16; ++c)
+                    if (c < jw && tin && t >= c) setL(j0 + t, j0 + c, r[c]);
+                if (lane == 0) cw.fail = fail;
             }
             __syncthreads();
+            fail = cw.fail;
+        } else {
+            // Unblocked Cholesky of the panel in LDS, thread t = row j0 + t, one
+            // barrier per column: every thread derives the pivot of column c from
+            // LDS, reads column c unscaled (scaling each entry itself) and writes
+            // its scaled entry of column c straight to L in global memory, so
+            // column c is never rewritten in LDS while others read it.
+            const int t = tid;
+            const bool tin = t < N - j0;
+            for (int c = 0; c < jw; ++c) {
+                const real piv = cw.pan[c * PLD + c].x;
+                if (!(piv > real(0.0))) {
+                    fail = j0 + c + 1;
+                    break;
+                }
+                const real sq = sqrt_r(piv);
+                const real ri = real(1.0) / sq;
+                if (t >= c && tin) {
+                    const cx ltc = (t == c) ? cm(sq, 0.0) : cw.pan[c * PLD + t] * ri;
+                    setL(j0 + t, j0 + c, ltc);
+                    for (int c2 = c + 1; c2 < jw && c2 <= t; ++c2)
+                        cw.pan[c2 * PLD + t] = cw.pan[c2 * PLD + t] - cmulc(ltc, cw.pan[c * PLD + c2] * ri);
+                }
+                __syncthreads();
+            }
         }
         if (fail) {
             real* Wb = W + (size_t)b * N;
