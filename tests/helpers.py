@@ -13,7 +13,7 @@ EPS = 2.0 ** -52
 TOL_EIG = 1e-10        # reference bar (tests/static/test_full_n54.cpp:144-182)
 TOL_PHASE = 1e-10
 DEGEN_GAP = 1e-6
-RESID_FACTOR = 50.0    # residual, S-orthogonality <= 50 N eps
+RESID_FACTOR = 50.0    # bar: residual and S-orthogonality <= 50 N eps (1-norms)
 
 
 def fixtures_index():
