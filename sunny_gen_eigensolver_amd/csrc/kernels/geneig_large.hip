@@ -31,7 +31,10 @@
 
 namespace gk {
 
-constexpr int LNT = 256;                      // workgroup size of the staged kernels
+// Workgroup size of the staged kernels: 256; for orders up to 64 (where only
+// the Cholesky is built from this file) one wave per 16-row tile, so that
+// small orders do not carry idle waves.
+constexpr int LNT = (N <= 64) ? 64 * ((N + 15) / 16) : 256;
 constexpr int LNW = LNT / 64;
 constexpr int ND = (N + 63) / 64;             // rows per lane when lanes index rows
 constexpr int LNB = (N + 15) / 16;             // 16-blocks per dimension

@@ -496,10 +496,12 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
     const unsigned grid_w = (unsigned)((batch + bpb_ - 1) / bpb_);
     const unsigned bd = (unsigned)block_dim_;
     // FP64 Cholesky and W = L^-1 (full, upper zero): the staged kernels'
-    // panel factorisation, one 256-thread workgroup per matrix.
+    // panel factorisation, one workgroup per matrix of one wave per 16-row
+    // tile (the kernel's size for orders up to 64).
     long long wstride = (long long)(n * n * 16);
     void* a1[] = {&S, &W, &U, &info, &Wd, &wstride, &batch};
-    SG_HIP(hipModuleLaunchKernel(k_mx_chol_, grid_b, 1, 1, 256, 1, 1, 0, stream, a1, nullptr));
+    const unsigned bd_chol = (unsigned)(64 * ((n_ + 15) / 16));
+    SG_HIP(hipModuleLaunchKernel(k_mx_chol_, grid_b, 1, 1, bd_chol, 1, 1, 0, stream, a1, nullptr));
     mark_(stream);
     // M = W H W^H once in FP64 on the matrix cores (one wave per 16-column block).
     const unsigned bd_ref = (unsigned)(64 * ((n_ + 15) / 16));
