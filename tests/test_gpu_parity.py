@@ -98,3 +98,16 @@ def test_non_hpd_info_codes(kernel, n):
     good = [b for b in range(6) if b not in bad]
     T.assert_parity(H[good], S[good], W[good], U[good], info[good], Wl[good], Ul[good],
                     tag=f"{kernel} beside non-HPD n={n}")
+
+
+@pytest.mark.parametrize("kernel", MODES)
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 9, 13])
+def test_tiny_orders_vs_lapack(kernel, n):
+    """Orders far below a 16-row tile or a wave (one matrix element, 2 x 2, odd
+    orders) in every mode: the padded MFMA tiles, the partial panels and the
+    lane masks all see mostly-empty ranges."""
+    H, S = O.gen_reference(n, 16, 4200 + n, "random")
+    Wl, Ul, il = O.zhegv_batch(H, S, threads=8)
+    W, U, info = T.run_gpu(H, S, kernel=kernel)
+    assert (il == 0).all()
+    T.assert_parity(H, S, W, U, info, Wl, Ul, tag=f"{kernel} tiny n={n}")
