@@ -68,7 +68,10 @@ typedef enum {
 } sunny_geneig_kernel_mode_t;
 
 /* Reference: src/common/c_api.h:50-55 (c_api.cpp:76-107). Returns NULL on
- * failure. batches_per_block_request <= 0 selects automatically. */
+ * failure. batches_per_block_request <= 0 selects automatically; 1..4 is
+ * honoured (one wave per matrix); a larger request returns NULL with a message
+ * naming the limit, as the reference does for an explicit BPB it cannot build
+ * (unified_solver.cpp:1021-1028; its ladder goes to 16, this kernel's to 4). */
 sunny_geneig_solver_t sunny_geneig_create(
     int                         n,
     int                         device_id,
@@ -101,7 +104,11 @@ sunny_geneig_status_t sunny_geneig_launch(
     int                   batch_size,
     uintptr_t             stream);
 
-/* Reference: src/common/c_api.h:87-91 (c_api.cpp:188-206). -1 / 0 on NULL. */
+/* Reference: src/common/c_api.h:87-91 (c_api.cpp:188-206). -1 / 0 on NULL.
+ * batches_per_block is 1 in the split (HYBRID) and MIXED modes, as the
+ * reference's split mode reports (c_api.cpp:41-49); shared_mem_bytes is the
+ * fused kernel's LDS, the split mode's front kernel's, or the largest of the
+ * mixed mode's kernels. */
 int          sunny_geneig_matrix_size      (sunny_geneig_solver_t solver);
 int          sunny_geneig_batches_per_block(sunny_geneig_solver_t solver);
 int          sunny_geneig_block_dim_x      (sunny_geneig_solver_t solver);
@@ -116,6 +123,30 @@ const char*                sunny_geneig_path_name  (sunny_geneig_solver_t solver
 sunny_geneig_kernel_mode_t sunny_geneig_kernel_mode(sunny_geneig_solver_t solver);
 
 /* ---- additive entry points (no counterpart in the reference ABI) --------- */
+
+/* Chunked launch over several streams: contiguous chunks of chunk_size
+ * matrices (the last one shorter) go round-robin to streams[0..num_streams),
+ * each as one sunny_geneig_launch on its stream (pointers offset by the
+ * chunk's first matrix). C-ABI form of UnifiedGeneigSolver::launch_chunked
+ * (reference: src/unified/unified_solver.cpp:1078-1107) and of the offset
+ * arithmetic of run_kernel_chunked! (SunnyGeneigBench.jl:191-251).
+ * Asynchronous; the caller synchronises the streams. In the fused mode the
+ * chunks may run concurrently; the split and mixed modes and the staged
+ * orders share one workspace per handle, so their chunks run in submission
+ * order whatever the stream. total_matrices == 0 is a no-op; a negative
+ * total, chunk_size <= 0, num_streams <= 0 or a NULL streams array ->
+ * INVALID_ARG. A stream value of 0 is the default stream. */
+sunny_geneig_status_t sunny_geneig_launch_chunked(
+    sunny_geneig_solver_t solver,
+    uintptr_t             d_H,
+    uintptr_t             d_S,
+    uintptr_t             d_W,
+    uintptr_t             d_U,
+    uintptr_t             d_info,
+    int                   total_matrices,
+    int                   chunk_size,
+    const uintptr_t*      streams,
+    int                   num_streams);
 
 /* Compile (hipRTC) and store in the on-disk cache the code objects for order n
  * with `block_dim_x` threads (0 = suggested) for `arch` (e.g. "gfx950"): the
@@ -208,10 +239,11 @@ int sunny_geneig_varlen_bucket_waves(sunny_geneig_varlen_t solver, int i);
 
 /* ---- serving session (no counterpart in the reference ABI) ---------------
  * A request loop over resident solvers: requests (count pencils of one order
- * from `orders`) are queued on submission and admitted in arrival order when
- * one of `num_slots` slots (a stream with its own solver handles, workspaces
- * reserved for max_count pencils at creation) is free and the request's
- * device footprint fits `hbm_budget_bytes`. Admitted requests run
+ * from `orders`) are queued on submission. The session cuts them into slices
+ * (see sunny_geneig_session_set_policy) and starts a slice whenever one of
+ * `num_slots` slots (a stream with its own solver handles, workspaces
+ * reserved for max_count pencils at creation) is free and, for a request's
+ * first slice, its device footprint fits `hbm_budget_bytes`. Slices run
  * concurrently: copy in, solve, copy out. With on_device != 0 the five
  * buffers are caller-owned device memory (no copies, no footprint); otherwise
  * they are host memory (pinned memory gives asynchronous copies), staged
@@ -229,10 +261,22 @@ void sunny_geneig_session_destroy(sunny_geneig_session_t session);
 
 /* Returns the request id (>= 0) or a negative status (INVALID_ARG for an
  * order the session does not serve, a count outside [1, max_count], NULL
- * buffers, or a footprint larger than the budget). */
+ * buffers, or a footprint larger than the budget). `stream` is the stream on
+ * which the caller produced the buffers (a hipStream_t; 0 = the null stream):
+ * the session's copies and kernels for the request wait for the work that
+ * stream holds at submission. For host buffers filled synchronously pass 0
+ * (no wait is recorded). */
 int64_t sunny_geneig_session_submit(sunny_geneig_session_t session, int n, int count,
                                     uintptr_t H, uintptr_t S, uintptr_t W, uintptr_t U,
-                                    uintptr_t info, int on_device);
+                                    uintptr_t info, int on_device, uintptr_t stream);
+
+/* Scheduling: policy 0 serves requests in arrival order, 1 serves first the
+ * request with the least work left (pencils not yet started x n^3, oldest
+ * first on ties); slices of at most `slice` pencils (0 = whole requests) let
+ * one request use several slots at once and let the scheduler switch
+ * requests between slices. Default: policy 0, slice 0. Applies to slices
+ * started after the call. Returns 0 or INVALID_ARG. */
+int sunny_geneig_session_set_policy(sunny_geneig_session_t session, int policy, int slice);
 
 /* 1 when the request is done, 0 while queued or running, a negative status
  * for an unknown id. */
@@ -245,7 +289,7 @@ int sunny_geneig_session_wait(sunny_geneig_session_t session, int64_t request_id
                               double* latency_ms, double* service_ms);
 
 /* Counters since creation: requests submitted / completed, the arena's
- * high-water mark in bytes and the most requests running at once. */
+ * high-water mark in bytes and the most slices running at once. */
 int sunny_geneig_session_stats(sunny_geneig_session_t session, int64_t* submitted, int64_t* completed,
                                uint64_t* hbm_peak_bytes, int* max_concurrent);
 

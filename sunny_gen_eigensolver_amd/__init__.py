@@ -51,7 +51,8 @@ EXPORTED_SYMBOLS = (
     "sunny_geneig_set_graphs", "sunny_geneig_graph_stats",
     "sunny_geneig_session_create", "sunny_geneig_session_destroy", "sunny_geneig_session_submit",
     "sunny_geneig_session_poll", "sunny_geneig_session_wait", "sunny_geneig_session_stats",
-    "sunny_geneig_session_request_bytes",
+    "sunny_geneig_session_request_bytes", "sunny_geneig_launch_chunked",
+    "sunny_geneig_session_set_policy",
 )
 
 _lib = None
@@ -85,6 +86,8 @@ def lib() -> ctypes.CDLL:
     L.sunny_geneig_destroy.restype = None
     L.sunny_geneig_launch.argtypes = [vp, cu, cu, cu, cu, cu, ci, cu]
     L.sunny_geneig_launch.restype = ci
+    L.sunny_geneig_launch_chunked.argtypes = [vp, cu, cu, cu, cu, cu, ci, ci, ctypes.POINTER(cu), ci]
+    L.sunny_geneig_launch_chunked.restype = ci
     for name in ("sunny_geneig_matrix_size", "sunny_geneig_batches_per_block",
                  "sunny_geneig_block_dim_x", "sunny_geneig_kernel_mode"):
         getattr(L, name).argtypes = [vp]
@@ -131,8 +134,10 @@ def lib() -> ctypes.CDLL:
     L.sunny_geneig_session_create.restype = vp
     L.sunny_geneig_session_destroy.argtypes = [vp]
     L.sunny_geneig_session_destroy.restype = None
-    L.sunny_geneig_session_submit.argtypes = [vp, ci, ci, cu, cu, cu, cu, cu, ci]
+    L.sunny_geneig_session_submit.argtypes = [vp, ci, ci, cu, cu, cu, cu, cu, ci, cu]
     L.sunny_geneig_session_submit.restype = i64
+    L.sunny_geneig_session_set_policy.argtypes = [vp, ci, ci]
+    L.sunny_geneig_session_set_policy.restype = ci
     L.sunny_geneig_session_poll.argtypes = [vp, i64]
     L.sunny_geneig_session_poll.restype = ci
     L.sunny_geneig_session_wait.argtypes = [vp, i64, pd, pd]
@@ -262,21 +267,54 @@ def _stream_handle(stream) -> int:
     return int(stream.cuda_stream)
 
 
+def _check_tensor(name, t, shape, dtype, device):
+    """The positional checks of run_kernel! (SunnyGeneigBench.jl:152-156):
+    shape, element type, device and a dense layout the kernels can index."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name} must be a torch tensor")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {tuple(shape)}")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} has dtype {t.dtype}, expected {dtype}")
+    if t.device != device:
+        raise ValueError(f"{name} is on {t.device}, expected {device}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+def _check_batch(solver: Solver, H, S, W, U, info):
+    """Validate H, S (and W, U, info when given); returns (batch, device)."""
+    import torch
+    if H.dim() != 3:
+        raise ValueError(f"H must be (batch, n, n), got shape {tuple(H.shape)}")
+    batch, n, n2 = H.shape
+    if n != solver.n or n2 != solver.n:
+        raise ValueError(f"matrix size mismatch: solver.n={solver.n}, H={tuple(H.shape)}")
+    dev = H.device
+    if not H.is_cuda:
+        raise ValueError("H must be a device tensor")
+    _check_tensor("H", H, (batch, n, n), torch.complex128, dev)
+    _check_tensor("S", S, (batch, n, n), torch.complex128, dev)
+    if W is not None:
+        _check_tensor("W", W, (batch, n), torch.float64, dev)
+    if U is not None:
+        _check_tensor("U", U, (batch, n, n), torch.complex128, dev)
+    if info is not None:
+        _check_tensor("info", info, (batch,), torch.int32, dev)
+    return batch, dev
+
+
 def run_kernel(solver: Solver, H, S, W=None, U=None, info=None, stream=None):
     """run_kernel!(solver, H, S; W, U, info, stream) -- SunnyGeneigBench.jl:104-171.
 
     H, S: complex128 cuda tensors (B, n, n) in column-major-per-matrix layout.
-    Returns (W, U, info); pass output tensors to reuse them across calls."""
+    Returns (W, U, info); pass output tensors to reuse them across calls
+    (W float64 (B, n), U complex128 (B, n, n), info int32 (B,), contiguous, on
+    H's device; anything else raises ValueError before the launch)."""
     import torch
-    batch, n, n2 = H.shape
-    if n != solver.n or n2 != solver.n:
-        raise ValueError(f"matrix size mismatch: solver.n={solver.n}, H={tuple(H.shape)}")
-    if tuple(S.shape) != tuple(H.shape):
-        raise ValueError("S size must match H")
-    for name, t in (("H", H), ("S", S)):
-        if t.dtype != torch.complex128 or not t.is_cuda or not t.is_contiguous():
-            raise ValueError(f"{name} must be a contiguous complex128 device tensor")
-    dev = H.device
+    batch, dev = _check_batch(solver, H, S, W, U, info)
+    n = solver.n
     W = torch.empty((batch, n), dtype=torch.float64, device=dev) if W is None else W
     U = torch.empty((batch, n, n), dtype=torch.complex128, device=dev) if U is None else U
     info = torch.zeros((batch,), dtype=torch.int32, device=dev) if info is None else info
@@ -286,23 +324,22 @@ def run_kernel(solver: Solver, H, S, W=None, U=None, info=None, stream=None):
 
 
 def run_kernel_chunked(solver: Solver, H, S, W, U, info, chunk_size: int, streams):
-    """run_kernel_chunked! (SunnyGeneigBench.jl:191-251): launch contiguous
-    chunks round-robin over `streams`, then synchronise each stream."""
-    batch = H.shape[0]
-    n = solver.n
-    mat = n * n * 16
-    launched = 0
-    s = 0
-    while launched < batch:
-        cnt = min(chunk_size, batch - launched)
-        st = streams[s % len(streams)]
-        solver.launch_raw(H.data_ptr() + launched * mat, S.data_ptr() + launched * mat,
-                          W.data_ptr() + launched * n * 8, U.data_ptr() + launched * mat,
-                          info.data_ptr() + launched * 4, cnt, _stream_handle(st))
-        launched += cnt
-        s += 1
-    for st in streams:
-        st.synchronize()
+    """run_kernel_chunked! (SunnyGeneigBench.jl:191-251): contiguous chunks of
+    chunk_size matrices round-robin over `streams` through
+    sunny_geneig_launch_chunked, then every stream is synchronised."""
+    batch, _ = _check_batch(solver, H, S, W, U, info)
+    if W is None or U is None or info is None:
+        raise ValueError("run_kernel_chunked needs preallocated W, U and info")
+    if chunk_size <= 0 or not streams:
+        raise ValueError("chunk_size must be > 0 and at least one stream given")
+    handles = (ctypes.c_uint64 * len(streams))(*[_stream_handle(st) for st in streams])
+    st = lib().sunny_geneig_launch_chunked(solver.handle, H.data_ptr(), S.data_ptr(), W.data_ptr(),
+                                           U.data_ptr(), info.data_ptr(), batch, chunk_size,
+                                           handles, len(streams))
+    if st != STATUS_OK:
+        raise GeneigError(st, last_error())
+    for s_ in streams:
+        s_.synchronize()
     return W, U, info
 
 
@@ -426,13 +463,19 @@ class ServingSession:
     """Request loop over resident solvers (sunny_geneig_session_*).
 
     ServingSession(orders=[24, 54], kernel="mixed", max_count=4096, slots=4,
-    hbm_budget=2 << 30). submit(H, S, W, U, info) queues one request: with
-    device tensors the buffers are used in place, with (pinned) CPU tensors the
-    session stages them through its device arena. Returns a request id;
-    wait(id) -> (latency_ms, service_ms)."""
+    hbm_budget=2 << 30, policy="fifo", slice=0). submit(H, S, W, U, info)
+    queues one request: with device tensors the buffers are used in place (the
+    session waits for the current torch stream as it is at submission), with
+    (pinned) CPU tensors the session stages them through its device arena.
+    policy "fifo" serves requests in arrival order, "srpt" the one with the
+    least work left; slice > 0 cuts requests into slices of that many pencils.
+    Returns a request id; wait(id) -> (latency_ms, service_ms)."""
+
+    POLICIES = {"fifo": 0, "srpt": 1}
 
     def __init__(self, orders, kernel: str = "mixed", max_count: int = 4096, slots: int = 4,
-                 hbm_budget: int = 2 << 30, device_id: int = 0, cache: str = "auto"):
+                 hbm_budget: int = 2 << 30, device_id: int = 0, cache: str = "auto",
+                 policy: str = "fifo", slice: int = 0):
         L = lib()
         k = {"hybrid": KERNEL_HYBRID, "mixed": KERNEL_MIXED}.get(kernel, KERNEL_UNIFIED)
         c = {"nodisk": CACHE_NODISK, "nocache": CACHE_NOCACHE}.get(cache, CACHE_AUTO)
@@ -442,14 +485,39 @@ class ServingSession:
         if not h:
             raise GeneigError(STATUS_INVALID_ARG, f"sunny_geneig_session_create failed: {last_error()}")
         self.handle = h
+        self.device_id = device_id
         self._keep = {}
+        self.set_policy(policy, slice)
+
+    def set_policy(self, policy: str = "fifo", slice: int = 0) -> None:
+        if policy not in self.POLICIES:
+            raise ValueError(f"policy must be one of {sorted(self.POLICIES)}")
+        st = lib().sunny_geneig_session_set_policy(self.handle, self.POLICIES[policy], int(slice))
+        if st != STATUS_OK:
+            raise GeneigError(st, last_error())
 
     def submit(self, H, S, W, U, info) -> int:
-        n = H.shape[-1]
-        count = H.shape[0]
-        on_dev = 1 if H.is_cuda else 0
+        """Queue one request of K pencils: H, S complex128 (K, n, n), W float64
+        (K, n), U complex128 (K, n, n), info int32 (K,), all contiguous and all
+        on the session's device or all in host memory."""
+        import torch
+        if H.dim() != 3 or H.shape[1] != H.shape[2]:
+            raise ValueError(f"H must be (count, n, n), got shape {tuple(H.shape)}")
+        count, n = H.shape[0], H.shape[-1]
+        on_dev = H.is_cuda
+        dev = H.device
+        if on_dev and dev.index not in (None, self.device_id):
+            raise ValueError(f"H is on {dev}, the session serves cuda:{self.device_id}")
+        for name, t, shape, dtype in (("H", H, (count, n, n), torch.complex128),
+                                      ("S", S, (count, n, n), torch.complex128),
+                                      ("W", W, (count, n), torch.float64),
+                                      ("U", U, (count, n, n), torch.complex128),
+                                      ("info", info, (count,), torch.int32)):
+            _check_tensor(name, t, shape, dtype, dev)
+        stream = int(torch.cuda.current_stream(dev).cuda_stream) if on_dev else 0
         rid = lib().sunny_geneig_session_submit(self.handle, n, count, H.data_ptr(), S.data_ptr(),
-                                                W.data_ptr(), U.data_ptr(), info.data_ptr(), on_dev)
+                                                W.data_ptr(), U.data_ptr(), info.data_ptr(),
+                                                1 if on_dev else 0, stream)
         if rid < 0:
             raise GeneigError(int(rid), last_error())
         self._keep[rid] = (H, S, W, U, info)     # buffers stay alive until wait()

@@ -133,6 +133,39 @@ sunny_geneig_status_t sunny_geneig_launch(sunny_geneig_solver_t solver, uintptr_
     return SUNNY_GENEIG_LAUNCH_ERROR;
 }
 
+sunny_geneig_status_t sunny_geneig_launch_chunked(sunny_geneig_solver_t solver, uintptr_t d_H,
+                                                  uintptr_t d_S, uintptr_t d_W, uintptr_t d_U,
+                                                  uintptr_t d_info, int total_matrices, int chunk_size,
+                                                  const uintptr_t* streams, int num_streams) {
+    clear_error();
+    if (!solver) {
+        set_error("sunny_geneig_launch_chunked: solver is NULL");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    if (total_matrices < 0 || chunk_size <= 0 || num_streams <= 0 || !streams) {
+        set_error("sunny_geneig_launch_chunked: need total_matrices >= 0, chunk_size > 0 and "
+                  "num_streams > 0 streams");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    if (total_matrices > 0 && (!d_H || !d_S || !d_W || !d_U || !d_info)) {
+        set_error("sunny_geneig_launch_chunked: NULL device buffer");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    try {
+        std::vector<hipStream_t> st((size_t)num_streams);
+        for (int i = 0; i < num_streams; ++i) st[(size_t)i] = reinterpret_cast<hipStream_t>(streams[i]);
+        solver->impl->launch_chunked(reinterpret_cast<const void*>(d_H), reinterpret_cast<const void*>(d_S),
+                                     reinterpret_cast<double*>(d_W), reinterpret_cast<void*>(d_U),
+                                     reinterpret_cast<int*>(d_info), total_matrices, chunk_size, st);
+        return SUNNY_GENEIG_OK;
+    } catch (const std::exception& e) {
+        set_error(std::string("sunny_geneig_launch_chunked: ") + e.what());
+    } catch (...) {
+        set_error("unknown C++ exception in sunny_geneig_launch_chunked");
+    }
+    return SUNNY_GENEIG_LAUNCH_ERROR;
+}
+
 int sunny_geneig_matrix_size(sunny_geneig_solver_t s) { return s ? s->impl->n() : -1; }
 int sunny_geneig_batches_per_block(sunny_geneig_solver_t s) {
     return s ? s->impl->batches_per_block() : -1;
@@ -352,7 +385,8 @@ void sunny_geneig_session_destroy(sunny_geneig_session_t s) {
 }
 
 int64_t sunny_geneig_session_submit(sunny_geneig_session_t s, int n, int count, uintptr_t H, uintptr_t S,
-                                    uintptr_t W, uintptr_t U, uintptr_t info, int on_device) {
+                                    uintptr_t W, uintptr_t U, uintptr_t info, int on_device,
+                                    uintptr_t stream) {
     clear_error();
     if (!s) {
         set_error("sunny_geneig_session_submit: session is NULL");
@@ -361,13 +395,33 @@ int64_t sunny_geneig_session_submit(sunny_geneig_session_t s, int n, int count, 
     try {
         return s->impl->submit(n, count, reinterpret_cast<const void*>(H), reinterpret_cast<const void*>(S),
                                reinterpret_cast<double*>(W), reinterpret_cast<void*>(U),
-                               reinterpret_cast<int*>(info), on_device != 0);
+                               reinterpret_cast<int*>(info), on_device != 0,
+                               reinterpret_cast<hipStream_t>(stream));
     } catch (const std::invalid_argument& e) {
         set_error(std::string("sunny_geneig_session_submit: ") + e.what());
         return SUNNY_GENEIG_INVALID_ARG;
     } catch (const std::exception& e) {
         set_error(std::string("sunny_geneig_session_submit: ") + e.what());
         return SUNNY_GENEIG_LAUNCH_ERROR;
+    }
+}
+
+int sunny_geneig_session_set_policy(sunny_geneig_session_t s, int policy, int slice) {
+    clear_error();
+    if (!s) {
+        set_error("sunny_geneig_session_set_policy: session is NULL");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    if (policy != 0 && policy != 1) {
+        set_error("sunny_geneig_session_set_policy: policy must be 0 (arrival order) or 1 (least work left)");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    try {
+        s->impl->set_policy(static_cast<sgeig::Policy>(policy), slice);
+        return SUNNY_GENEIG_OK;
+    } catch (const std::exception& e) {
+        set_error(std::string("sunny_geneig_session_set_policy: ") + e.what());
+        return SUNNY_GENEIG_INVALID_ARG;
     }
 }
 

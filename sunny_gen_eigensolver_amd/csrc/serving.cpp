@@ -111,12 +111,22 @@ ServingSession::~ServingSession() {
         if (s.done) (void)hipEventDestroy(s.done);
         if (s.stream) (void)hipStreamDestroy(s.stream);
     }
+    for (auto& kv : requests_)
+        if (kv.second->ready) (void)hipEventDestroy(kv.second->ready);
+    for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
     if (dev_base_) (void)hipFree(dev_base_);
     (void)hipSetDevice(cur);
 }
 
+void ServingSession::set_policy(Policy policy, int slice) {
+    if (slice < 0) throw std::invalid_argument("slice must be >= 0 (0 = whole requests)");
+    std::lock_guard<std::mutex> lk(mu_);
+    policy_ = policy;
+    slice_ = slice;
+}
+
 int64_t ServingSession::submit(int n, int count, const void* H, const void* S, double* W, void* U,
-                               int* info, bool on_device) {
+                               int* info, bool on_device, hipStream_t ready) {
     if (std::find(orders_.begin(), orders_.end(), n) == orders_.end())
         throw std::invalid_argument("order " + std::to_string(n) + " is not served by this session");
     if (count < 1 || count > max_count_)
@@ -135,6 +145,27 @@ int64_t ServingSession::submit(int n, int count, const void* H, const void* S, d
     if (r->footprint > arena_->size())
         throw std::invalid_argument("request needs " + std::to_string(r->footprint) +
                                     " B of device memory, more than the session's HBM budget");
+    if (on_device || ready) {
+        // Capture the caller's stream as it is now: the session's copies and
+        // kernels for this request wait for it.
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            if (!event_pool_.empty()) {
+                r->ready = event_pool_.back();
+                event_pool_.pop_back();
+            }
+        }
+        int cur = 0;
+        SG_HIP(hipGetDevice(&cur));
+        if (cur != device_) SG_HIP(hipSetDevice(device_));
+        if (!r->ready) SG_HIP(hipEventCreateWithFlags(&r->ready, hipEventDisableTiming));
+        const hipError_t e = hipEventRecord(r->ready, ready);
+        if (cur != device_) (void)hipSetDevice(cur);
+        if (e != hipSuccess) {
+            (void)hipEventDestroy(r->ready);
+            SG_HIP(e);
+        }
+    }
     r->t_submit = std::chrono::steady_clock::now();
     {
         std::lock_guard<std::mutex> lk(mu_);
@@ -147,56 +178,99 @@ int64_t ServingSession::submit(int n, int count, const void* H, const void* S, d
     return r->id;
 }
 
-bool ServingSession::try_admit_(const std::shared_ptr<Request>& r) {
+bool ServingSession::admit_one_() {
     int free_slot = -1;
     for (int i = 0; i < (int)slots_.size(); ++i)
         if (!slots_[i].req) {
             free_slot = i;
             break;
         }
-    if (free_slot < 0) return false;
-    if (!r->on_device) {
-        const size_t off = arena_->alloc(r->footprint);
-        if (off == SIZE_MAX) return false;       // head of line waits for memory
-        r->offset = off;
-        stats_.hbm_peak = std::max(stats_.hbm_peak, arena_->in_use());
+    if (free_slot < 0 || pending_.empty()) return false;
+    // The request to serve next: the oldest one, or the one with the least
+    // work left (pencils not yet handed out x n^3), oldest first on ties.
+    auto pick = pending_.begin();
+    if (policy_ == Policy::ShortestRemaining) {
+        double best = 1e300;
+        for (auto it = pending_.begin(); it != pending_.end(); ++it) {
+            const double nn = (double)(*it)->n;
+            const double work = (double)((*it)->count - (*it)->dispatched) * nn * nn * nn;
+            if (work < best) {
+                best = work;
+                pick = it;
+            }
+        }
     }
+    std::shared_ptr<Request> r = *pick;
+    if (!r->admitted) {
+        if (!r->on_device) {
+            const size_t off = arena_->alloc(r->footprint);
+            if (off == SIZE_MAX) return false;       // the chosen request waits for memory
+            r->offset = off;
+            stats_.hbm_peak = std::max(stats_.hbm_peak, arena_->in_use());
+        }
+        r->admitted = true;
+        r->t_admit = std::chrono::steady_clock::now();
+    }
+    const int off = r->dispatched;
+    const int cnt = (slice_ > 0) ? std::min(slice_, r->count - off) : r->count - off;
+    r->dispatched += cnt;
+    ++r->inflight;
+    if (r->dispatched == r->count) pending_.erase(pick);
     Slot& s = slots_[free_slot];
-    r->slot = free_slot;
-    r->t_admit = std::chrono::steady_clock::now();
     s.req = r;
     int running = 0;
     for (auto& sl : slots_) running += sl.req ? 1 : 0;
     stats_.max_concurrent = std::max(stats_.max_concurrent, running);
     try {
-        launch_(s, *r);
+        launch_(s, *r, off, cnt);
     } catch (const std::exception& e) {
-        r->error = e.what();
-        (void)hipEventRecord(s.done, s.stream);   // completes with the stream's prior work
+        if (r->error.empty()) r->error = e.what();
+        if (r->dispatched < r->count) {             // nothing more of a failed request
+            r->dispatched = r->count;
+            pending_.erase(std::find(pending_.begin(), pending_.end(), r));
+        }
+        (void)hipEventRecord(s.done, s.stream);     // completes with the stream's prior work
     }
     return true;
 }
 
-void ServingSession::launch_(Slot& s, Request& r) {
+void ServingSession::launch_(Slot& s, Request& r, int off, int cnt) {
     Solver& sol = *s.solvers.at(r.n);
+    if (r.ready) SG_HIP(hipStreamWaitEvent(s.stream, r.ready, 0));
+    const size_t nn = (size_t)r.n * r.n, c = (size_t)r.count, o = (size_t)off, k = (size_t)cnt;
+    const size_t mo = o * nn * 16, mk = k * nn * 16;
     if (r.on_device) {
-        sol.launch(r.H, r.S, r.W, r.U, r.info, r.count, s.stream);
+        sol.launch(static_cast<const char*>(r.H) + mo, static_cast<const char*>(r.S) + mo, r.W + o * r.n,
+                   static_cast<char*>(r.U) + mo, r.info + o, cnt, s.stream);
     } else {
-        const size_t nn = (size_t)r.n * r.n, c = (size_t)r.count;
+        // The request's block of the arena: H, S, U, W, info for all `count`
+        // pencils; this slice uses its own part of each.
         char* base = static_cast<char*>(dev_base_) + r.offset;
         char* dH = base;
         char* dS = dH + align_up(nn * c * 16);
         char* dU = dS + align_up(nn * c * 16);
-        char* dW = dU + align_up(nn * c * 16);
-        char* dI = dW + align_up((size_t)r.n * c * 8);
-        SG_HIP(hipMemcpyAsync(dH, r.H, nn * c * 16, hipMemcpyHostToDevice, s.stream));
-        SG_HIP(hipMemcpyAsync(dS, r.S, nn * c * 16, hipMemcpyHostToDevice, s.stream));
-        sol.launch(dH, dS, reinterpret_cast<double*>(dW), dU, reinterpret_cast<int*>(dI), r.count, s.stream);
-        SG_HIP(hipMemcpyAsync(r.W, dW, (size_t)r.n * c * 8, hipMemcpyDeviceToHost, s.stream));
-        SG_HIP(hipMemcpyAsync(r.U, dU, nn * c * 16, hipMemcpyDeviceToHost, s.stream));
-        SG_HIP(hipMemcpyAsync(r.info, dI, c * 4, hipMemcpyDeviceToHost, s.stream));
+        double* dW = reinterpret_cast<double*>(dU + align_up(nn * c * 16));
+        int* dI = reinterpret_cast<int*>(reinterpret_cast<char*>(dW) + align_up((size_t)r.n * c * 8));
+        SG_HIP(hipMemcpyAsync(dH + mo, static_cast<const char*>(r.H) + mo, mk, hipMemcpyHostToDevice, s.stream));
+        SG_HIP(hipMemcpyAsync(dS + mo, static_cast<const char*>(r.S) + mo, mk, hipMemcpyHostToDevice, s.stream));
+        sol.launch(dH + mo, dS + mo, dW + o * r.n, dU + mo, dI + o, cnt, s.stream);
+        SG_HIP(hipMemcpyAsync(r.W + o * r.n, dW + o * r.n, k * r.n * 8, hipMemcpyDeviceToHost, s.stream));
+        SG_HIP(hipMemcpyAsync(static_cast<char*>(r.U) + mo, dU + mo, mk, hipMemcpyDeviceToHost, s.stream));
+        SG_HIP(hipMemcpyAsync(r.info + o, dI + o, k * 4, hipMemcpyDeviceToHost, s.stream));
     }
     SG_HIP(hipEventRecord(s.done, s.stream));
+}
+
+void ServingSession::finish_(const std::shared_ptr<Request>& r) {
+    r->t_done = std::chrono::steady_clock::now();
+    r->done = true;
+    if (!r->on_device && r->offset != SIZE_MAX) arena_->release(r->offset);
+    if (r->ready) {
+        event_pool_.push_back(r->ready);
+        r->ready = nullptr;
+    }
+    ++stats_.completed;
+    cv_done_.notify_all();
 }
 
 void ServingSession::loop_() {
@@ -213,19 +287,12 @@ void ServingSession::loop_() {
                 continue;
             }
             auto r = s.req;
-            if (q != hipSuccess && r->error.empty()) r->error = hipGetErrorString(q);
-            r->t_done = std::chrono::steady_clock::now();
-            r->done = true;
-            if (!r->on_device) arena_->release(r->offset);
             s.req.reset();
-            ++stats_.completed;
-            cv_done_.notify_all();
+            if (q != hipSuccess && r->error.empty()) r->error = hipGetErrorString(q);
+            if (--r->inflight == 0 && r->dispatched == r->count) finish_(r);
         }
-        // Admission, in arrival order.
-        while (!pending_.empty() && try_admit_(pending_.front())) {
-            pending_.pop_front();
-            ++running;
-        }
+        // Admission: one slice per free slot.
+        while (admit_one_()) ++running;
         if (stop_ && pending_.empty() && running == 0) break;
         if (running == 0 && pending_.empty()) {
             cv_work_.wait(lk);

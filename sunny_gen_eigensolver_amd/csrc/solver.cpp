@@ -126,9 +126,15 @@ Solver::Solver(int n, int device_id, int bpb_request, Tuning tuning, CacheMode c
     if (const char* v = std::getenv("SUNNY_GENEIG_GRAPH")) graphs_ = std::atoi(v) != 0;
     if (const char* v = std::getenv("SUNNY_GENEIG_QL_SWEEP_CAP")) ql_sweep_cap_ = std::max(0, std::atoi(v));
     if (const char* v = std::getenv("SUNNY_GENEIG_QL_LOG_MIN_BATCH")) ql_log_min_batch_ = std::max(0, std::atoi(v));
+    // One matrix per wave: a workgroup of k waves solves k matrices, and its
+    // LDS work area and register budget allow k <= 4. The reference returns
+    // NULL for an explicit request it cannot build (unified_solver.cpp:1021-1028);
+    // so does this library, with this message.
     if (bpb_request > 4)
-        throw std::invalid_argument("batches_per_block must be <= 4 (one wave per matrix, "
-                                    "LDS-resident work area)");
+        throw std::invalid_argument("batches_per_block_request=" + std::to_string(bpb_request) +
+                                    " is not supported: at most 4 matrices per workgroup (one wave "
+                                    "per matrix, LDS work area shared by the workgroup); pass 1..4, "
+                                    "or <= 0 to select automatically");
     int count = 0;
     SG_HIP(hipGetDeviceCount(&count));
     if (device_id < 0 || device_id >= count)
@@ -145,10 +151,15 @@ Solver::Solver(int n, int device_id, int bpb_request, Tuning tuning, CacheMode c
         autotuned_ = true;
     }
     bind_module_(bd, cache);
+    SG_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
 }
 
 Solver::~Solver() {
     for (auto& e : ev_) (void)hipEventDestroy(e);
+    if (done_) {
+        (void)hipEventSynchronize(done_);
+        (void)hipEventDestroy(done_);
+    }
     clear_graphs_();
     if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
     if (lbuf_ || mbuf_ || gbuf_ || qlog_) {
@@ -244,13 +255,23 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
         if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_ITERS")) refine_iters_ = std::max(0, std::atoi(v));
         if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_DELTA")) refine_delta_ = std::atof(v);
     }
-    int lds = 0;
-    SG_HIP(hipFuncGetAttribute(&lds, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, fused_));
+    // Reported LDS: the fused kernel's; in the split mode the front kernel's
+    // (the reference reports its reduce kernel's, c_api.cpp:41-49); in the
+    // mixed mode the largest of its kernels'.
+    std::vector<hipFunction_t> fns = {fused_};
+    if (mode_ == Mode::Hybrid) fns = {k_reduce_};
+    if (mode_ == Mode::Mixed) fns = {k_mx_chol_, k_mx_formm_, k_mx_front_, k_back32_, k_refine_};
     int max_lds = 0;
     SG_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device_));
-    if (lds > max_lds)
-        throw std::runtime_error("kernel needs " + std::to_string(lds) + " B of LDS, device allows " +
-                                 std::to_string(max_lds));
+    int lds = 0;
+    for (hipFunction_t f : fns) {
+        int l = 0;
+        SG_HIP(hipFuncGetAttribute(&l, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, f));
+        if (l > max_lds)
+            throw std::runtime_error("kernel needs " + std::to_string(l) + " B of LDS, device allows " +
+                                     std::to_string(max_lds));
+        lds = std::max(lds, l);
+    }
     lds_bytes_ = (unsigned)lds;
 }
 
@@ -343,6 +364,9 @@ void Solver::launch(const void* H, const void* S, double* W, void* U, int* info,
     if (cur != device_) SG_HIP(hipSetDevice(device_));
     ev_used_ = 0;
     const bool multi = large_ || mode_ != Mode::Unified;
+    // The workspace is shared by every launch of this handle: order this one
+    // after the previous one, whatever stream that ran on.
+    if (multi) SG_HIP(hipStreamWaitEvent(stream, done_, 0));
     if (graphs_ && multi && !timing_) {
         reserve((size_t)batch);                    // any allocation happens outside capture
         GraphEntry* hit = nullptr;
@@ -378,7 +402,24 @@ void Solver::launch(const void* H, const void* S, double* W, void* U, int* info,
         mark_(stream);
         dispatch_(H, S, W, U, info, batch, stream);
     }
+    if (multi) SG_HIP(hipEventRecord(done_, stream));
     if (cur != device_) SG_HIP(hipSetDevice(cur));
+}
+
+void Solver::launch_chunked(const void* H, const void* S, double* W, void* U, int* info, int total,
+                            int chunk, const std::vector<hipStream_t>& streams) {
+    if (total < 0) throw std::invalid_argument("total_matrices < 0");
+    if (total == 0) return;
+    if (chunk <= 0) throw std::invalid_argument("chunk_size must be > 0");
+    if (streams.empty()) throw std::invalid_argument("at least one stream is required");
+    const size_t mat = (size_t)n_ * n_ * 16;
+    int k = 0;
+    for (int off = 0; off < total; off += chunk, ++k) {
+        const int cnt = std::min(chunk, total - off);
+        launch(static_cast<const char*>(H) + (size_t)off * mat, static_cast<const char*>(S) + (size_t)off * mat,
+               W + (size_t)off * n_, static_cast<char*>(U) + (size_t)off * mat, info + off, cnt,
+               streams[(size_t)k % streams.size()]);
+    }
 }
 
 void Solver::dispatch_(const void* H, const void* S, double* W, void* U, int* info, int batch,

@@ -25,12 +25,23 @@ public:
     Solver(const Solver&) = delete;
     Solver& operator=(const Solver&) = delete;
 
-    // Asynchronous on `stream`; batch == 0 is a no-op.
+    // Asynchronous on `stream`; batch == 0 is a no-op. In the modes with a
+    // workspace (split, mixed, staged orders) a launch is ordered after the
+    // handle's previous launch even when the caller switches streams.
     void launch(const void* H, const void* S, double* W, void* U, int* info, int batch,
                 hipStream_t stream);
 
+    // Contiguous chunks of `chunk` matrices launched round-robin over
+    // `streams` (reference: UnifiedGeneigSolver::launch_chunked,
+    // unified_solver.cpp:1078-1107). Fused-mode chunks run concurrently;
+    // chunks of the modes with a workspace run in submission order.
+    void launch_chunked(const void* H, const void* S, double* W, void* U, int* info, int total,
+                        int chunk, const std::vector<hipStream_t>& streams);
+
     int n() const { return n_; }
-    int batches_per_block() const { return bpb_; }
+    // Matrices per workgroup of the fused kernel; the split and mixed modes
+    // report 1, as the reference's split mode does (c_api.cpp:41-49).
+    int batches_per_block() const { return (mode_ == Mode::Unified) ? bpb_ : 1; }
     int block_dim_x() const { return block_dim_; }
     unsigned shared_mem_bytes() const { return lds_bytes_; }
     Mode mode() const { return mode_; }
@@ -135,6 +146,10 @@ private:
     long long graph_captures_ = 0, graph_replays_ = 0;
     void clear_graphs_();
     void dispatch_(const void* H, const void* S, double* W, void* U, int* info, int batch, hipStream_t stream);
+
+    // Recorded after every launch of a mode with a workspace; the next launch
+    // waits for it on its own stream.
+    hipEvent_t done_ = nullptr;
 
     // Stage timing: when enabled, events bracket every kernel of a launch on
     // its stream; stage_times() reads the last launch's durations.

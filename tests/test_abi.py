@@ -67,6 +67,13 @@ def test_null_handle_accessors():
     L.sunny_geneig_destroy(None)
 
 
+def test_chunked_launch_null_and_bad_arguments():
+    L = sg.lib()
+    one = (ctypes.c_uint64 * 1)(0)
+    assert L.sunny_geneig_launch_chunked(None, 1, 1, 1, 1, 1, 8, 4, one, 1) == sg.STATUS_INVALID_ARG
+    assert "NULL" in sg.last_error()
+
+
 def test_null_launch_is_invalid_arg_with_message():
     L = sg.lib()
     st = L.sunny_geneig_launch(None, 0, 0, 0, 0, 0, 8, 0)
@@ -129,7 +136,8 @@ def test_varlen_null_and_bad_arguments():
 def test_session_null_and_bad_arguments():
     L = sg.lib()
     L.sunny_geneig_session_destroy(None)
-    assert L.sunny_geneig_session_submit(None, 8, 1, 0, 0, 0, 0, 0, 1) == sg.STATUS_INVALID_ARG
+    assert L.sunny_geneig_session_submit(None, 8, 1, 0, 0, 0, 0, 0, 1, 0) == sg.STATUS_INVALID_ARG
+    assert L.sunny_geneig_session_set_policy(None, 0, 0) == sg.STATUS_INVALID_ARG
     assert L.sunny_geneig_session_poll(None, 0) == sg.STATUS_INVALID_ARG
     assert L.sunny_geneig_session_wait(None, 0, None, None) == sg.STATUS_INVALID_ARG
     assert L.sunny_geneig_session_stats(None, None, None, None, None) == sg.STATUS_INVALID_ARG

@@ -45,11 +45,16 @@ def _outputs(n, c, device):
             torch.zeros((c,), dtype=torch.int32, **kw))
 
 
-@pytest.mark.parametrize("kernel", ["mixed", "unified"])
-def test_alone_equals_mixed(kernel):
+@pytest.mark.parametrize("kernel,policy,slice_", [("mixed", "fifo", 0), ("unified", "fifo", 0),
+                                                  ("mixed", "srpt", 32), ("unified", "fifo", 16)])
+def test_alone_equals_mixed(kernel, policy, slice_):
+    """Slices (a request cut into launches of at most slice_ pencils, spread
+    over the slots) and either admission order give the same bits as whole
+    requests solved alone."""
     import sunny_gen_eigensolver_amd as sg
     data = _inputs()
-    with sg.ServingSession([24, 54], kernel=kernel, max_count=256, slots=4, hbm_budget=64 << 20) as ss:
+    with sg.ServingSession([24, 54], kernel=kernel, max_count=256, slots=4, hbm_budget=64 << 20,
+                           policy=policy, slice=slice_) as ss:
         alone = []
         for (H, S), (n, c) in zip(data, REQS):
             dH, dS = _dev((H, S))
@@ -77,11 +82,13 @@ def test_alone_equals_mixed(kernel):
                         tag=f"serving {kernel} request {k}")
 
 
-def test_host_buffers_match_device_buffers():
+@pytest.mark.parametrize("slice_", [0, 24])
+def test_host_buffers_match_device_buffers(slice_):
     import torch
     import sunny_gen_eigensolver_amd as sg
     data = _inputs(400)
-    with sg.ServingSession([24, 54], kernel="mixed", max_count=256, slots=3, hbm_budget=64 << 20) as ss:
+    with sg.ServingSession([24, 54], kernel="mixed", max_count=256, slots=3, hbm_budget=64 << 20,
+                           policy="srpt", slice=slice_) as ss:
         dev_out, host_out, ids = [], [], []
         for (H, S), (n, c) in zip(data, REQS):
             dH, dS = _dev((H, S))
@@ -175,3 +182,59 @@ def test_staged_orders_in_the_mix():
         Wl, Ul, il = O.zhegv_batch(H, S, threads=8)
         T.assert_parity(H, S, alone[k][0], alone[k][1], alone[k][2], Wl, Ul, tag=f"serving staged request {k}")
 
+
+
+def test_waits_for_the_callers_stream():
+    """Inputs written on the caller's stream just before submission, behind a
+    long kernel: the session's launch waits for that stream, so the solve sees
+    the new inputs (the buffers first hold a non-HPD S that would fail)."""
+    import torch
+    import sunny_gen_eigensolver_amd as sg
+    n, c = 54, 64
+    H, S = O.gen_reference(n, c, 611, "random")
+    srcH, srcS = _dev((H, S))
+    with sg.ServingSession([n], kernel="mixed", max_count=c, slots=2, hbm_budget=1 << 20) as ss:
+        W0, U0, i0 = _outputs(n, c, "cuda")
+        ss.wait(ss.submit(srcH, srcS, W0, U0, i0))
+        side = torch.cuda.Stream()
+        dH = torch.zeros_like(srcH)
+        dS = torch.zeros_like(srcS)              # zero S: every Cholesky would fail
+        torch.cuda.synchronize()
+        with torch.cuda.stream(side):
+            a = torch.randn((4096, 4096), device="cuda", dtype=torch.float64)
+            for _ in range(4):
+                a = a @ a / 4096.0                # keep the side stream busy
+            dH.copy_(srcH)
+            dS.copy_(srcS)
+            W, U, info = _outputs(n, c, "cuda")
+            rid = ss.submit(dH, dS, W, U, info)
+        ss.wait(rid)
+        torch.cuda.synchronize()
+        del a
+    assert (info.cpu().numpy() == 0).all()
+    assert np.array_equal(W.cpu().numpy(), W0.cpu().numpy())
+    assert np.array_equal(U.cpu().numpy(), U0.cpu().numpy())
+
+
+def test_submit_validates_buffers():
+    """Mismatched devices, shapes, dtypes or layouts are refused before any
+    device work (a host W beside device H would otherwise be written through
+    a host pointer by the kernels)."""
+    import torch
+    import sunny_gen_eigensolver_amd as sg
+    n, c = 24, 4
+    with sg.ServingSession([n], kernel="unified", max_count=8, slots=1, hbm_budget=1 << 20) as ss:
+        H = torch.zeros((c, n, n), dtype=torch.complex128, device="cuda")
+        W, U, info = _outputs(n, c, "cuda")
+        Wc, _, _ = _outputs(n, c, "cpu")
+        bad = [
+            (H, H, Wc, U, info),                                        # W on the host
+            (H, H, W, U.to(torch.complex64), info),                     # U dtype
+            (H, H, W[:, :n - 1], U, info),                              # W shape
+            (H, H, W, U, info.to(torch.int64)),                         # info dtype
+            (H, H.transpose(1, 2), W, U, info),                         # S not contiguous
+            (H, H[:2], W, U, info),                                     # S count
+        ]
+        for k, args in enumerate(bad):
+            with pytest.raises(ValueError):
+                ss.submit(*args)
