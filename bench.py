@@ -56,6 +56,17 @@ def bytes_per_solve(n: int) -> float:
     return 48.0 * n * n + 8.0 * n + 4.0
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -103,7 +114,8 @@ def cpu_baseline_sample(Hs, Ss, threads: int):
 
 
 def serving_bench(kernel: str, pools, peak_requests: int, poisson_requests: int, load: float,
-                  seed: int, slots: int = 4):
+                  seed: int, slots: int = 4, policy: str = "fifo", slice_: int = 0, host: bool = False,
+                  hbm_budget: int = 1 << 30):
     """Continuous batching through the library's serving session.
 
     Requests are q-point batches: K pencils, K log-uniform in [16, 4096], one
@@ -112,7 +124,10 @@ def serving_bench(kernel: str, pools, peak_requests: int, poisson_requests: int,
     pool and gets its own output buffers). First a burst of `peak_requests`
     gives the peak sustained rate; then `poisson_requests` arrive as a Poisson
     process at `load` x that rate. Latency = submission to results ready, as
-    measured by the session."""
+    measured by the session. policy / slice_: the session's admission order
+    and slice size. host: inputs and outputs in pinned host memory, staged
+    through the session's arena of hbm_budget bytes (admission by footprint
+    then limits how many requests are in flight)."""
     import numpy as np
     import torch
     import sunny_gen_eigensolver_amd as sg
@@ -124,10 +139,21 @@ def serving_bench(kernel: str, pools, peak_requests: int, poisson_requests: int,
         ns = rng.choice(orders, size=count)
         return list(zip(ns.tolist(), ks.tolist()))
 
-    ss = sg.ServingSession(orders, kernel=kernel, max_count=4096, slots=slots, hbm_budget=1 << 30)
+    ss = sg.ServingSession(orders, kernel=kernel, max_count=4096, slots=slots, hbm_budget=hbm_budget,
+                           policy=policy, slice=slice_)
+    if host:
+        pools = {n: (H.cpu().pin_memory(), S.cpu().pin_memory()) for n, (H, S) in pools.items()}
+        outs = {n: (torch.empty((4096, n), dtype=torch.float64).pin_memory(),
+                    torch.empty((4096, n, n), dtype=torch.complex128).pin_memory(),
+                    torch.zeros((4096,), dtype=torch.int32).pin_memory()) for n in orders}
 
     def submit(n, k):
         H, S = pools[n]
+        if host:
+            # Requests share their order's pinned output area: only latency is
+            # measured here, and every request's results are written in full.
+            W, U, info = (t[:k] for t in outs[n])
+            return ss.submit(H[:k], S[:k], W, U, info)
         W = torch.empty((k, n), dtype=torch.float64, device=H.device)
         U = torch.empty((k, n, n), dtype=torch.complex128, device=H.device)
         info = torch.zeros((k,), dtype=torch.int32, device=H.device)
@@ -169,7 +195,9 @@ def serving_bench(kernel: str, pools, peak_requests: int, poisson_requests: int,
     # lower bound on the p99 latency of any schedule for this request mix.
     iso = np.array([np.interp(np.log(k), np.log(solo_k), solo[n]) for n, k in reqs])
     return {
-        "mode": kernel, "orders": orders, "requests": len(reqs), "slots": slots,
+        "mode": kernel, "orders": orders, "requests": len(reqs), "slots": slots, "policy": policy,
+        "slice": slice_, "buffers": "pinned host, staged through the arena" if host else "device",
+        "hbm_budget_bytes": hbm_budget if host else None, "arena_peak_bytes": st["hbm_peak_bytes"],
         "peak_solves_per_s": peak_rate, "offered_load": load,
         "offered_solves_per_s": lam * mean_k,
         "achieved_solves_per_s": sum(k for _, k in reqs) / span,
@@ -177,9 +205,10 @@ def serving_bench(kernel: str, pools, peak_requests: int, poisson_requests: int,
         "max_ms": float(lats.max()), "max_concurrent": st["max_concurrent"],
         "p99_lower_bound_ms": float(np.percentile(iso, 99)), "p50_lower_bound_ms": float(np.percentile(iso, 50)),
         "solo_ms": {str(n): dict(zip(map(str, solo_k), solo[n])) for n in orders},
-        "note": "K log-uniform in [16, 4096] per request, N uniform over the served orders; "
-                "device-resident inputs and outputs (no PCIe in the latency); "
-                f"{slots} slots; latency from submission to results ready in HBM",
+        "note": "K log-uniform in [16, 4096] per request, N uniform over the served orders; " +
+                ("pinned host inputs and outputs (PCIe copies inside the latency); " if host else
+                 "device-resident inputs and outputs (no PCIe in the latency); ") +
+                f"{slots} slots; latency from submission to results ready",
     }
 
 
@@ -300,18 +329,31 @@ def main():
         acc["max_eig_rel_diff_vs_fp64_fused"] = (
             ((W - W_fp64).abs() / W_fp64.abs().clamp(min=1.0)).max().item())
 
-    # Roofline of the dominant kernel (the refinement, on the matrix cores):
-    # its own algorithmic GEMM flops per launch / its launch time.
-    dom_name, dom_ms = max(st, key=lambda kv: kv[1])
-    rf = refine_flops_per_solve(n, iters) if dom_name == "geneig_mx_refine" else flops_per_solve(n)
-    achieved = B * rf / (dom_ms * 1e-3) / 1e12
+    # Roofline of the path (the algorithmic work of a zhegv solve, 32 N^3
+    # real flops, refinement not credited) over the whole step: the mixed
+    # step's kernels run back to back as one hipGraph replay, so the step time
+    # is the sum of their launch durations (timed on the launch stream).
+    path_tflops = B * flops_per_solve(n) / (ms * 1e-3) / 1e12
     traffic = None
-    tpath = os.path.join(REPO, "profiles", f"traffic_mx_refine_n{n}.json")
+    tpath = os.path.join(REPO, "profiles", f"traffic_mixed_n{n}_b{B}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
-        if tr.get("batch") == B:
-            traffic = tr.get("hbm_bytes_per_launch")
+        if tr.get("batch") == B and tr.get("n") == n:
+            traffic = tr.get("hbm_bytes_per_step")
+    # The refinement kernel's own matrix-core efficiency (its GEMM flops are
+    # extra work of the mixed form and are not credited to the path above).
+    refine_ms = dict(st).get("geneig_mx_refine")
+    refine = None
+    if refine_ms:
+        rf = refine_flops_per_solve(n, iters)
+        refine = {"kernel": "geneig_mx_refine", "ms": refine_ms, "gemm_flops_per_solve": rf,
+                  "achieved_TFLOPs": B * rf / (refine_ms * 1e-3) / 1e12,
+                  "frac_of_fp64_mfma_peak": B * rf / (refine_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                  "note": "the refinement's complex products (per step M V, V^H X, V^H V, V E; then "
+                          "U = W^H V at its triangular size), 8 real flops per complex multiply-add, "
+                          "over the kernel's launch time"}
+    dom_name, dom_ms = max(st, key=lambda kv: kv[1])
 
     out = {
         "metric": "geneig solves/s at N=54 complex, mixed+refined vs FP64 at <=50Ne resid; p99 ms",
@@ -332,17 +374,17 @@ def main():
                    "n": n, "batch": B, "seed": hex(a.seed), "mode": "mixed", "refine_iters": iters},
         "accuracy": acc,
         "stages_ms": {k: v for k, v in st},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": dom_name,
-                     "note": f"{dom_name}: {rf:.0f} algorithmic flops per solve (the refinement's complex "
-                             f"products, triangular ones at their true size, 8 real flops per complex "
-                             f"multiply-add) x batch / its launch time, timed with HIP events on the launch "
-                             f"stream; FP64 MFMA peak"},
-        "path_fp64_equivalent": {
-            "achieved_TFLOPs": B * flops_per_solve(n) / (ms * 1e-3) / 1e12,
-            "frac_of_fp64_peak": B * flops_per_solve(n) / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
-            "note": "32 N^3 algorithmic flops per solve (refinement not credited) / whole step"},
+        "roofline": {"bound": "mfma", "achieved": path_tflops, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": path_tflops / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": "mixed step: " + " + ".join(k for k, _ in st),
+                     "note": f"32 N^3 = {flops_per_solve(n):.0f} algorithmic flops per solve (the LAPACK "
+                             f"zhegv path, LAWN 41 counting; refinement not credited) x batch / the whole "
+                             f"step's time; the step's kernels run back to back (one hipGraph replay), "
+                             f"timed with HIP events on the launch stream; FP64 peak (vector = matrix); "
+                             f"traffic = HBM bytes of all kernels of one step (rocprofv3 FETCH_SIZE + "
+                             f"WRITE_SIZE, calibrated)"},
+        "dominant_kernel": {"kernel": dom_name, "ms": dom_ms, "share_of_step": dom_ms / ms},
+        "refine_gemm": refine,
         "hbm": {"compulsory_GBs": B * bytes_per_solve(n) / (ms * 1e-3) / 1e9,
                 "frac_of_peak": B * bytes_per_solve(n) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "fp64_modes": modes,
@@ -522,10 +564,13 @@ def main():
         del pools
 
     if not a.no_cpu:
+        # The host cores this GPU's share of the box allows (16 per GPU; the
+        # machine's own count is reported beside it).
         try:
-            threads = min(16, len(os.sched_getaffinity(0)))
+            visible = len(os.sched_getaffinity(0))
         except Exception:
-            threads = min(16, os.cpu_count() or 1)
+            visible = os.cpu_count() or 1
+        threads = min(int(os.environ.get("OMP_NUM_THREADS", "16") or 16), visible)
         # The mixed result for the sample, recomputed (W holds the last mode's).
         m2 = sg.Solver(n, kernel="mixed")
         sg.run_kernel(m2, H, S, W, U, info, stream=stream)
@@ -536,6 +581,8 @@ def main():
         Wg = W[:Wl.shape[0]].cpu().numpy()
         out["accuracy"]["max_eig_rel_err_vs_zhegv_sample"] = float(
             (np.abs(Wg - Wl) / np.maximum(np.abs(Wl), 1.0)).max())
+        cb["cpu_model"] = cpu_model()
+        cb["host_cpus_visible"] = visible
         out["cpu_baseline"] = cb
         # The uneven batch's sample: zhegv's own residual and S-orthogonality
         # on the same pencils (cond(S) up to 1e4 lifts both above 50 N eps).

@@ -111,3 +111,25 @@ def test_tiny_orders_vs_lapack(kernel, n):
     W, U, info = T.run_gpu(H, S, kernel=kernel)
     assert (il == 0).all()
     T.assert_parity(H, S, W, U, info, Wl, Ul, tag=f"{kernel} tiny n={n}")
+
+
+@pytest.mark.parametrize("kernel", MODES)
+@pytest.mark.parametrize("n", [54, 96])
+def test_ill_conditioned_s(kernel, n):
+    """cond(S) log-uniform in [1e1, 1e4] (S = Q_S diag(sigma) Q_S^H, sigma
+    geometric from 1 to cond) at an LDS-resident and a staged order, every
+    mode: the mixed mode's FP32 stages see M = L^-1 H L^-H, whose conditioning
+    grows with cond(S), and the refinement must still land within the
+    conditioning-aware bars against zhegv on the same pencils."""
+    rng = np.random.default_rng(0x1C0 + n)
+    b = 24
+    conds = np.exp(rng.uniform(np.log(1e1), np.log(1e4), size=b))
+    conds[0] = 1e4
+    Hs, Ss = [], []
+    for k in range(b):
+        H1, S1 = O.gen_spectrum(n, 1, 0xBAD5 + 97 * n + k, cond_s=float(conds[k]))
+        Hs.append(H1[0])
+        Ss.append(S1[0])
+    H, S = np.stack(Hs), np.stack(Ss)
+    W, U, info = T.run_gpu(H, S, kernel=kernel)
+    T.assert_parity_conditioned(H, S, W, U, info, tag=f"{kernel} n={n} cond(S) <= 1e4")

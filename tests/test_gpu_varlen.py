@@ -253,3 +253,39 @@ def test_autotuned_buckets_vs_lapack():
         Hs, Ss = _gen_mix(orders, 8100)
         W, U, info = _solve(vs, Hs, Ss, orders)
     _check_groups(Hs, Ss, W, U, info, orders, "varlen autotuned")
+
+
+def _gen_c3(orders, conds, seed):
+    """The uneven workload's law: per pencil the benchmark generator with
+    S = Q_S diag(sigma) Q_S^H, sigma geometric from 1 to cond(S)."""
+    Hs, Ss = [], []
+    for k, (n, c) in enumerate(zip(orders, conds)):
+        H, S = O.gen_spectrum(int(n), 1, seed + k, cond_s=float(c))
+        Hs.append(H[0])
+        Ss.append(S[0])
+    return Hs, Ss
+
+
+def test_c3_law_vs_zhegv(vs128):
+    """Orders log-uniform over 12..128 (LDS-resident and staged buckets in one
+    call) with cond(S) log-uniform in [1e1, 1e4]: each pencil against zhegv
+    on the same pencil, with the conditioning-aware bars (residual and
+    S-orthogonality <= max(50 N eps, 2 x zhegv's own), eigenvalues within
+    1e-12 ||H||_2)."""
+    rng = np.random.default_rng(0xC3)
+    cnt = 240
+    orders = np.clip(np.floor(np.exp(rng.uniform(np.log(12), np.log(129), size=cnt))), 12, 128).astype(int)
+    orders[:4] = [12, 64, 65, 128]                   # both sides of the LDS / staged boundary
+    conds = np.exp(rng.uniform(np.log(1e1), np.log(1e4), size=cnt))
+    conds[:4] = 1e4
+    Hs, Ss = _gen_c3(orders.tolist(), conds, 0xC3000)
+    W, U, info = _solve(vs128, Hs, Ss, orders.tolist())
+    by_n = {}
+    for k, n in enumerate(orders.tolist()):
+        by_n.setdefault(n, []).append(k)
+    # Groups of neighbouring orders keep zhegv batches reasonably sized.
+    for n, ks in sorted(by_n.items()):
+        H = np.stack([Hs[k] for k in ks])
+        S = np.stack([Ss[k] for k in ks])
+        T.assert_parity_conditioned(H, S, np.stack([W[k] for k in ks]), np.stack([U[k] for k in ks]),
+                                    info[ks], tag=f"C3 n={n} x{len(ks)}")

@@ -225,3 +225,79 @@ def test_metrics_catch_planted_mistakes():
     # Transposed instead of conjugate-transposed factor: residual blows up.
     U4 = np.conj(U)
     assert T.metrics(H, S, W, U4, W, U)[:, 3].max() > 1e-6
+
+
+def _spectrum_properties(H, S, n, cond_s, tag):
+    """Checks of the benchmark generator's stated law on one batch (matrices
+    stored column-major: the stored array is the transpose)."""
+    npairs = min(6, n // 2)
+    for b in range(H.shape[0]):
+        h, s = H[b].T, S[b].T
+        assert np.abs(h - h.conj().T).max() <= 1e-14 * np.abs(h).max(), f"{tag}: H not Hermitian"
+        assert np.abs(s - s.conj().T).max() <= 1e-14 * np.abs(s).max(), f"{tag}: S not Hermitian"
+        lam = np.linalg.eigvalsh(h)
+        assert lam.min() >= -10.0 - 1e-12 and lam.max() <= 10.0 + 1e-12, f"{tag}: lambda outside [-10, 10]"
+        gaps = np.diff(lam)
+        near = np.abs(gaps - 1e-8) <= 1e-12
+        assert near.sum() >= npairs, f"{tag}: {near.sum()} pairs split by 1e-8, expected {npairs}"
+        assert (gaps[~near] > 1e-9).all() or npairs == 0, f"{tag}: unexpected near-degenerate pair"
+        sig = np.linalg.eigvalsh(s)
+        assert sig.min() > 0, f"{tag}: S not positive definite"
+        cond = sig.max() / sig.min()
+        if cond_s:
+            assert abs(cond / cond_s - 1.0) <= 1e-8, f"{tag}: cond(S) {cond:.6e} != {cond_s:.6e}"
+            want = cond_s ** (np.arange(n) / max(n - 1, 1))
+            assert np.allclose(sig, want, rtol=1e-10), f"{tag}: S spectrum not geometric"
+        else:
+            # The stated band (1e2..1e3) is for the benchmark's order, 54; the
+            # ratio of B B^H's extreme eigenvalues shrinks at small orders.
+            lo = 1e2 if n >= 32 else 1e1
+            assert lo <= cond <= 1e3, f"{tag}: cond(S) {cond:.3e} outside [{lo:g}, 1e3]"
+
+
+@pytest.mark.parametrize("n", [12, 54, 96])
+def test_spectrum_generator_law(n):
+    """The benchmark generator (oracle restatement used by the tests): H = Q
+    diag(lambda) Q^H with lambda in [-10, 10] and 6 pairs split by 1e-8;
+    S = B B^H + 0.05 N I with cond(S) in [1e2, 1e3]; with cond_s, S's spectrum
+    is geometric from 1 to cond_s exactly."""
+    H, S = O.gen_spectrum(n, 6, 0x5EED0000 + n)
+    _spectrum_properties(H, S, n, 0.0, f"oracle n={n}")
+    for c in (1e1, 1e4):
+        H, S = O.gen_spectrum(n, 3, 0x5EED1000 + n, cond_s=c)
+        _spectrum_properties(H, S, n, c, f"oracle n={n} cond {c:g}")
+
+
+@pytest.mark.parametrize("n", [12, 54])
+def test_device_spectrum_generator_law(n):
+    """The device generator the benchmark times on (torch, run here on the
+    CPU device) obeys the same law, including a per-matrix cond(S)."""
+    import torch
+    from sunny_gen_eigensolver_amd import inputs
+    H, S = inputs.spectrum_batch(n, 4, 0x5EED0002, device="cpu")
+    _spectrum_properties(H.numpy(), S.numpy(), n, 0.0, f"device n={n}")
+    conds = torch.tensor([1e1, 3e2, 1e4], dtype=torch.float64)
+    H, S = inputs.spectrum_batch(n, 3, 0x5EED0003, device="cpu", cond_s=conds)
+    for b in range(3):
+        _spectrum_properties(H.numpy()[b:b + 1], S.numpy()[b:b + 1], n, float(conds[b]),
+                             f"device n={n} cond {float(conds[b]):g}")
+
+
+def test_spectrum_property_checks_catch_planted_mistakes():
+    """A dropped pair split, a lambda range off by a factor and a wrong S
+    spectrum each fail the checks above (planted in the oracle's output)."""
+    n = 24
+    H, S = O.gen_spectrum(n, 1, 0x5EED2000)
+    Q = np.linalg.eigh(H[0].T)[1]
+    lam = np.linalg.eigvalsh(H[0].T)
+    bad = [lam.copy(), lam * 1.5]
+    i = int(np.argmin(np.abs(np.diff(lam) - 1e-8)))  # one of the split pairs
+    bad[0][i + 1] = bad[0][i] + 1e-3                 # that pair no longer split by 1e-8
+    for lb in bad:
+        Hb = (Q * lb) @ Q.conj().T
+        Hb = 0.5 * (Hb + Hb.conj().T)
+        with pytest.raises(AssertionError):
+            _spectrum_properties(Hb.T[None], S, n, 0.0, "planted H")
+    Hc, Sc = O.gen_spectrum(n, 1, 0x5EED2001, cond_s=1e3)
+    with pytest.raises(AssertionError):
+        _spectrum_properties(Hc, Sc, n, 1e4, "planted cond")
