@@ -367,8 +367,8 @@ void Solver::launch(const void* H, const void* S, double* W, void* U, int* info,
     // The workspace is shared by every launch of this handle: order this one
     // after the previous one, whatever stream that ran on.
     if (multi) SG_HIP(hipStreamWaitEvent(stream, done_, 0));
+    const Workspace ws = multi ? own_workspace_((size_t)batch) : Workspace{};   // allocates outside capture
     if (graphs_ && multi && !timing_) {
-        reserve((size_t)batch);                    // any allocation happens outside capture
         GraphEntry* hit = nullptr;
         for (auto& e : graph_cache_)
             if (e.H == H && e.S == S && e.W == W && e.U == U && e.info == info && e.batch == batch) hit = &e;
@@ -376,7 +376,7 @@ void Solver::launch(const void* H, const void* S, double* W, void* U, int* info,
             if (!cap_stream_) SG_HIP(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking));
             SG_HIP(hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal));
             try {
-                dispatch_(H, S, W, U, info, batch, cap_stream_);
+                dispatch_(H, S, W, U, info, batch, cap_stream_, ws);
             } catch (...) {
                 hipGraph_t g = nullptr;
                 (void)hipStreamEndCapture(cap_stream_, &g);
@@ -400,7 +400,7 @@ void Solver::launch(const void* H, const void* S, double* W, void* U, int* info,
         SG_HIP(hipGraphLaunch(hit->exec, stream));
     } else {
         mark_(stream);
-        dispatch_(H, S, W, U, info, batch, stream);
+        dispatch_(H, S, W, U, info, batch, stream, ws);
     }
     if (multi) SG_HIP(hipEventRecord(done_, stream));
     if (cur != device_) SG_HIP(hipSetDevice(cur));
@@ -423,11 +423,33 @@ void Solver::launch_chunked(const void* H, const void* S, double* W, void* U, in
 }
 
 void Solver::dispatch_(const void* H, const void* S, double* W, void* U, int* info, int batch,
-                       hipStream_t stream) {
-    if (large_) launch_large_(H, S, W, U, info, batch, stream);
+                       hipStream_t stream, const Workspace& ws) {
+    if (large_) launch_large_(H, S, W, U, info, batch, stream, ws);
     else if (mode_ == Mode::Unified) launch_unified_(H, S, W, U, info, batch, stream);
-    else if (mode_ == Mode::Hybrid) launch_hybrid_(H, S, W, U, info, batch, stream);
-    else launch_mixed_(H, S, W, U, info, batch, stream);
+    else if (mode_ == Mode::Hybrid) launch_hybrid_(H, S, W, U, info, batch, stream, ws);
+    else launch_mixed_(H, S, W, U, info, batch, stream, ws);
+}
+
+Solver::Workspace Solver::own_workspace_(size_t batch) {
+    Workspace ws;
+    if (large_) {
+        ensure_large_workspace_(batch);
+        ws.staged = gbuf_;
+        ws.staged_batch = gbuf_batch_;
+        return ws;
+    }
+    if (mode_ == Mode::Hybrid) {
+        ensure_workspace_(batch);
+        ws.split = lbuf_;
+    } else if (mode_ == Mode::Mixed) {
+        ensure_mixed_workspace_(batch);
+        ws.mixed = mbuf_;
+    }
+    if (mode_ != Mode::Unified && batch >= (size_t)ql_log_min_batch_) {
+        ensure_qlog_(batch);
+        ws.qlog = qlog_;
+    }
+    return ws;
 }
 
 void Solver::clear_graphs_() {
@@ -466,20 +488,18 @@ void Solver::ensure_workspace_(size_t batch) {
 // Split mode: three kernels (front half, QL, back half), each at the
 // occupancy its own footprint allows (reference: hybrid_solver.cpp:295-393).
 void Solver::launch_hybrid_(const void* H, const void* S, double* W, void* U, int* info, int batch,
-                            hipStream_t stream) {
-    ensure_workspace_((size_t)batch);
-    double* E = static_cast<double*>(lbuf_);
+                            hipStream_t stream, const Workspace& ws) {
+    double* E = static_cast<double*>(ws.split);
     double* Zw = E + (size_t)batch * n_;
     void* a1[] = {&H, &S, &W, &U, &info, &E, &batch};
     SG_HIP(hipModuleLaunchKernel(k_reduce_, (unsigned)batch, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream,
                                  a1, nullptr));
     mark_(stream);
     const unsigned grid2 = (unsigned)((batch + bpb_ - 1) / bpb_);
-    ensure_qlog_((size_t)batch);
-    char* qlog = static_cast<char*>(qlog_);
+    char* qlog = static_cast<char*>(ws.qlog);
     long long qstride = (long long)ql_log_bytes(n_);
     int cap = ql_sweep_cap_;
-    int all = batch < ql_log_min_batch_ ? 1 : 0;
+    int all = (batch < ql_log_min_batch_ || !qlog) ? 1 : 0;
     last_ql_log_ = !all;
     if (!all) {
         void* ar[] = {&W, &E, &info, &qlog, &qstride, &batch, &cap};
@@ -520,12 +540,11 @@ void Solver::ensure_mixed_workspace_(size_t batch) {
 // / QL / V = Q Z (the split kernels built in FP32), FP64 refinement of (M, V)
 // in the Cholesky frame and U = W^H V on the matrix cores.
 void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int* info, int batch,
-                           hipStream_t stream) {
-    ensure_mixed_workspace_((size_t)batch);
+                           hipStream_t stream, const Workspace& ws) {
     const size_t n = (size_t)n_, B = (size_t)batch;
     // Offsets for this batch (the buffer holds at least this many matrices).
     const MixedLayout ml = mixed_layout(n, B);
-    char* base = static_cast<char*>(mbuf_);
+    char* base = static_cast<char*>(ws.mixed);
     void* Wd = base + ml.wd;
     void* V32 = base + ml.v32;
     void* W32 = base + ml.w32;
@@ -552,11 +571,10 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
     void* a2[] = {&M32, &W32, &V32, &info, &E32, &batch};
     SG_HIP(hipModuleLaunchKernel(k_mx_front_, grid_b, 1, 1, bd, 1, 1, 0, stream, a2, nullptr));
     mark_(stream);
-    ensure_qlog_((size_t)batch);
-    char* qlog = static_cast<char*>(qlog_);
+    char* qlog = static_cast<char*>(ws.qlog);
     long long qstride = (long long)ql_log_bytes(n_);
     int cap = ql_sweep_cap_;
-    int all = batch < ql_log_min_batch_ ? 1 : 0;
+    int all = (batch < ql_log_min_batch_ || !qlog) ? 1 : 0;
     last_ql_log_ = !all;
     if (!all) {
         void* ar[] = {&W32, &E32, &info, &qlog, &qstride, &batch, &cap};
@@ -637,13 +655,12 @@ void Solver::ensure_large_workspace_(size_t batch) {
 // L^-1, M = W H W^H, tridiagonalisation, QL, V = Q Z and U = W^H V, all in
 // FP64. Mixed: the middle three in FP32, then FP64 refinement of (M, V).
 void Solver::launch_large_(const void* H, const void* S, double* W, void* U, int* info, int batch,
-                           hipStream_t stream) {
-    ensure_large_workspace_((size_t)batch);
+                           hipStream_t stream, const Workspace& ws_) {
     const size_t n = (size_t)n_, nn = n * n;
     const long long wss = (long long)large_matrix_bytes(n_);
     const unsigned bd = 256, bd_ql = (unsigned)(64 * ((n_ + 63) / 64));
-    const int chunk = (int)gbuf_batch_;
-    char* ws = static_cast<char*>(gbuf_);
+    const int chunk = (int)std::min<size_t>(ws_.staged_batch, (size_t)batch);
+    char* ws = static_cast<char*>(ws_.staged);
     for (int b0 = 0; b0 < batch; b0 += chunk) {
         int cnt = std::min(chunk, batch - b0);
         const void* Hc = static_cast<const char*>(H) + (size_t)b0 * nn * 16;
@@ -696,8 +713,10 @@ void Solver::launch_large_(const void* H, const void* S, double* W, void* U, int
 size_t Solver::workspace_bytes(Mode mode, int n_, size_t batch) {
     if (n_ > kMaxLdsOrder) return std::min(batch, large_chunk(n_)) * large_matrix_bytes(n_);
     const size_t n = (size_t)n_;
-    if (mode == Mode::Hybrid) return batch * ((n + n * n) * 8 + ql_log_bytes(n_));
-    if (mode == Mode::Mixed) return mixed_layout(n, batch).total + batch * ql_log_bytes(n_);
+    // The QL rotation logs only for batches that run QL through them.
+    const size_t qlog = batch >= (size_t)kQlLogMinBatch ? batch * ql_log_bytes(n_) : 0;
+    if (mode == Mode::Hybrid) return batch * (n + n * n) * 8 + qlog;
+    if (mode == Mode::Mixed) return mixed_layout(n, batch).total + qlog;
     return 0;
 }
 
@@ -708,7 +727,7 @@ size_t Solver::reserve(size_t batch) {
     if (large_) ensure_large_workspace_(batch);
     else if (mode_ == Mode::Hybrid) ensure_workspace_(batch);
     else if (mode_ == Mode::Mixed) ensure_mixed_workspace_(batch);
-    if (!large_ && mode_ != Mode::Unified) ensure_qlog_(batch);
+    if (!large_ && mode_ != Mode::Unified && batch >= (size_t)ql_log_min_batch_) ensure_qlog_(batch);
     if (cur != device_) SG_HIP(hipSetDevice(cur));
     return workspace_bytes(mode_, n_, batch);
 }

@@ -53,16 +53,29 @@ public:
     static int suggested_block_dim(int n);
 
 private:
+    // Device workspace of one launch: the split mode's off-diagonals and Z,
+    // the mixed mode's hand-offs, the QL rotation logs (null: QL without the
+    // log) and the staged kernels' per-matrix areas with the matrices one
+    // round may hold.
+    struct Workspace {
+        void* split = nullptr;
+        void* mixed = nullptr;
+        void* qlog = nullptr;
+        void* staged = nullptr;
+        size_t staged_batch = 0;
+    };
+    Workspace own_workspace_(size_t batch);   // grows the handle's workspace as needed
+
     void bind_module_(int block_dim, CacheMode cache);
     int autotune_(CacheMode cache);
     void launch_unified_(const void* H, const void* S, double* W, void* U, int* info, int batch,
                          hipStream_t stream);
     void launch_hybrid_(const void* H, const void* S, double* W, void* U, int* info, int batch,
-                        hipStream_t stream);
+                        hipStream_t stream, const Workspace& ws);
     void launch_mixed_(const void* H, const void* S, double* W, void* U, int* info, int batch,
-                       hipStream_t stream);
+                       hipStream_t stream, const Workspace& ws);
     void launch_large_(const void* H, const void* S, double* W, void* U, int* info, int batch,
-                       hipStream_t stream);
+                       hipStream_t stream, const Workspace& ws);
     void ensure_workspace_(size_t batch);
     void ensure_mixed_workspace_(size_t batch);
     void ensure_large_workspace_(size_t batch);
@@ -145,7 +158,8 @@ private:
     std::vector<GraphEntry> graph_cache_;
     long long graph_captures_ = 0, graph_replays_ = 0;
     void clear_graphs_();
-    void dispatch_(const void* H, const void* S, double* W, void* U, int* info, int batch, hipStream_t stream);
+    void dispatch_(const void* H, const void* S, double* W, void* U, int* info, int batch, hipStream_t stream,
+                   const Workspace& ws);
 
     // Recorded after every launch of a mode with a workspace; the next launch
     // waits for it on its own stream.
