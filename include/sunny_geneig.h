@@ -148,6 +148,30 @@ sunny_geneig_status_t sunny_geneig_launch_chunked(
     const uintptr_t*      streams,
     int                   num_streams);
 
+/* No-allocation launch. No counterpart in the reference ABI: it is for
+ * callers that cannot allocate device memory (code under hipGraph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same arguments,
+ * results and error behaviour as sunny_geneig_launch, but the split, mixed
+ * and staged-order workspace is carved from a device array of
+ * SUNNY_GENEIG_NOALLOC_SCRATCH_BYTES that the library holds on each GPU it
+ * runs on (set up when a handle is created there) until the process ends; the
+ * call allocates nothing and captures no graph. A batch whose workspace does
+ * not fit returns INVALID_ARG with the limit in the message (at N = 54: about
+ * 1,000 matrices per call in the mixed mode, 5,600 in the split mode; the
+ * fused mode needs no workspace; orders above 64 run in rounds that fit).
+ * Calls on one GPU share its array and are serialised on the device through
+ * an event, whatever stream or handle they use. */
+#define SUNNY_GENEIG_NOALLOC_SCRATCH_BYTES ((uint64_t)128 << 20)
+sunny_geneig_status_t sunny_geneig_launch_noalloc(
+    sunny_geneig_solver_t solver,
+    uintptr_t             d_H,
+    uintptr_t             d_S,
+    uintptr_t             d_W,
+    uintptr_t             d_U,
+    uintptr_t             d_info,
+    int                   batch_size,
+    uintptr_t             stream);
+
 /* Compile (hipRTC) and store in the on-disk cache the code objects for order n
  * with `block_dim_x` threads (0 = suggested) for `arch` (e.g. "gfx950"): the
  * FP64 module and the FP32 module of the mixed mode, without touching a device. Returns 0 or a status. Lets a build step warm the

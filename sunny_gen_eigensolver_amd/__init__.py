@@ -52,7 +52,7 @@ EXPORTED_SYMBOLS = (
     "sunny_geneig_session_create", "sunny_geneig_session_destroy", "sunny_geneig_session_submit",
     "sunny_geneig_session_poll", "sunny_geneig_session_wait", "sunny_geneig_session_stats",
     "sunny_geneig_session_request_bytes", "sunny_geneig_launch_chunked",
-    "sunny_geneig_session_set_policy",
+    "sunny_geneig_session_set_policy", "sunny_geneig_launch_noalloc",
 )
 
 _lib = None
@@ -86,6 +86,8 @@ def lib() -> ctypes.CDLL:
     L.sunny_geneig_destroy.restype = None
     L.sunny_geneig_launch.argtypes = [vp, cu, cu, cu, cu, cu, ci, cu]
     L.sunny_geneig_launch.restype = ci
+    L.sunny_geneig_launch_noalloc.argtypes = [vp, cu, cu, cu, cu, cu, ci, cu]
+    L.sunny_geneig_launch_noalloc.restype = ci
     L.sunny_geneig_launch_chunked.argtypes = [vp, cu, cu, cu, cu, cu, ci, ci, ctypes.POINTER(cu), ci]
     L.sunny_geneig_launch_chunked.restype = ci
     for name in ("sunny_geneig_matrix_size", "sunny_geneig_batches_per_block",

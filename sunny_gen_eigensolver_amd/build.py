@@ -68,12 +68,18 @@ def embed_kernel_source() -> str:
 def build_library() -> str:
     embed_kernel_source()
     os.makedirs(LIBDIR, exist_ok=True)
+    os.makedirs(BUILD, exist_ok=True)
     srcs = [os.path.join(CSRC, f) for f in ("codeobj.cpp", "solver.cpp", "varlen.cpp", "serving.cpp", "c_api.cpp")]
+    inc = [f"-I{os.path.join(REPO, 'include')}", f"-I{CSRC}"]
+    # The no-allocation launch holds its scratch in a file-scope device array,
+    # so its translation unit is compiled by hipcc for gfx950.
+    noalloc_o = os.path.join(BUILD, "noalloc.o")
+    _run(["hipcc", "--offload-arch=gfx950", "-O2", "-std=c++17", "-fPIC", *HIP_FLAGS, *inc, "-c",
+          os.path.join(CSRC, "noalloc.hip"), "-o", noalloc_o])
     tmp = LIB + ".tmp"
     _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-pthread",
-          "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", f"-I{os.path.join(REPO, 'include')}",
-          f"-I{CSRC}", *srcs, "-o", tmp, f"-L{ROCM}/lib", "-lamdhip64", "-lhiprtc",
-          f"-Wl,-rpath,{ROCM}/lib"])
+          "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", *inc, *srcs, noalloc_o, "-o", tmp,
+          f"-L{ROCM}/lib", "-lamdhip64", "-lhiprtc", f"-Wl,-rpath,{ROCM}/lib"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -1,30 +1,13 @@
 // C ABI of libsunny_geneig.so (declared in include/sunny_geneig.h). Wraps the
 // HIP solver in an opaque handle and turns C++ exceptions into status codes
 // with a thread-local message, as the reference's src/common/c_api.cpp does.
-#include "sunny_geneig.h"
-
-#include "solver.hpp"
-#include "serving.hpp"
-#include "varlen.hpp"
+#include "handles.hpp"
 
 #include <exception>
 #include <memory>
 #include <new>
 #include <string>
 #include <vector>
-
-struct sunny_geneig_solver {
-    std::unique_ptr<sgeig::Solver> impl;
-    sunny_geneig_kernel_mode_t mode = SUNNY_GENEIG_KERNEL_UNIFIED;
-};
-
-struct sunny_geneig_varlen_solver {
-    std::unique_ptr<sgeig::VarlenSolver> impl;
-};
-
-struct sunny_geneig_session {
-    std::unique_ptr<sgeig::ServingSession> impl;
-};
 
 namespace {
 
@@ -56,6 +39,9 @@ sunny_geneig_solver_t create_impl(int n, int device_id, int bpb, sunny_geneig_tu
             mode == SUNNY_GENEIG_KERNEL_HYBRID  ? sgeig::Mode::Hybrid
             : mode == SUNNY_GENEIG_KERNEL_MIXED ? sgeig::Mode::Mixed
                                                 : sgeig::Mode::Unified);
+        // Per-device set-up of the no-allocation launch's scratch (once per
+        // device, with that device current).
+        sgeig::noalloc_setup(device_id);
         return h.release();
     } catch (const std::bad_alloc&) {
         set_error(std::string("out of memory in ") + fn);
@@ -68,6 +54,11 @@ sunny_geneig_solver_t create_impl(int n, int device_id, int bpb, sunny_geneig_tu
 }
 
 }  // namespace
+
+namespace sgeig {
+void api_set_error(const std::string& m) { set_error(m); }
+void api_clear_error() { clear_error(); }
+}  // namespace sgeig
 
 extern "C" {
 

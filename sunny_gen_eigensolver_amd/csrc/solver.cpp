@@ -452,6 +452,48 @@ Solver::Workspace Solver::own_workspace_(size_t batch) {
     return ws;
 }
 
+size_t Solver::scratch_bytes_needed(Mode mode, int n, size_t batch) {
+    if (n > kMaxLdsOrder) return large_matrix_bytes(n);
+    const size_t nn = (size_t)n;
+    if (mode == Mode::Hybrid) return (batch * (nn + nn * nn) * 8 + 255) / 256 * 256;
+    if (mode == Mode::Mixed) return mixed_layout(nn, batch).total;
+    return 0;
+}
+
+void Solver::launch_in_scratch(const void* H, const void* S, double* W, void* U, int* info, int batch,
+                               hipStream_t stream, void* scratch, size_t scratch_bytes) {
+    if (batch < 0) throw std::invalid_argument("batch_size < 0");
+    if (batch == 0) return;
+    const size_t need = scratch_bytes_needed(mode_, n_, (size_t)batch);
+    if (need > scratch_bytes) {
+        const size_t per = scratch_bytes_needed(mode_, n_, 1);
+        throw std::invalid_argument("batch of " + std::to_string(batch) + " matrices of order " +
+                                    std::to_string(n_) + " needs " + std::to_string(need) +
+                                    " B of scratch, more than the " + std::to_string(scratch_bytes) +
+                                    " B available (at most about " + std::to_string(per ? scratch_bytes / per : 0) +
+                                    " matrices per call in this mode)");
+    }
+    int cur = 0;
+    SG_HIP(hipGetDevice(&cur));
+    if (cur != device_) SG_HIP(hipSetDevice(device_));
+    Workspace ws;
+    if (large_) {
+        ws.staged = scratch;
+        ws.staged_batch = scratch_bytes / large_matrix_bytes(n_);
+    } else if (mode_ == Mode::Hybrid) {
+        ws.split = scratch;
+    } else if (mode_ == Mode::Mixed) {
+        ws.mixed = scratch;
+    }
+    const bool multi = large_ || mode_ != Mode::Unified;
+    ev_used_ = 0;
+    if (multi) SG_HIP(hipStreamWaitEvent(stream, done_, 0));
+    mark_(stream);
+    dispatch_(H, S, W, U, info, batch, stream, ws);
+    if (multi) SG_HIP(hipEventRecord(done_, stream));
+    if (cur != device_) SG_HIP(hipSetDevice(cur));
+}
+
 void Solver::clear_graphs_() {
     for (auto& e : graph_cache_) (void)hipGraphExecDestroy(e.exec);
     graph_cache_.clear();

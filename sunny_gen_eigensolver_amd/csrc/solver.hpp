@@ -39,6 +39,7 @@ public:
                         int chunk, const std::vector<hipStream_t>& streams);
 
     int n() const { return n_; }
+    int device() const { return device_; }
     // Matrices per workgroup of the fused kernel; the split and mixed modes
     // report 1, as the reference's split mode does (c_api.cpp:41-49).
     int batches_per_block() const { return (mode_ == Mode::Unified) ? bpb_ : 1; }
@@ -51,6 +52,17 @@ public:
 
     // Suggested workgroup size for order n (a model, no measurement).
     static int suggested_block_dim(int n);
+
+    // Same results as launch(), with the mode's device workspace carved from
+    // the caller's `scratch` (`scratch_bytes` long, 256-byte aligned) instead
+    // of the handle's own: no allocation, no graph capture, QL without the
+    // rotation log. Throws std::invalid_argument when the batch needs more
+    // than scratch_bytes (the staged orders run in rounds that fit instead).
+    void launch_in_scratch(const void* H, const void* S, double* W, void* U, int* info, int batch,
+                           hipStream_t stream, void* scratch, size_t scratch_bytes);
+    // Scratch bytes launch_in_scratch needs for `batch` matrices (for the
+    // staged orders: one matrix's area, the least a round needs).
+    static size_t scratch_bytes_needed(Mode mode, int n, size_t batch);
 
 private:
     // Device workspace of one launch: the split mode's off-diagonals and Z,
