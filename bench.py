@@ -229,6 +229,8 @@ def main():
     ap.add_argument("--no-uneven", action="store_true",
                     help="skip the uneven batch (200000 pencils, N log-uniform over 12..128)")
     ap.add_argument("--uneven-count", type=int, default=200000)
+    ap.add_argument("--no-sunny", action="store_true",
+                    help="skip the H = Sigma (diagonal) measurement")
     ap.add_argument("--no-large", action="store_true",
                     help="skip the staged-kernel orders (N=96 batch 16384, N=192 batch 2048)")
     ap.add_argument("--serve-requests", type=int, default=1000)
@@ -394,6 +396,54 @@ def main():
         out["mixed_vs_fp64_fused"] = value / modes["unified"]["solves_per_s"]
         out["mixed_vs_fp64_split"] = value / modes["hybrid"]["solves_per_s"]
     mixed.close()
+
+    # Sunny's call pattern (H = Sigma = diag(I, -I), the paraunitary metric;
+    # S positive definite) on the same S: the mixed mode through
+    # sunny_geneig_launch with Sigma written out in full, then through
+    # sunny_geneig_launch_hdiag with its diagonal (M formed without H).
+    if not a.no_sunny:
+        sig = torch.cat([torch.ones(n // 2), -torch.ones(n - n // 2)]).to(torch.float64).to(dev)
+        Hs = torch.diag(sig).to(torch.complex128).expand(B, n, n).contiguous()
+        ms_d = {}
+        with sg.Solver(n, kernel="mixed") as s3:
+            for form in ("full", "diagonal"):
+                def go():
+                    if form == "full":
+                        sg.run_kernel(s3, Hs, S, W, U, info, stream=stream)
+                    else:
+                        sg.run_kernel_hdiag(s3, sig, S, W, U, info, stream=stream)
+                for _ in range(a.warmup):
+                    go()
+                torch.cuda.synchronize()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(a.steps):
+                    go()
+                e1.record(stream)
+                torch.cuda.synchronize()
+                ms_d[form] = e0.elapsed_time(e1) / a.steps
+                res, orth = inputs.quality(Hs, S, W, U)
+                ms_d[form + "_acc"] = {"max_residual_over_N_eps": res.max().item() / (n * EPS),
+                                       "max_s_orth_over_N_eps": orth.max().item() / (n * EPS),
+                                       "info_nonzero": int((info != 0).sum().item())}
+                s3.set_stage_timing(True)
+                go()
+                ms_d[form + "_stages"] = dict(s3.stage_times())
+                s3.set_stage_timing(False)
+        out["sunny_sigma"] = {
+            "workload": f"mixed mode, H = Sigma = diag(I_{n // 2}, -I_{n - n // 2}), S = the benchmark's S, "
+                        f"N={n}, batch {B}",
+            "full_h_ms_per_step": ms_d["full"], "diagonal_h_ms_per_step": ms_d["diagonal"],
+            "full_h_solves_per_s": B / (ms_d["full"] * 1e-3),
+            "diagonal_h_solves_per_s": B / (ms_d["diagonal"] * 1e-3),
+            "speedup": ms_d["full"] / ms_d["diagonal"],
+            "accuracy": {"full": ms_d["full_acc"], "diagonal": ms_d["diagonal_acc"]},
+            "stages_ms": {"full": ms_d["full_stages"], "diagonal": ms_d["diagonal_stages"]}}
+        log(f"sunny Sigma/S: full H {ms_d['full']:.2f} ms, diagonal H {ms_d['diagonal']:.2f} ms "
+            f"({ms_d['full'] / ms_d['diagonal']:.2f}x); accuracy {ms_d['diagonal_acc']}")
+        del Hs
+        torch.cuda.empty_cache()
 
     # Mixed precision at the staged orders (FP64 Cholesky / products /
     # refinement around FP32 tridiagonalisation and eigenvectors) beside the

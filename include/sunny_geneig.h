@@ -148,6 +148,27 @@ sunny_geneig_status_t sunny_geneig_launch_chunked(
     const uintptr_t*      streams,
     int                   num_streams);
 
+/* Launch for a real diagonal H (no counterpart in the reference ABI): the
+ * Sunny call pattern passes H = Sigma = diag(I, -I), the paraunitary metric,
+ * with S = the dynamical matrix (julia/SunnyGeneigBench/scripts/
+ * setup_sw08.jl:6-18). d_Hdiag is float64: matrix b's diagonal is the n
+ * values at d_Hdiag + b * hdiag_stride (hdiag_stride 0 = one diagonal for
+ * the whole batch). Results and errors as sunny_geneig_launch on the full H.
+ * In the MIXED mode at orders up to 64, M = L^-1 H L^-H is formed from the
+ * diagonal directly (one signed Gram product, H never materialised); the
+ * other paths write the full H into a handle workspace first.
+ * hdiag_stride < 0 or a NULL diagonal with batch_size > 0 -> INVALID_ARG. */
+sunny_geneig_status_t sunny_geneig_launch_hdiag(
+    sunny_geneig_solver_t solver,
+    uintptr_t             d_Hdiag,
+    int64_t               hdiag_stride,
+    uintptr_t             d_S,
+    uintptr_t             d_W,
+    uintptr_t             d_U,
+    uintptr_t             d_info,
+    int                   batch_size,
+    uintptr_t             stream);
+
 /* No-allocation launch. No counterpart in the reference ABI: it is for
  * callers that cannot allocate device memory (code under hipGraph capture,
  * tight serving loops, harnesses that forbid hipMalloc). Same arguments,

@@ -52,7 +52,7 @@ EXPORTED_SYMBOLS = (
     "sunny_geneig_session_create", "sunny_geneig_session_destroy", "sunny_geneig_session_submit",
     "sunny_geneig_session_poll", "sunny_geneig_session_wait", "sunny_geneig_session_stats",
     "sunny_geneig_session_request_bytes", "sunny_geneig_launch_chunked",
-    "sunny_geneig_session_set_policy", "sunny_geneig_launch_noalloc",
+    "sunny_geneig_session_set_policy", "sunny_geneig_launch_noalloc", "sunny_geneig_launch_hdiag",
 )
 
 _lib = None
@@ -86,6 +86,8 @@ def lib() -> ctypes.CDLL:
     L.sunny_geneig_destroy.restype = None
     L.sunny_geneig_launch.argtypes = [vp, cu, cu, cu, cu, cu, ci, cu]
     L.sunny_geneig_launch.restype = ci
+    L.sunny_geneig_launch_hdiag.argtypes = [vp, cu, ctypes.c_int64, cu, cu, cu, cu, ci, cu]
+    L.sunny_geneig_launch_hdiag.restype = ci
     L.sunny_geneig_launch_noalloc.argtypes = [vp, cu, cu, cu, cu, cu, ci, cu]
     L.sunny_geneig_launch_noalloc.restype = ci
     L.sunny_geneig_launch_chunked.argtypes = [vp, cu, cu, cu, cu, cu, ci, ci, ctypes.POINTER(cu), ci]
@@ -322,6 +324,31 @@ def run_kernel(solver: Solver, H, S, W=None, U=None, info=None, stream=None):
     info = torch.zeros((batch,), dtype=torch.int32, device=dev) if info is None else info
     solver.launch_raw(H.data_ptr(), S.data_ptr(), W.data_ptr(), U.data_ptr(), info.data_ptr(),
                       batch, _stream_handle(stream))
+    return W, U, info
+
+
+def run_kernel_hdiag(solver: Solver, hdiag, S, W=None, U=None, info=None, stream=None):
+    """run_kernel for a real diagonal H (Sunny's H = Sigma = diag(I, -I)):
+    hdiag is float64 on S's device, shape (n,) (one diagonal for the batch) or
+    (B, n) (one per matrix). Returns (W, U, info) as run_kernel."""
+    import torch
+    batch, dev = _check_batch(solver, S, S, W, U, info)
+    n = solver.n
+    if hdiag.dtype != torch.float64 or hdiag.device != dev or not hdiag.is_contiguous():
+        raise ValueError("hdiag must be a contiguous float64 tensor on S's device")
+    if tuple(hdiag.shape) == (n,):
+        stride = 0
+    elif tuple(hdiag.shape) == (batch, n):
+        stride = n
+    else:
+        raise ValueError(f"hdiag must have shape ({n},) or ({batch}, {n}), got {tuple(hdiag.shape)}")
+    W = torch.empty((batch, n), dtype=torch.float64, device=dev) if W is None else W
+    U = torch.empty((batch, n, n), dtype=torch.complex128, device=dev) if U is None else U
+    info = torch.zeros((batch,), dtype=torch.int32, device=dev) if info is None else info
+    st = lib().sunny_geneig_launch_hdiag(solver.handle, hdiag.data_ptr(), stride, S.data_ptr(), W.data_ptr(),
+                                         U.data_ptr(), info.data_ptr(), batch, _stream_handle(stream))
+    if st != STATUS_OK:
+        raise GeneigError(st, last_error())
     return W, U, info
 
 

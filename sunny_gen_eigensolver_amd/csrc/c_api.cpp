@@ -124,6 +124,37 @@ sunny_geneig_status_t sunny_geneig_launch(sunny_geneig_solver_t solver, uintptr_
     return SUNNY_GENEIG_LAUNCH_ERROR;
 }
 
+sunny_geneig_status_t sunny_geneig_launch_hdiag(sunny_geneig_solver_t solver, uintptr_t d_Hdiag,
+                                                int64_t hdiag_stride, uintptr_t d_S, uintptr_t d_W,
+                                                uintptr_t d_U, uintptr_t d_info, int batch_size,
+                                                uintptr_t stream) {
+    clear_error();
+    if (!solver) {
+        set_error("sunny_geneig_launch_hdiag: solver is NULL");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    if (batch_size < 0 || hdiag_stride < 0) {
+        set_error("sunny_geneig_launch_hdiag: batch_size and hdiag_stride must be >= 0");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    if (batch_size > 0 && (!d_Hdiag || !d_S || !d_W || !d_U || !d_info)) {
+        set_error("sunny_geneig_launch_hdiag: NULL device buffer");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    try {
+        solver->impl->launch_hdiag(reinterpret_cast<const double*>(d_Hdiag), (long long)hdiag_stride,
+                                   reinterpret_cast<const void*>(d_S), reinterpret_cast<double*>(d_W),
+                                   reinterpret_cast<void*>(d_U), reinterpret_cast<int*>(d_info), batch_size,
+                                   reinterpret_cast<hipStream_t>(stream));
+        return SUNNY_GENEIG_OK;
+    } catch (const std::exception& e) {
+        set_error(std::string("sunny_geneig_launch_hdiag: ") + e.what());
+    } catch (...) {
+        set_error("unknown C++ exception in sunny_geneig_launch_hdiag");
+    }
+    return SUNNY_GENEIG_LAUNCH_ERROR;
+}
+
 sunny_geneig_status_t sunny_geneig_launch_chunked(sunny_geneig_solver_t solver, uintptr_t d_H,
                                                   uintptr_t d_S, uintptr_t d_W, uintptr_t d_U,
                                                   uintptr_t d_info, int total_matrices, int chunk_size,
@@ -238,7 +269,7 @@ int sunny_geneig_stage_times(sunny_geneig_solver_t s, double* ms, const char** n
         for (int i = 0; i < n && i < max_stages; ++i) {
             if (ms) ms[i] = tmp[i];
             if (names)
-                names[i] = sgeig::Solver::stage_name(s->impl->mode(), i, s->impl->n(), s->impl->last_ql_log());
+                names[i] = s->impl->stage_label(i);
         }
         return n;
     } catch (const std::exception& e) {

@@ -2471,6 +2471,55 @@ geneig_mx_formm(const cx* __restrict__ H, const cx* __restrict__ Wd, const int* 
     }
 }
 
+// M = W diag(h) W^H for a diagonal H (Sunny's paraunitary metric H = Sigma =
+// diag(I, -I), or any real diagonal): one product on the matrix cores,
+// M(i, j) = sum_{k <= min(i, j)} W(i, k) h_k conj(W(j, k)), with the right
+// operand formed from W's packed lower triangle and h as it is read; H itself
+// is never loaded. h is row b * hd_stride of Hd (stride 0: one diagonal for
+// the whole batch). Outputs as geneig_mx_formm.
+extern "C" __global__ void __launch_bounds__(gk::REF_NT, 2)
+geneig_mx_formm_diag(const real* __restrict__ Hd, long long hd_stride, const cx* __restrict__ Wd,
+                     const int* __restrict__ info, cx* __restrict__ Mp, gk::cf* __restrict__ M32, int batch) {
+    using namespace gk;
+    const int b = blockIdx.x;
+    if (b >= batch || info[b] != 0) return;
+    const int l = lane_id(), cb = wave_id();
+    const int g4 = l >> 4;
+    const int j = cb * 16 + (l & 15);          // this lane's column
+    const bool jin = j < N;
+    const size_t off = (size_t)b * N * N;
+    __shared__ cx Ws[NPK];
+    __shared__ real hs[KEND];
+    stage_lower_packed(Ws, Wd + off, threadIdx.x, REF_NT);
+    if (threadIdx.x < KEND) hs[threadIdx.x] = threadIdx.x < N ? Hd[(size_t)b * hd_stride + threadIdx.x] : real(0.0);
+    __syncthreads();
+    // Column block cb needs k <= j < 16 (cb + 1).
+    const int kz = 16 * (cb + 1) < KEND ? 16 * (cb + 1) : KEND;
+    ctile M[NB];
+    cgemm(M, [&](int i, int k) { return lelem(Ws, i, k); },
+          [&](int k, int jj) { return cconj(lelem(Ws, jj, k)) * hs[k]; }, cb, kz);
+    cx* Mb = Mp + (size_t)b * NPK;
+    cf* M32b = M32 + off;
+#pragma unroll
+    for (int rt = 0; rt < NB; ++rt) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = rt * 16 + g4 + 4 * v;
+            if (i < N && jin && i >= j) {
+                const cx mv = tile_get(M[rt], v);
+                Mb[pk(j) + i - j] = mv;
+                cf lo, up;
+                lo.x = (float)mv.x;
+                lo.y = (i == j) ? 0.0f : (float)mv.y;
+                up.x = lo.x;
+                up.y = -lo.y;
+                M32b[(size_t)j * N + i] = lo;
+                if (i != j) M32b[(size_t)i * N + j] = up;
+            }
+        }
+    }
+}
+
 // Refinement in the Cholesky frame: with W = L^-1 (FP64), M = W H W^H and the
 // FP32 eigenvectors V of M, iters Ogita-Aishima steps on the matrix cores
 // (T = V^H M V, G = V^H V, lambda_j = T_jj / G_jj, V <- V + V E with E as
@@ -2627,3 +2676,20 @@ geneig_mx_refine(const cx* __restrict__ Mp, const cx* __restrict__ Wd, real* __r
 #endif  // !GENEIG_FP32
 
 #endif  // GENEIG_N <= 64
+
+#ifndef GENEIG_FP32
+// A diagonal H written out in full (complex128, column-major) from its real
+// diagonal, row b * hd_stride of Hd, for the paths that read H itself.
+extern "C" __global__ void __launch_bounds__(256)
+geneig_expand_hdiag(const double* __restrict__ Hd, long long hd_stride, gk::cx* __restrict__ H, int batch) {
+    using namespace gk;
+    const int b = blockIdx.x;
+    if (b >= batch) return;
+    cx* Hb = H + (size_t)b * N * N;
+    const double* h = Hd + (size_t)b * hd_stride;
+    for (int e = threadIdx.x; e < N * N; e += 256) {
+        const int i = e % N, j = e / N;
+        Hb[e] = cm(i == j ? h[i] : 0.0, 0.0);
+    }
+}
+#endif  // !GENEIG_FP32

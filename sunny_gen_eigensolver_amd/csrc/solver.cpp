@@ -162,7 +162,7 @@ Solver::~Solver() {
     }
     clear_graphs_();
     if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
-    if (lbuf_ || mbuf_ || gbuf_ || qlog_) {
+    if (lbuf_ || mbuf_ || gbuf_ || qlog_ || hbuf_) {
         int cur = 0;
         (void)hipGetDevice(&cur);
         (void)hipSetDevice(device_);
@@ -170,6 +170,7 @@ Solver::~Solver() {
         if (mbuf_) (void)hipFree(mbuf_);
         if (gbuf_) (void)hipFree(gbuf_);
         if (qlog_) (void)hipFree(qlog_);
+        if (hbuf_) (void)hipFree(hbuf_);
         (void)hipSetDevice(cur);
     }
 }
@@ -179,6 +180,7 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
     module_ = acquire_module(key, cache, &layer_, &acquire_ms_);
     block_dim_ = block_dim;
     bpb_ = block_dim / 64;
+    k_expand_ = module_->get("geneig_expand_hdiag");
     if (large_) {
         bpb_ = 1;
         k_lg_chol_ = module_->get("geneig_lg_chol");
@@ -241,6 +243,7 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
         k_mx_chol_ = module_->get("geneig_lg_chol");
         k_refine_ = module_->get("geneig_mx_refine");
         k_mx_formm_ = module_->get("geneig_mx_formm");
+        k_mx_formm_diag_ = module_->get("geneig_mx_formm_diag");
         ModuleKey k32 = key;
         k32.fp32 = true;
         CacheLayer l32;
@@ -357,21 +360,43 @@ int Solver::autotune_(CacheMode cache) {
 
 void Solver::launch(const void* H, const void* S, double* W, void* U, int* info, int batch,
                     hipStream_t stream) {
+    launch_impl_(H, nullptr, 0, S, W, U, info, batch, stream);
+}
+
+void Solver::launch_hdiag(const double* Hd, long long hdiag_stride, const void* S, double* W, void* U, int* info,
+                          int batch, hipStream_t stream) {
+    if (hdiag_stride < 0) throw std::invalid_argument("hdiag_stride < 0");
+    if (batch > 0 && !Hd) throw std::invalid_argument("diagonal of H is NULL");
+    launch_impl_(nullptr, Hd, hdiag_stride, S, W, U, info, batch, stream);
+}
+
+void Solver::launch_impl_(const void* H, const double* Hd, long long hds, const void* S, double* W, void* U,
+                          int* info, int batch, hipStream_t stream) {
     if (batch < 0) throw std::invalid_argument("batch_size < 0");
     if (batch == 0) return;
     int cur = 0;
     SG_HIP(hipGetDevice(&cur));
     if (cur != device_) SG_HIP(hipSetDevice(device_));
     ev_used_ = 0;
-    const bool multi = large_ || mode_ != Mode::Unified;
+    const bool multi = large_ || mode_ != Mode::Unified || Hd;
     // The workspace is shared by every launch of this handle: order this one
     // after the previous one, whatever stream that ran on.
     if (multi) SG_HIP(hipStreamWaitEvent(stream, done_, 0));
-    const Workspace ws = multi ? own_workspace_((size_t)batch) : Workspace{};   // allocates outside capture
+    Workspace ws = multi ? own_workspace_((size_t)batch) : Workspace{};   // allocates outside capture
+    if (Hd) {
+        ws.hdiag = Hd;
+        ws.hdiag_stride = hds;
+        if (!hdiag_direct_()) {
+            ensure_hbuf_((size_t)batch);
+            ws.hfull = hbuf_;
+        }
+    }
     if (graphs_ && multi && !timing_) {
         GraphEntry* hit = nullptr;
         for (auto& e : graph_cache_)
-            if (e.H == H && e.S == S && e.W == W && e.U == U && e.info == info && e.batch == batch) hit = &e;
+            if (e.H == H && e.hd == Hd && e.hds == hds && e.S == S && e.W == W && e.U == U && e.info == info &&
+                e.batch == batch)
+                hit = &e;
         if (!hit) {
             if (!cap_stream_) SG_HIP(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking));
             SG_HIP(hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal));
@@ -391,7 +416,7 @@ void Solver::launch(const void* H, const void* S, double* W, void* U, int* info,
             (void)hipGraphDestroy(g);
             SG_HIP(ie);
             if (graph_cache_.size() >= 8) clear_graphs_();
-            graph_cache_.push_back({H, S, W, U, info, batch, ex});
+            graph_cache_.push_back({H, Hd, hds, S, W, U, info, batch, ex});
             hit = &graph_cache_.back();
             ++graph_captures_;
         } else {
@@ -424,6 +449,16 @@ void Solver::launch_chunked(const void* H, const void* S, double* W, void* U, in
 
 void Solver::dispatch_(const void* H, const void* S, double* W, void* U, int* info, int batch,
                        hipStream_t stream, const Workspace& ws) {
+    if (ws.hdiag && ws.hfull) {
+        // Paths that read H: write the diagonal out in full first.
+        const double* hd = ws.hdiag;
+        long long hds = ws.hdiag_stride;
+        void* hf = ws.hfull;
+        void* ae[] = {&hd, &hds, &hf, &batch};
+        launch_(k_expand_, (unsigned)batch, 256, stream, ae);
+        mark_(stream);
+        H = hf;
+    }
     if (large_) launch_large_(H, S, W, U, info, batch, stream, ws);
     else if (mode_ == Mode::Unified) launch_unified_(H, S, W, U, info, batch, stream);
     else if (mode_ == Mode::Hybrid) launch_hybrid_(H, S, W, U, info, batch, stream, ws);
@@ -508,8 +543,7 @@ void Solver::launch_unified_(const void* H, const void* S, double* W, void* U, i
                              hipStream_t stream) {
     void* args[] = {&H, &S, &W, &U, &info, &batch};
     const unsigned grid = (unsigned)((batch + bpb_ - 1) / bpb_);
-    SG_HIP(hipModuleLaunchKernel(fused_, grid, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream, args,
-                                 nullptr));
+    launch_(fused_, grid, (unsigned)block_dim_, stream, args);
     mark_(stream);
 }
 
@@ -534,8 +568,7 @@ void Solver::launch_hybrid_(const void* H, const void* S, double* W, void* U, in
     double* E = static_cast<double*>(ws.split);
     double* Zw = E + (size_t)batch * n_;
     void* a1[] = {&H, &S, &W, &U, &info, &E, &batch};
-    SG_HIP(hipModuleLaunchKernel(k_reduce_, (unsigned)batch, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream,
-                                 a1, nullptr));
+    launch_(k_reduce_, (unsigned)batch, (unsigned)block_dim_, stream, a1);
     mark_(stream);
     const unsigned grid2 = (unsigned)((batch + bpb_ - 1) / bpb_);
     char* qlog = static_cast<char*>(ws.qlog);
@@ -545,21 +578,17 @@ void Solver::launch_hybrid_(const void* H, const void* S, double* W, void* U, in
     last_ql_log_ = !all;
     if (!all) {
         void* ar[] = {&W, &E, &info, &qlog, &qstride, &batch, &cap};
-        SG_HIP(hipModuleLaunchKernel(k_qlrec_, (unsigned)((batch + 63) / 64), 1, 1, 64, 1, 1, 0, stream, ar,
-                                     nullptr));
+        launch_(k_qlrec_, (unsigned)((batch + 63) / 64), 64, stream, ar);
         mark_(stream);
         void* a2[] = {&W, &E, &Zw, &info, &qlog, &qstride, &batch};
-        SG_HIP(hipModuleLaunchKernel(k_heev_, grid2, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream, a2,
-                                     nullptr));
+        launch_(k_heev_, grid2, (unsigned)block_dim_, stream, a2);
         mark_(stream);
     }
     void* af[] = {&W, &E, &Zw, &info, &qlog, &qstride, &batch, &all};
-    SG_HIP(hipModuleLaunchKernel(k_qlfull_, grid2, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream, af,
-                                 nullptr));
+    launch_(k_qlfull_, grid2, (unsigned)block_dim_, stream, af);
     mark_(stream);
     void* a3[] = {&U, &Zw, &info, &batch};
-    SG_HIP(hipModuleLaunchKernel(k_back_, grid2, 1, 1, (unsigned)block_dim_, 1, 1, 0, stream, a3,
-                                 nullptr));
+    launch_(k_back_, grid2, (unsigned)block_dim_, stream, a3);
     mark_(stream);
 }
 
@@ -603,15 +632,23 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
     long long wstride = (long long)(n * n * 16);
     void* a1[] = {&S, &W, &U, &info, &Wd, &wstride, &batch};
     const unsigned bd_chol = (unsigned)(64 * ((n_ + 15) / 16));
-    SG_HIP(hipModuleLaunchKernel(k_mx_chol_, grid_b, 1, 1, bd_chol, 1, 1, 0, stream, a1, nullptr));
+    launch_(k_mx_chol_, grid_b, bd_chol, stream, a1);
     mark_(stream);
     // M = W H W^H once in FP64 on the matrix cores (one wave per 16-column block).
     const unsigned bd_ref = (unsigned)(64 * ((n_ + 15) / 16));
-    void* af[] = {&H, &Wd, &info, &Mp, &M32, &batch};
-    SG_HIP(hipModuleLaunchKernel(k_mx_formm_, grid_b, 1, 1, bd_ref, 1, 1, 0, stream, af, nullptr));
+    if (ws.hdiag && !ws.hfull) {
+        // Diagonal H: M = W diag(h) W^H straight from the diagonal.
+        const double* hd = ws.hdiag;
+        long long hds = ws.hdiag_stride;
+        void* ad[] = {&hd, &hds, &Wd, &info, &Mp, &M32, &batch};
+        launch_(k_mx_formm_diag_, grid_b, bd_ref, stream, ad);
+    } else {
+        void* af[] = {&H, &Wd, &info, &Mp, &M32, &batch};
+        launch_(k_mx_formm_, grid_b, bd_ref, stream, af);
+    }
     mark_(stream);
     void* a2[] = {&M32, &W32, &V32, &info, &E32, &batch};
-    SG_HIP(hipModuleLaunchKernel(k_mx_front_, grid_b, 1, 1, bd, 1, 1, 0, stream, a2, nullptr));
+    launch_(k_mx_front_, grid_b, bd, stream, a2);
     mark_(stream);
     char* qlog = static_cast<char*>(ws.qlog);
     long long qstride = (long long)ql_log_bytes(n_);
@@ -620,24 +657,23 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
     last_ql_log_ = !all;
     if (!all) {
         void* ar[] = {&W32, &E32, &info, &qlog, &qstride, &batch, &cap};
-        SG_HIP(hipModuleLaunchKernel(k_qlrec_, (unsigned)((batch + 63) / 64), 1, 1, 64, 1, 1, 0, stream, ar,
-                                     nullptr));
+        launch_(k_qlrec_, (unsigned)((batch + 63) / 64), 64, stream, ar);
         mark_(stream);
         void* a3[] = {&W32, &E32, &Z32, &info, &qlog, &qstride, &batch};
-        SG_HIP(hipModuleLaunchKernel(k_ql32_, grid_w, 1, 1, bd, 1, 1, 0, stream, a3, nullptr));
+        launch_(k_ql32_, grid_w, bd, stream, a3);
         mark_(stream);
     }
     void* aq[] = {&W32, &E32, &Z32, &info, &qlog, &qstride, &batch, &all};
-    SG_HIP(hipModuleLaunchKernel(k_qlfull32_, grid_w, 1, 1, bd, 1, 1, 0, stream, aq, nullptr));
+    launch_(k_qlfull32_, grid_w, bd, stream, aq);
     mark_(stream);
     void* a4[] = {&V32, &Z32, &info, &batch};
-    SG_HIP(hipModuleLaunchKernel(k_back32_, grid_w, 1, 1, bd, 1, 1, 0, stream, a4, nullptr));
+    launch_(k_back32_, grid_w, bd, stream, a4);
     mark_(stream);
     int iters = refine_iters_;
     double delta = refine_delta_;
     void* a5[] = {&Mp, &Wd, &W, &U, &info, &V32, &W32, &batch, &iters, &delta};
     // The refinement runs one wave per 16-column block of the matrix.
-    SG_HIP(hipModuleLaunchKernel(k_refine_, grid_b, 1, 1, bd_ref, 1, 1, 0, stream, a5, nullptr));
+    launch_(k_refine_, grid_b, bd_ref, stream, a5);
     mark_(stream);
 }
 
@@ -661,6 +697,19 @@ size_t Solver::ql_log_bytes(int n_) {
     const size_t lam = al(rank + 4 * n, 16);
     const size_t rot = al(lam + 8 * n, 16);
     return al(rot + 16 * (2 * n * n), 256);
+}
+
+void Solver::ensure_hbuf_(size_t batch) {
+    if (batch <= hbuf_batch_) return;
+    clear_graphs_();
+    if (hbuf_) {
+        SG_HIP(hipDeviceSynchronize());
+        SG_HIP(hipFree(hbuf_));
+        hbuf_ = nullptr;
+        hbuf_batch_ = 0;
+    }
+    SG_HIP(hipMalloc(&hbuf_, batch * (size_t)n_ * n_ * 16));
+    hbuf_batch_ = batch;
 }
 
 void Solver::ensure_qlog_(size_t batch) {
@@ -715,13 +764,13 @@ void Solver::launch_large_(const void* H, const void* S, double* W, void* U, int
             if (marks) mark_(stream);
         };
         void* a1[] = {&Sc, &Wc, &Uc, &Ic, &ws, (void*)&wss, &cnt};
-        SG_HIP(hipModuleLaunchKernel(k_lg_chol_, (unsigned)cnt, 1, 1, bd, 1, 1, 0, stream, a1, nullptr));
+        launch_(k_lg_chol_, (unsigned)cnt, bd, stream, a1);
         mark();
         void* a2[] = {&Hc, &Ic, &ws, (void*)&wss, &cnt};
-        SG_HIP(hipModuleLaunchKernel(k_lg_formm_, (unsigned)cnt, 1, 1, bd, 1, 1, 0, stream, a2, nullptr));
+        launch_(k_lg_formm_, (unsigned)cnt, bd, stream, a2);
         mark();
         void* a3[] = {&Ic, &ws, (void*)&wss, &cnt};
-        SG_HIP(hipModuleLaunchKernel(k_lg_hetrd_, (unsigned)cnt, 1, 1, bd, 1, 1, 0, stream, a3, nullptr));
+        launch_(k_lg_hetrd_, (unsigned)cnt, bd, stream, a3);
         mark();
         // QL eigenvalues: the caller's W (FP64), or the workspace's FP32 slot.
         void* wout = Wc;
@@ -732,20 +781,20 @@ void Solver::launch_large_(const void* H, const void* S, double* W, void* U, int
         }
         int cap = ql_sweep_cap_;
         void* a4[] = {&wout, &wstride, &Ic, &ws, (void*)&wss, &cnt, &cap};
-        SG_HIP(hipModuleLaunchKernel(k_lg_ql_, (unsigned)cnt, 1, 1, 64, 1, 1, 0, stream, a4, nullptr));
+        launch_(k_lg_ql_, (unsigned)cnt, 64, stream, a4);
         mark();
-        SG_HIP(hipModuleLaunchKernel(k_lg_qlz_, (unsigned)cnt, 1, 1, bd_ql, 1, 1, 0, stream, a4, nullptr));
+        launch_(k_lg_qlz_, (unsigned)cnt, bd_ql, stream, a4);
         mark();
-        SG_HIP(hipModuleLaunchKernel(k_lg_back_, (unsigned)cnt, 1, 1, bd, 1, 1, 0, stream, a3, nullptr));
+        launch_(k_lg_back_, (unsigned)cnt, bd, stream, a3);
         mark();
         if (mode_ == Mode::Mixed) {
             int iters = refine_iters_;
             double delta = refine_delta_;
             void* a6[] = {&Wc, &Uc, &Ic, &ws, (void*)&wss, &cnt, &iters, &delta};
-            SG_HIP(hipModuleLaunchKernel(k_lg_refine_, (unsigned)cnt, 1, 1, bd, 1, 1, 0, stream, a6, nullptr));
+            launch_(k_lg_refine_, (unsigned)cnt, bd, stream, a6);
         } else {
             void* a6[] = {&Uc, &Ic, &ws, (void*)&wss, &cnt};
-            SG_HIP(hipModuleLaunchKernel(k_lg_backtri_, (unsigned)cnt, 1, 1, bd, 1, 1, 0, stream, a6, nullptr));
+            launch_(k_lg_backtri_, (unsigned)cnt, bd, stream, a6);
         }
         mark();
     }
@@ -778,14 +827,33 @@ void Solver::set_stage_timing(bool on) {
     timing_ = on;
     if (on && ev_.empty()) {
         ev_.resize(12);
+        ev_fn_.assign(12, nullptr);
         for (auto& e : ev_) SG_HIP(hipEventCreate(&e));
     }
     ev_used_ = 0;
 }
 
+void Solver::launch_(hipFunction_t f, unsigned grid, unsigned block, hipStream_t stream, void** args) {
+    SG_HIP(hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, 0, stream, args, nullptr));
+    last_fn_ = f;
+}
+
 void Solver::mark_(hipStream_t stream) {
     if (!timing_ || ev_used_ >= (int)ev_.size()) return;
+    // Event i closes the stage launched last before it (event 0 opens the launch).
+    ev_fn_[ev_used_] = ev_used_ ? last_fn_ : nullptr;
     SG_HIP(hipEventRecord(ev_[ev_used_++], stream));
+}
+
+const char* Solver::stage_label(int i) const {
+    if (i < 0 || i + 1 >= ev_used_) return "";
+    hipFunction_t f = ev_fn_[i + 1];
+    for (const auto* m : {module_.get(), module32_.get()}) {
+        if (!m) continue;
+        for (const auto& kv : m->functions)
+            if (kv.second == f) return kv.first.c_str();
+    }
+    return "";
 }
 
 int Solver::stage_times(double* ms, int n) {
@@ -800,28 +868,5 @@ int Solver::stage_times(double* ms, int n) {
     return stages;
 }
 
-const char* Solver::stage_name(Mode mode, int i, int n, bool ql_log) {
-    static const char* large64[] = {"geneig_lg_chol", "geneig_lg_formm", "geneig_lg_hetrd", "geneig_lg_qlr",
-                                    "geneig_lg_qlz",  "geneig_lg_back",  "geneig_lg_backtri"};
-    static const char* large_mx[] = {"geneig_lg_chol", "geneig_lg_formm", "geneig_lg_hetrd", "geneig_lg_qlr",
-                                     "geneig_lg_qlz",  "geneig_lg_back",  "geneig_lg_refine"};
-    if (n > kMaxLdsOrder) {
-        if (i < 0 || i >= 7) return "";
-        return (mode == Mode::Mixed) ? large_mx[i] : large64[i];
-    }
-    static const char* unified[] = {"geneig_fused"};
-    static const char* hybrid[] = {"geneig_split_front", "geneig_ql_record", "geneig_split_ql", "geneig_ql_full",
-                                   "geneig_split_back"};
-    static const char* mixed[] = {"geneig_lg_chol",   "geneig_mx_formm", "geneig_mx_front", "geneig_ql_record",
-                                  "geneig_split_ql",  "geneig_ql_full",  "geneig_mx_back",  "geneig_mx_refine"};
-    // Without the QL log: the no-log kernel alone in place of recorder + applier.
-    static const char* hybrid_nolog[] = {"geneig_split_front", "geneig_ql_full", "geneig_split_back"};
-    static const char* mixed_nolog[] = {"geneig_lg_chol", "geneig_mx_formm", "geneig_mx_front",
-                                        "geneig_ql_full", "geneig_mx_back",  "geneig_mx_refine"};
-    if (mode == Mode::Unified) return (i == 0) ? unified[0] : "";
-    if (mode == Mode::Hybrid)
-        return ql_log ? ((i >= 0 && i < 5) ? hybrid[i] : "") : ((i >= 0 && i < 3) ? hybrid_nolog[i] : "");
-    return ql_log ? ((i >= 0 && i < 8) ? mixed[i] : "") : ((i >= 0 && i < 6) ? mixed_nolog[i] : "");
-}
 
 }  // namespace sgeig

@@ -31,6 +31,14 @@ public:
     void launch(const void* H, const void* S, double* W, void* U, int* info, int batch,
                 hipStream_t stream);
 
+    // As launch() for a real diagonal H: row b * hdiag_stride of Hd (FP64, n
+    // entries; stride 0 = one diagonal for the whole batch, e.g. Sunny's
+    // paraunitary metric diag(I, -I)). The mixed mode at orders up to 64 forms
+    // M = W diag(h) W^H directly without H; the other paths expand the diagonal
+    // into a full H in a workspace first.
+    void launch_hdiag(const double* Hd, long long hdiag_stride, const void* S, double* W, void* U, int* info,
+                      int batch, hipStream_t stream);
+
     // Contiguous chunks of `chunk` matrices launched round-robin over
     // `streams` (reference: UnifiedGeneigSolver::launch_chunked,
     // unified_solver.cpp:1078-1107). Fused-mode chunks run concurrently;
@@ -75,7 +83,20 @@ private:
         void* qlog = nullptr;
         void* staged = nullptr;
         size_t staged_batch = 0;
+        // A diagonal H (launch_hdiag): its diagonals, their stride, and the
+        // full H it is expanded into for the paths that read H (null: formed
+        // from the diagonal directly).
+        const double* hdiag = nullptr;
+        long long hdiag_stride = 0;
+        void* hfull = nullptr;
     };
+    void launch_impl_(const void* H, const double* Hd, long long hds, const void* S, double* W, void* U,
+                      int* info, int batch, hipStream_t stream);
+    bool hdiag_direct_() const { return mode_ == Mode::Mixed && !large_; }
+    void ensure_hbuf_(size_t batch);
+    void* hbuf_ = nullptr;             // full H expanded from a diagonal
+    size_t hbuf_batch_ = 0;
+    hipFunction_t k_expand_ = nullptr, k_mx_formm_diag_ = nullptr;
     Workspace own_workspace_(size_t batch);   // grows the handle's workspace as needed
 
     void bind_module_(int block_dim, CacheMode cache);
@@ -158,6 +179,8 @@ private:
     // launches the kernels directly.
     struct GraphEntry {
         const void* H;
+        const double* hd;
+        long long hds;
         const void* S;
         double* W;
         void* U;
@@ -181,8 +204,11 @@ private:
     // its stream; stage_times() reads the last launch's durations.
     bool timing_ = false;
     std::vector<hipEvent_t> ev_;
+    std::vector<hipFunction_t> ev_fn_;      // kernel each event closes
     int ev_used_ = 0;
+    hipFunction_t last_fn_ = nullptr;       // kernel launched last
     void mark_(hipStream_t stream);
+    void launch_(hipFunction_t f, unsigned grid, unsigned block, hipStream_t stream, void** args);
 
 public:
     int refine_iters() const { return refine_iters_; }
@@ -211,9 +237,8 @@ public:
     // Fills ms[0 .. min(n, stages)) and returns the number of stages of the
     // last timed launch (0 if none); waits for that launch to finish.
     int stage_times(double* ms, int n);
-    // Kernel of stage i of a launch; ql_log: whether that launch ran QL through
-    // the log (see last_ql_log()).
-    static const char* stage_name(Mode mode, int i, int n = 1, bool ql_log = true);
+    // Kernel name of stage i of the last timed launch ("" if none).
+    const char* stage_label(int i) const;
     bool last_ql_log() const { return last_ql_log_; }
 };
 
