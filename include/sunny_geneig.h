@@ -79,7 +79,10 @@ sunny_geneig_solver_t sunny_geneig_create(
     sunny_geneig_tuning_mode_t  tuning_mode,
     sunny_geneig_cache_mode_t   cache_mode);
 
-/* Reference: src/common/c_api.h:57-69 (c_api.cpp:109-137). */
+/* Reference: src/common/c_api.h:57-69 (c_api.cpp:109-137). Unlike the
+ * reference, AUTOTUNE also applies to the HYBRID and MIXED modes here: the
+ * timed, correctness-gated sweep covers the workgroup size (= matrices per
+ * workgroup) and, from order 48, LDS-resident versus HBM-staged blocking. */
 sunny_geneig_solver_t sunny_geneig_create_ex(
     int                         n,
     int                         device_id,
@@ -194,10 +197,19 @@ sunny_geneig_status_t sunny_geneig_launch_noalloc(
     uintptr_t             stream);
 
 /* Compile (hipRTC) and store in the on-disk cache the code objects for order n
- * with `block_dim_x` threads (0 = suggested) for `arch` (e.g. "gfx950"): the
+ * with `block_dim_x` threads (0 = suggested; -1 = the HBM-staged pipeline at
+ * an order up to 64, the autotuner's other blocking) for `arch` (e.g. "gfx950"): the
  * FP64 module and the FP32 module of the mixed mode, without touching a device. Returns 0 or a status. Lets a build step warm the
  * cache so the first create on a fresh machine skips compilation. */
 int sunny_geneig_precompile(int n, int block_dim_x, const char* arch);
+
+/* Blocking of the handle's kernels: 0 = LDS-resident (one matrix, or a few,
+ * per workgroup with the pencil in LDS; orders up to 64), 1 = HBM-staged (a
+ * kernel per stage handing over through a per-matrix workspace; every order
+ * above 64, and at or below 64 when TUNING_AUTOTUNE found it faster or the
+ * environment variable SUNNY_GENEIG_STAGED was set to 1 at creation). -1 for
+ * NULL. No counterpart in the reference ABI. */
+int sunny_geneig_blocking(sunny_geneig_solver_t solver);
 
 /* Where the cache layer that served construction came from: "in-process",
  * "disk" or "compile" (reference: unified_solver.hpp:129-136). "" for NULL. */

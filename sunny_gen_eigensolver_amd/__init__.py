@@ -53,6 +53,7 @@ EXPORTED_SYMBOLS = (
     "sunny_geneig_session_poll", "sunny_geneig_session_wait", "sunny_geneig_session_stats",
     "sunny_geneig_session_request_bytes", "sunny_geneig_launch_chunked",
     "sunny_geneig_session_set_policy", "sunny_geneig_launch_noalloc", "sunny_geneig_launch_hdiag",
+    "sunny_geneig_blocking",
 )
 
 _lib = None
@@ -93,7 +94,7 @@ def lib() -> ctypes.CDLL:
     L.sunny_geneig_launch_chunked.argtypes = [vp, cu, cu, cu, cu, cu, ci, ci, ctypes.POINTER(cu), ci]
     L.sunny_geneig_launch_chunked.restype = ci
     for name in ("sunny_geneig_matrix_size", "sunny_geneig_batches_per_block",
-                 "sunny_geneig_block_dim_x", "sunny_geneig_kernel_mode"):
+                 "sunny_geneig_block_dim_x", "sunny_geneig_kernel_mode", "sunny_geneig_blocking"):
         getattr(L, name).argtypes = [vp]
         getattr(L, name).restype = ci
     L.sunny_geneig_shared_mem_bytes.argtypes = [vp]
@@ -222,6 +223,12 @@ class Solver:
     def kernel_mode(self) -> str:
         return {KERNEL_HYBRID: "hybrid", KERNEL_MIXED: "mixed"}.get(
             lib().sunny_geneig_kernel_mode(self.handle), "unified")
+
+    @property
+    def staged(self) -> bool:
+        """True when the handle runs the HBM-staged pipeline (orders above 64,
+        or the autotuner's choice below)."""
+        return lib().sunny_geneig_blocking(self.handle) == 1
 
     @property
     def cache_layer(self) -> str:

@@ -39,6 +39,9 @@ HIP_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 WARM_ORDERS = [16, 24, 32, 48, 54, 64]
 BUCKET_ORDERS = list(range(4, 65, 4))
 LARGE_ORDERS = list(range(72, 129, 8)) + [150, 192]   # staged kernels (orders above 64)
+# The staged pipeline at LDS-resident orders, which the autotuner weighs from
+# order 48 (the uneven-batch buckets and the fixtures' orders there).
+STAGED_LDS_ORDERS = [48, 52, 54, 56, 60, 64]
 
 
 def _run(cmd, **kw):
@@ -106,7 +109,7 @@ def _precompile(job):
     return n, bd, rc, (lib.sunny_geneig_last_error().decode() if rc else "")
 
 
-def warm_cache(orders=WARM_ORDERS, buckets=BUCKET_ORDERS, large=LARGE_ORDERS) -> None:
+def warm_cache(orders=WARM_ORDERS, buckets=BUCKET_ORDERS, large=LARGE_ORDERS, staged=STAGED_LDS_ORDERS) -> None:
     from concurrent.futures import ProcessPoolExecutor
     os.makedirs(CACHE, exist_ok=True)
     # Entries are keyed by a hash of source + options: drop stale ones.
@@ -116,7 +119,7 @@ def warm_cache(orders=WARM_ORDERS, buckets=BUCKET_ORDERS, large=LARGE_ORDERS) ->
     os.environ["SUNNY_GENEIG_CACHE_DIR"] = CACHE
     jobs = [(n, bd) for n in orders for bd in (64, 128, 256)]
     jobs += [(n, 0) for n in buckets if n not in orders]
-    jobs = [(n, 0) for n in large] + jobs           # the longest compiles first
+    jobs = [(n, 0) for n in large] + [(n, -1) for n in staged] + jobs   # the longest compiles first
     workers = max(1, min(8, len(os.sched_getaffinity(0))))
     with ProcessPoolExecutor(workers) as ex:
         for n, bd, rc, msg in ex.map(_precompile, jobs):

@@ -78,11 +78,11 @@ sunny_geneig_solver_t sunny_geneig_create_ex(int n, int device_id, int batches_p
         set_error("sunny_geneig_create_ex: unknown kernel_mode");
         return nullptr;
     }
-    // As in the reference, the split mode ignores tuning and batches-per-block;
-    // so does the mixed mode, which is built on the split kernels.
+    // As in the reference, the split mode ignores batches-per-block (so does
+    // the mixed mode, built on the split kernels); the tuning mode applies to
+    // every mode here.
     if (kernel_mode != SUNNY_GENEIG_KERNEL_UNIFIED)
-        return create_impl(n, device_id, 0, SUNNY_GENEIG_TUNING_SUGGESTED, cache_mode, kernel_mode,
-                           "sunny_geneig_create_ex");
+        return create_impl(n, device_id, 0, tuning_mode, cache_mode, kernel_mode, "sunny_geneig_create_ex");
     return create_impl(n, device_id, batches_per_block_request, tuning_mode, cache_mode, kernel_mode,
                        "sunny_geneig_create_ex");
 }
@@ -216,8 +216,10 @@ int sunny_geneig_precompile(int n, int block_dim_x, const char* arch) {
     try {
         if (n < 1 || n > sgeig::Solver::kMaxOrder) throw std::invalid_argument("n out of range");
         int bd = block_dim_x > 0 ? block_dim_x : sgeig::Solver::suggested_block_dim(n);
-        if (n > sgeig::Solver::kMaxLdsOrder) bd = 256;     // the staged kernels' only workgroup size
+        const bool staged = block_dim_x < 0 && n <= sgeig::Solver::kMaxLdsOrder;
+        if (n > sgeig::Solver::kMaxLdsOrder || staged) bd = 256;   // the staged kernels' only workgroup size
         sgeig::ModuleKey key{n, bd, arch && arch[0] ? arch : "gfx950"};
+        key.staged = staged;
         sgeig::write_disk(key, sgeig::compile_code_object(key));
         key.fp32 = true;
         sgeig::write_disk(key, sgeig::compile_code_object(key));
@@ -230,6 +232,8 @@ int sunny_geneig_precompile(int n, int block_dim_x, const char* arch) {
         return SUNNY_GENEIG_INVALID_ARG;
     }
 }
+
+int sunny_geneig_blocking(sunny_geneig_solver_t s) { return s ? (s->impl->staged() ? 1 : 0) : -1; }
 
 const char* sunny_geneig_cache_layer(sunny_geneig_solver_t s) { return s ? s->impl->cache_layer() : ""; }
 

@@ -73,6 +73,7 @@ std::vector<std::string> compile_options(const ModuleKey& key) {
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     };
     if (key.fp32) o.push_back("-DGENEIG_FP32");
+    if (key.staged) o.push_back("-DGENEIG_STAGED");
     return o;
 }
 
@@ -159,8 +160,9 @@ std::string cache_dir() {
 
 std::string code_object_path(const ModuleKey& key) {
     char name[160];
-    std::snprintf(name, sizeof(name), "geneig_n%d_wg%d_%s%s_%016llx.hsaco", key.n, key.block_dim,
-                  key.arch.c_str(), key.fp32 ? "_fp32" : "", (unsigned long long)key_hash(key));
+    std::snprintf(name, sizeof(name), "geneig_n%d_wg%d_%s%s%s_%016llx.hsaco", key.n, key.block_dim,
+                  key.arch.c_str(), key.fp32 ? "_fp32" : "", key.staged ? "_staged" : "",
+                  (unsigned long long)key_hash(key));
     return cache_dir() + "/" + name;
 }
 

@@ -41,10 +41,12 @@ struct ModuleKey {
     int block_dim = 0;
     std::string arch;   // base gfx name, e.g. "gfx950"
     bool fp32 = false;  // FP32 build of the split stages (mixed-precision path)
+    bool staged = false; // the HBM-staged pipeline at an order up to 64
     bool operator<(const ModuleKey& o) const {
         if (n != o.n) return n < o.n;
         if (block_dim != o.block_dim) return block_dim < o.block_dim;
         if (fp32 != o.fp32) return fp32 < o.fp32;
+        if (staged != o.staged) return staged < o.staged;
         return arch < o.arch;
     }
 };

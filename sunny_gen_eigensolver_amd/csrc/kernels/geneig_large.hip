@@ -31,10 +31,20 @@
 
 namespace gk {
 
-// Workgroup size of the staged kernels: 256; for orders up to 64 (where only
-// the Cholesky is built from this file) one wave per 16-row tile, so that
-// small orders do not carry idle waves.
-constexpr int LNT = (N <= 64) ? 64 * ((N + 15) / 16) : 256;
+// The whole staged pipeline is built above order 64, and at any order when
+// GENEIG_STAGED is defined (the HBM-staged alternative the autotuner weighs
+// against the LDS-resident kernels); otherwise only geneig_lg_chol is, for the
+// mixed mode's FP64 Cholesky and W = L^-1.
+#if GENEIG_N > 64 || defined(GENEIG_STAGED)
+#define GENEIG_LG_PIPELINE 1
+#else
+#define GENEIG_LG_PIPELINE 0
+#endif
+
+// Workgroup size of the staged kernels: 256; where only the Cholesky is
+// built (orders up to 64) one wave per 16-row tile, so that small orders do
+// not carry idle waves.
+constexpr int LNT = (N <= 64 && !GENEIG_LG_PIPELINE) ? 64 * ((N + 15) / 16) : 256;
 constexpr int LNW = LNT / 64;
 constexpr int ND = (N + 63) / 64;             // rows per lane when lanes index rows
 constexpr int LNB = (N + 15) / 16;             // 16-blocks per dimension
@@ -663,7 +673,7 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
     if (tid == 0) info[b] = 0;
 }
 
-#if GENEIG_N > 64
+#if GENEIG_LG_PIPELINE
 // M = W H W^H: X = W H (W lower triangular: k <= row), then the lower tiles
 // of X W^H, mirrored so that M is exactly Hermitian.
 extern "C" __global__ void __launch_bounds__(gk::LNT)
@@ -752,10 +762,10 @@ geneig_lg_backtri(cx* __restrict__ U, const int* __restrict__ info, char* __rest
             }
     }
 }
-#endif  // GENEIG_N > 64
+#endif  // GENEIG_LG_PIPELINE
 #endif  // !GENEIG_FP32
 
-#if GENEIG_N > 64
+#if GENEIG_LG_PIPELINE
 // Bytes of one matrix's workspace, for the host's consistency check.
 extern "C" __device__ const unsigned long long geneig_lg_ws_bytes = gk::WS_BYTES;
 
@@ -1452,4 +1462,4 @@ geneig_lg_scatter(const real* __restrict__ Wp, const cx* __restrict__ Up, const 
     }
 }
 #endif  // !GENEIG_FP32
-#endif  // GENEIG_N > 64
+#endif  // GENEIG_LG_PIPELINE

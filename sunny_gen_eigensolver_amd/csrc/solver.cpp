@@ -20,11 +20,22 @@ namespace {
 // Workspace of one round of the staged kernels (orders above 64).
 constexpr size_t kLargeRoundBytes = size_t(32) << 30;
 
-// Persisted autotune winners: "<n> <arch> <block_dim>" lines in
-// <cache_dir>/autotune.tsv. The reference memoises per process only
+// Persisted autotune winners: "<n> <arch> <mode> <block_dim> <staged>" lines
+// in <cache_dir>/autotune.tsv (mode 0 fused, 1 split, 2 mixed; staged 1 = the
+// HBM-staged pipeline). The reference memoises per process only
 // (unified_solver.cpp:647, :970-993); this table also survives restarts.
+struct TuneKey {
+    int n;
+    std::string arch;
+    int mode;
+    bool operator<(const TuneKey& o) const {
+        if (n != o.n) return n < o.n;
+        if (mode != o.mode) return mode < o.mode;
+        return arch < o.arch;
+    }
+};
 std::mutex g_tune_mutex;
-std::map<std::pair<int, std::string>, int> g_tune;
+std::map<TuneKey, std::pair<int, int>> g_tune;
 bool g_tune_loaded = false;
 
 std::string tune_path() { return cache_dir() + "/autotune.tsv"; }
@@ -33,9 +44,13 @@ void load_tune_table() {
     if (g_tune_loaded) return;
     g_tune_loaded = true;
     std::ifstream f(tune_path());
-    int n, bd;
-    std::string arch;
-    while (f >> n >> arch >> bd) g_tune[{n, arch}] = bd;
+    std::string line;
+    while (std::getline(f, line)) {
+        std::istringstream ls(line);
+        TuneKey k;
+        int bd = 0, staged = 0;
+        if (ls >> k.n >> k.arch >> k.mode >> bd >> staged) g_tune[k] = {bd, staged};
+    }
 }
 
 void save_tune_table() {
@@ -43,7 +58,9 @@ void save_tune_table() {
     const std::string tmp = path + ".tmp";
     std::ofstream f(tmp, std::ios::trunc);
     if (!f) return;
-    for (auto& kv : g_tune) f << kv.first.first << ' ' << kv.first.second << ' ' << kv.second << '\n';
+    for (auto& kv : g_tune)
+        f << kv.first.n << ' ' << kv.first.arch << ' ' << kv.first.mode << ' ' << kv.second.first << ' '
+          << kv.second.second << '\n';
     f.close();
     std::rename(tmp.c_str(), path.c_str());
 }
@@ -141,17 +158,24 @@ Solver::Solver(int n, int device_id, int bpb_request, Tuning tuning, CacheMode c
         throw std::invalid_argument("device_id " + std::to_string(device_id) + " out of range");
     SG_HIP(hipSetDevice(device_id));
     arch_ = device_arch(device_id);
+    SG_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
     // A workgroup of W waves solves W matrices (one QL chain per wave), so a
     // requested matrices-per-block count fixes the workgroup size.
     int bd = (bpb_request > 0) ? 64 * bpb_request : suggested_block_dim(n);
-    // Staged kernels: one matrix per 256-thread workgroup, nothing to tune.
-    if (large_) bd = 256;
-    if (mode == Mode::Unified && tuning == Tuning::Autotune && bpb_request <= 0 && !large_) {
-        bd = autotune_(cache);
+    // SUNNY_GENEIG_STAGED=1 runs the HBM-staged pipeline at orders up to 64
+    // as well (the autotuner's other blocking; tests use it to check it).
+    if (const char* v = std::getenv("SUNNY_GENEIG_STAGED"))
+        if (std::atoi(v) != 0 && bpb_request <= 0) staged_ = true;
+    if (tuning == Tuning::Autotune && bpb_request <= 0 && !large_ && !staged_) {
+        const auto choice = autotune_(cache);
+        bd = choice.first;
+        staged_ = choice.second != 0;
         autotuned_ = true;
     }
+    large_ = n > kMaxLdsOrder || staged_;
+    // Staged kernels: one matrix per 256-thread workgroup.
+    if (large_) bd = 256;
     bind_module_(bd, cache);
-    SG_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
 }
 
 Solver::~Solver() {
@@ -176,7 +200,9 @@ Solver::~Solver() {
 }
 
 void Solver::bind_module_(int block_dim, CacheMode cache) {
+    clear_graphs_();                         // captured launches hold the old kernels
     ModuleKey key{n_, block_dim, arch_};
+    key.staged = staged_ && n_ <= kMaxLdsOrder;
     module_ = acquire_module(key, cache, &layer_, &acquire_ms_);
     block_dim_ = block_dim;
     bpb_ = block_dim / 64;
@@ -278,61 +304,107 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
     lds_bytes_ = (unsigned)lds;
 }
 
-// Timed, correctness-gated sweep over workgroup sizes, generalising the
-// reference's BlockDim sweep (unified_solver.cpp:649-925): every candidate is
-// compiled, checked against the suggested size's output on the tuning batch
-// (eigenvalues to 1e-10 relative), and timed as the median of 5 launches.
-int Solver::autotune_(CacheMode cache) {
+// Timed, correctness-gated sweep, generalising the reference's BlockDim sweep
+// (unified_solver.cpp:649-925) to the choices this library has per order and
+// mode: the workgroup size, which with one wave per matrix is also the number
+// of matrices per workgroup (64 / 128 / 256 threads = 1 / 2 / 4 matrices),
+// and, from order kMinStagedTuneOrder up, LDS-resident versus HBM-staged
+// blocking (the staged pipeline: one matrix per 256-thread workgroup). Every
+// candidate is compiled and run on the tuning batch; it must reproduce the
+// suggested candidate's eigenvalues (1e-10 relative) and, on spot matrices,
+// its eigenvectors (1 - |u^H S u_ref| < 1e-10 for eigenvalues separated by
+// 1e-6 or more, the reference's checks at :858-885); the fastest valid one
+// (median of 5 launches) wins and is persisted per (order, arch, mode).
+std::pair<int, int> Solver::autotune_(CacheMode cache) {
+    const TuneKey key{n_, arch_, (int)mode_};
     {
         std::lock_guard<std::mutex> lk(g_tune_mutex);
         load_tune_table();
-        auto it = g_tune.find({n_, arch_});
+        auto it = g_tune.find(key);
         if (it != g_tune.end()) return it->second;
     }
     const int suggested = suggested_block_dim(n_);
-    std::vector<int> cands = {64, 128, 256};
+    std::vector<std::pair<int, int>> cands = {{suggested, 0}};
+    for (int c : {64, 128, 256})
+        if (c != suggested) cands.push_back({c, 0});
+    if (n_ >= kMinStagedTuneOrder) cands.push_back({256, 1});
     const int batch = 1024;
     const size_t nn = (size_t)n_ * n_;
     std::vector<std::complex<double>> H, S;
-    tune_inputs(n_, batch, 0xA5A5A5A5ULL ^ ((uint64_t)n_ << 16), H, S);
+    tune_inputs(n_, batch, 0xA5A5A5A5ULL ^ ((uint64_t)n_ << 16) ^ (uint64_t)mode_, H, S);
     DevBuf dH(nn * batch * 16), dS(nn * batch * 16), dU(nn * batch * 16), dW((size_t)n_ * batch * 8),
         dI((size_t)batch * 4);
     SG_HIP(hipMemcpy(dH.p, H.data(), nn * batch * 16, hipMemcpyHostToDevice));
     SG_HIP(hipMemcpy(dS.p, S.data(), nn * batch * 16, hipMemcpyHostToDevice));
+    constexpr int kSpots = 8;
     std::vector<double> ref_w, w((size_t)n_ * batch);
+    std::vector<std::complex<double>> ref_u, u(nn * kSpots);
     hipEvent_t e0, e1;
     SG_HIP(hipEventCreate(&e0));
     SG_HIP(hipEventCreate(&e1));
-    int best = suggested;
+    std::pair<int, int> best = cands[0];
     double best_ms = 1e300;
-    // Suggested first: its output is the validation reference.
-    std::stable_partition(cands.begin(), cands.end(), [&](int c) { return c == suggested; });
-    for (int c : cands) {
+    const bool graphs = graphs_;
+    graphs_ = false;                         // direct launches: candidates change kernels
+    auto run = [&]() {
+        launch(dH.p, dS.p, (double*)dW.p, dU.p, (int*)dI.p, batch, nullptr);
+    };
+    // Spot matrices spread over the batch.
+    auto spot = [&](int s) { return (size_t)s * (batch / kSpots); };
+    for (const auto& c : cands) {
         try {
-            bind_module_(c, cache);
+            staged_ = c.second != 0;
+            large_ = n_ > kMaxLdsOrder || staged_;
+            bind_module_(staged_ ? 256 : c.first, cache);
+            run();
+            SG_HIP(hipDeviceSynchronize());
         } catch (const std::exception& ex) {
-            std::fprintf(stderr, "[sunny_geneig autotune] N=%d wg=%d skipped: %s\n", n_, c, ex.what());
+            std::fprintf(stderr, "[sunny_geneig autotune] N=%d wg=%d %s skipped: %s\n", n_, c.first,
+                         c.second ? "staged" : "lds", ex.what());
             continue;
         }
-        launch_unified_(dH.p, dS.p, (double*)dW.p, dU.p, (int*)dI.p, batch, nullptr);
-        SG_HIP(hipDeviceSynchronize());
         SG_HIP(hipMemcpy(w.data(), dW.p, w.size() * 8, hipMemcpyDeviceToHost));
+        for (int sp = 0; sp < kSpots; ++sp)
+            SG_HIP(hipMemcpy(u.data() + nn * sp, static_cast<char*>(dU.p) + spot(sp) * nn * 16, nn * 16,
+                             hipMemcpyDeviceToHost));
         if (ref_w.empty()) {
             ref_w = w;
+            ref_u = u;
         } else {
-            double worst = 0.0;
+            double worst = 0.0, worst_phase = 0.0;
             for (size_t i = 0; i < w.size(); ++i)
                 worst = std::max(worst, std::fabs(w[i] - ref_w[i]) / std::max(std::fabs(ref_w[i]), 1.0));
-            if (!(worst < 1e-10)) {
-                std::fprintf(stderr, "[sunny_geneig autotune] N=%d wg=%d rejected (eig diff %.3e)\n",
-                             n_, c, worst);
+            std::vector<std::complex<double>> Su(n_);
+            for (int sp = 0; sp < kSpots; ++sp) {
+                const std::complex<double>* Sb = S.data() + nn * spot(sp);
+                const double* wr = ref_w.data() + (size_t)n_ * spot(sp);
+                for (int col = 0; col < n_; ++col) {
+                    const double gap = std::min(col > 0 ? wr[col] - wr[col - 1] : 1e300,
+                                                col + 1 < n_ ? wr[col + 1] - wr[col] : 1e300);
+                    if (gap < 1e-6) continue;
+                    const std::complex<double>* ur = ref_u.data() + nn * sp + (size_t)col * n_;
+                    const std::complex<double>* uc = u.data() + nn * sp + (size_t)col * n_;
+                    for (int i = 0; i < n_; ++i) {
+                        std::complex<double> acc = 0.0;
+                        for (int k = 0; k < n_; ++k) acc += Sb[(size_t)k * n_ + i] * ur[k];
+                        Su[i] = acc;
+                    }
+                    std::complex<double> inner = 0.0;
+                    for (int i = 0; i < n_; ++i) inner += std::conj(uc[i]) * Su[i];
+                    worst_phase = std::max(worst_phase, 1.0 - std::abs(inner));
+                }
+            }
+            if (!(worst < 1e-10) || !(worst_phase < 1e-10)) {
+                std::fprintf(stderr,
+                             "[sunny_geneig autotune] N=%d wg=%d %s rejected (eig diff %.3e, phase %.3e)\n", n_,
+                             c.first, c.second ? "staged" : "lds", worst, worst_phase);
                 continue;
             }
         }
         std::vector<float> samples;
-        for (int s = 0; s < 5; ++s) {
+        for (int sm = 0; sm < 5; ++sm) {
             SG_HIP(hipEventRecord(e0, nullptr));
-            launch_unified_(dH.p, dS.p, (double*)dW.p, dU.p, (int*)dI.p, batch, nullptr);
+            run();
             SG_HIP(hipEventRecord(e1, nullptr));
             SG_HIP(hipEventSynchronize(e1));
             float ms = 0.f;
@@ -341,18 +413,21 @@ int Solver::autotune_(CacheMode cache) {
         }
         std::sort(samples.begin(), samples.end());
         const double med = samples[2];
-        std::fprintf(stderr, "[sunny_geneig autotune] N=%d wg=%d median %.3f ms (%.0f solves/s)\n", n_,
-                     c, med, batch / (med * 1e-3));
+        std::fprintf(stderr, "[sunny_geneig autotune] N=%d mode=%d wg=%d %s median %.3f ms (%.0f solves/s)\n", n_,
+                     (int)mode_, c.first, c.second ? "staged" : "lds", med, batch / (med * 1e-3));
         if (med < best_ms) {
             best_ms = med;
             best = c;
         }
     }
+    graphs_ = graphs;
+    staged_ = false;
+    large_ = n_ > kMaxLdsOrder;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     {
         std::lock_guard<std::mutex> lk(g_tune_mutex);
-        g_tune[{n_, arch_}] = best;
+        g_tune[key] = best;
         if (cache == CacheMode::Auto) save_tune_table();
     }
     return best;

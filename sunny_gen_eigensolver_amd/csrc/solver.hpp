@@ -57,6 +57,11 @@ public:
     const char* cache_layer() const { return layer_name(layer_); }
     double acquire_ms() const { return acquire_ms_; }
     bool was_autotuned() const { return autotuned_; }
+    // HBM-staged pipeline (always above order 64; below it when the autotuner
+    // or SUNNY_GENEIG_STAGED chose it) rather than the LDS-resident kernels.
+    bool staged() const { return large_; }
+    // The autotuner weighs the staged pipeline from this order up.
+    static constexpr int kMinStagedTuneOrder = 48;
 
     // Suggested workgroup size for order n (a model, no measurement).
     static int suggested_block_dim(int n);
@@ -100,7 +105,8 @@ private:
     Workspace own_workspace_(size_t batch);   // grows the handle's workspace as needed
 
     void bind_module_(int block_dim, CacheMode cache);
-    int autotune_(CacheMode cache);
+    // (block_dim, staged) chosen by the timed sweep, or read from its table.
+    std::pair<int, int> autotune_(CacheMode cache);
     void launch_unified_(const void* H, const void* S, double* W, void* U, int* info, int batch,
                          hipStream_t stream);
     void launch_hybrid_(const void* H, const void* S, double* W, void* U, int* info, int batch,
@@ -153,7 +159,8 @@ private:
     size_t mbuf_batch_ = 0;
     // Orders above 64 (the staged kernels, every mode): per-matrix workspace
     // for up to large_chunk(n) matrices at a time.
-    bool large_ = false;
+    bool large_ = false;     // staged kernels: order above 64, or staged_
+    bool staged_ = false;    // staged kernels chosen at an order up to 64
     hipFunction_t k_lg_chol_ = nullptr, k_lg_formm_ = nullptr, k_lg_hetrd_ = nullptr, k_lg_ql_ = nullptr,
                   k_lg_qlz_ = nullptr, k_lg_back_ = nullptr, k_lg_backtri_ = nullptr, k_lg_refine_ = nullptr;
     void* gbuf_ = nullptr;

@@ -45,14 +45,19 @@ VarlenSolver::VarlenSolver(int max_n, int device_id, const std::vector<int>& buc
         Bucket b;
         b.order = o;
         b.block_dim = Solver::suggested_block_dim(o);   // same warm code objects
+        std::unique_ptr<Solver> tuned;
         if (tuning == Tuning::Autotune && o <= Solver::kMaxLdsOrder) {
-            Solver probe(o, device_id, 0, Tuning::Autotune, cache, Mode::Unified);
-            b.block_dim = probe.block_dim_x();
+            // The fused solver's sweep for this order: workgroup size, or the
+            // HBM-staged pipeline (then this bucket takes the staged route).
+            tuned = std::make_unique<Solver>(o, device_id, 0, Tuning::Autotune, cache, Mode::Unified);
+            b.block_dim = tuned->block_dim_x();
         }
         b.nw = b.block_dim / 64;
-        if (o > Solver::kMaxLdsOrder) {
+        if (o > Solver::kMaxLdsOrder || (tuned && tuned->staged())) {
             b.nw = 1;
-            b.large = std::make_unique<Solver>(o, device_id, 0, Tuning::Suggested, cache, Mode::Unified);
+            b.large = (tuned && tuned->staged())
+                          ? std::move(tuned)
+                          : std::make_unique<Solver>(o, device_id, 0, Tuning::Suggested, cache, Mode::Unified);
             b.gather = b.large->function("geneig_lg_gather");
             b.scatter = b.large->function("geneig_lg_scatter");
         } else {
