@@ -17,7 +17,9 @@
  *   d_U      : complex128 [batch][n][n], eigenvectors, U^H S U = I.
  *   d_info   : int32      [batch]; 0 = success, n + k = leading minor k of S
  *              not positive definite (zhegv INFO), 1..n-1 = QL did not
- *              converge (that many off-diagonals left).
+ *              converge (that many off-diagonals left), n = (MIXED mode only)
+ *              the FP64 refinement's last correction was still above its
+ *              tolerance (W and U hold the last iterate).
  */
 #ifndef SUNNY_GENEIG_H
 #define SUNNY_GENEIG_H

@@ -2337,6 +2337,7 @@ struct RefWork {
     cx U[N * LDR];        // eigenvector estimate V
     cx M[NPK];            // M = W H W^H, lower triangle packed by columns
     real lam[N];
+    real emax[NB];        // per wave: largest entry of the last step's correction
 };
 
 // M(i, k) from its packed lower triangle (Hermitian), zero outside.
@@ -2527,11 +2528,14 @@ geneig_mx_formm_diag(const real* __restrict__ Hd, long long hd_stride, const cx*
 // eigenvalues with adjacent gaps below delta: one exact 2 x 2 step for a
 // pair, Jacobi sweeps over all its pairs otherwise), then U = W^H V. In step
 // it (from 0), delta = 2^it delta_rel max |lambda|. U^H S U = V^H V up to the
-// accuracy of W, as in the FP64 modes.
+// accuracy of W, as in the FP64 modes. Convergence test: the error left
+// after a step is of the order of the square of its correction, so a last
+// step whose correction E has an entry above `tol` marks the matrix
+// info = N (a value zhegv's INFO never takes), with W and U as computed.
 extern "C" __global__ void __launch_bounds__(gk::REF_NT, 2)
 geneig_mx_refine(const cx* __restrict__ Mp, const cx* __restrict__ Wd, real* __restrict__ W,
-                 cx* __restrict__ U, const int* __restrict__ info, const gk::cf* __restrict__ V32,
-                 const float* __restrict__ W32, int batch, int iters, real delta_rel) {
+                 cx* __restrict__ U, int* __restrict__ info, const gk::cf* __restrict__ V32,
+                 const float* __restrict__ W32, int batch, int iters, real delta_rel, real tol) {
     using namespace gk;
     __shared__ RefWork r;
     const int b = blockIdx.x;
@@ -2554,6 +2558,7 @@ geneig_mx_refine(const cx* __restrict__ Mp, const cx* __restrict__ Wd, real* __r
         __syncthreads();
     }
     real delta = real(0.0);
+    real emax = real(0.0);                       // this lane's largest |E| entry, last step
     for (int it = 0; it < iters; ++it) {
         ctile T[NB], G[NB];
         {
@@ -2593,6 +2598,7 @@ geneig_mx_refine(const cx* __restrict__ Mp, const cx* __restrict__ Wd, real* __r
                 if (i == j) e = cm(real(0.5) * (real(1.0) - g.x), 0.0);
                 else if (!(fabs(gap) < delta)) e = e + (t - g * (real(0.5) * (li + lj))) * (real(1.0) / gap);
                 if (i >= N || !jin) e = cm(0.0, 0.0);
+                if (it == iters - 1) emax = fmax(emax, fmax(fabs(e.x), fabs(e.y)));
                 T[rt].re[v] = e.x;
                 T[rt].im[v] = e.y;
             }
@@ -2656,7 +2662,15 @@ geneig_mx_refine(const cx* __restrict__ Mp, const cx* __restrict__ Wd, real* __r
             c0 = c1 + 1;
         }
     }
+    emax = wave_max(emax);
+    if (l == 0) r.emax[cb] = emax;
     __syncthreads();
+    if (threadIdx.x == 0 && iters > 0 && info[b] == 0) {
+        real m = r.emax[0];
+#pragma unroll
+        for (int q = 1; q < NB; ++q) m = fmax(m, r.emax[q]);
+        if (!(m <= tol)) info[b] = N;
+    }
     // U = W^H V, W's lower triangle staged again over M (no longer needed).
     stage_lower_packed(r.M, Wb, threadIdx.x, REF_NT);
     __syncthreads();

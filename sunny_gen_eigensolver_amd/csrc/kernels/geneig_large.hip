@@ -1240,9 +1240,11 @@ __device__ __forceinline__ void ritz_pair_g(const cx* __restrict__ M, cx* V, rea
 // pair, Jacobi sweeps over its pairs otherwise), then U = W^H V. In step it
 // (from 0), delta = 2^it delta_rel max |lambda|. Buffers: V in the scratch
 // and V slots alternately, P in the reflector slot, E in the caller's U.
+// Convergence test as in geneig_mx_refine: a last correction with an entry
+// above `tol` marks the matrix info = N.
 extern "C" __global__ void __launch_bounds__(gk::LNT, 2)
-geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, const int* __restrict__ info, char* __restrict__ ws,
-                 long long wss, int batch, int iters, real delta_rel) {
+geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, int* __restrict__ info, char* __restrict__ ws,
+                 long long wss, int batch, int iters, real delta_rel, real tol) {
     using namespace gk;
     __shared__ real lam[N];
     __shared__ real wmax[LNW];
@@ -1264,6 +1266,7 @@ geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, const int* __restrict
     __syncthreads();
     constexpr int NRG = (LNB + RRG - 1) / RRG;
     real delta = 0.0;
+    real emax = 0.0;                             // this lane's largest |E| entry, last step
     for (int it = 0; it < iters; ++it) {
         const cx* V = Va;
         // P = M V.
@@ -1328,6 +1331,7 @@ geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, const int* __restrict
                         const real gap = lj - li;
                         if (i == j) e = cm(real(0.5) * (real(1.0) - gv.x), 0.0);
                         else if (!(fabs(gap) < delta)) e = e + (tv - gv * (real(0.5) * (li + lj))) * (real(1.0) / gap);
+                        if (it == iters - 1) emax = fmax(emax, fmax(fabs(e.x), fabs(e.y)));
                         E[(size_t)j * N + i] = e;
                     }
                 }
@@ -1389,6 +1393,15 @@ geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, const int* __restrict
         }
     }
     __syncthreads();
+    emax = wave_max(emax);
+    if (lane == 0) wmax[wave] = emax;
+    __syncthreads();
+    if (tid == 0 && iters > 0 && info[b] == 0) {
+        real m = wmax[0];
+#pragma unroll
+        for (int u = 1; u < LNW; ++u) m = fmax(m, wmax[u]);
+        if (!(m <= tol)) info[b] = N;
+    }
     // U = W^H V.
     const cx* V = Va;
     for (int t = wave; t < NRG * LNB; t += LNW) {

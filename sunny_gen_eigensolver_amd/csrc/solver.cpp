@@ -227,6 +227,7 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
             k_lg_refine_ = module_->get("geneig_lg_refine");
             if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_ITERS")) refine_iters_ = std::max(0, std::atoi(v));
             if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_DELTA")) refine_delta_ = std::atof(v);
+            if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_TOL")) refine_tol_ = std::atof(v);
         } else {
             k_lg_hetrd_ = module_->get("geneig_lg_hetrd");
             k_lg_ql_ = module_->get("geneig_lg_qlr");
@@ -283,6 +284,7 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
         k_back32_ = module32_->get("geneig_mx_back");
         if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_ITERS")) refine_iters_ = std::max(0, std::atoi(v));
         if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_DELTA")) refine_delta_ = std::atof(v);
+        if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_TOL")) refine_tol_ = std::atof(v);
     }
     // Reported LDS: the fused kernel's; in the split mode the front kernel's
     // (the reference reports its reduce kernel's, c_api.cpp:41-49); in the
@@ -746,7 +748,8 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
     mark_(stream);
     int iters = refine_iters_;
     double delta = refine_delta_;
-    void* a5[] = {&Mp, &Wd, &W, &U, &info, &V32, &W32, &batch, &iters, &delta};
+    double tol = refine_tol_;
+    void* a5[] = {&Mp, &Wd, &W, &U, &info, &V32, &W32, &batch, &iters, &delta, &tol};
     // The refinement runs one wave per 16-column block of the matrix.
     launch_(k_refine_, grid_b, bd_ref, stream, a5);
     mark_(stream);
@@ -865,7 +868,8 @@ void Solver::launch_large_(const void* H, const void* S, double* W, void* U, int
         if (mode_ == Mode::Mixed) {
             int iters = refine_iters_;
             double delta = refine_delta_;
-            void* a6[] = {&Wc, &Uc, &Ic, &ws, (void*)&wss, &cnt, &iters, &delta};
+            double tol = refine_tol_;
+            void* a6[] = {&Wc, &Uc, &Ic, &ws, (void*)&wss, &cnt, &iters, &delta, &tol};
             launch_(k_lg_refine_, (unsigned)cnt, bd, stream, a6);
         } else {
             void* a6[] = {&Uc, &Ic, &ws, (void*)&wss, &cnt};

@@ -167,6 +167,12 @@ private:
     size_t gbuf_batch_ = 0;
     int refine_iters_ = 2;
     double refine_delta_ = 1e-4;
+    // Largest entry the last refinement step's correction may have; above it
+    // info = N. Measured on the benchmark generator (N = 54, 96, 192) and the
+    // fixtures: converged matrices stay below 1e-6 (a tail of pairs with
+    // gaps just above the closeness threshold reaches 1e-7), so 1e-5 leaves
+    // a factor 10. SUNNY_GENEIG_REFINE_TOL overrides it.
+    double refine_tol_ = 1e-5;
     // Sweeps a QL log may hold before the matrix is redone without a log (the
     // kernels clamp it to the log's capacity); SUNNY_GENEIG_QL_SWEEP_CAP lowers
     // it, which tests use to run the no-log path.

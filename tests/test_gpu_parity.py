@@ -133,3 +133,21 @@ def test_ill_conditioned_s(kernel, n):
     H, S = np.stack(Hs), np.stack(Ss)
     W, U, info = T.run_gpu(H, S, kernel=kernel)
     T.assert_parity_conditioned(H, S, W, U, info, tag=f"{kernel} n={n} cond(S) <= 1e4")
+
+
+@pytest.mark.parametrize("n", [54, 96])
+def test_refinement_convergence_test_reports_through_info(n, monkeypatch):
+    """The mixed mode's refinement checks its last correction: with the
+    tolerance forced to (almost) zero from the host every matrix reports
+    info = n (W and U still the refined iterate, within the bars); at the
+    default tolerance none does."""
+    H, S = O.gen_spectrum(n, 16, 0xF1A6 + n)
+    monkeypatch.setenv("SUNNY_GENEIG_REFINE_TOL", "1e-300")
+    W, U, info = T.run_gpu(H, S, kernel="mixed")
+    assert (info == n).all(), info
+    Wl, Ul, _ = O.zhegv_batch(H, S, threads=8)
+    T.assert_parity(H, S, W, U, np.zeros_like(info), Wl, Ul, tag=f"flagged mixed n={n}")
+    monkeypatch.delenv("SUNNY_GENEIG_REFINE_TOL")
+    W2, U2, info2 = T.run_gpu(H, S, kernel="mixed")
+    assert (info2 == 0).all()
+    assert np.array_equal(W, W2) and np.array_equal(U, U2)
