@@ -99,9 +99,22 @@ __device__ __forceinline__ cx wave_sum_cx(cx v) { return cm(wave_sum(v.x), wave_
 // Bytes beside the packed triangle: v (two steps), w, per-wave partial sums
 // of A v and the column sums of its upper part.
 constexpr size_t HT_EXTRA = (size_t)(2 + 1 + LNW + 1) * N * sizeof(cx) + 64;
+// LDS budget of the tridiagonalisation: in FP64 half the CU's LDS, so that
+// two workgroups share a CU; the columns whose packed triangle does not fit
+// are reduced in place in global memory first (K0 below). Measured at batch
+// 16384 / 2048: N = 96 46.8 -> 26.6 ms (K0 = 5), N = 192 39.0 -> 25.3 ms
+// (K0 = 94) against the whole LDS at one workgroup per CU; a third of the
+// LDS (three workgroups) was slower (29.5 ms at N = 96). The FP32 build
+// keeps the whole triangle in LDS (its in-place global path would overwrite
+// the FP64 M the refinement still reads).
+#ifdef GENEIG_FP32
+constexpr size_t HT_CAP = LDS_CAP;
+#else
+constexpr size_t HT_CAP = LDS_CAP / 2;
+#endif
 constexpr int ht_fit() {
     int nl = N;
-    while (nl > 1 && (size_t)nl * (nl + 1) / 2 * sizeof(cx) + HT_EXTRA > LDS_CAP) --nl;
+    while (nl > 1 && (size_t)nl * (nl + 1) / 2 * sizeof(cx) + HT_EXTRA > HT_CAP) --nl;
     return nl;
 }
 constexpr int NL = ht_fit();                  // order of the LDS-resident trailing triangle
