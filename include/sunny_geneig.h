@@ -277,7 +277,11 @@ sunny_geneig_varlen_t sunny_geneig_varlen_create_ex(int max_n, int device_id,
 void sunny_geneig_varlen_destroy(sunny_geneig_varlen_t solver);
 
 /* Asynchronous on `stream`; n_host (host memory, `count` orders) is read
- * before return. A handle is not reentrant: one call at a time per handle.
+ * before return. Calls on one handle are ordered after each other whatever
+ * the stream; the first call with given buffers and orders is captured into
+ * one hipGraph (every grid, the staged buckets included) that later calls
+ * with the same buffers and orders replay without any host-side sorting or
+ * upload. A handle is not reentrant: one call at a time per handle.
  * count == 0 is a no-op; an order outside [1, max_n] -> INVALID_ARG. */
 sunny_geneig_status_t sunny_geneig_varlen_launch(
     sunny_geneig_varlen_t solver,
@@ -292,6 +296,13 @@ sunny_geneig_status_t sunny_geneig_varlen_launch(
 
 /* Number of buckets, the order of bucket i and its matrices per workgroup
  * (-1 on NULL / out of range). */
+/* hipGraph replay of uneven-batch calls (see sunny_geneig_varlen_launch):
+ * enable = 0 launches the grids directly (as the environment variable
+ * SUNNY_GENEIG_GRAPH set to 0 at creation does); graph_stats reports captures
+ * and replays so far. Return 0 or INVALID_ARG. */
+int sunny_geneig_varlen_set_graphs(sunny_geneig_varlen_t solver, int enable);
+int sunny_geneig_varlen_graph_stats(sunny_geneig_varlen_t solver, long long* captures, long long* replays);
+
 int sunny_geneig_varlen_num_buckets (sunny_geneig_varlen_t solver);
 int sunny_geneig_varlen_bucket_order(sunny_geneig_varlen_t solver, int i);
 int sunny_geneig_varlen_bucket_waves(sunny_geneig_varlen_t solver, int i);

@@ -53,7 +53,7 @@ EXPORTED_SYMBOLS = (
     "sunny_geneig_session_poll", "sunny_geneig_session_wait", "sunny_geneig_session_stats",
     "sunny_geneig_session_request_bytes", "sunny_geneig_launch_chunked",
     "sunny_geneig_session_set_policy", "sunny_geneig_launch_noalloc", "sunny_geneig_launch_hdiag",
-    "sunny_geneig_blocking",
+    "sunny_geneig_blocking", "sunny_geneig_varlen_set_graphs", "sunny_geneig_varlen_graph_stats",
 )
 
 _lib = None
@@ -120,6 +120,11 @@ def lib() -> ctypes.CDLL:
     L.sunny_geneig_varlen_destroy.restype = None
     L.sunny_geneig_varlen_launch.argtypes = [vp, cu, cu, cu, cu, cu, pi, ci, cu]
     L.sunny_geneig_varlen_launch.restype = ci
+    L.sunny_geneig_varlen_set_graphs.argtypes = [vp, ci]
+    L.sunny_geneig_varlen_set_graphs.restype = ci
+    L.sunny_geneig_varlen_graph_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong),
+                                                  ctypes.POINTER(ctypes.c_longlong)]
+    L.sunny_geneig_varlen_graph_stats.restype = ci
     L.sunny_geneig_varlen_num_buckets.argtypes = [vp]
     L.sunny_geneig_varlen_num_buckets.restype = ci
     L.sunny_geneig_varlen_bucket_order.argtypes = [vp, ci]
@@ -420,6 +425,21 @@ class VarlenSolver:
         L = lib()
         return [L.sunny_geneig_varlen_bucket_order(self.handle, i)
                 for i in range(L.sunny_geneig_varlen_num_buckets(self.handle))]
+
+    def set_graphs(self, on: bool = True) -> None:
+        """Capture each call into a hipGraph and replay it when the buffers
+        and orders repeat (default on)."""
+        st = lib().sunny_geneig_varlen_set_graphs(self.handle, 1 if on else 0)
+        if st != STATUS_OK:
+            raise GeneigError(st, last_error())
+
+    def graph_stats(self):
+        """(captures, replays) of this handle's call graphs."""
+        c, r = ctypes.c_longlong(), ctypes.c_longlong()
+        st = lib().sunny_geneig_varlen_graph_stats(self.handle, ctypes.byref(c), ctypes.byref(r))
+        if st != STATUS_OK:
+            raise GeneigError(st, last_error())
+        return c.value, r.value
 
     @property
     def bucket_waves(self):

@@ -503,6 +503,32 @@ uint64_t sunny_geneig_session_request_bytes(int n, int count) {
     return sgeig::ServingSession::request_bytes(n, count);
 }
 
+int sunny_geneig_varlen_set_graphs(sunny_geneig_varlen_t s, int enable) {
+    clear_error();
+    if (!s) {
+        set_error("sunny_geneig_varlen_set_graphs: solver is NULL");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    try {
+        s->impl->set_graphs(enable != 0);
+        return SUNNY_GENEIG_OK;
+    } catch (const std::exception& e) {
+        set_error(std::string("sunny_geneig_varlen_set_graphs: ") + e.what());
+        return SUNNY_GENEIG_DEVICE_ERROR;
+    }
+}
+
+int sunny_geneig_varlen_graph_stats(sunny_geneig_varlen_t s, long long* captures, long long* replays) {
+    clear_error();
+    if (!s) {
+        set_error("sunny_geneig_varlen_graph_stats: solver is NULL");
+        return SUNNY_GENEIG_INVALID_ARG;
+    }
+    if (captures) *captures = s->impl->graph_captures();
+    if (replays) *replays = s->impl->graph_replays();
+    return SUNNY_GENEIG_OK;
+}
+
 int sunny_geneig_varlen_num_buckets(sunny_geneig_varlen_t s) {
     return s ? (int)s->impl->buckets().size() : -1;
 }

@@ -508,6 +508,21 @@ void Solver::launch_impl_(const void* H, const double* Hd, long long hds, const 
     if (cur != device_) SG_HIP(hipSetDevice(cur));
 }
 
+void Solver::launch_unordered(const void* H, const void* S, double* W, void* U, int* info, int batch,
+                              hipStream_t stream) {
+    if (batch < 0) throw std::invalid_argument("batch_size < 0");
+    if (batch == 0) return;
+    int cur = 0;
+    SG_HIP(hipGetDevice(&cur));
+    if (cur != device_) SG_HIP(hipSetDevice(device_));
+    ev_used_ = 0;
+    const bool multi = large_ || mode_ != Mode::Unified;
+    const Workspace ws = multi ? own_workspace_((size_t)batch) : Workspace{};
+    mark_(stream);
+    dispatch_(H, S, W, U, info, batch, stream, ws);
+    if (cur != device_) SG_HIP(hipSetDevice(cur));
+}
+
 void Solver::launch_chunked(const void* H, const void* S, double* W, void* U, int* info, int total,
                             int chunk, const std::vector<hipStream_t>& streams) {
     if (total < 0) throw std::invalid_argument("total_matrices < 0");

@@ -39,6 +39,13 @@ public:
     void launch_hdiag(const double* Hd, long long hdiag_stride, const void* S, double* W, void* U, int* info,
                       int batch, hipStream_t stream);
 
+    // As launch(), but without ordering against this handle's other launches
+    // and without graph capture, so that a caller which orders its launches
+    // itself can issue it into a stream it is capturing (the uneven-batch
+    // path). The workspace must already hold `batch` matrices (reserve()).
+    void launch_unordered(const void* H, const void* S, double* W, void* U, int* info, int batch,
+                          hipStream_t stream);
+
     // Contiguous chunks of `chunk` matrices launched round-robin over
     // `streams` (reference: UnifiedGeneigSolver::launch_chunked,
     // unified_solver.cpp:1078-1107). Fused-mode chunks run concurrently;

@@ -3,6 +3,7 @@
 #include "solver.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -70,6 +71,7 @@ VarlenSolver::VarlenSolver(int max_n, int device_id, const std::vector<int>& buc
     }
     SG_HIP(hipEventCreateWithFlags(&staged_, hipEventDisableTiming));
     SG_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
+    if (const char* v = std::getenv("SUNNY_GENEIG_GRAPH")) graphs_ = std::atoi(v) != 0;
 }
 
 VarlenSolver::~VarlenSolver() {
@@ -77,6 +79,12 @@ VarlenSolver::~VarlenSolver() {
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(device_);
     if (done_) (void)hipEventSynchronize(done_);
+    for (auto& e : graph_cache_) {
+        (void)hipGraphExecDestroy(e.exec);
+        (void)hipFree(e.desc);
+    }
+    graph_cache_.clear();
+    if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
     if (staged_) (void)hipEventDestroy(staged_);
     if (done_) (void)hipEventDestroy(done_);
     if (host_desc_) (void)hipHostFree(host_desc_);
@@ -101,6 +109,33 @@ void VarlenSolver::ensure_capacity_(int count) {
     desc_count_ = (size_t)count;
 }
 
+namespace {
+uint64_t orders_hash(const int* n, int count) {
+    uint64_t h = 0xcbf29ce484222325ULL;
+    const unsigned char* p = reinterpret_cast<const unsigned char*>(n);
+    for (size_t i = 0; i < (size_t)count * sizeof(int); ++i) {
+        h ^= p[i];
+        h *= 0x100000001b3ULL;
+    }
+    return h;
+}
+}  // namespace
+
+void VarlenSolver::clear_graphs_() {
+    if (graph_cache_.empty()) return;
+    SG_HIP(hipEventSynchronize(done_));
+    for (auto& e : graph_cache_) {
+        (void)hipGraphExecDestroy(e.exec);
+        (void)hipFree(e.desc);
+    }
+    graph_cache_.clear();
+}
+
+void VarlenSolver::set_graphs(bool on) {
+    graphs_ = on;
+    if (!on) clear_graphs_();
+}
+
 void VarlenSolver::launch(const void* H, const void* S, double* W, void* U, int* info, const int* n,
                           int count, hipStream_t stream) {
     if (count < 0) throw std::invalid_argument("count < 0");
@@ -120,9 +155,25 @@ void VarlenSolver::launch(const void* H, const void* S, double* W, void* U, int*
     int cur = 0;
     SG_HIP(hipGetDevice(&cur));
     if (cur != device_) SG_HIP(hipSetDevice(device_));
+    const uint64_t hash = graphs_ ? orders_hash(n, count) : 0;
+    if (graphs_) {
+        for (auto& e : graph_cache_) {
+            if (e.H == H && e.S == S && e.W == W && e.U == U && e.info == info && e.count == count &&
+                e.hash == hash) {
+                ++graph_replays_;
+                SG_HIP(hipStreamWaitEvent(stream, done_, 0));
+                SG_HIP(hipGraphLaunch(e.exec, stream));
+                SG_HIP(hipEventRecord(done_, stream));
+                if (cur != device_) SG_HIP(hipSetDevice(cur));
+                return;
+            }
+        }
+    }
 
     ensure_capacity_(count);
     size_t scratch_need = 0, pad_need = 0;
+    bool grew = false;
+    if (large_reserved_.size() != (size_t)nb) large_reserved_.assign(nb, 0);
     for (int b = 0; b < nb; ++b) {
         if (!per_bucket[b]) continue;
         const Bucket& B = buckets_[b];
@@ -130,7 +181,11 @@ void VarlenSolver::launch(const void* H, const void* S, double* W, void* U, int*
         if (B.large) {
             const size_t r = std::min<size_t>(per_bucket[b], kLargeRound);
             pad_need = std::max(pad_need, 3 * al256(r * no * no * 16) + al256(r * no * 8) + al256(r * 4));
-            B.large->reserve(r);
+            if (r > large_reserved_[b]) {
+                if (large_reserved_[b]) grew = true;      // its workspace may move
+                B.large->reserve(r);
+                large_reserved_[b] = r;
+            }
             continue;
         }
         const size_t wgs = std::min<size_t>((per_bucket[b] + B.nw - 1) / B.nw, kMaxGrid);
@@ -143,6 +198,7 @@ void VarlenSolver::launch(const void* H, const void* S, double* W, void* U, int*
         pad_bytes_ = 0;
         SG_HIP(hipMalloc(&pad_, pad_need));
         pad_bytes_ = pad_need;
+        grew = true;
     }
     if (scratch_need > scratch_bytes_) {
         SG_HIP(hipEventSynchronize(done_));
@@ -151,10 +207,13 @@ void VarlenSolver::launch(const void* H, const void* S, double* W, void* U, int*
         scratch_bytes_ = 0;
         SG_HIP(hipMalloc(&scratch_, scratch_need));
         scratch_bytes_ = scratch_need;
+        grew = true;
     }
+    if (grew) clear_graphs_();               // captured grids hold the old areas
 
     // The previous upload must have left the pinned staging area.
     if (staged_pending_) SG_HIP(hipEventSynchronize(staged_));
+    staged_pending_ = false;
     int64_t* moff = static_cast<int64_t*>(host_desc_);
     int64_t* woff = moff + count;
     int* ids = reinterpret_cast<int*>(woff + count);
@@ -174,14 +233,60 @@ void VarlenSolver::launch(const void* H, const void* S, double* W, void* U, int*
         std::vector<int> fill(start.begin(), start.end() - 1);
         for (int k = 0; k < count; ++k) ids[fill[bucket_of[k]]++] = k;
     }
-    // Descriptors and scratch are reused: order this call after the last one
-    // even when the caller switches streams.
+    const size_t desc_bytes = (size_t)count * 24;
+    if (!graphs_) {
+        // Descriptors and scratch are reused: order this call after the last
+        // one even when the caller switches streams.
+        SG_HIP(hipStreamWaitEvent(stream, done_, 0));
+        SG_HIP(hipMemcpyAsync(dev_desc_, host_desc_, desc_bytes, hipMemcpyHostToDevice, stream));
+        SG_HIP(hipEventRecord(staged_, stream));
+        staged_pending_ = true;
+        enqueue_(H, S, W, U, info, dev_desc_, count, per_bucket, start, stream);
+        SG_HIP(hipEventRecord(done_, stream));
+        if (cur != device_) SG_HIP(hipSetDevice(cur));
+        return;
+    }
+    // Capture: the graph gets its own descriptors (a later call with other
+    // orders must not overwrite what it reads).
+    if (graph_cache_.size() >= 4) clear_graphs_();
+    void* desc = nullptr;
+    SG_HIP(hipMalloc(&desc, desc_bytes));
+    try {
+        SG_HIP(hipMemcpy(desc, host_desc_, desc_bytes, hipMemcpyHostToDevice));
+        if (!cap_stream_) SG_HIP(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking));
+        SG_HIP(hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue_(H, S, W, U, info, desc, count, per_bucket, start, cap_stream_);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(cap_stream_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hipGraph_t g = nullptr;
+        SG_HIP(hipStreamEndCapture(cap_stream_, &g));
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        SG_HIP(ie);
+        graph_cache_.push_back({H, S, W, U, info, count, hash, desc, ex});
+        ++graph_captures_;
+    } catch (...) {
+        (void)hipFree(desc);
+        if (cur != device_) (void)hipSetDevice(cur);
+        throw;
+    }
     SG_HIP(hipStreamWaitEvent(stream, done_, 0));
-    SG_HIP(hipMemcpyAsync(dev_desc_, host_desc_, (size_t)count * 24, hipMemcpyHostToDevice, stream));
-    SG_HIP(hipEventRecord(staged_, stream));
-    staged_pending_ = true;
+    SG_HIP(hipGraphLaunch(graph_cache_.back().exec, stream));
+    SG_HIP(hipEventRecord(done_, stream));
+    if (cur != device_) SG_HIP(hipSetDevice(cur));
+}
 
-    const int64_t* d_moff = static_cast<const int64_t*>(dev_desc_);
+void VarlenSolver::enqueue_(const void* H, const void* S, double* W, void* U, int* info, const void* desc,
+                            int count, const std::vector<int>& per_bucket, const std::vector<int>& start,
+                            hipStream_t stream) {
+    const int nb = (int)buckets_.size();
+    const int64_t* d_moff = static_cast<const int64_t*>(desc);
     const int64_t* d_woff = d_moff + count;
     const int* d_ids = reinterpret_cast<const int*>(d_woff + count);
     const int* d_nn = d_ids + count;
@@ -204,7 +309,7 @@ void VarlenSolver::launch(const void* H, const void* S, double* W, void* U, int*
                 const int* list = d_ids + start[b] + off;
                 void* ga[] = {&H, &S, &list, &d_moff, &d_nn, &cnt, &Hp, &Sp};
                 SG_HIP(hipModuleLaunchKernel(B.gather, (unsigned)cnt, 1, 1, 256, 1, 1, 0, stream, ga, nullptr));
-                B.large->launch(Hp, Sp, Wp, Up, Ip, cnt, stream);
+                B.large->launch_unordered(Hp, Sp, Wp, Up, Ip, cnt, stream);
                 void* sa[] = {&Wp, &Up, &Ip, &list, &d_moff, &d_woff, &d_nn, &cnt, &W, &U, &info};
                 SG_HIP(hipModuleLaunchKernel(B.scatter, (unsigned)cnt, 1, 1, 256, 1, 1, 0, stream, sa, nullptr));
             }
@@ -219,8 +324,6 @@ void VarlenSolver::launch(const void* H, const void* S, double* W, void* U, int*
                                          args, nullptr));
         }
     }
-    SG_HIP(hipEventRecord(done_, stream));
-    if (cur != device_) SG_HIP(hipSetDevice(cur));
 }
 
 }  // namespace sgeig

@@ -38,9 +38,19 @@ public:
     // Matrices packed back to back: matrix k (order n[k]) at element offset
     // sum_{j<k} n[j]^2 of H, S and U, its eigenvalues at sum_{j<k} n[j] of W,
     // its status at info[k]. Asynchronous on `stream`; `n` is read before
-    // return. Not reentrant for one handle (one staging buffer).
+    // return. Calls are ordered after the handle's previous call whatever the
+    // stream. The first call with given buffers and orders captures every
+    // grid of the call (the LDS-resident buckets, and the staged buckets'
+    // gather, solve and scatter) into one hipGraph with its own copy of the
+    // work-list descriptors; a later call with the same buffers and orders
+    // (compared by count and a 64-bit hash of the orders) replays it: one
+    // graph launch, no host sort, no upload. SUNNY_GENEIG_GRAPH=0 at creation
+    // or set_graphs(false) launches the grids directly.
     void launch(const void* H, const void* S, double* W, void* U, int* info, const int* n,
                 int count, hipStream_t stream);
+    void set_graphs(bool on);
+    long long graph_captures() const { return graph_captures_; }
+    long long graph_replays() const { return graph_replays_; }
 
     int max_n() const { return max_n_; }
     const std::vector<int>& buckets() const { return orders_; }
@@ -65,6 +75,28 @@ private:
         hipFunction_t gather = nullptr, scatter = nullptr;
     };
     void ensure_capacity_(int count);
+    // Every grid of one call, on `stream`, reading the work-list descriptors
+    // at `desc` (moff | woff | ids | nn for `count` matrices).
+    void enqueue_(const void* H, const void* S, double* W, void* U, int* info, const void* desc, int count,
+                  const std::vector<int>& per_bucket, const std::vector<int>& start, hipStream_t stream);
+
+    struct GraphEntry {
+        const void* H;
+        const void* S;
+        double* W;
+        void* U;
+        int* info;
+        int count;
+        uint64_t hash;
+        void* desc;              // the descriptors this graph reads (device)
+        hipGraphExec_t exec;
+    };
+    bool graphs_ = true;
+    hipStream_t cap_stream_ = nullptr;
+    std::vector<GraphEntry> graph_cache_;
+    long long graph_captures_ = 0, graph_replays_ = 0;
+    std::vector<size_t> large_reserved_;   // matrices each staged bucket's solver holds
+    void clear_graphs_();
 
     int max_n_ = 0;
     int device_ = 0;

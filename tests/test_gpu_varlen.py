@@ -289,3 +289,51 @@ def test_c3_law_vs_zhegv(vs128):
         S = np.stack([Ss[k] for k in ks])
         T.assert_parity_conditioned(H, S, np.stack([W[k] for k in ks]), np.stack([U[k] for k in ks]),
                                     info[ks], tag=f"C3 n={n} x{len(ks)}")
+
+
+def test_graph_replay_of_uneven_calls(vs128):
+    """A call is captured once per (buffers, orders) into one hipGraph that
+    holds every grid, staged buckets included; repeated calls replay it with
+    bit-identical results, a call with other orders in between gets its own
+    graph and does not disturb the first, and direct launches agree."""
+    import torch
+    import sunny_gen_eigensolver_amd as sg
+    rng = np.random.default_rng(0x6A)
+    o1 = [int(x) for x in rng.integers(12, 129, size=40)] + [70, 128, 64]
+    o2 = [int(x) for x in rng.integers(1, 129, size=25)]
+    H1, S1 = _gen_mix(o1, 7100)
+    H2, S2 = _gen_mix(o2, 7200)
+    dev = torch.device("cuda", 0)
+    packs = {}
+    for tag, (Hs, Ss, oo) in (("a", (H1, S1, o1)), ("b", (H2, S2, o2))):
+        dH = sg.pack([torch.from_numpy(np.ascontiguousarray(h)) for h in Hs]).to(dev)
+        dS = sg.pack([torch.from_numpy(np.ascontiguousarray(s_)) for s_ in Ss]).to(dev)
+        W = torch.empty((sum(oo),), dtype=torch.float64, device=dev)
+        U = torch.empty((sum(x * x for x in oo),), dtype=torch.complex128, device=dev)
+        I = torch.zeros((len(oo),), dtype=torch.int32, device=dev)
+        packs[tag] = (dH, dS, W, U, I, oo)
+    c0, r0 = vs128.graph_stats()
+    outs = []
+    for tag in ("a", "a", "b", "a", "b"):
+        dH, dS, W, U, I, oo = packs[tag]
+        W.fill_(np.nan)
+        U.zero_()
+        vs128.run(dH, dS, oo, W, U, I)
+        torch.cuda.synchronize()
+        outs.append((tag, W.cpu().numpy().copy(), U.cpu().numpy().copy(), I.cpu().numpy().copy()))
+    c1, r1 = vs128.graph_stats()
+    assert (c1 - c0, r1 - r0) == (2, 3), (c1 - c0, r1 - r0)
+    ref = {t: o for t, *o in outs[:1] + outs[2:3]}
+    for tag, W, U, I in outs:
+        assert np.array_equal(W, ref[tag][0]) and np.array_equal(U, ref[tag][1]) and np.array_equal(I, ref[tag][2])
+    vs128.set_graphs(False)
+    try:
+        dH, dS, W, U, I, oo = packs["a"]
+        vs128.run(dH, dS, oo, W, U, I)
+        torch.cuda.synchronize()
+        assert np.array_equal(W.cpu().numpy(), ref["a"][0]) and np.array_equal(U.cpu().numpy(), ref["a"][1])
+    finally:
+        vs128.set_graphs(True)
+    Wl = [w for w in sg.unpack_vectors(torch.from_numpy(ref["a"][0]), o1)]
+    Ul = [u.numpy() for u in sg.unpack_matrices(torch.from_numpy(ref["a"][1]), o1)]
+    _check_groups(H1, S1, [w.numpy() for w in Wl], Ul, ref["a"][2], o1, "graph replay")
