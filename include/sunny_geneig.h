@@ -294,8 +294,6 @@ sunny_geneig_status_t sunny_geneig_varlen_launch(
     int                   count,
     uintptr_t             stream);
 
-/* Number of buckets, the order of bucket i and its matrices per workgroup
- * (-1 on NULL / out of range). */
 /* hipGraph replay of uneven-batch calls (see sunny_geneig_varlen_launch):
  * enable = 0 launches the grids directly (as the environment variable
  * SUNNY_GENEIG_GRAPH set to 0 at creation does); graph_stats reports captures
@@ -303,6 +301,8 @@ sunny_geneig_status_t sunny_geneig_varlen_launch(
 int sunny_geneig_varlen_set_graphs(sunny_geneig_varlen_t solver, int enable);
 int sunny_geneig_varlen_graph_stats(sunny_geneig_varlen_t solver, long long* captures, long long* replays);
 
+/* Number of buckets, the order of bucket i and its matrices per workgroup
+ * (-1 on NULL / out of range). */
 int sunny_geneig_varlen_num_buckets (sunny_geneig_varlen_t solver);
 int sunny_geneig_varlen_bucket_order(sunny_geneig_varlen_t solver, int i);
 int sunny_geneig_varlen_bucket_waves(sunny_geneig_varlen_t solver, int i);
