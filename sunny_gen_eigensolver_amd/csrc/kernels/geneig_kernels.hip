@@ -232,20 +232,36 @@ __device__ __forceinline__ void eig2_pencil(real t11, real t22, cx t12, real g11
     X[0][1] = (y2a - cmul(cl21, X[1][1])) * (real(1.0) / l11);
 }
 
-#ifndef GENEIG_FP32
-// A 16 x 16 complex FP64 tile on the matrix cores: v_mfma_f64_16x16x4f64
-// takes A(i, k) from lane i + 16 k and B(k, j) from lane j + 16 k (i, j < 16,
-// k < 4) and leaves D(i, j) in lane j + 16 (i % 4), element i / 4; a complex
-// product is four real MFMAs into a (re, im) pair of accumulators.
-typedef double d4 __attribute__((ext_vector_type(4)));
+// A 16 x 16 complex tile on the matrix cores in the module precision:
+// v_mfma_f64_16x16x4f64, or v_mfma_f32_16x16x4f32 in the FP32 build. Both take
+// A(i, k) from lane i + 16 k and B(k, j) from lane j + 16 k (i, j < 16,
+// k < 4) and leave D(i, j) in lane j + 16 g, element e: FP64 with g = i % 4,
+// e = i / 4; FP32 with g = i / 4, e = i % 4 (acc_row gives i). Element e of
+// a result is therefore the B operand of the k-step over rows acc_row(e, g).
+// A complex product is four real MFMAs into a (re, im) pair of accumulators.
+typedef real r4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ r4 mfma16(real a, real b, r4 c) {
+#ifdef GENEIG_FP32
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+#else
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+#endif
+}
+__device__ __forceinline__ constexpr int acc_row(int e, int g) {
+#ifdef GENEIG_FP32
+    return 4 * g + e;
+#else
+    return 4 * e + g;
+#endif
+}
 struct ctile {
-    d4 re, im;
+    r4 re, im;
 };
 __device__ __forceinline__ void cmfma(ctile& c, cx a, cx b) {
-    c.re = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, c.re, 0, 0, 0);
-    c.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, c.re, 0, 0, 0);
-    c.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, c.im, 0, 0, 0);
-    c.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, c.im, 0, 0, 0);
+    c.re = mfma16(a.x, b.x, c.re);
+    c.re = mfma16(-a.y, b.y, c.re);
+    c.im = mfma16(a.x, b.y, c.im);
+    c.im = mfma16(a.y, b.x, c.im);
 }
 __device__ __forceinline__ cx tile_get(const ctile& t, int v) { return cm(t.re[v], t.im[v]); }
 
@@ -254,17 +270,17 @@ __device__ __forceinline__ cx tile_get(const ctile& t, int v) { return cm(t.re[v
 // rounding error of im is bounded by eps (|ar| + |ai|)(|br| + |bi|) per term,
 // at most twice the four-product form's. bs = br + bi, formed once per B.
 struct ctile3 {
-    d4 p, q, s;
+    r4 p, q, s;
 };
 __device__ __forceinline__ void c3_zero(ctile3& c) {
-    c.p = d4{0.0, 0.0, 0.0, 0.0};
-    c.q = d4{0.0, 0.0, 0.0, 0.0};
-    c.s = d4{0.0, 0.0, 0.0, 0.0};
+    c.p = r4{real(0.0), real(0.0), real(0.0), real(0.0)};
+    c.q = c.p;
+    c.s = c.p;
 }
-__device__ __forceinline__ void cmfma3(ctile3& c, cx a, cx b, double bs) {
-    c.p = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, c.p, 0, 0, 0);
-    c.q = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, c.q, 0, 0, 0);
-    c.s = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x + a.y, bs, c.s, 0, 0, 0);
+__device__ __forceinline__ void cmfma3(ctile3& c, cx a, cx b, real bs) {
+    c.p = mfma16(a.x, b.x, c.p);
+    c.q = mfma16(a.y, b.y, c.q);
+    c.s = mfma16(a.x + a.y, bs, c.s);
 }
 __device__ __forceinline__ ctile c3_final(const ctile3& c) {
     ctile t;
@@ -272,7 +288,6 @@ __device__ __forceinline__ ctile c3_final(const ctile3& c) {
     t.im = c.s - c.p - c.q;
     return t;
 }
-#endif
 
 }  // namespace gk
 

@@ -244,6 +244,19 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
                 throw std::runtime_error("staged workspace layout mismatch: module " + std::to_string(dev_ws) +
                                          " B, host " + std::to_string(large_matrix_bytes(n_)) + " B");
         }
+        {
+            // V = Q Z runs one wave per 16-column panel where it is built on
+            // the matrix cores: the module states its workgroup size.
+            hipDeviceptr_t sym = nullptr;
+            size_t sym_bytes = 0;
+            int nt = 0;
+            LoadedModule* bm = (mode_ == Mode::Mixed) ? module32_.get() : module_.get();
+            SG_HIP(hipModuleGetGlobal(&sym, &sym_bytes, bm->module, "geneig_lg_back_threads"));
+            SG_HIP(hipMemcpyDtoH(&nt, sym, sizeof(nt)));
+            if (nt <= 0 || nt % 64 != 0 || nt > 1024)
+                throw std::runtime_error("geneig_lg_back states " + std::to_string(nt) + " threads per block");
+            lg_back_threads_ = (unsigned)nt;
+        }
         lds_probe = k_lg_hetrd_;
         int lds = 0;
         SG_HIP(hipFuncGetAttribute(&lds, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, lds_probe));
@@ -878,7 +891,7 @@ void Solver::launch_large_(const void* H, const void* S, double* W, void* U, int
         mark();
         launch_(k_lg_qlz_, (unsigned)cnt, bd_ql, stream, a4);
         mark();
-        launch_(k_lg_back_, (unsigned)cnt, bd, stream, a3);
+        launch_(k_lg_back_, (unsigned)cnt, lg_back_threads_, stream, a3);
         mark();
         if (mode_ == Mode::Mixed) {
             int iters = refine_iters_;
