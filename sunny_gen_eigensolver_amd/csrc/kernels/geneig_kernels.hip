@@ -2226,28 +2226,24 @@ geneig_mx_front(const gk::cf* __restrict__ M32, real* __restrict__ W32, cx* __re
     }
 }
 
-// V = Q Z per wave (the FP32 back half without the W^H factor, which the FP64
-// refinement applies): reflectors staged from the stash in U32, V over it.
-namespace gk {
-constexpr size_t REFL_SLICE = ((size_t)(NREFL + N) * sizeof(cx) + 15) / 16 * 16;   // reflectors + tau
-}  // namespace gk
-
-extern "C" __global__ void __launch_bounds__(GENEIG_NT)
+// V = Q Z on the matrix cores (back_wy; the FP32 back half without the W^H
+// factor, which the FP64 refinement applies), one workgroup of one wave per
+// 16-column panel per matrix: reflectors from the stash in U32 (v_k(i) at
+// row k, column i; tau_k at row k, column k + 1), V over it. Four waves per
+// SIMD: the compiler would otherwise spend registers on three.
+extern "C" __global__ void __launch_bounds__(gk::BNT) __attribute__((amdgpu_waves_per_eu(4)))
 geneig_mx_back(cx* __restrict__ U32, const real* __restrict__ Zw, const int* __restrict__ info, int batch) {
     using namespace gk;
-    __shared__ __attribute__((aligned(16))) char slices[NW][REFL_SLICE];
-    const int wave = wave_id(), lane = lane_id();
-    const int b = blockIdx.x * NW + wave;
-    if (!(b < batch && info[b] <= N)) return;       // no workgroup barrier below
+    __shared__ BackWork bw;
+    const int b = blockIdx.x;
+    if (b >= batch || info[b] > N) return;
     const size_t off = (size_t)b * N * N;
-    cx* sl = reinterpret_cast<cx*>(slices[wave]);
-    cx v[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = cm((lane < N) ? Zw[off + lane * N + i] : real(0.0), 0.0);
-    stage_reflectors(sl, U32 + off);
-    wave_lds_sync();
-    apply_q_wave(sl, v);
-    store_columns(U32 + off, v);
+    cx* Ub = U32 + off;
+    const real* Zb = Zw + off;
+    back_wy<true>(
+        bw, [&](int k, int row) { return Ub[(size_t)row * N + k]; }, [&](int k) { return Ub[(size_t)(k + 1) * N + k]; },
+        [&](int row, int col) { return Zb[(size_t)col * N + row]; },
+        [&](int row, int col, cx v) { Ub[(size_t)col * N + row] = v; });
 }
 #endif  // GENEIG_FP32
 

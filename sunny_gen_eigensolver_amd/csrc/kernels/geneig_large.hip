@@ -93,6 +93,189 @@ __device__ __forceinline__ void dset(real (&a)[ND], int idx, real x) {
 
 __device__ __forceinline__ cx wave_sum_cx(cx v) { return cm(wave_sum(v.x), wave_sum(v.y)); }
 
+// V = Q Z on the matrix cores, for the staged pipeline (geneig_lg_back) and
+// the mixed mode's FP32 back half up to order 64 (geneig_mx_back), which keep
+// the reflectors in different places. The reflectors are taken 16 at a
+// time: H_k0 H_k0+1 ... H_k0+15 = I - Y T Y^H with Y the block's reflector
+// columns (v_k(k+1) = 1, zero above) and T upper triangular (T_jj = tau_j,
+// T(0:j, j) = -tau_j T(0:j, 0:j) Y(:, 0:j)^H v_j, the forward columnwise form
+// of LAPACK's zlarft). Blocks are applied last first: X <- X - Y (T (Y^H X)).
+// One wave per 16-column panel of X, held in registers as row tiles in the
+// accumulator layout; a block touches only the row tiles at or below its
+// first reflector's row.
+//   prologue: wave w forms T for blocks w, w + BRT, ... (Gram Y^H Y on the
+//             matrix cores, the recurrence on 16 lanes) into LDS;
+//   per block: Y staged in LDS (coalesced, masked), then per wave
+//             W1 = Y^H X, C2 = T W1, X -= Y C2, three complex products in the
+//             3M form.
+// Every read of the reflectors happens before the barrier that follows the
+// last staging, so V may overwrite them.
+constexpr int BRT = (N + 15) / 16;          // row tiles = column panels = waves
+constexpr int BNT = 64 * BRT;
+constexpr int NBK = (N - 1 + 15) / 16;      // blocks of the N - 1 reflectors
+constexpr int BYLD = 16 * BRT + 1;          // staged column stride (odd: no bank conflicts)
+constexpr int TLD = 17;                     // row stride of T in LDS
+struct BackWork {
+    cx T[NBK > 0 ? NBK : 1][16 * TLD];
+    cx Y[16 * BYLD];                          // Y(row, m) at [m * BYLD + row]
+};
+static_assert(BRT * 256 <= 16 * BYLD, "the Gram scratch of every wave fits the staged block");
+static_assert(16 * 16 * BRT <= 4 * BNT, "a block is at most four elements per thread");
+
+// vget(k, row): the stored v_k(row), k <= N-2, k+2 <= row < N; tget(k): tau_k;
+// zget(row, col): Z; vput(row, col, v): V(row, col) = v. KFAST: consecutive
+// reflectors of one row are adjacent in memory (else consecutive rows of one
+// reflector), which sets the order in which threads stage a block. Each
+// thread holds at most 4 elements of a block; the next block's are loaded
+// while the current one is applied.
+constexpr int BSPT = 4;                     // staged elements per thread: 16 x 16 BRT / BNT
+template <bool KFAST, class FV, class FT, class FZ, class FP>
+__device__ __forceinline__ void back_wy(BackWork& bw, FV vget, FT tget, FZ zget, FP vput) {
+    const int lane = lane_id(), wave = wave_id();
+    const int c16 = lane & 15, k4 = lane >> 4;
+    // Element (row, m) of reflector block k0: v_{k0+m}(row).
+    auto refl = [&](int k0, int m, int row) -> cx {
+        const int k = k0 + m;
+        if (k > N - 2 || row <= k || row >= N) return cm(0.0, 0.0);
+        if (row == k + 1) return cm(1.0, 0.0);
+        return vget(k, row);
+    };
+
+    // T of every block. The Gram matrix goes through this wave's slice of the
+    // (not yet used) staging buffer; lane i < 16 then forms row i of T.
+    cx* G = bw.Y + wave * 256;
+    for (int bi = wave; bi < NBK; bi += BRT) {
+        const int k0 = 16 * bi;
+        ctile3 a3;
+        c3_zero(a3);
+#pragma unroll
+        for (int rt = 0; rt < BRT; ++rt) {
+            if (rt < bi) continue;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const cx y = refl(k0, c16, 16 * rt + acc_row(e, k4));
+                cmfma3(a3, cconj(y), y, y.x + y.y);
+            }
+        }
+        const ctile g = c3_final(a3);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) G[acc_row(e, k4) * 16 + c16] = tile_get(g, e);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < 16) {
+            const int i = lane;
+            cx t[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const cx tau = (k0 + j <= N - 2) ? tget(k0 + j) : cm(0.0, 0.0);
+                cx s = cm(0.0, 0.0);
+#pragma unroll
+                for (int m = 0; m < j; ++m)
+                    if (m >= i) s = s + cmul(t[m], G[m * 16 + j]);
+                t[j] = (i < j) ? cm(0.0, 0.0) - cmul(tau, s) : (i == j ? tau : cm(0.0, 0.0));
+            }
+#pragma unroll
+            for (int j = 0; j < 16; ++j) bw.T[bi][i * TLD + j] = t[j];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+
+    // This wave's panel: columns 16 wave .. 16 wave + 15 of Z.
+    const int col = 16 * wave + c16;
+    ctile x[BRT];
+#pragma unroll
+    for (int rt = 0; rt < BRT; ++rt)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int row = 16 * rt + acc_row(e, k4);
+            x[rt].re[e] = (row < N && col < N) ? zget(row, col) : real(0.0);
+            x[rt].im[e] = real(0.0);
+        }
+
+    // Element s of this thread's share of block bi: (m, row), or m < 0 past its end.
+    auto slot = [&](int bi, int sidx, int& m, int& row) {
+        const int r0 = 16 * bi, nr = 16 * BRT - r0;
+        const int idx = (int)threadIdx.x + BNT * sidx;
+        m = -1;
+        row = 0;
+        if (idx < 16 * nr) {
+            m = KFAST ? idx % 16 : idx / nr;
+            row = r0 + (KFAST ? idx / 16 : idx % nr);
+        }
+    };
+    cx pre[BSPT];
+    auto fetch = [&](int bi) {
+#pragma unroll
+        for (int u = 0; u < BSPT; ++u) {
+            int m, row;
+            slot(bi, u, m, row);
+            pre[u] = (m >= 0) ? refl(16 * bi, m, row) : cm(0.0, 0.0);
+        }
+    };
+    if (NBK > 0) fetch(NBK - 1);
+    for (int bi = NBK - 1; bi >= 0; --bi) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < BSPT; ++u) {
+            int m, row;
+            slot(bi, u, m, row);
+            if (m >= 0) bw.Y[m * BYLD + row] = pre[u];
+        }
+        __syncthreads();
+        if (bi > 0) fetch(bi - 1);
+        // W1 = Y^H X (16 x 16): A(m, row) = conj(Y(row, m)), B = the X tiles.
+        ctile3 a3;
+        c3_zero(a3);
+#pragma unroll
+        for (int rt = 0; rt < BRT; ++rt) {
+            if (rt < bi) continue;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const cx y = bw.Y[c16 * BYLD + 16 * rt + acc_row(e, k4)];
+                const cx xe = tile_get(x[rt], e);
+                cmfma3(a3, cconj(y), xe, xe.x + xe.y);
+            }
+        }
+        const ctile w1 = c3_final(a3);
+        // C2 = T W1.
+        c3_zero(a3);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const cx t = bw.T[bi][c16 * TLD + acc_row(e, k4)];
+            const cx we = tile_get(w1, e);
+            cmfma3(a3, t, we, we.x + we.y);
+        }
+        const ctile c2 = c3_final(a3);
+        // X(row tile) -= Y(row tile, :) C2.
+        real c2s[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) c2s[e] = c2.re[e] + c2.im[e];
+#pragma unroll
+        for (int rt = 0; rt < BRT; ++rt) {
+            if (rt < bi) continue;
+            c3_zero(a3);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const cx y = bw.Y[acc_row(e, k4) * BYLD + 16 * rt + c16];
+                cmfma3(a3, y, tile_get(c2, e), c2s[e]);
+            }
+            const ctile d = c3_final(a3);
+            x[rt].re = x[rt].re - d.re;
+            x[rt].im = x[rt].im - d.im;
+        }
+    }
+#pragma unroll
+    for (int rt = 0; rt < BRT; ++rt)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int row = 16 * rt + acc_row(e, k4);
+            if (row < N && col < N) vput(row, col, tile_get(x[rt], e));
+        }
+}
+
 // ---------------------------------------------------------------------------
 // Tridiagonalisation (module precision).
 // ---------------------------------------------------------------------------
@@ -1069,41 +1252,8 @@ geneig_lg_qlz(real* __restrict__ wout, long long wout_stride, int* __restrict__ 
 }
 
 #if defined(GENEIG_FP32) || GENEIG_N <= 128
-namespace gk {
-// V = Q Z on the matrix cores (the FP32 build at every staged order, the FP64
-// build up to order 128, where its register tiles fit). The reflectors are
-// taken 16 at a time: H_k0 H_k0+1 ... H_k0+15 = I - Y T Y^H with Y the block's
-// reflector columns (v_k(k+1) = 1, zero above) and T upper triangular
-// (T_jj = tau_j, T(0:j, j) = -tau_j T(0:j, 0:j) Y(:, 0:j)^H v_j, the
-// forward columnwise form of LAPACK's zlarft). Blocks are applied last first:
-// X <- X - Y (T (Y^H X)). One wave per 16-column panel of X, held in
-// registers as row tiles in the accumulator layout; a block touches only the
-// row tiles at or below its first reflector's row.
-//   prologue: wave w forms T for blocks w, w + BRT, ... (Gram Y^H Y on the
-//             matrix cores, the recurrence on 16 lanes) into LDS;
-//   per block: Y staged in LDS (coalesced, masked), then per wave
-//             W1 = Y^H X, C2 = T W1, X -= Y C2, three complex products in the
-//             3M form.
-constexpr int BRT = (N + 15) / 16;          // row tiles = column panels = waves
-constexpr int BNT = 64 * BRT;
-constexpr int NBK = (N - 1 + 15) / 16;      // blocks of the N - 1 reflectors
-constexpr int BYLD = 16 * BRT + 1;           // staged column stride (odd: no bank conflicts)
-constexpr int TLD = 17;                     // row stride of T in LDS
-struct BackWork {
-    cx T[NBK][16 * TLD];
-    cx Y[16 * BYLD];                           // Y(row, m) at [m * BYLD + row]
-};
-static_assert(BRT * 256 <= 16 * BYLD, "the Gram scratch of every wave fits the staged block");
-
-// Element (row, m) of reflector block k0: v_{k0+m}(row).
-__device__ __forceinline__ cx refl(const cx* __restrict__ R, int k0, int m, int row) {
-    const int k = k0 + m;
-    if (k > N - 2 || row <= k || row >= N) return cm(0.0, 0.0);
-    if (row == k + 1) return cm(1.0, 0.0);
-    return R[(size_t)k * N + row];
-}
-}  // namespace gk
-
+// V = Q Z on the matrix cores (back_wy): the FP32 build at every staged
+// order, the FP64 build up to order 128, where its register tiles fit.
 // Workgroup size of geneig_lg_back, read by the host before launching it.
 extern "C" __device__ const int geneig_lg_back_threads = gk::BNT;
 
@@ -1113,124 +1263,14 @@ geneig_lg_back(const int* __restrict__ info, char* __restrict__ ws, long long ws
     __shared__ BackWork bw;
     const int b = blockIdx.x;
     if (b >= batch || info[b] > N) return;
-    const int lane = lane_id(), wave = wave_id();
-    const int c16 = lane & 15, k4 = lane >> 4;
     char* wsb = ws_of(ws, wss, b);
     const cx* R = reinterpret_cast<const cx*>(wsb + WS_R);
     const real* Z = reinterpret_cast<const real*>(wsb + WS_Z);
     cx* V = reinterpret_cast<cx*>(wsb + WS_V);
-
-    // T of every block. The Gram matrix goes through this wave's slice of the
-    // (not yet used) staging buffer; lane i < 16 then forms row i of T.
-    cx* G = bw.Y + wave * 256;
-    for (int bi = wave; bi < NBK; bi += BRT) {
-        const int k0 = 16 * bi;
-        ctile3 a3;
-        c3_zero(a3);
-#pragma unroll
-        for (int rt = 0; rt < BRT; ++rt) {
-            if (rt < bi) continue;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const cx y = refl(R, k0, c16, 16 * rt + acc_row(e, k4));
-                cmfma3(a3, cconj(y), y, y.x + y.y);
-            }
-        }
-        const ctile g = c3_final(a3);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) G[acc_row(e, k4) * 16 + c16] = tile_get(g, e);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (lane < 16) {
-            const int i = lane;
-            cx t[16];
-#pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                const cx tau = (k0 + j <= N - 2) ? R[(size_t)(k0 + j) * N + k0 + j + 1] : cm(0.0, 0.0);
-                cx s = cm(0.0, 0.0);
-#pragma unroll
-                for (int m = 0; m < j; ++m)
-                    if (m >= i) s = s + cmul(t[m], G[m * 16 + j]);
-                t[j] = (i < j) ? cm(0.0, 0.0) - cmul(tau, s) : (i == j ? tau : cm(0.0, 0.0));
-            }
-#pragma unroll
-            for (int j = 0; j < 16; ++j) bw.T[bi][i * TLD + j] = t[j];
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-
-    // This wave's panel: columns 16 wave .. 16 wave + 15 of Z.
-    const int col = 16 * wave + c16;
-    ctile x[BRT];
-#pragma unroll
-    for (int rt = 0; rt < BRT; ++rt)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int row = 16 * rt + acc_row(e, k4);
-            x[rt].re[e] = (row < N && col < N) ? Z[(size_t)col * N + row] : real(0.0);
-            x[rt].im[e] = real(0.0);
-        }
-
-    for (int bi = NBK - 1; bi >= 0; --bi) {
-        const int k0 = 16 * bi;
-        const int r0 = 16 * bi, nr = 16 * BRT - r0;
-        __syncthreads();
-        for (int idx = threadIdx.x; idx < 16 * nr; idx += BNT) {
-            const int m = idx / nr, row = r0 + idx % nr;
-            bw.Y[m * BYLD + row] = refl(R, k0, m, row);
-        }
-        __syncthreads();
-        // W1 = Y^H X (16 x 16): A(m, row) = conj(Y(row, m)), B = the X tiles.
-        ctile3 a3;
-        c3_zero(a3);
-#pragma unroll
-        for (int rt = 0; rt < BRT; ++rt) {
-            if (rt < bi) continue;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const cx y = bw.Y[c16 * BYLD + 16 * rt + acc_row(e, k4)];
-                const cx xe = tile_get(x[rt], e);
-                cmfma3(a3, cconj(y), xe, xe.x + xe.y);
-            }
-        }
-        const ctile w1 = c3_final(a3);
-        // C2 = T W1.
-        c3_zero(a3);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const cx t = bw.T[bi][c16 * TLD + acc_row(e, k4)];
-            const cx we = tile_get(w1, e);
-            cmfma3(a3, t, we, we.x + we.y);
-        }
-        const ctile c2 = c3_final(a3);
-        // X(row tile) -= Y(row tile, :) C2.
-        real c2s[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) c2s[e] = c2.re[e] + c2.im[e];
-#pragma unroll
-        for (int rt = 0; rt < BRT; ++rt) {
-            if (rt < bi) continue;
-            c3_zero(a3);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const cx y = bw.Y[acc_row(e, k4) * BYLD + 16 * rt + c16];
-                cmfma3(a3, y, tile_get(c2, e), c2s[e]);
-            }
-            const ctile d = c3_final(a3);
-            x[rt].re = x[rt].re - d.re;
-            x[rt].im = x[rt].im - d.im;
-        }
-    }
-#pragma unroll
-    for (int rt = 0; rt < BRT; ++rt)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int row = 16 * rt + acc_row(e, k4);
-            if (row < N && col < N) V[(size_t)col * N + row] = tile_get(x[rt], e);
-        }
+    back_wy<false>(
+        bw, [&](int k, int row) { return R[(size_t)k * N + row]; }, [&](int k) { return R[(size_t)k * N + k + 1]; },
+        [&](int row, int col) { return Z[(size_t)col * N + row]; },
+        [&](int row, int col, cx v) { V[(size_t)col * N + row] = v; });
 }
 #else
 // V = Q Z = H_0 H_1 ... H_{N-2} Z (reflectors applied last first), eight

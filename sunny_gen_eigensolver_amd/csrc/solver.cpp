@@ -771,8 +771,9 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
     void* aq[] = {&W32, &E32, &Z32, &info, &qlog, &qstride, &batch, &all};
     launch_(k_qlfull32_, grid_w, bd, stream, aq);
     mark_(stream);
+    // V = Q Z: one workgroup per matrix, one wave per 16-column panel.
     void* a4[] = {&V32, &Z32, &info, &batch};
-    launch_(k_back32_, grid_w, bd, stream, a4);
+    launch_(k_back32_, grid_b, bd_ref, stream, a4);
     mark_(stream);
     int iters = refine_iters_;
     double delta = refine_delta_;
