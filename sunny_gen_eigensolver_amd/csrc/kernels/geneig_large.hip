@@ -1183,26 +1183,110 @@ geneig_lg_qlr(real* __restrict__ wout, long long wout_stride, int* __restrict__ 
     }
 }
 
-// QL, second half: Z = product of the logged rotations, one row per thread
-// (rows in LDS, or in global memory for the rows that do not fit), then Z with
-// its columns in ascending order to the workspace. A matrix whose log
-// overflowed is solved here inline instead.
+namespace gk {
+constexpr int QZ_CH = 8;                      // columns per branch-free chunk of the applier
+}  // namespace gk
+
+// Rows of Z in registers: every staged order in FP32, up to order 96 in FP64
+// (a row is N registers in FP32, 2N in FP64).
+#if defined(GENEIG_FP32) || GENEIG_N <= 96
+// QL, second half: Z = product of the logged rotations, one row of Z per
+// thread held in registers, then Z with its columns in ascending order to the
+// workspace. Every sweep's rotations are staged by the wave in its own LDS
+// slice at their column index, identity (1, 0) outside the sweep's window, so
+// the columns run unrolled at compile-time addresses: chunks of QZ_CH columns
+// that miss the window are skipped (a uniform branch), the rest are applied
+// without branches (an identity rotation leaves both entries unchanged,
+// exactly). The next sweep's rotations are fetched while the current one is
+// applied. Matrices whose log overflowed are left to geneig_lg_qlz_inline.
 extern "C" __global__ void __launch_bounds__(gk::QNT)
 geneig_lg_qlz(real* __restrict__ wout, long long wout_stride, int* __restrict__ info, char* __restrict__ ws,
               long long wss, int batch) {
     using namespace gk;
-    __shared__ real zs[ZR_LDS > 0 ? ZR_LDS * N : 1];
-    __shared__ real rotb[ND][2][N];
-    __shared__ int rank_s[N];
+    __shared__ real rcs[ND][2][2][N];             // [wave][sweep parity][c or s][column]
     const int b = blockIdx.x;
     if (b >= batch || info[b] > N) return;
     char* wsb = ws_of(ws, wss, b);
     const char* qb = wsb + WS_Q;
     const int* hdr = reinterpret_cast<const int*>(qb);
-    if (hdr[1]) {
-        gk::ql_inline(b, wout, wout_stride, info, wsb, zs, rotb, rank_s);
-        return;
+    if (hdr[1]) return;
+    const int* sw = reinterpret_cast<const int*>(qb + Q_SW);
+    const int* rk = reinterpret_cast<const int*>(qb + Q_RANK);
+    const rpair* rot = reinterpret_cast<const rpair*>(qb + Q_ROT);
+    const int lane = lane_id(), wave = wave_id();
+    const int r = wave * 64 + lane;
+    real* zout = reinterpret_cast<real*>(wsb + WS_Z);
+    real z[N];
+#pragma unroll
+    for (int c = 0; c < N; ++c) z[c] = (c == r) ? real(1.0) : real(0.0);
+    // Rotation of column p (lane p % 64, register p / 64) of the sweep whose
+    // rotations start at base (rotation l + j at base[j]).
+    auto fetch = [&](rpair (&f)[ND], const rpair* base, int l, int m) {
+#pragma unroll
+        for (int q = 0; q < ND; ++q) {
+            const int p = 64 * q + lane;
+            f[q] = (p >= l && p < m) ? base[p - l] : rpair{real(1.0), real(0.0)};
+        }
+    };
+    const int nsw = hdr[0];
+    int off = 0;
+    int l = nsw > 0 ? sw[0] : 0, m = nsw > 0 ? sw[1] : 0;
+    rpair cur[ND];
+    fetch(cur, rot, l, m);
+    for (int t = 0; t < nsw; ++t) {
+        real* rc = rcs[wave][t & 1][0];
+        real* rs = rcs[wave][t & 1][1];
+#pragma unroll
+        for (int q = 0; q < ND; ++q) {
+            const int p = 64 * q + lane;
+            if (p < N) {
+                rc[p] = cur[q].c;
+                rs[p] = cur[q].s;
+            }
+        }
+        wave_lds_sync();
+        const int off_n = off + (m - l);
+        const bool last = t + 1 == nsw;
+        const int ln = last ? 0 : sw[2 * t + 2], mn = last ? 0 : sw[2 * t + 3];
+        fetch(cur, rot + off_n, ln, mn);
+        // Columns i = m-1 down to l, rotation i acting on (i, i+1).
+#pragma unroll
+        for (int c0 = (N - 2) / QZ_CH * QZ_CH; c0 >= 0; c0 -= QZ_CH) {
+            if (c0 + QZ_CH - 1 < l || c0 >= m) continue;
+#pragma unroll
+            for (int i = c0 + QZ_CH - 1; i >= c0; --i) {
+                if (i > N - 2) continue;
+                const real c = rc[i], s = rs[i];
+                const real zi = z[i], zi1 = z[i + 1];
+                z[i + 1] = fma(s, zi, c * zi1);
+                z[i] = fma(c, zi, -(s * zi1));
+            }
+        }
+        off = off_n;
+        l = ln;
+        m = mn;
     }
+    if (r < N) {
+#pragma unroll
+        for (int c = 0; c < N; ++c) zout[(size_t)rk[c] * N + r] = z[c];
+    }
+}
+#else
+// QL, second half: Z = product of the logged rotations, one row per thread
+// (rows in LDS, or in global memory for the rows that do not fit), then Z with
+// its columns in ascending order to the workspace. Matrices whose log
+// overflowed are left to geneig_lg_qlz_inline.
+extern "C" __global__ void __launch_bounds__(gk::QNT)
+geneig_lg_qlz(real* __restrict__ wout, long long wout_stride, int* __restrict__ info, char* __restrict__ ws,
+              long long wss, int batch) {
+    using namespace gk;
+    __shared__ real zs[ZR_LDS > 0 ? ZR_LDS * N : 1];
+    const int b = blockIdx.x;
+    if (b >= batch || info[b] > N) return;
+    char* wsb = ws_of(ws, wss, b);
+    const char* qb = wsb + WS_Q;
+    const int* hdr = reinterpret_cast<const int*>(qb);
+    if (hdr[1]) return;
     const int* sw = reinterpret_cast<const int*>(qb + Q_SW);
     const int* rk = reinterpret_cast<const int*>(qb + Q_RANK);
     const rpair* rot = reinterpret_cast<const rpair*>(qb + Q_ROT);
@@ -1249,6 +1333,25 @@ geneig_lg_qlz(real* __restrict__ wout, long long wout_stride, int* __restrict__ 
             for (int c = 0; c < N; ++c) zout[(size_t)rk[c] * N + r] = zg[(size_t)c * N + r];
         }
     }
+}
+
+#endif  // register rows
+
+// Matrices whose QL rotation log overflowed: QL solved inline (ql_inline),
+// eigenvalues to wout, Z to the workspace; every other matrix returns at once.
+extern "C" __global__ void __launch_bounds__(gk::QNT)
+geneig_lg_qlz_inline(real* __restrict__ wout, long long wout_stride, int* __restrict__ info,
+                     char* __restrict__ ws, long long wss, int batch) {
+    using namespace gk;
+    __shared__ real zs[ZR_LDS > 0 ? ZR_LDS * N : 1];
+    __shared__ real rotb[ND][2][N];
+    __shared__ int rank_s[N];
+    const int b = blockIdx.x;
+    if (b >= batch || info[b] > N) return;
+    char* wsb = ws_of(ws, wss, b);
+    const int* hdr = reinterpret_cast<const int*>(wsb + WS_Q);
+    if (!hdr[1]) return;
+    gk::ql_inline(b, wout, wout_stride, info, wsb, zs, rotb, rank_s);
 }
 
 #if defined(GENEIG_FP32) || GENEIG_N <= 128

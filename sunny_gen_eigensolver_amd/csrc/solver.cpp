@@ -223,6 +223,7 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
             k_lg_hetrd_ = module32_->get("geneig_lg_hetrd");
             k_lg_ql_ = module32_->get("geneig_lg_qlr");
             k_lg_qlz_ = module32_->get("geneig_lg_qlz");
+            k_lg_qlzi_ = module32_->get("geneig_lg_qlz_inline");
             k_lg_back_ = module32_->get("geneig_lg_back");
             k_lg_refine_ = module_->get("geneig_lg_refine");
             if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_ITERS")) refine_iters_ = std::max(0, std::atoi(v));
@@ -232,6 +233,7 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
             k_lg_hetrd_ = module_->get("geneig_lg_hetrd");
             k_lg_ql_ = module_->get("geneig_lg_qlr");
             k_lg_qlz_ = module_->get("geneig_lg_qlz");
+            k_lg_qlzi_ = module_->get("geneig_lg_qlz_inline");
             k_lg_back_ = module_->get("geneig_lg_back");
         }
         {
@@ -890,6 +892,9 @@ void Solver::launch_large_(const void* H, const void* S, double* W, void* U, int
         void* a4[] = {&wout, &wstride, &Ic, &ws, (void*)&wss, &cnt, &cap};
         launch_(k_lg_ql_, (unsigned)cnt, 64, stream, a4);
         mark();
+        // Matrices whose rotation log overflowed (none, usually: each
+        // workgroup returns after one header read), then every other one.
+        launch_(k_lg_qlzi_, (unsigned)cnt, bd_ql, stream, a4);
         launch_(k_lg_qlz_, (unsigned)cnt, bd_ql, stream, a4);
         mark();
         launch_(k_lg_back_, (unsigned)cnt, lg_back_threads_, stream, a3);

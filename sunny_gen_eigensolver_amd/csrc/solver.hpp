@@ -169,7 +169,7 @@ private:
     bool large_ = false;     // staged kernels: order above 64, or staged_
     bool staged_ = false;    // staged kernels chosen at an order up to 64
     hipFunction_t k_lg_chol_ = nullptr, k_lg_formm_ = nullptr, k_lg_hetrd_ = nullptr, k_lg_ql_ = nullptr,
-                  k_lg_qlz_ = nullptr, k_lg_back_ = nullptr, k_lg_backtri_ = nullptr, k_lg_refine_ = nullptr;
+                  k_lg_qlz_ = nullptr, k_lg_qlzi_ = nullptr, k_lg_back_ = nullptr, k_lg_backtri_ = nullptr, k_lg_refine_ = nullptr;
     unsigned lg_back_threads_ = 256;   // workgroup size of geneig_lg_back (its launch bound)
     void* gbuf_ = nullptr;
     size_t gbuf_batch_ = 0;
