@@ -1187,9 +1187,10 @@ namespace gk {
 constexpr int QZ_CH = 8;                      // columns per branch-free chunk of the applier
 }  // namespace gk
 
-// Rows of Z in registers: every staged order in FP32, up to order 96 in FP64
-// (a row is N registers in FP32, 2N in FP64).
-#if defined(GENEIG_FP32) || GENEIG_N <= 96
+// Rows of Z in registers: every staged order in FP32, up to order 128 in FP64
+// (a row is N registers in FP32, 2N in FP64; above order 104 the compiler
+// keeps part of it in the accumulation registers, one wave per SIMD).
+#if defined(GENEIG_FP32) || GENEIG_N <= 128
 // QL, second half: Z = product of the logged rotations, one row of Z per
 // thread held in registers, then Z with its columns in ascending order to the
 // workspace. Every sweep's rotations are staged by the wave in its own LDS
