@@ -2042,7 +2042,9 @@ extern "C" __global__ void __launch_bounds__(GENEIG_NT)
 geneig_split_ql(real* __restrict__ W, const real* __restrict__ E, real* __restrict__ Zw,
                 int* __restrict__ info, const char* __restrict__ qlog, long long log_stride, int batch) {
     using namespace gk;
-    const int b = blockIdx.x * NW + wave_id();
+    __shared__ real rcs[NW][2][2][N];             // [wave][sweep parity][c or s][column]
+    const int wave = wave_id();
+    const int b = blockIdx.x * NW + wave;
     if (b >= batch) return;
     if (info[b] != 0) return;
     const int lane = lane_id();
@@ -2058,25 +2060,38 @@ geneig_split_ql(real* __restrict__ W, const real* __restrict__ E, real* __restri
     const int nsw = hdr[0];
     int off = 0;
     int l = nsw > 0 ? sw[0] : 0, m = nsw > 0 ? sw[1] : 0;
-    real cr = (lane < m - l) ? rot[2 * lane] : real(1.0);
-    real sr = (lane < m - l) ? rot[2 * lane + 1] : real(0.0);
+    // Lane p holds the rotation of column p (identity outside the window).
+    real cr = (lane >= l && lane < m) ? rot[2 * (lane - l)] : real(1.0);
+    real sr = (lane >= l && lane < m) ? rot[2 * (lane - l) + 1] : real(0.0);
     for (int t = 0; t < nsw; ++t) {
+        // The sweep's rotations at their column index in this wave's LDS
+        // slice: the columns then run unrolled at compile-time addresses,
+        // chunks of QZ_CH that miss the window skipped, the rest without
+        // branches (an identity rotation changes nothing, exactly).
+        real* rc = rcs[wave][t & 1][0];
+        real* rs = rcs[wave][t & 1][1];
+        if (lane < N) {
+            rc[lane] = cr;
+            rs[lane] = sr;
+        }
+        wave_lds_sync();
         const int off_n = off + (m - l);
         const bool last = t + 1 == nsw;
         const int ln = last ? 0 : sw[2 * t + 2], mn = last ? 0 : sw[2 * t + 3];
-        const real cn = (lane < mn - ln) ? rot[2 * (off_n + lane)] : real(1.0);
-        const real sn = (lane < mn - ln) ? rot[2 * (off_n + lane) + 1] : real(0.0);
+        cr = (lane >= ln && lane < mn) ? rot[2 * (off_n + lane - ln)] : real(1.0);
+        sr = (lane >= ln && lane < mn) ? rot[2 * (off_n + lane - ln) + 1] : real(0.0);
 #pragma unroll
-        for (int i = N - 2; i >= 0; --i) {
-            if (i < m && i >= l) {
-                const real c = bcast(cr, i - l), s = bcast(sr, i - l);
+        for (int c0 = (N - 2) / QZ_CH * QZ_CH; c0 >= 0; c0 -= QZ_CH) {
+            if (c0 + QZ_CH - 1 < l || c0 >= m) continue;
+#pragma unroll
+            for (int i = c0 + QZ_CH - 1; i >= c0; --i) {
+                if (i > N - 2) continue;
+                const real c = rc[i], s = rs[i];
                 const real zi = z[i], zi1 = z[i + 1];
                 z[i + 1] = fma(s, zi, c * zi1);
                 z[i] = fma(c, zi, -(s * zi1));
             }
         }
-        cr = cn;
-        sr = sn;
         off = off_n;
         l = ln;
         m = mn;

@@ -49,6 +49,7 @@ constexpr int LNW = LNT / 64;
 constexpr int ND = (N + 63) / 64;             // rows per lane when lanes index rows
 constexpr int LNB = (N + 15) / 16;             // 16-blocks per dimension
 constexpr int LKEND = (N + 3) / 4 * 4;         // k-steps that touch rows/columns < N
+constexpr int QZ_CH = 8;                       // columns per branch-free chunk of the QL appliers
 constexpr size_t NN = (size_t)N * N;
 constexpr size_t LDS_CAP = 160 * 1024;
 
@@ -1182,10 +1183,6 @@ geneig_lg_qlr(real* __restrict__ wout, long long wout_stride, int* __restrict__ 
         if (unconverged) info[b] = unconverged;
     }
 }
-
-namespace gk {
-constexpr int QZ_CH = 8;                      // columns per branch-free chunk of the applier
-}  // namespace gk
 
 // Rows of Z in registers: every staged order in FP32, up to order 128 in FP64
 // (a row is N registers in FP32, 2N in FP64; above order 104 the compiler
