@@ -172,15 +172,42 @@ __device__ __forceinline__ float rsqrt_nr(float t) {
 }
 #else
 __device__ __forceinline__ double rsqrt_nr(double t) {
-    double y = __builtin_amdgcn_rsq(t);
-    const double h = 0.5 * t;
-    double r = fma(-(h * y), y, 0.5);
-    y = fma(y, r, y);
-    r = fma(-(h * y), y, 0.5);
-    y = fma(y, r, y);
-    return y;
+    // v_rsq_f64 is good to about 2^-24 (measured on gfx950), so one
+    // third-order step y (1 + e/2 + 3 e^2 / 8), e = 1 - t y^2 (the binomial
+    // series of (1 - e)^(-1/2)), leaves an error of order e^3 (below 2^-70)
+    // plus the final rounding: four dependent operations instead of the six
+    // of two Newton steps.
+    const double y = __builtin_amdgcn_rsq(t);
+    const double e = fma(-(t * y), y, 1.0);
+    const double q = fma(e, 0.375, 0.5);
+    return fma(y * e, q, y);
 }
 #endif
+
+// One rotation of an implicit QL sweep (Handbook II/3, tql2's inner loop)
+// on the pair (d_i, e_i) with the sweep's running (s, c, g, p) and d_up (the
+// original d[i+1]); returns the new e[i+1] and d[i+1] through e_next and
+// d_next. 2 c b is formed as (2 g b) y, so that only s, c and that product
+// wait for the reciprocal square root. t = f^2 + g^2 > 0 inside the window
+// (f = s e_i != 0 there); the smallest normal added to g^2 keeps t, and so
+// the rotation, finite for inputs scaled near underflow.
+__device__ __forceinline__ void ql_rotation(real e_i, real d_i, real& s, real& c, real& g, real& p, real& d_up,
+                                            real& e_next, real& d_next) {
+    const real f = s * e_i;
+    const real b = c * e_i;
+    const real t = fma(f, f, fma(g, g, REAL_TINY));
+    const real y = rsqrt_nr(t);
+    const real gb2 = real(2.0) * g * b;
+    e_next = t * y;
+    s = f * y;
+    c = g * y;
+    g = d_up - p;
+    const real r2 = fma(d_i - g, s, gb2 * y);
+    p = s * r2;
+    d_next = g + p;
+    g = fma(c, r2, -b);
+    d_up = d_i;
+}
 
 // Orders this wave's LDS writes before its later LDS reads.
 __device__ __forceinline__ void wave_lds_sync() {
@@ -588,30 +615,20 @@ __device__ __forceinline__ int ql_wave(real& dreg, real& ereg, real (&z)[N]) {
         g = bcast(dreg, m) - d_l + e_l / (g + copysign(r, g));
         real s = 1.0, c = 1.0, p = 0.0;
         real d_up = bcast(dreg, m);      // original d[i+1] for the current i
+        // Step i reads lane i before any step of this sweep writes it (step i
+        // writes lane i + 1), so the reads come from copies taken here: they
+        // then wait for nothing written during the sweep.
+        const real d0 = dreg, e0 = ereg;
 #pragma unroll
         for (int i = N - 2; i >= 0; --i) {
             if (i < m && i >= l) {
-                const real e_i = bcast(ereg, i);
-                const real d_i = bcast(dreg, i);
-                const real f = s * e_i;
-                const real b = c * e_i;
-                // t > 0: every e_i in [l, m) is above the deflation threshold,
-                // so f = s e_i != 0 (s starts at 1 and stays nonzero). A
-                // floor keeps t finite even for inputs scaled near underflow.
-                const real t = fmax(fma(f, f, g * g), REAL_TINY);
-                const real y = rsqrt_nr(t);
-                ereg = sel_lane(ereg, t * y, i + 1);
-                s = f * y;
-                c = g * y;
-                g = d_up - p;
-                const real r2 = fma(d_i - g, s, real(2.0) * c * b);
-                p = s * r2;
-                dreg = sel_lane(dreg, g + p, i + 1);
-                g = fma(c, r2, -b);
+                real e_next, d_next;
+                ql_rotation(bcast(e0, i), bcast(d0, i), s, c, g, p, d_up, e_next, d_next);
+                ereg = sel_lane(ereg, e_next, i + 1);
+                dreg = sel_lane(dreg, d_next, i + 1);
                 const real zi = z[i], zi1 = z[i + 1];
                 z[i + 1] = fma(s, zi, c * zi1);
                 z[i] = fma(c, zi, -(s * zi1));
-                d_up = d_i;
             }
         }
         dreg = sel_lane(dreg, d_l - p, l);
@@ -1973,23 +1990,12 @@ geneig_ql_record(const real* __restrict__ W, const real* __restrict__ E, const i
                 }
             }
         } else if (phase == SWEEP) {
-            const real e_i = es[i][lane], d_i = ds[i][lane];
-            const real f = s * e_i;
-            const real bb = c * e_i;
-            // f = s e_i != 0 inside the window; the floor keeps t finite near underflow.
-            const real tt = fmax(fma(f, f, g * g), REAL_TINY);
-            const real y = rsqrt_nr(tt);
-            es[i + 1][lane] = tt * y;
-            s = f * y;
-            c = g * y;
-            g = d_up - p;
-            const real r2 = fma(d_i - g, s, real(2.0) * c * bb);
-            p = s * r2;
-            ds[i + 1][lane] = g + p;
-            g = fma(c, r2, -bb);
+            real e_next, d_next;
+            ql_rotation(es[i][lane], ds[i][lane], s, c, g, p, d_up, e_next, d_next);
+            es[i + 1][lane] = e_next;
+            ds[i + 1][lane] = d_next;
             rot[2 * (nrot + i - l)] = c;
             rot[2 * (nrot + i - l) + 1] = s;
-            d_up = d_i;
             if (--i < l) {
                 ds[l][lane] = d_l - p;
                 es[l][lane] = g;
@@ -2171,15 +2177,16 @@ geneig_split_back(cx* __restrict__ U, const real* __restrict__ Zw, const int* __
 // Mixed precision (FP32 tridiagonalisation and eigenvectors, FP64 Cholesky
 // and refinement). Per matrix:
 //   geneig_lg_chol    (FP64 module): S = L L^H and W = L^-1 in FP64 by 16-column
-//                     panels with the products on the matrix cores; W (full,
-//                     upper zero) to Wd. A non-positive pivot ends the matrix
-//                     here with zhegv's INFO, as in the other modes.
+//                     panels with the products on the matrix cores; W to Wd
+//                     (its lower triangle packed by columns, or in full with
+//                     the upper zero where it did not fit LDS). A non-positive
+//                     pivot ends the matrix here with zhegv's INFO, as in the
+//                     other modes.
 //   geneig_mx_formm   (FP64 module): M = W H W^H on the matrix cores; its
-//                     lower triangle packed to Mp (complex128) and, made
-//                     Hermitian from that triangle, rounded in full to M32
-//                     (complex64).
-//   geneig_mx_front   (FP32 module): Householder tridiagonalisation of M32;
-//                     reflectors and tau to U32, T to W32 / E32.
+//                     lower triangle packed to Mp (complex128).
+//   geneig_mx_front   (FP32 module): Householder tridiagonalisation of M (Mp
+//                     rounded to complex64, Hermitian from its lower
+//                     triangle); reflectors and tau to U32, T to W32 / E32.
 //   geneig_split_ql   (FP32 module): QL with FP32 unit roundoff, Z to Z32.
 //   geneig_mx_back    (FP32 module): V = Q Z into U32.
 //   geneig_mx_refine  (FP64 module): Ogita-Aishima refinement of (M, V) in
@@ -2219,19 +2226,31 @@ __device__ __forceinline__ void stash_refl_reg(const WorkF& w, const cx (&m)[CPW
 }  // namespace gk
 
 extern "C" __global__ void __launch_bounds__(GENEIG_NT, 2)
-geneig_mx_front(const gk::cf* __restrict__ M32, real* __restrict__ W32, cx* __restrict__ U32,
+geneig_mx_front(const gk::cd* __restrict__ Mp, real* __restrict__ W32, cx* __restrict__ U32,
                 const int* __restrict__ info, real* __restrict__ E32, int batch) {
     using namespace gk;
     __shared__ WorkF w;
+    __shared__ cx ms[NPK];
     const int b = blockIdx.x;
     if (b >= batch || info[b] != 0) return;
     const size_t off = (size_t)b * N * N;
     const int lane = lane_id(), wave = wave_id();
+    // M's lower triangle (complex128, packed by columns) rounded into LDS, then
+    // each owned column made Hermitian from it: M(i, j) for i >= j, conj(M(j, i))
+    // above the diagonal, a real diagonal.
+    {
+        const cd* Mb = Mp + (size_t)b * NPK;
+        for (int e = threadIdx.x; e < NPK; e += NT) ms[e] = to_cx(Mb[e]);
+        __syncthreads();
+    }
     cx m[CPW];
 #pragma unroll
     for (int c = 0; c < CPW; ++c) {
         const int j = wave + NW * c;
-        m[c] = (lane < N && j < N) ? to_cx(M32[off + (size_t)j * N + lane]) : cm(0.0f, 0.0f);
+        const bool in = lane < N && j < N;
+        const bool lo = lane >= j;
+        const cx v = ms[in ? (lo ? pk(j) + lane - j : pk(lane) + j - lane) : 0];
+        m[c] = !in ? cm(0.0f, 0.0f) : (lane == j ? cm(v.x, 0.0f) : (lo ? v : cconj(v)));
     }
     hetrd_reg(w, m);
     stash_refl_reg(w, m, U32 + off);
@@ -2392,6 +2411,25 @@ __device__ __forceinline__ void stage_lower_packed(cx* P, const cx* A, int tid, 
     }
 }
 
+// W = L^-1 of matrix b as geneig_lg_chol leaves it for the mixed mode: its
+// lower triangle packed by columns (W_PACKED_OUT), or in full (upper zero).
+__device__ __forceinline__ const cx* w_of(const cx* Wd, int b) {
+    return Wd + (size_t)b * (W_PACKED_OUT ? (size_t)NPK : (size_t)N * N);
+}
+// W's lower triangle into P, packed by columns.
+__device__ __forceinline__ void stage_w(cx* P, const cx* Wb, int tid, int nt) {
+    if constexpr (W_PACKED_OUT) {
+        for (int e = tid; e < NPK; e += nt) P[e] = Wb[e];
+    } else {
+        stage_lower_packed(P, Wb, tid, nt);
+    }
+}
+// W(i, k) straight from Wb (global), zero above the diagonal and outside.
+__device__ __forceinline__ cx w_elem(const cx* Wb, int i, int k) {
+    if constexpr (W_PACKED_OUT) return lelem(Wb, i, k);
+    else return elem(Wb, N, i, k);
+}
+
 // Rayleigh-Ritz on columns (a, b) of V for the Hermitian M (held in r.M), by
 // one wave: the 2 x 2 projected pencil (V^H M V, V^H V) of the pair, then
 // V(:, [a b]) <- V(:, [a b]) X and lam[a], lam[b] <- the Ritz values.
@@ -2434,16 +2472,28 @@ __device__ __forceinline__ void ritz_pair(RefWork& r, int a, int b) {
 namespace gk {
 // geneig_mx_formm stages H and W in LDS when two workgroups still fit a CU.
 constexpr bool FORMM_LDS = ((size_t)N * LDR + NPK) * sizeof(cx) <= 80 * 1024;
+
+// The lower triangle of this wave's column block of M (tiles in the MFMA
+// result layout, column j in this lane) to Mb, packed by columns.
+__device__ __forceinline__ void store_m_packed(cx* __restrict__ Mb, const ctile (&M)[NB], int j, bool jin, int g4) {
+#pragma unroll
+    for (int rt = 0; rt < NB; ++rt) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = rt * 16 + g4 + 4 * v;
+            if (i < N && jin && i >= j) Mb[pk(j) + i - j] = tile_get(M[rt], v);
+        }
+    }
+}
 }  // namespace gk
 
 // M = W H W^H of the mixed mode, once, on the matrix cores (W = L^-1 lower
 // triangular: Z(:, j) = H W^H(:, j) needs k <= j, M(i, j) = W(i, :) Z(:, j)
-// needs k <= i). Its lower triangle goes packed by columns to Mp (complex128,
-// for the refinement) and, made Hermitian from that triangle (real
-// diagonal), rounded to complex64 in full to M32 (for the FP32 front half).
+// needs k <= i). Its lower triangle goes packed by columns to Mp (complex128),
+// which both the FP32 front half and the refinement read.
 extern "C" __global__ void __launch_bounds__(gk::REF_NT, 2)
 geneig_mx_formm(const cx* __restrict__ H, const cx* __restrict__ Wd, const int* __restrict__ info,
-                cx* __restrict__ Mp, gk::cf* __restrict__ M32, int batch) {
+                cx* __restrict__ Mp, int batch) {
     using namespace gk;
     const int b = blockIdx.x;
     if (b >= batch || info[b] != 0) return;
@@ -2461,7 +2511,7 @@ geneig_mx_formm(const cx* __restrict__ H, const cx* __restrict__ Wd, const int* 
         __shared__ cx Ws[FORMM_LDS ? NPK : 1];
         const cx* Hb = H + off;
         for (int e = threadIdx.x; e < N * N; e += REF_NT) Hs[(e / N) * LDR + e % N] = Hb[e];
-        stage_lower_packed(Ws, Wd + off, threadIdx.x, REF_NT);
+        stage_w(Ws, w_of(Wd, b), threadIdx.x, REF_NT);
         __syncthreads();
         auto fW = [&](int i, int k) { return lelem(Ws, i, k); };
         cgemm(Z, [&](int i, int k) { return elem(Hs, LDR, i, k); },
@@ -2471,31 +2521,12 @@ geneig_mx_formm(const cx* __restrict__ H, const cx* __restrict__ Wd, const int* 
         // Operands straight from global memory (no LDS, so more workgroups
         // per CU hide the latency).
         const cx* Hb = H + off;
-        const cx* Wb = Wd + off;
+        const cx* Wb = w_of(Wd, b);
         cgemm(Z, [&](int i, int k) { return elem(Hb, N, i, k); },
-              [&](int k, int jj) { return cconj(elem(Wb, N, jj, k)); }, cb, kz);
-        cgemm_reg<true>(M, [&](int i, int k) { return elem(Wb, N, i, k); }, Z, cb);
+              [&](int k, int jj) { return cconj(w_elem(Wb, jj, k)); }, cb, kz);
+        cgemm_reg<true>(M, [&](int i, int k) { return w_elem(Wb, i, k); }, Z, cb);
     }
-    cx* Mb = Mp + (size_t)b * NPK;
-    cf* M32b = M32 + off;
-#pragma unroll
-    for (int rt = 0; rt < NB; ++rt) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int i = rt * 16 + g4 + 4 * v;
-            if (i < N && jin && i >= j) {
-                const cx mv = tile_get(M[rt], v);
-                Mb[pk(j) + i - j] = mv;
-                cf lo, up;
-                lo.x = (float)mv.x;
-                lo.y = (i == j) ? 0.0f : (float)mv.y;
-                up.x = lo.x;
-                up.y = -lo.y;
-                M32b[(size_t)j * N + i] = lo;
-                if (i != j) M32b[(size_t)i * N + j] = up;
-            }
-        }
-    }
+    store_m_packed(Mp + (size_t)b * NPK, M, j, jin, g4);
 }
 
 // M = W diag(h) W^H for a diagonal H (Sunny's paraunitary metric H = Sigma =
@@ -2506,7 +2537,7 @@ geneig_mx_formm(const cx* __restrict__ H, const cx* __restrict__ Wd, const int* 
 // the whole batch). Outputs as geneig_mx_formm.
 extern "C" __global__ void __launch_bounds__(gk::REF_NT, 2)
 geneig_mx_formm_diag(const real* __restrict__ Hd, long long hd_stride, const cx* __restrict__ Wd,
-                     const int* __restrict__ info, cx* __restrict__ Mp, gk::cf* __restrict__ M32, int batch) {
+                     const int* __restrict__ info, cx* __restrict__ Mp, int batch) {
     using namespace gk;
     const int b = blockIdx.x;
     if (b >= batch || info[b] != 0) return;
@@ -2517,7 +2548,7 @@ geneig_mx_formm_diag(const real* __restrict__ Hd, long long hd_stride, const cx*
     const size_t off = (size_t)b * N * N;
     __shared__ cx Ws[NPK];
     __shared__ real hs[KEND];
-    stage_lower_packed(Ws, Wd + off, threadIdx.x, REF_NT);
+    stage_w(Ws, w_of(Wd, b), threadIdx.x, REF_NT);
     if (threadIdx.x < KEND) hs[threadIdx.x] = threadIdx.x < N ? Hd[(size_t)b * hd_stride + threadIdx.x] : real(0.0);
     __syncthreads();
     // Column block cb needs k <= j < 16 (cb + 1).
@@ -2525,26 +2556,7 @@ geneig_mx_formm_diag(const real* __restrict__ Hd, long long hd_stride, const cx*
     ctile M[NB];
     cgemm(M, [&](int i, int k) { return lelem(Ws, i, k); },
           [&](int k, int jj) { return cconj(lelem(Ws, jj, k)) * hs[k]; }, cb, kz);
-    cx* Mb = Mp + (size_t)b * NPK;
-    cf* M32b = M32 + off;
-#pragma unroll
-    for (int rt = 0; rt < NB; ++rt) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int i = rt * 16 + g4 + 4 * v;
-            if (i < N && jin && i >= j) {
-                const cx mv = tile_get(M[rt], v);
-                Mb[pk(j) + i - j] = mv;
-                cf lo, up;
-                lo.x = (float)mv.x;
-                lo.y = (i == j) ? 0.0f : (float)mv.y;
-                up.x = lo.x;
-                up.y = -lo.y;
-                M32b[(size_t)j * N + i] = lo;
-                if (i != j) M32b[(size_t)i * N + j] = up;
-            }
-        }
-    }
+    store_m_packed(Mp + (size_t)b * NPK, M, j, jin, g4);
 }
 
 // Refinement in the Cholesky frame: with W = L^-1 (FP64), M = W H W^H and the
@@ -2571,7 +2583,7 @@ geneig_mx_refine(const cx* __restrict__ Mp, const cx* __restrict__ Wd, real* __r
     const int j = cb * 16 + (l & 15);          // this lane's column
     const bool jin = j < N;
     const size_t off = (size_t)b * N * N;
-    const cx* Wb = Wd + (size_t)b * N * N;          // W = L^-1, full (upper zero)
+    const cx* Wb = w_of(Wd, b);                     // W = L^-1
     auto fM = [&](int i, int k) { return melem(r.M, i, k); };
     auto fV = [&](int i, int k) { return elem(r.U, LDR, i, k); };
     auto fVh = [&](int i, int k) { return cconj(elem(r.U, LDR, k, i)); };
@@ -2698,7 +2710,7 @@ geneig_mx_refine(const cx* __restrict__ Mp, const cx* __restrict__ Wd, real* __r
         if (!(m <= tol)) info[b] = N;
     }
     // U = W^H V, W's lower triangle staged again over M (no longer needed).
-    stage_lower_packed(r.M, Wb, threadIdx.x, REF_NT);
+    stage_w(r.M, Wb, threadIdx.x, REF_NT);
     __syncthreads();
     ctile Uo[NB];
     cgemm<true>(Uo, [&](int i, int k) { return cconj(fW(k, i)); }, fV, cb);

@@ -598,33 +598,25 @@ __device__ __forceinline__ void ql_sweep(real (&dr)[ND], real (&er)[ND], int l, 
     g = d_m - d_l + e_l / (g + copysign(rr, g));
     real s = real(1.0), c = real(1.0), p = real(0.0);
     real d_up = d_m;
+    // Step i reads element i before any step of this sweep writes it (step i
+    // writes i + 1): copies taken here keep those reads off the sweep's chain.
+    real d0[ND], e0[ND];
 #pragma unroll
     for (int q = 0; q < ND; ++q) {
         cr[q] = real(1.0);
         sr[q] = real(0.0);
+        d0[q] = dr[q];
+        e0[q] = er[q];
     }
 #pragma unroll
     for (int i = N - 2; i >= 0; --i) {
         if (i < m && i >= l) {
-            const real e_i = bcast(er[i >> 6], i & 63);
-            const real d_i = bcast(dr[i >> 6], i & 63);
-            const real f = s * e_i;
-            const real bb = c * e_i;
-            // f = s e_i != 0 inside the window (every e_i there is above the
-            // deflation threshold); the floor keeps t finite near underflow.
-            const real tt = fmax(fma(f, f, g * g), REAL_TINY);
-            const real y = rsqrt_nr(tt);
-            er[(i + 1) >> 6] = sel_lane(er[(i + 1) >> 6], tt * y, (i + 1) & 63);
-            s = f * y;
-            c = g * y;
-            g = d_up - p;
-            const real r2 = fma(d_i - g, s, real(2.0) * c * bb);
-            p = s * r2;
-            dr[(i + 1) >> 6] = sel_lane(dr[(i + 1) >> 6], g + p, (i + 1) & 63);
-            g = fma(c, r2, -bb);
+            real e_next, d_next;
+            ql_rotation(bcast(e0[i >> 6], i & 63), bcast(d0[i >> 6], i & 63), s, c, g, p, d_up, e_next, d_next);
+            er[(i + 1) >> 6] = sel_lane(er[(i + 1) >> 6], e_next, (i + 1) & 63);
+            dr[(i + 1) >> 6] = sel_lane(dr[(i + 1) >> 6], d_next, (i + 1) & 63);
             cr[i >> 6] = sel_lane(cr[i >> 6], c, i & 63);
             sr[i >> 6] = sel_lane(sr[i >> 6], s, i & 63);
-            d_up = d_i;
         }
     }
     dset(dr, l, d_l - p);
@@ -702,13 +694,22 @@ using gk::real;
 // non-positive pivot k (0-based) of a leading minor: info = N + k + 1, W NaN,
 // U zero (LAPACK zhegv's convention), and the later stages skip the matrix.
 // For small orders (while three workgroups still fit a CU), L and then W live
-// packed in LDS (the products read LDS), and W goes out in full at the end;
-// otherwise they live in the workspace.
+// packed in LDS (the products read LDS), and W goes out at the end (packed
+// where W_PACKED_OUT below holds, in full otherwise); otherwise they live in
+// the workspace.
 namespace gk {
 constexpr bool CHOL_LDS = N <= 64 && sizeof(CholWork) + (size_t)N * (N + 1) / 2 * sizeof(cx) <= 52 * 1024;
 constexpr int LPK = N * (N + 1) / 2;
 __device__ __forceinline__ int lpk(int j) { return j * N - (j * (j - 1)) / 2; }
+// Where only the Cholesky is built (the mixed mode's module, orders up to 64)
+// and W was formed in LDS, W goes out as its lower triangle packed by columns
+// (element (i, j), i >= j, at lpk(j) + i - j; matrix b at b LPK): half the
+// bytes of the full layout, which the staged pipeline's consumers read.
+constexpr bool W_PACKED_OUT = CHOL_LDS && !GENEIG_LG_PIPELINE;
 }  // namespace gk
+
+// Whether geneig_lg_chol writes W packed (the host sets its stride by it).
+extern "C" __device__ const int geneig_w_packed = gk::W_PACKED_OUT ? 1 : 0;
 
 extern "C" __global__ void __launch_bounds__(gk::LNT, 3)
 geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ U, int* __restrict__ info,
@@ -861,7 +862,9 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
         }
         __syncthreads();
     }
-    if constexpr (CHOL_LDS) {
+    if constexpr (W_PACKED_OUT) {
+        for (int e = tid; e < LPK; e += LNT) Lg[e] = Lp[e];
+    } else if constexpr (CHOL_LDS) {
         for (int e = tid; e < (int)NN; e += LNT) {
             const int i = e % N, j = e / N;
             Lg[e] = (i >= j) ? Lp[lpk(j) + i - j] : cm(0.0, 0.0);

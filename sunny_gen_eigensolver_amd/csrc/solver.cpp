@@ -116,7 +116,7 @@ namespace {
 // and off-diagonals, Z (FP32), M's lower triangle packed (complex128) and M
 // in full (complex64).
 struct MixedLayout {
-    size_t wd, v32, w32, e32, z32, mp, m32, total;
+    size_t wd, v32, w32, e32, z32, mp, total;
 };
 MixedLayout mixed_layout(size_t n, size_t B) {
     auto al = [](size_t x) { return (x + 255) / 256 * 256; };
@@ -127,8 +127,7 @@ MixedLayout mixed_layout(size_t n, size_t B) {
     m.e32 = al(m.w32 + B * n * 4);
     m.z32 = al(m.e32 + B * n * 4);
     m.mp = al(m.z32 + B * n * n * 4);
-    m.m32 = al(m.mp + B * n * (n + 1) / 2 * 16);
-    m.total = al(m.m32 + B * n * n * 8);
+    m.total = al(m.mp + B * n * (n + 1) / 2 * 16);
     return m;
 }
 }  // namespace
@@ -286,6 +285,15 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
         k_refine_ = module_->get("geneig_mx_refine");
         k_mx_formm_ = module_->get("geneig_mx_formm");
         k_mx_formm_diag_ = module_->get("geneig_mx_formm_diag");
+        {
+            // geneig_lg_chol leaves W packed (its lower triangle) where it fits LDS.
+            hipDeviceptr_t sym = nullptr;
+            size_t sym_bytes = 0;
+            int packed = 0;
+            SG_HIP(hipModuleGetGlobal(&sym, &sym_bytes, module_->module, "geneig_w_packed"));
+            SG_HIP(hipMemcpyDtoH(&packed, sym, sizeof(packed)));
+            w_packed_ = packed != 0;
+        }
         ModuleKey k32 = key;
         k32.fp32 = true;
         CacheLayer l32;
@@ -708,8 +716,8 @@ void Solver::ensure_mixed_workspace_(size_t batch) {
         mbuf_ = nullptr;
         mbuf_batch_ = 0;
     }
-    // complex128 W = L^-1 (full), complex64 stash / V, float eigenvalues,
-    // off-diagonals and Z, M (packed complex128 and full complex64).
+    // complex128 W = L^-1 (room for the full matrix), complex64 stash / V,
+    // float eigenvalues, off-diagonals and Z, M (packed complex128).
     SG_HIP(hipMalloc(&mbuf_, mixed_layout((size_t)n_, batch).total));
     mbuf_batch_ = batch;
 }
@@ -729,14 +737,14 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
     void* E32 = base + ml.e32;
     void* Z32 = base + ml.z32;
     void* Mp = base + ml.mp;
-    void* M32 = base + ml.m32;
     const unsigned grid_b = (unsigned)batch;
     const unsigned grid_w = (unsigned)((batch + bpb_ - 1) / bpb_);
     const unsigned bd = (unsigned)block_dim_;
-    // FP64 Cholesky and W = L^-1 (full, upper zero): the staged kernels'
-    // panel factorisation, one workgroup per matrix of one wave per 16-row
-    // tile (the kernel's size for orders up to 64).
-    long long wstride = (long long)(n * n * 16);
+    // FP64 Cholesky and W = L^-1 (its lower triangle packed by columns, or in
+    // full with the upper zero): the staged kernels' panel factorisation, one
+    // workgroup per matrix of one wave per 16-row tile (the kernel's size for
+    // orders up to 64).
+    long long wstride = (long long)((w_packed_ ? n * (n + 1) / 2 : n * n) * 16);
     void* a1[] = {&S, &W, &U, &info, &Wd, &wstride, &batch};
     const unsigned bd_chol = (unsigned)(64 * ((n_ + 15) / 16));
     launch_(k_mx_chol_, grid_b, bd_chol, stream, a1);
@@ -747,14 +755,14 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
         // Diagonal H: M = W diag(h) W^H straight from the diagonal.
         const double* hd = ws.hdiag;
         long long hds = ws.hdiag_stride;
-        void* ad[] = {&hd, &hds, &Wd, &info, &Mp, &M32, &batch};
+        void* ad[] = {&hd, &hds, &Wd, &info, &Mp, &batch};
         launch_(k_mx_formm_diag_, grid_b, bd_ref, stream, ad);
     } else {
-        void* af[] = {&H, &Wd, &info, &Mp, &M32, &batch};
+        void* af[] = {&H, &Wd, &info, &Mp, &batch};
         launch_(k_mx_formm_, grid_b, bd_ref, stream, af);
     }
     mark_(stream);
-    void* a2[] = {&M32, &W32, &V32, &info, &E32, &batch};
+    void* a2[] = {&Mp, &W32, &V32, &info, &E32, &batch};
     launch_(k_mx_front_, grid_b, bd, stream, a2);
     mark_(stream);
     char* qlog = static_cast<char*>(ws.qlog);

@@ -160,6 +160,7 @@ private:
     hipFunction_t k_ql32_ = nullptr;
     hipFunction_t k_qlfull32_ = nullptr;
     hipFunction_t k_mx_formm_ = nullptr;
+    bool w_packed_ = false;          // geneig_lg_chol leaves W packed (mixed mode)
     hipFunction_t k_back32_ = nullptr;
     hipFunction_t k_refine_ = nullptr;
     void* mbuf_ = nullptr;
