@@ -1031,442 +1031,17 @@ __device__ __forceinline__ void back_half_waves(Work& w, bool active, const real
     }
 }
 
-// ===========================================================================
-// Packed front half (used by geneig_fused). Hermitian and triangular N x N
-// matrices are kept as their lower triangle, column-major packed: element
-// (i, j), i >= j, at pk(j) + i - j. Two such triangles (L then M, and
-// W = L^-1) plus an 8-column block buffer make the whole LDS footprint, so
-// two workgroups share a CU.
-// ===========================================================================
+// Hermitian and triangular N x N matrices packed by columns: element (i, j),
+// i >= j, of the lower triangle at pk(j) + i - j.
 constexpr int NPK = N * (N + 1) / 2;
-constexpr int CB = 8;                              // columns per branch-free chunk
-constexpr int ZB = 8;                              // columns per block of M = W H W^H
-constexpr int NZB = (N + ZB - 1) / ZB;
-constexpr int ZPW = (ZB + NW - 1) / NW;            // block columns per wave
 __device__ __forceinline__ int pk(int j) { return j * N - (j * (j - 1)) / 2; }
 
-// A copy of `x` the compiler cannot prove loop-invariant: per-column LDS
-// addresses derived from it are recomputed each step instead of being held
-// live (dozens of VGPRs) across a whole loop.
-__device__ __forceinline__ int fresh(int x) {
-    x = __builtin_amdgcn_readfirstlane(x);   // wave-uniform by construction
-    asm volatile("" : "+s"(x));
-    return x;
-}
-
-struct WorkP {
-    cx L[NPK];            // S -> L; then M -> T (reflectors below the subdiagonal)
-    cx Wi[NPK];           // W = L^-1
-    union {
-        cx zj[N * ZB];    // Z block of H W^H
-        struct {
-            cx vb[2][N];
-            cx wb[NW][N];
-            cx part[NW * N];
-        } h;
-    } u;
-    cx tau[N];
-    real invL[N];
-    real d[N];
-    real e[N];
-    int flag[2];
-};
-
-__device__ __forceinline__ void load_lower_packed(cx* P, const cx* __restrict__ src) {
-    for (int e = threadIdx.x; e < N * N; e += NT) {
-        const int i = e % N, j = e / N;
-        if (i >= j) P[pk(j) + i - j] = src[e];
-    }
-}
-
-// Cholesky on the packed lower triangle: column j belongs to wave j % NW; the
-// owner of column k+1 finishes it (pivot check, scaling, publication in
-// vb[(k+1) & 1]) right after its own updates of step k. One barrier per step.
-__device__ __forceinline__ int potrf_packed(WorkP& w) {
-    const int lane = lane_id();
-    const int lr = lane < N ? lane : N - 1;
-    auto finish_column = [&](int k) {
-        const cx a = (lane >= k && lane < N) ? w.L[pk(k) + lane - k] : cm(0.0, 0.0);
-        const real dkk = bcast(a.x, k);
-        if (!(dkk > real(0.0))) {
-            if (lane == 0) w.flag[1] = k + 1;
-            return;
-        }
-        const real rinv = real(1.0) / sqrt_r(dkk);
-        const cx l = a * rinv;
-        if (lane >= k && lane < N) w.L[pk(k) + lane - k] = l;
-        if (lane < N) w.u.h.vb[k & 1][lane] = l;
-        if (lane == 0) w.invL[k] = rinv;
-    };
-    if (threadIdx.x == 0) w.flag[1] = 0;
-    __syncthreads();
-    if (wave_id() == 0) finish_column(0);
-    __syncthreads();
-    for (int k = 0; k < N; ++k) {
-        const int fail = w.flag[1];
-        if (fail) return fail;
-        const int p = k & 1;
-        const int wave = fresh(wave_id());
-        const cx li = w.u.h.vb[p][lr];
-#pragma unroll
-        for (int c0 = 0; c0 < CPW; c0 += CB) {
-            __builtin_amdgcn_sched_barrier(0);   // keep chunks from interleaving (register pressure)
-            cx a[CB], lj[CB];
-#pragma unroll
-            for (int cc = 0; cc < CB; ++cc) {
-                const int c = c0 + cc;
-                const int j = (c < CPW && wave + NW * c < N) ? wave + NW * c : N - 1;
-                a[cc] = w.L[pk(j) + (lr >= j ? lr - j : 0)];
-                lj[cc] = w.u.h.vb[p][j];
-            }
-#pragma unroll
-            for (int cc = 0; cc < CB; ++cc) {
-                const int c = c0 + cc;
-                const int j = wave + NW * c;
-                if (c < CPW && j > k && j < N && lane >= j && lane < N)
-                    w.L[pk(j) + lane - j] = a[cc] - cmulc(li, lj[cc]);
-            }
-        }
-        if (k + 1 < N && wave == (k + 1) % NW) finish_column(k + 1);
-        __syncthreads();
-    }
-    return 0;
-}
-
-// W = L^-1 by forward substitution on the identity, column j owned by wave
-// j % NW (lane = row); L is read only, so no barrier is needed between steps.
-__device__ __forceinline__ void trtri_packed(WorkP& w) {
-    const int lane = lane_id();
-    const int lr = lane < N ? lane : N - 1;
-#pragma unroll
-    for (int c = 0; c < CPW; ++c) {
-        const int j = wave_id() + NW * c;
-        if (j < N && lane >= j && lane < N) w.Wi[pk(j) + lane - j] = cm(lane == j ? 1.0 : 0.0, 0.0);
-    }
-    for (int k = 0; k < N; ++k) {
-        const int wave = fresh(wave_id());
-        const real inv = w.invL[k];
-        const cx lik = (lr > k) ? w.L[pk(k) + lr - k] : cm(0.0, 0.0);
-#pragma unroll
-        for (int c0 = 0; c0 < CPW; c0 += CB) {
-            __builtin_amdgcn_sched_barrier(0);   // keep chunks from interleaving (register pressure)
-            cx pv[CB], bv[CB];
-#pragma unroll
-            for (int cc = 0; cc < CB; ++cc) {
-                const int j0 = wave + NW * (c0 + cc);
-                const int j = (c0 + cc < CPW && j0 <= k) ? j0 : 0;
-                pv[cc] = w.Wi[pk(j) + k - j];
-                bv[cc] = w.Wi[pk(j) + (lr >= j ? lr - j : 0)];
-            }
-#pragma unroll
-            for (int cc = 0; cc < CB; ++cc) {
-                const int j = wave + NW * (c0 + cc);
-                const cx x = pv[cc] * inv;
-                const cx nb = (lane == k) ? x : bv[cc] - cmul(lik, x);
-                if (c0 + cc < CPW && j <= k && lane >= k && lane < N) w.Wi[pk(j) + lane - j] = nb;
-            }
-        }
-    }
-}
-
-// M = W H W^H (lower triangle, packed into w.L), eight columns at a time:
-// Z = H (W rows J)^H with H read from global memory (coalesced columns), then
-// M(:, J) = W Z on rows i >= j. Column c of a block belongs to wave c % NW.
-__device__ __forceinline__ void form_m(WorkP& w, const cx* __restrict__ Hb) {
-    const int lane = lane_id(), wave = wave_id();
-    const int lr = lane < N ? lane : N - 1;
-#pragma unroll
-    for (int jb = 0; jb < NZB; ++jb) {
-        const int j0 = jb * ZB;
-        cx acc[ZPW];
-#pragma unroll
-        for (int cc = 0; cc < ZPW; ++cc) acc[cc] = cm(0.0, 0.0);
-        const int kend = (j0 + ZB < N) ? j0 + ZB : N;
-#pragma unroll 4
-        for (int k = 0; k < kend; ++k) {
-            const cx h = Hb[k * N + lr];
-#pragma unroll
-            for (int cc = 0; cc < ZPW; ++cc) {
-                const int j = j0 + wave + NW * cc;
-                const bool on = (j < N) && (k <= j) && (wave + NW * cc < ZB);
-                const cx wjk = w.Wi[on ? pk(k) + j - k : 0];
-                acc[cc] = acc[cc] + cmulc(h, on ? wjk : cm(0.0, 0.0));
-            }
-        }
-#pragma unroll
-        for (int cc = 0; cc < ZPW; ++cc) {
-            const int c = wave + NW * cc;
-            if (c < ZB && lane < N) w.u.zj[c * N + lane] = acc[cc];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int cc = 0; cc < ZPW; ++cc) acc[cc] = cm(0.0, 0.0);
-#pragma unroll 4
-        for (int k = 0; k < N; ++k) {
-            const cx wik = (lr >= k) ? w.Wi[pk(k) + lr - k] : cm(0.0, 0.0);
-#pragma unroll
-            for (int cc = 0; cc < ZPW; ++cc) {
-                const int c = wave + NW * cc;
-                const cx z = w.u.zj[(c < ZB ? c : 0) * N + k];
-                acc[cc] = acc[cc] + cmul(wik, z);
-            }
-        }
-#pragma unroll
-        for (int cc = 0; cc < ZPW; ++cc) {
-            const int c = wave + NW * cc;
-            const int j = j0 + c;
-            if (c < ZB && j < N && lane >= j && lane < N) w.L[pk(j) + lane - j] = acc[cc];
-        }
-        __syncthreads();
-    }
-}
-
-// Householder vector for column k (packed M), by the wave owning column k.
-__device__ __forceinline__ void house_packed(WorkP& w, int k) {
-    const int lane = lane_id();
-    const int m0 = k + 1;
-    const cx x = (lane >= m0 && lane < N) ? w.L[pk(k) + lane - k] : cm(0.0, 0.0);
-    const cx alpha = bcast(x, m0);
-    const real xn2 = wave_sum(lane > m0 ? cabs2(x) : real(0.0));
-    cx t = cm(0.0, 0.0);
-    real beta = alpha.x;
-    cx v = cm(0.0, 0.0);
-    if (xn2 == real(0.0) && alpha.y == real(0.0)) {
-        if (lane == m0) v = cm(1.0, 0.0);
-    } else {
-        const real nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
-        beta = (alpha.x >= real(0.0)) ? -nrm : nrm;
-        t = cm((beta - alpha.x) / beta, -alpha.y / beta);
-        const cx den = cm(alpha.x - beta, alpha.y);
-        const real dd = real(1.0) / cabs2(den);
-        const cx scal = cm(den.x * dd, -den.y * dd);   // 1 / (alpha - beta)
-        if (lane == m0) v = cm(1.0, 0.0);
-        else if (lane > m0) v = cmul(x, scal);
-    }
-    if (lane > m0 && lane < N) w.L[pk(k) + lane - k] = v;
-    if (lane < N) w.u.h.vb[k & 1][lane] = v;
-    if (lane == 0) {
-        w.e[k] = beta;
-        w.d[k] = w.L[pk(k)].x;
-        w.tau[k] = t;
-    }
-}
-
-// Tridiagonalisation of the packed Hermitian M (see hetrd for the schedule).
-// Lane i reads row i of the full matrix from the lower triangle: M(i, j) for
-// i >= j, conj(M(j, i)) for i < j; the rank-2 update touches the lower part.
-__device__ __forceinline__ void hetrd_packed(WorkP& w) {
-    const int lane = lane_id();
-    const int lr = lane < N ? lane : N - 1;
-    if (wave_id() == 0) house_packed(w, 0);
-    __syncthreads();
-    for (int k = 0; k < N - 1; ++k) {
-        const int m0 = k + 1;
-        const int p = k & 1;
-        const int wave = fresh(wave_id());
-        const cx t = w.tau[k];
-        if (t.x != real(0.0) || t.y != real(0.0)) {
-            const cx vl = (lane < N) ? w.u.h.vb[p][lr] : cm(0.0, 0.0);
-            cx acc = cm(0.0, 0.0);
-#pragma unroll
-            for (int c0 = 0; c0 < CPW; c0 += CB) {
-                __builtin_amdgcn_sched_barrier(0);   // keep chunks from interleaving (register pressure)
-                cx a[CB], vj[CB];
-#pragma unroll
-                for (int cc = 0; cc < CB; ++cc) {
-                    const int c = c0 + cc;
-                    const int j = (c < CPW && wave + NW * c < N) ? wave + NW * c : N - 1;
-                    const cx m = w.L[(lr >= j) ? pk(j) + lr - j : pk(lr) + j - lr];
-                    a[cc] = (lr >= j) ? m : cconj(m);
-                    vj[cc] = w.u.h.vb[p][j];
-                }
-#pragma unroll
-                for (int cc = 0; cc < CB; ++cc) {
-                    const int c = c0 + cc;
-                    const int j = wave + NW * c;
-                    const cx prod = cmul(a[cc], vj[cc]);
-                    if (c < CPW && j >= m0 && j < N) acc = acc + prod;
-                }
-            }
-            if (lane < N) w.u.h.part[wave * N + lane] = acc;
-            __syncthreads();
-            cx pi = cm(0.0, 0.0);
-            if (lane < N) {
-#pragma unroll
-                for (int q = 0; q < NW; ++q) pi = pi + w.u.h.part[q * N + lane];
-            }
-            pi = cmul(t, pi);
-            const cx pvl = cconjmul(pi, vl);
-            const cx pv = cm(wave_sum(pvl.x), wave_sum(pvl.y));
-            const cx cc = cmul(t, pv) * -0.5;
-            const cx wi = pi + cmul(cc, vl);
-            if (lane < N) w.u.h.wb[wave][lane] = wi;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-            for (int c0 = 0; c0 < CPW; c0 += CB) {
-                __builtin_amdgcn_sched_barrier(0);   // keep chunks from interleaving (register pressure)
-                cx a[CB], vj[CB], wj[CB];
-#pragma unroll
-                for (int cc = 0; cc < CB; ++cc) {
-                    const int c = c0 + cc;
-                    const int j = (c < CPW && wave + NW * c < N) ? wave + NW * c : N - 1;
-                    a[cc] = w.L[pk(j) + (lr >= j ? lr - j : 0)];
-                    vj[cc] = w.u.h.vb[p][j];
-                    wj[cc] = w.u.h.wb[wave][j];
-                }
-#pragma unroll
-                for (int cc = 0; cc < CB; ++cc) {
-                    const int c = c0 + cc;
-                    const int j = wave + NW * c;
-                    if (c < CPW && j >= m0 && j < N && lane >= j && lane < N)
-                        w.L[pk(j) + lane - j] = a[cc] - cmulc(vl, wj[cc]) - cmulc(wi, vj[cc]);
-                }
-            }
-        }
-        if (k + 1 < N - 1 && wave == (k + 1) % NW) house_packed(w, k + 1);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        w.d[N - 1] = w.L[pk(N - 1)].x;
-        w.e[N - 1] = 0.0;
-        w.tau[N - 1] = cm(0.0, 0.0);
-    }
-    __syncthreads();
-}
-
-// Stash in the matrix's U slot: W on and below the diagonal, tau_k at
-// (k, k+1), reflector entries v_k(i) at (k, i), i >= k+2.
-__device__ __forceinline__ void stash_packed(const WorkP& w, cx* __restrict__ Ub) {
-    for (int e = threadIdx.x; e < N * N; e += NT) {
-        const int i = e % N, j = e / N;
-        cx v;
-        if (i >= j) v = w.Wi[pk(j) + i - j];
-        else if (i == j - 1) v = w.tau[i];
-        else v = w.L[pk(i) + j - i];
-        Ub[e] = v;
-    }
-}
-
 // ===========================================================================
-// Register-resident front half (split mode): each wave keeps its own columns
-// (j = wave + NW c) in registers, lane = row, so the Cholesky updates and the
-// tridiagonalisation's matrix-vector products and rank-2 updates never read
-// the matrix from LDS; LDS carries only the published pivot column / v / w.
+// Register-resident tridiagonalisation (geneig_mx_front): each wave keeps its
+// own columns (j = wave + NW c) in registers, lane = row, so the
+// matrix-vector products and rank-2 updates never read the matrix from LDS;
+// LDS carries only the published v / w and the partial sums.
 // ===========================================================================
-
-// Cholesky with the lower part of owned columns in registers; L is written to
-// the packed LDS triangle at the end (for the triangular inverse).
-__device__ __forceinline__ int potrf_reg(WorkP& w, const cx* __restrict__ Sb) {
-    const int lane = lane_id(), wave = wave_id();
-    const int lr = lane < N ? lane : N - 1;
-    cx a[CPW];
-#pragma unroll
-    for (int c = 0; c < CPW; ++c) {
-        const int j = wave + NW * c;
-        a[c] = (j < N && lane >= j && lane < N) ? Sb[(size_t)j * N + lane] : cm(0.0, 0.0);
-    }
-    if (threadIdx.x == 0) w.flag[1] = 0;
-    // Finish column k (pivot, scaling, publication) in the owning wave.
-    auto finish = [&](int k) {
-#pragma unroll
-        for (int c = 0; c < CPW; ++c) {
-            if (wave + NW * c == k) {
-                const real dkk = bcast(a[c].x, k);
-                if (!(dkk > real(0.0))) {
-                    if (lane == 0) w.flag[1] = k + 1;
-                } else {
-                    const real rinv = real(1.0) / sqrt_r(dkk);
-                    a[c] = a[c] * rinv;
-                    if (lane < N) w.u.h.vb[k & 1][lane] = a[c];
-                    if (lane == 0) w.invL[k] = rinv;
-                }
-            }
-        }
-    };
-    if (wave == 0) finish(0);
-    __syncthreads();
-    for (int k = 0; k < N; ++k) {
-        const int fail = w.flag[1];
-        if (fail) return fail;
-        const int p = k & 1;
-        const cx li = w.u.h.vb[p][lr];
-        cx lj[CPW];
-#pragma unroll
-        for (int c = 0; c < CPW; ++c) {
-            const int j = wave + NW * c;
-            lj[c] = w.u.h.vb[p][j < N ? j : N - 1];
-        }
-#pragma unroll
-        for (int c = 0; c < CPW; ++c) {
-            const int j = wave + NW * c;
-            const cx na = a[c] - cmulc(li, lj[c]);
-            if (j > k && j < N && lane >= j && lane < N) a[c] = na;
-        }
-        if (k + 1 < N && wave == (k + 1) % NW) finish(k + 1);
-        __syncthreads();
-    }
-#pragma unroll
-    for (int c = 0; c < CPW; ++c) {
-        const int j = wave + NW * c;
-        if (j < N && lane >= j && lane < N) w.L[pk(j) + lane - j] = a[c];
-    }
-    return 0;
-}
-
-// M = W H W^H, full columns, straight into the owning waves' registers
-// (column j = j0 + wave + NW cc of block jb is owned by wave j % NW, slot
-// j / NW). Z blocks go through LDS as in form_m.
-template <class HT>
-__device__ __forceinline__ void form_m_reg(WorkP& w, const HT* __restrict__ Hb, cx (&m)[CPW]) {
-    const int lane = lane_id(), wave = wave_id();
-    const int lr = lane < N ? lane : N - 1;
-    static_assert(ZB % NW == 0, "block columns must map onto owning waves");
-#pragma unroll
-    for (int jb = 0; jb < NZB; ++jb) {
-        const int j0 = jb * ZB;
-        cx acc[ZPW];
-#pragma unroll
-        for (int cc = 0; cc < ZPW; ++cc) acc[cc] = cm(0.0, 0.0);
-        const int kend = (j0 + ZB < N) ? j0 + ZB : N;
-#pragma unroll 4
-        for (int k = 0; k < kend; ++k) {
-            const cx h = to_cx(Hb[k * N + lr]);
-#pragma unroll
-            for (int cc = 0; cc < ZPW; ++cc) {
-                const int j = j0 + wave + NW * cc;
-                const bool on = (j < N) && (k <= j);
-                const cx wjk = w.Wi[on ? pk(k) + j - k : 0];
-                acc[cc] = acc[cc] + cmulc(h, on ? wjk : cm(0.0, 0.0));
-            }
-        }
-#pragma unroll
-        for (int cc = 0; cc < ZPW; ++cc) {
-            const int c = wave + NW * cc;
-            if (lane < N) w.u.zj[c * N + lane] = acc[cc];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int cc = 0; cc < ZPW; ++cc) acc[cc] = cm(0.0, 0.0);
-#pragma unroll 4
-        for (int k = 0; k < N; ++k) {
-            const cx wik = (lr >= k) ? w.Wi[pk(k) + lr - k] : cm(0.0, 0.0);
-#pragma unroll
-            for (int cc = 0; cc < ZPW; ++cc) {
-                const int c = wave + NW * cc;
-                acc[cc] = acc[cc] + cmul(wik, w.u.zj[c * N + k]);
-            }
-        }
-#pragma unroll
-        for (int cc = 0; cc < ZPW; ++cc) {
-            const int slot = jb * (ZB / NW) + cc;      // (j0 + wave + NW cc) / NW
-            if (slot < CPW) m[slot] = (lane < N) ? acc[cc] : cm(0.0, 0.0);
-        }
-        __syncthreads();
-    }
-}
-
 // Householder vector for column k held in registers (slot c) of its owner.
 template <class WK>
 __device__ __forceinline__ void house_reg(WK& w, int k, cx& col) {
@@ -1574,69 +1149,6 @@ __device__ __forceinline__ void hetrd_reg(WK& w, cx (&m)[CPW]) {
     __syncthreads();
 }
 
-// Stash: W (LDS) on and below the diagonal, tau on the superdiagonal, and the
-// reflector entries from the owning waves' registers (column k, rows >= k+2,
-// go to (k, i) of the stash).
-__device__ __forceinline__ void stash_reg(const WorkP& w, const cx (&m)[CPW], cx* __restrict__ Ub) {
-    const int lane = lane_id(), wave = wave_id();
-    for (int e = threadIdx.x; e < N * N; e += NT) {
-        const int i = e % N, j = e / N;
-        if (i >= j) Ub[e] = w.Wi[pk(j) + i - j];
-        else if (i == j - 1) Ub[e] = w.tau[i];
-    }
-#pragma unroll
-    for (int c = 0; c < CPW; ++c) {
-        const int k = wave + NW * c;
-        if (k < N && lane >= k + 2 && lane < N) Ub[(size_t)lane * N + k] = m[c];
-    }
-}
-
-// Stage W (columns packed) from the stash into a wave's slice.
-__device__ __forceinline__ void stage_w(cx* sl, const cx* __restrict__ Ub) {
-    const int lane = lane_id();
-    for (int e = lane; e < N * N; e += 64) {
-        const int i = e % N, j = e / N;
-        if (i >= j) sl[pk(j) + i - j] = Ub[e];
-    }
-}
-
-// U = W^H V for this lane's column, in place in ascending row order:
-// u_i = sum_{k >= i} conj(W(k, i)) v_k.
-__device__ __forceinline__ void trmm_wh_wave(const cx* sl, cx (&b)[N]) {
-    for (int i = 0; i < N; ++i) {
-        const int col = pk(i) - i;            // W(k, i) at col + k, k >= i
-        cx acc = cm(0.0, 0.0);
-#pragma unroll
-        for (int blk = 0; blk < NRB; ++blk) {
-            if (blk * RB + RB - 1 < i) continue;
-            cx ww[RB];
-#pragma unroll
-            for (int ii = 0; ii < RB; ++ii) {
-                const int k = blk * RB + ii;
-                if (k < N) ww[ii] = sl[(k >= i) ? col + k : 0];
-            }
-#pragma unroll
-            for (int ii = 0; ii < RB; ++ii) {
-                const int k = blk * RB + ii;
-                if (k < N) {
-                    const cx t = cconjmul(ww[ii], b[k]);
-                    if (k >= i) acc = acc + t;
-                }
-            }
-        }
-        const int ib = i / RB;
-#pragma unroll
-        for (int blk = 0; blk < NRB; ++blk) {
-            if (blk != ib) continue;
-#pragma unroll
-            for (int ii = 0; ii < RB; ++ii) {
-                const int k = blk * RB + ii;
-                if (k < N) b[k] = (k == i) ? opq(acc) : b[k];
-            }
-        }
-    }
-}
-
 // Stage data kept per matrix of a workgroup between the front and back halves.
 struct Staged {
     real d[NW][N];
@@ -1650,38 +1162,6 @@ struct Staged {
 using gk::cx;
 using gk::real;
 
-namespace gk {
-// Back half of the fused kernel: each wave finishes its own matrix in its
-// LDS slice of the (then idle) work area: Z -> V = Q Z -> U = W^H V.
-constexpr int PSLICES = (int)(sizeof(WorkP) / SLICE_BYTES) < NW ? (int)(sizeof(WorkP) / SLICE_BYTES) : NW;
-static_assert(PSLICES >= 1, "packed work area must hold one slice");
-
-__device__ __forceinline__ void back_half_packed(WorkP& w, bool active, const real (&z)[N], int rank,
-                                                 cx* __restrict__ Ub) {
-    const int wave = wave_id();
-    char* base = reinterpret_cast<char*>(&w);
-    for (int r = 0; r < (NW + PSLICES - 1) / PSLICES; ++r) {
-        if (active && wave / PSLICES == r) {
-            cx* sl = reinterpret_cast<cx*>(base + (size_t)(wave % PSLICES) * SLICE_BYTES);
-            real* zbuf = reinterpret_cast<real*>(sl);
-            cx b[N];
-            store_z_slice(zbuf, z, rank);
-            wave_lds_sync();
-            load_z_columns(zbuf, b);
-            wave_lds_sync();
-            stage_reflectors(sl, Ub);
-            wave_lds_sync();
-            apply_q_wave(sl, b);
-            wave_lds_sync();
-            stage_w(sl, Ub);
-            wave_lds_sync();
-            trmm_wh_wave(sl, b);
-            store_columns(Ub, b);
-        }
-        __syncthreads();
-    }
-}
-}  // namespace gk
 
 // ---------------------------------------------------------------------------
 // Fused solve. A workgroup of NW waves takes NW consecutive matrices: it runs
@@ -1851,45 +1331,6 @@ geneig_varlen(const cx* __restrict__ H, const cx* __restrict__ S, real* __restri
     }
 }
 
-// ---------------------------------------------------------------------------
-// Split mode (reference: src/hybrid/hybrid_solver.cpp:295-393): the same
-// stages as three kernels, each launched with the occupancy its register and
-// LDS footprint allow.
-//   geneig_split_front : Cholesky, W = L^-1, M = W H W^H, tridiagonalisation
-//                        (packed LDS, one matrix per workgroup). The stash
-//                        (W, reflectors, tau) goes to U, the diagonal of T to
-//                        W and its off-diagonal to E.
-//   geneig_split_ql    : one wave per matrix, no LDS: QL on (d, e), sorted
-//                        eigenvalues to W, Z (ascending columns) to Zw.
-//   geneig_split_back  : one wave per matrix: V = Q Z, U = W^H V.
-// A matrix whose Cholesky failed carries info > N and is skipped downstream.
-// ---------------------------------------------------------------------------
-extern "C" __global__ void __launch_bounds__(GENEIG_NT, 2)
-geneig_split_front(const cx* __restrict__ H, const cx* __restrict__ S, real* __restrict__ W,
-                   cx* __restrict__ U, int* __restrict__ info, real* __restrict__ E, int batch) {
-    using namespace gk;
-    __shared__ WorkP w;
-    const int b = blockIdx.x;
-    if (b >= batch) return;
-    const size_t off = (size_t)b * N * N;
-    const int pinfo = potrf_reg(w, S + off);
-    if (pinfo) {
-        fail_store(N + pinfo, W + (size_t)b * N, U + off, info + b);
-        return;
-    }
-    __syncthreads();
-    trtri_packed(w);
-    __syncthreads();
-    cx m[CPW];
-    form_m_reg(w, H + off, m);
-    hetrd_reg(w, m);
-    stash_reg(w, m, U + off);
-    if (threadIdx.x < N) {
-        W[(size_t)b * N + threadIdx.x] = w.d[threadIdx.x];
-        E[(size_t)b * N + threadIdx.x] = w.e[threadIdx.x];
-    }
-    if (threadIdx.x == 0) info[b] = 0;
-}
 #endif  // !GENEIG_FP32
 
 // ---------------------------------------------------------------------------
@@ -2137,41 +1578,6 @@ geneig_ql_full(real* __restrict__ W, const real* __restrict__ E, real* __restric
     store_z(Zw + (size_t)b * N * N, z, rank, info, b, qi);
 }
 
-namespace gk {
-constexpr int BACK_SLICES = (150 * 1024) / (int)SLICE_BYTES < NW ? (150 * 1024) / (int)SLICE_BYTES : NW;
-static_assert(BACK_SLICES >= 1, "one back-half slice must fit");
-}  // namespace gk
-
-extern "C" __global__ void __launch_bounds__(GENEIG_NT)
-geneig_split_back(cx* __restrict__ U, const real* __restrict__ Zw, const int* __restrict__ info,
-                  int batch) {
-    using namespace gk;
-    // One wave per matrix: lane j holds column j of V; reflectors and W are
-    // staged (packed) in the wave's LDS slice; waves beyond the slice count
-    // take turns.
-    __shared__ __attribute__((aligned(16))) char slices[BACK_SLICES][SLICE_BYTES];
-    const int wave = wave_id(), lane = lane_id();
-    const int b = blockIdx.x * NW + wave;
-    const bool active = b < batch && info[b] <= N;
-    for (int r = 0; r < (NW + BACK_SLICES - 1) / BACK_SLICES; ++r) {
-        if (active && wave / BACK_SLICES == r) {
-            const size_t off = (size_t)b * N * N;
-            cx* sl = reinterpret_cast<cx*>(slices[wave % BACK_SLICES]);
-            cx v[N];
-#pragma unroll
-            for (int i = 0; i < N; ++i) v[i] = cm((lane < N) ? Zw[off + lane * N + i] : 0.0, 0.0);
-            stage_reflectors(sl, U + off);
-            wave_lds_sync();
-            apply_q_wave(sl, v);
-            wave_lds_sync();
-            stage_w(sl, U + off);
-            wave_lds_sync();
-            trmm_wh_wave(sl, v);
-            store_columns(U + off, v);
-        }
-        __syncthreads();
-    }
-}
 
 // ---------------------------------------------------------------------------
 // Mixed precision (FP32 tridiagonalisation and eigenvectors, FP64 Cholesky
@@ -2192,12 +1598,16 @@ geneig_split_back(cx* __restrict__ U, const real* __restrict__ Zw, const int* __
 //   geneig_mx_refine  (FP64 module): Ogita-Aishima refinement of (M, V) in
 //                     the Cholesky frame on the matrix cores (M from Mp),
 //                     Rayleigh-Ritz on clusters, then U = W^H V; W, U out.
+// The FP64 split mode runs the same stages in the FP64 module, without the
+// refinement: geneig_lg_chol, geneig_mx_formm, geneig_mx_front (T to W and E,
+// the stash in U), geneig_ql_record + geneig_split_ql (or geneig_ql_full),
+// geneig_mx_back (V = Q Z in U), then geneig_mx_backtri (U = W^H V in place).
 // ---------------------------------------------------------------------------
 
-#ifdef GENEIG_FP32
+// Built in both precisions: FP32 for the mixed mode, FP64 for the split mode.
 namespace gk {
-// Shared memory of the FP32 front half: the tridiagonalisation's buffers
-// only (M arrives formed, straight into the owning waves' registers).
+// Shared memory of the front half: the tridiagonalisation's buffers only
+// (M arrives formed, straight into the owning waves' registers).
 struct WorkF {
     union {
         struct {
@@ -2230,17 +1640,31 @@ geneig_mx_front(const gk::cd* __restrict__ Mp, real* __restrict__ W32, cx* __res
                 const int* __restrict__ info, real* __restrict__ E32, int batch) {
     using namespace gk;
     __shared__ WorkF w;
-    __shared__ cx ms[NPK];
+    // FP32: M in full (rounded); FP64: its packed lower triangle (half the
+    // LDS, so that two workgroups still share a CU).
+    constexpr bool MS_FULL = sizeof(real) == 4;
+    __shared__ cx ms[MS_FULL ? N * LDA : NPK];
     const int b = blockIdx.x;
     if (b >= batch || info[b] != 0) return;
     const size_t off = (size_t)b * N * N;
     const int lane = lane_id(), wave = wave_id();
-    // M's lower triangle (complex128, packed by columns) rounded into LDS, then
-    // each owned column made Hermitian from it: M(i, j) for i >= j, conj(M(j, i))
-    // above the diagonal, a real diagonal.
+    // M's lower triangle (complex128, packed by columns; one coalesced column
+    // per wave and step) into LDS; each owned column is then Hermitian from
+    // it: M(i, j) below the diagonal, conj(M(j, i)) above, a real diagonal.
     {
         const cd* Mb = Mp + (size_t)b * NPK;
-        for (int e = threadIdx.x; e < NPK; e += NT) ms[e] = to_cx(Mb[e]);
+        for (int j = wave; j < N; j += NW) {
+            const int i = j + lane;
+            if (i < N) {
+                const cx v = to_cx(Mb[pk(j) + lane]);
+                if constexpr (MS_FULL) {
+                    ms[j * LDA + i] = (i == j) ? cm(v.x, real(0.0)) : v;
+                    if (i != j) ms[i * LDA + j] = cconj(v);
+                } else {
+                    ms[pk(j) + lane] = (i == j) ? cm(v.x, real(0.0)) : v;
+                }
+            }
+        }
         __syncthreads();
     }
     cx m[CPW];
@@ -2248,9 +1672,13 @@ geneig_mx_front(const gk::cd* __restrict__ Mp, real* __restrict__ W32, cx* __res
     for (int c = 0; c < CPW; ++c) {
         const int j = wave + NW * c;
         const bool in = lane < N && j < N;
-        const bool lo = lane >= j;
-        const cx v = ms[in ? (lo ? pk(j) + lane - j : pk(lane) + j - lane) : 0];
-        m[c] = !in ? cm(0.0f, 0.0f) : (lane == j ? cm(v.x, 0.0f) : (lo ? v : cconj(v)));
+        if constexpr (MS_FULL) {
+            m[c] = in ? ms[j * LDA + lane] : cm(real(0.0), real(0.0));
+        } else {
+            const bool lo = lane >= j;
+            const cx v = ms[in ? (lo ? pk(j) + lane - j : pk(lane) + j - lane) : 0];
+            m[c] = !in ? cm(real(0.0), real(0.0)) : (lo ? v : cconj(v));
+        }
     }
     hetrd_reg(w, m);
     stash_refl_reg(w, m, U32 + off);
@@ -2260,12 +1688,17 @@ geneig_mx_front(const gk::cd* __restrict__ Mp, real* __restrict__ W32, cx* __res
     }
 }
 
-// V = Q Z on the matrix cores (back_wy; the FP32 back half without the W^H
-// factor, which the FP64 refinement applies), one workgroup of one wave per
-// 16-column panel per matrix: reflectors from the stash in U32 (v_k(i) at
-// row k, column i; tau_k at row k, column k + 1), V over it. Four waves per
-// SIMD: the compiler would otherwise spend registers on three.
-extern "C" __global__ void __launch_bounds__(gk::BNT) __attribute__((amdgpu_waves_per_eu(4)))
+// V = Q Z on the matrix cores (back_wy; the back half without the W^H
+// factor, which the FP64 refinement, or geneig_mx_backtri in the FP64 split
+// mode, applies), one workgroup of one wave per 16-column panel per matrix:
+// reflectors from the stash in U32 (v_k(i) at row k, column i; tau_k at row
+// k, column k + 1), V over it. Four waves per SIMD in FP32 (the compiler
+// would otherwise spend registers on three), three in FP64 (whose tiles take
+// twice the registers).
+namespace gk {
+constexpr int BACK_WAVES_PER_EU = sizeof(real) == 4 ? 4 : 3;
+}
+extern "C" __global__ void __launch_bounds__(gk::BNT) __attribute__((amdgpu_waves_per_eu(gk::BACK_WAVES_PER_EU)))
 geneig_mx_back(cx* __restrict__ U32, const real* __restrict__ Zw, const int* __restrict__ info, int batch) {
     using namespace gk;
     __shared__ BackWork bw;
@@ -2279,7 +1712,7 @@ geneig_mx_back(cx* __restrict__ U32, const real* __restrict__ Zw, const int* __r
         [&](int row, int col) { return Zb[(size_t)col * N + row]; },
         [&](int row, int col, cx v) { Ub[(size_t)col * N + row] = v; });
 }
-#endif  // GENEIG_FP32
+
 
 #ifndef GENEIG_FP32
 
@@ -2557,6 +1990,37 @@ geneig_mx_formm_diag(const real* __restrict__ Hd, long long hd_stride, const cx*
     cgemm(M, [&](int i, int k) { return lelem(Ws, i, k); },
           [&](int k, int jj) { return cconj(lelem(Ws, jj, k)) * hs[k]; }, cb, kz);
     store_m_packed(Mp + (size_t)b * NPK, M, j, jin, g4);
+}
+
+// U = W^H V of the FP64 split mode, in place (V arrives in U): V and W's
+// lower triangle staged in LDS, then each wave's 16-column block on the
+// matrix cores (W^H is upper triangular: row tile rt takes k >= 16 rt).
+// Matrices whose Cholesky failed keep the outputs it wrote.
+extern "C" __global__ void __launch_bounds__(gk::REF_NT, 2)
+geneig_mx_backtri(const cx* __restrict__ Wd, cx* __restrict__ U, const int* __restrict__ info, int batch) {
+    using namespace gk;
+    __shared__ cx Vs[N * LDR];
+    __shared__ cx Ws[NPK];
+    const int b = blockIdx.x;
+    if (b >= batch || info[b] > N) return;
+    const int l = lane_id(), cb = wave_id();
+    const int g4 = l >> 4;
+    const int j = cb * 16 + (l & 15);          // this lane's column
+    cx* Ub = U + (size_t)b * N * N;
+    for (int e = threadIdx.x; e < N * N; e += REF_NT) Vs[(e / N) * LDR + e % N] = Ub[e];
+    stage_w(Ws, w_of(Wd, b), threadIdx.x, REF_NT);
+    __syncthreads();
+    ctile Uo[NB];
+    cgemm<true>(Uo, [&](int i, int k) { return cconj(lelem(Ws, k, i)); },
+                [&](int k, int jj) { return elem(Vs, LDR, k, jj); }, cb);
+#pragma unroll
+    for (int rt = 0; rt < NB; ++rt) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = rt * 16 + g4 + 4 * v;
+            if (i < N && j < N) Ub[(size_t)j * N + i] = tile_get(Uo[rt], v);
+        }
+    }
 }
 
 // Refinement in the Cholesky frame: with W = L^-1 (FP64), M = W H W^H and the

@@ -118,6 +118,23 @@ namespace {
 struct MixedLayout {
     size_t wd, v32, w32, e32, z32, mp, total;
 };
+// The FP64 split mode's workspace: W = L^-1 (room for the full matrix;
+// packed where geneig_lg_chol packs it), M (packed complex128), the
+// tridiagonal's off-diagonal and Z (float64).
+struct HybridLayout {
+    size_t wd, mp, e, z, total;
+};
+HybridLayout hybrid_layout(size_t n, size_t B) {
+    auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+    HybridLayout h;
+    h.wd = 0;
+    h.mp = al(h.wd + B * n * n * 16);
+    h.e = al(h.mp + B * n * (n + 1) / 2 * 16);
+    h.z = al(h.e + B * n * 8);
+    h.total = al(h.z + B * n * n * 8);
+    return h;
+}
+
 MixedLayout mixed_layout(size_t n, size_t B) {
     auto al = [](size_t x) { return (x + 255) / 256 * 256; };
     MixedLayout m;
@@ -265,10 +282,8 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
         return;
     }
     fused_ = module_->get("geneig_fused");
-    k_reduce_ = module_->get("geneig_split_front");
     k_heev_ = module_->get("geneig_split_ql");
     k_qlfull_ = module_->get("geneig_ql_full");
-    k_back_ = module_->get("geneig_split_back");
     k_qlrec_ = module_->get("geneig_ql_record");
     {
         hipDeviceptr_t sym = nullptr;
@@ -280,20 +295,26 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
             throw std::runtime_error("QL log layout mismatch: module " + std::to_string(dev_q) + " B, host " +
                                      std::to_string(ql_log_bytes(n_)) + " B");
     }
-    if (mode_ == Mode::Mixed) {
+    if (mode_ == Mode::Hybrid || mode_ == Mode::Mixed) {
+        // The FP64 stages both modes share: Cholesky and W = L^-1, M = W H W^H.
         k_mx_chol_ = module_->get("geneig_lg_chol");
-        k_refine_ = module_->get("geneig_mx_refine");
         k_mx_formm_ = module_->get("geneig_mx_formm");
         k_mx_formm_diag_ = module_->get("geneig_mx_formm_diag");
-        {
-            // geneig_lg_chol leaves W packed (its lower triangle) where it fits LDS.
-            hipDeviceptr_t sym = nullptr;
-            size_t sym_bytes = 0;
-            int packed = 0;
-            SG_HIP(hipModuleGetGlobal(&sym, &sym_bytes, module_->module, "geneig_w_packed"));
-            SG_HIP(hipMemcpyDtoH(&packed, sym, sizeof(packed)));
-            w_packed_ = packed != 0;
-        }
+        // geneig_lg_chol leaves W packed (its lower triangle) where it fits LDS.
+        hipDeviceptr_t sym = nullptr;
+        size_t sym_bytes = 0;
+        int packed = 0;
+        SG_HIP(hipModuleGetGlobal(&sym, &sym_bytes, module_->module, "geneig_w_packed"));
+        SG_HIP(hipMemcpyDtoH(&packed, sym, sizeof(packed)));
+        w_packed_ = packed != 0;
+    }
+    if (mode_ == Mode::Hybrid) {
+        k_front64_ = module_->get("geneig_mx_front");
+        k_back64_ = module_->get("geneig_mx_back");
+        k_backtri_ = module_->get("geneig_mx_backtri");
+    }
+    if (mode_ == Mode::Mixed) {
+        k_refine_ = module_->get("geneig_mx_refine");
         ModuleKey k32 = key;
         k32.fp32 = true;
         CacheLayer l32;
@@ -309,11 +330,11 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
         if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_DELTA")) refine_delta_ = std::atof(v);
         if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_TOL")) refine_tol_ = std::atof(v);
     }
-    // Reported LDS: the fused kernel's; in the split mode the front kernel's
-    // (the reference reports its reduce kernel's, c_api.cpp:41-49); in the
-    // mixed mode the largest of its kernels'.
+    // Reported LDS: the fused kernel's; in the split mode the largest of the
+    // reduction kernels' (the reference reports its reduce kernel's,
+    // c_api.cpp:41-49); in the mixed mode the largest of its kernels'.
     std::vector<hipFunction_t> fns = {fused_};
-    if (mode_ == Mode::Hybrid) fns = {k_reduce_};
+    if (mode_ == Mode::Hybrid) fns = {k_mx_chol_, k_mx_formm_, k_front64_};
     if (mode_ == Mode::Mixed) fns = {k_mx_chol_, k_mx_formm_, k_mx_front_, k_back32_, k_refine_};
     int max_lds = 0;
     SG_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device_));
@@ -605,7 +626,7 @@ Solver::Workspace Solver::own_workspace_(size_t batch) {
 size_t Solver::scratch_bytes_needed(Mode mode, int n, size_t batch) {
     if (n > kMaxLdsOrder) return large_matrix_bytes(n);
     const size_t nn = (size_t)n;
-    if (mode == Mode::Hybrid) return (batch * (nn + nn * nn) * 8 + 255) / 256 * 256;
+    if (mode == Mode::Hybrid) return hybrid_layout(nn, batch).total;
     if (mode == Mode::Mixed) return mixed_layout(nn, batch).total;
     return 0;
 }
@@ -671,21 +692,48 @@ void Solver::ensure_workspace_(size_t batch) {
         lbuf_ = nullptr;
         lbuf_batch_ = 0;
     }
-    // Per matrix: the tridiagonal's off-diagonal (N doubles) and Z (N x N doubles).
-    SG_HIP(hipMalloc(&lbuf_, workspace_bytes(Mode::Hybrid, n_, batch)));
+    // Per matrix: W = L^-1, packed M, the tridiagonal's off-diagonal and Z (hybrid_layout).
+    SG_HIP(hipMalloc(&lbuf_, hybrid_layout((size_t)n_, batch).total));
     lbuf_batch_ = batch;
 }
 
-// Split mode: three kernels (front half, QL, back half), each at the
-// occupancy its own footprint allows (reference: hybrid_solver.cpp:295-393).
+// Split mode (reference: hybrid_solver.cpp:295-393), FP64 throughout: the
+// reduction (Cholesky and W = L^-1, M = W H W^H on the matrix cores, the
+// tridiagonalisation of M), QL, then the back transform (V = Q Z and
+// U = W^H V on the matrix cores); each kernel at the occupancy its own
+// footprint allows.
 void Solver::launch_hybrid_(const void* H, const void* S, double* W, void* U, int* info, int batch,
                             hipStream_t stream, const Workspace& ws) {
-    double* E = static_cast<double*>(ws.split);
-    double* Zw = E + (size_t)batch * n_;
-    void* a1[] = {&H, &S, &W, &U, &info, &E, &batch};
-    launch_(k_reduce_, (unsigned)batch, (unsigned)block_dim_, stream, a1);
+    const size_t n = (size_t)n_, B = (size_t)batch;
+    const HybridLayout hl = hybrid_layout(n, B);
+    char* base = static_cast<char*>(ws.split);
+    void* Wd = base + hl.wd;
+    void* Mp = base + hl.mp;
+    double* E = reinterpret_cast<double*>(base + hl.e);
+    double* Zw = reinterpret_cast<double*>(base + hl.z);
+    const unsigned grid_b = (unsigned)batch;
+    const unsigned grid_w = (unsigned)((batch + bpb_ - 1) / bpb_);
+    const unsigned bd = (unsigned)block_dim_;
+    // One wave per 16-row (or 16-column) tile for the matrix-core kernels.
+    const unsigned bd_tiles = (unsigned)(64 * ((n_ + 15) / 16));
+    long long wstride = (long long)((w_packed_ ? n * (n + 1) / 2 : n * n) * 16);
+    void* a1[] = {&S, &W, &U, &info, &Wd, &wstride, &batch};
+    launch_(k_mx_chol_, grid_b, bd_tiles, stream, a1);
     mark_(stream);
-    const unsigned grid2 = (unsigned)((batch + bpb_ - 1) / bpb_);
+    if (ws.hdiag && !ws.hfull) {
+        const double* hd = ws.hdiag;
+        long long hds = ws.hdiag_stride;
+        void* ad[] = {&hd, &hds, &Wd, &info, &Mp, &batch};
+        launch_(k_mx_formm_diag_, grid_b, bd_tiles, stream, ad);
+    } else {
+        void* af[] = {&H, &Wd, &info, &Mp, &batch};
+        launch_(k_mx_formm_, grid_b, bd_tiles, stream, af);
+    }
+    mark_(stream);
+    // Tridiagonalisation: diagonal to W, off-diagonal to E, reflectors to U.
+    void* a2[] = {&Mp, &W, &U, &info, &E, &batch};
+    launch_(k_front64_, grid_b, bd, stream, a2);
+    mark_(stream);
     char* qlog = static_cast<char*>(ws.qlog);
     long long qstride = (long long)ql_log_bytes(n_);
     int cap = ql_sweep_cap_;
@@ -695,15 +743,18 @@ void Solver::launch_hybrid_(const void* H, const void* S, double* W, void* U, in
         void* ar[] = {&W, &E, &info, &qlog, &qstride, &batch, &cap};
         launch_(k_qlrec_, (unsigned)((batch + 63) / 64), 64, stream, ar);
         mark_(stream);
-        void* a2[] = {&W, &E, &Zw, &info, &qlog, &qstride, &batch};
-        launch_(k_heev_, grid2, (unsigned)block_dim_, stream, a2);
+        void* a3[] = {&W, &E, &Zw, &info, &qlog, &qstride, &batch};
+        launch_(k_heev_, grid_w, bd, stream, a3);
         mark_(stream);
     }
-    void* af[] = {&W, &E, &Zw, &info, &qlog, &qstride, &batch, &all};
-    launch_(k_qlfull_, grid2, (unsigned)block_dim_, stream, af);
+    void* aq[] = {&W, &E, &Zw, &info, &qlog, &qstride, &batch, &all};
+    launch_(k_qlfull_, grid_w, bd, stream, aq);
     mark_(stream);
-    void* a3[] = {&U, &Zw, &info, &batch};
-    launch_(k_back_, grid2, (unsigned)block_dim_, stream, a3);
+    void* a4[] = {&U, &Zw, &info, &batch};
+    launch_(k_back64_, grid_b, bd_tiles, stream, a4);
+    mark_(stream);
+    void* a5[] = {&Wd, &U, &info, &batch};
+    launch_(k_backtri_, grid_b, bd_tiles, stream, a5);
     mark_(stream);
 }
 
@@ -927,7 +978,7 @@ size_t Solver::workspace_bytes(Mode mode, int n_, size_t batch) {
     const size_t n = (size_t)n_;
     // The QL rotation logs only for batches that run QL through them.
     const size_t qlog = batch >= (size_t)kQlLogMinBatch ? batch * ql_log_bytes(n_) : 0;
-    if (mode == Mode::Hybrid) return batch * (n + n * n) * 8 + qlog;
+    if (mode == Mode::Hybrid) return hybrid_layout(n, batch).total + qlog;
     if (mode == Mode::Mixed) return mixed_layout(n, batch).total + qlog;
     return 0;
 }

@@ -104,7 +104,7 @@ private:
     };
     void launch_impl_(const void* H, const double* Hd, long long hds, const void* S, double* W, void* U,
                       int* info, int batch, hipStream_t stream);
-    bool hdiag_direct_() const { return mode_ == Mode::Mixed && !large_; }
+    bool hdiag_direct_() const { return (mode_ == Mode::Mixed || mode_ == Mode::Hybrid) && !large_; }
     void ensure_hbuf_(size_t batch);
     void* hbuf_ = nullptr;             // full H expanded from a diagonal
     size_t hbuf_batch_ = 0;
@@ -139,16 +139,20 @@ private:
     std::string arch_;
     std::shared_ptr<LoadedModule> module_;
     hipFunction_t fused_ = nullptr;
-    hipFunction_t k_reduce_ = nullptr;
     hipFunction_t k_heev_ = nullptr;
     hipFunction_t k_qlfull_ = nullptr;     // QL of the matrices whose log overflowed
-    hipFunction_t k_back_ = nullptr;
+    // FP64 split mode: tridiagonalisation, V = Q Z and U = W^H V (with the
+    // Cholesky and M = W H W^H it shares with the mixed mode, below).
+    hipFunction_t k_front64_ = nullptr;
+    hipFunction_t k_back64_ = nullptr;
+    hipFunction_t k_backtri_ = nullptr;
     // QL recurrence recorder (64 matrices per wave) of the split QL and its
     // per-matrix rotation logs (split and mixed modes, orders up to 64).
     hipFunction_t k_qlrec_ = nullptr;
     void* qlog_ = nullptr;
     size_t qlog_batch_ = 0;
-    // Split-mode workspace: tridiagonal off-diagonals and Z, one per matrix.
+    // Split-mode workspace: W = L^-1, packed M, tridiagonal off-diagonals and
+    // Z, one set per matrix.
     void* lbuf_ = nullptr;
     size_t lbuf_batch_ = 0;
     // Mixed mode: the FP32 module and the FP32 hand-offs between its stages
@@ -160,7 +164,7 @@ private:
     hipFunction_t k_ql32_ = nullptr;
     hipFunction_t k_qlfull32_ = nullptr;
     hipFunction_t k_mx_formm_ = nullptr;
-    bool w_packed_ = false;          // geneig_lg_chol leaves W packed (mixed mode)
+    bool w_packed_ = false;          // geneig_lg_chol leaves W packed (split and mixed modes)
     hipFunction_t k_back32_ = nullptr;
     hipFunction_t k_refine_ = nullptr;
     void* mbuf_ = nullptr;

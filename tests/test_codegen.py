@@ -19,7 +19,8 @@ import pytest
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(REPO, "sunny_gen_eigensolver_amd", "csrc", "kernels", "geneig_kernels.hip")
-KERNELS = ("geneig_fused", "geneig_varlen", "geneig_split_front", "geneig_split_ql", "geneig_ql_full", "geneig_split_back")
+KERNELS = ("geneig_fused", "geneig_varlen", "geneig_mx_front", "geneig_split_ql", "geneig_ql_full", "geneig_mx_back",
+           "geneig_mx_backtri")
 ORDERS = sorted(set(range(4, 65, 4)) | {27, 54})
 
 

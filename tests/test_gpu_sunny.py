@@ -73,10 +73,10 @@ def _run_hdiag(H, S, kernel, per_matrix=False):
 @pytest.mark.parametrize("L", [12, 27, 48])
 def test_diagonal_h_entry_point(kernel, L):
     """sunny_geneig_launch_hdiag with one shared Sigma for the batch: the
-    mixed mode at these orders forms M from the diagonal directly (H never
-    loaded), the other modes expand it and must match sunny_geneig_launch on
-    the full H bit for bit; all against zhegv with the usual bars and the
-    +- pairing."""
+    split and mixed modes at these orders form M from the diagonal directly
+    (H never loaded), the fused mode expands it and must match
+    sunny_geneig_launch on the full H bit for bit; all against zhegv with the
+    usual bars and the +- pairing."""
     H, S = bogoliubov_pencils(L, 8, 2000 + L)
     W, U, info = _run_hdiag(H, S, kernel)
     assert (info == 0).all()
@@ -84,12 +84,13 @@ def test_diagonal_h_entry_point(kernel, L):
     assert (np.abs(W + W[:, ::-1]) / scale).max() < 1e-12
     Wl, Ul, il = O.zhegv_batch(H, S, threads=8)
     T.assert_parity(H, S, W, U, info, Wl, Ul, tag=f"hdiag {kernel} Sigma/D L={L}")
-    if kernel != "mixed":
+    if kernel == "unified":
         Wf, Uf, _ = T.run_gpu(H, S, kernel=kernel)
         assert np.array_equal(W, Wf) and np.array_equal(U, Uf)
 
 
-@pytest.mark.parametrize("kernel,n", [("mixed", 54), ("mixed", 7), ("unified", 54), ("mixed", 96)])
+@pytest.mark.parametrize("kernel,n", [("mixed", 54), ("mixed", 7), ("unified", 54), ("hybrid", 54), ("hybrid", 7),
+                                      ("mixed", 96)])
 def test_diagonal_h_per_matrix(kernel, n):
     """A different real (indefinite) diagonal per matrix (stride n), including
     a staged order, against zhegv on the full diagonal H."""
