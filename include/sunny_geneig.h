@@ -57,12 +57,12 @@ typedef enum {
 } sunny_geneig_tuning_mode_t;
 
 /* Reference: src/common/c_api.h:43-46. UNIFIED = one fused kernel per batch;
- * HYBRID = split mode: front-half kernel (Cholesky, reduction,
- * tridiagonalisation), batched QL kernel, batched back-transform kernel (all
- * written for gfx950 by this project). MIXED is additive (no counterpart in
- * the reference): FP64 Cholesky, FP32 reduction / tridiagonalisation / QL /
- * back-transform, then FP64 eigenpair refinement on the original pencil; same
- * buffers and outputs as the other modes. */
+ * HYBRID = split mode, FP64: reduction kernels (Cholesky and W = L^-1,
+ * M = W H W^H, tridiagonalisation), batched QL kernels, batched
+ * back-transform kernels (V = Q Z, U = W^H V), all written for gfx950 by this
+ * project. MIXED is additive (no counterpart in the reference): the same FP64
+ * Cholesky and M, FP32 tridiagonalisation / QL / V = Q Z, then FP64
+ * eigenpair refinement; same buffers and outputs as the other modes. */
 typedef enum {
     SUNNY_GENEIG_KERNEL_UNIFIED = 0,
     SUNNY_GENEIG_KERNEL_HYBRID  = 1,
@@ -112,8 +112,8 @@ sunny_geneig_status_t sunny_geneig_launch(
 /* Reference: src/common/c_api.h:87-91 (c_api.cpp:188-206). -1 / 0 on NULL.
  * batches_per_block is 1 in the split (HYBRID) and MIXED modes, as the
  * reference's split mode reports (c_api.cpp:41-49); shared_mem_bytes is the
- * fused kernel's LDS, the split mode's front kernel's, or the largest of the
- * mixed mode's kernels. */
+ * fused kernel's LDS, the largest of the split mode's reduction kernels', or
+ * the largest of the mixed mode's kernels'. */
 int          sunny_geneig_matrix_size      (sunny_geneig_solver_t solver);
 int          sunny_geneig_batches_per_block(sunny_geneig_solver_t solver);
 int          sunny_geneig_block_dim_x      (sunny_geneig_solver_t solver);
@@ -183,7 +183,7 @@ sunny_geneig_status_t sunny_geneig_launch_hdiag(
  * runs on (set up when a handle is created there) until the process ends; the
  * call allocates nothing and captures no graph. A batch whose workspace does
  * not fit returns INVALID_ARG with the limit in the message (at N = 54: about
- * 1,000 matrices per call in the mixed mode, 5,600 in the split mode; the
+ * 1,260 matrices per call in the mixed mode, 1,420 in the split mode; the
  * fused mode needs no workspace; orders above 64 run in rounds that fit).
  * Calls on one GPU share its array and are serialised on the device through
  * an event, whatever stream or handle they use. */
@@ -246,9 +246,10 @@ int sunny_geneig_graph_stats(sunny_geneig_solver_t solver, long long* captures, 
  * One call solves `count` pencils of different orders n[k] in [1, max_n].
  * Every matrix goes to the smallest bucket order >= n[k] and is embedded in
  * that order as (H (+) pad I, S (+) I), which leaves its eigenpairs
- * unchanged. Buckets up to 64 are one grid of an LDS-resident kernel each;
- * buckets above 64 run the staged FP64 kernels of sunny_geneig_launch on a
- * padded copy (gathered and scattered on the device, in rounds of 4096).
+ * unchanged. Every bucket runs the FP64 split mode of sunny_geneig_launch at
+ * its order (the staged kernels above 64) on a padded copy, gathered and
+ * scattered on the device in rounds (4096 matrices above 64; below, as many
+ * as keep the padded H, S and U within 4 GiB, at most 65536).
  *
  * Packed buffers, matrix k column-major at element offset sum_{j<k} n[j]^2:
  *   d_H, d_S, d_U : complex128, sum n[k]^2 elements;
@@ -264,11 +265,12 @@ sunny_geneig_varlen_t sunny_geneig_varlen_create(int max_n, int device_id,
                                                  sunny_geneig_cache_mode_t cache_mode);
 
 /* As sunny_geneig_varlen_create, with a tuning mode: AUTOTUNE gives every
- * bucket up to 64 the workgroup size (matrices per workgroup) that the timed
- * sweep of sunny_geneig_create_ex's AUTOTUNE picks for that order (run once
- * per order and architecture, then read from the cache's table); SUGGESTED
- * is sunny_geneig_varlen_create. Buckets above 64 have one size. Counterpart:
- * the reference tunes one order per handle only (unified_solver.cpp:649-925). */
+ * bucket what the timed sweep of sunny_geneig_create_ex's AUTOTUNE picks for
+ * the split mode at that order (workgroup size, and from order 48
+ * LDS-resident or HBM-staged blocking; run once per order and architecture,
+ * then read from the cache's table); SUGGESTED is sunny_geneig_varlen_create.
+ * Buckets above 64 have one size. Counterpart: the reference tunes one order
+ * per handle only (unified_solver.cpp:649-925). */
 sunny_geneig_varlen_t sunny_geneig_varlen_create_ex(int max_n, int device_id,
                                                     const int* bucket_orders, int num_buckets,
                                                     sunny_geneig_cache_mode_t cache_mode,
@@ -279,7 +281,7 @@ void sunny_geneig_varlen_destroy(sunny_geneig_varlen_t solver);
 /* Asynchronous on `stream`; n_host (host memory, `count` orders) is read
  * before return. Calls on one handle are ordered after each other whatever
  * the stream; the first call with given buffers and orders is captured into
- * one hipGraph (every grid, the staged buckets included) that later calls
+ * one hipGraph (every bucket's gather, solve and scatter rounds) that later calls
  * with the same buffers and orders replay without any host-side sorting or
  * upload. A handle is not reentrant: one call at a time per handle.
  * count == 0 is a no-op; an order outside [1, max_n] -> INVALID_ARG. */
@@ -302,7 +304,7 @@ int sunny_geneig_varlen_set_graphs(sunny_geneig_varlen_t solver, int enable);
 int sunny_geneig_varlen_graph_stats(sunny_geneig_varlen_t solver, long long* captures, long long* replays);
 
 /* Number of buckets, the order of bucket i and its matrices per workgroup
- * (-1 on NULL / out of range). */
+ * in the QL grids (1 on the staged kernels; -1 on NULL / out of range). */
 int sunny_geneig_varlen_num_buckets (sunny_geneig_varlen_t solver);
 int sunny_geneig_varlen_bucket_order(sunny_geneig_varlen_t solver, int i);
 int sunny_geneig_varlen_bucket_waves(sunny_geneig_varlen_t solver, int i);

@@ -404,7 +404,7 @@ class VarlenSolver:
     def __init__(self, max_n: int = 64, device_id: int = 0, buckets=None, cache: str = "auto",
                  tuning: str = "suggested"):
         """tuning: "suggested" (fixed workgroup size per order) or "autotune"
-        (per bucket up to 64, the fused solver's timed sweep for that order)."""
+        (per bucket, the split-mode solver's timed sweep for that order)."""
         L = lib()
         c = {"nodisk": CACHE_NODISK, "nocache": CACHE_NOCACHE}.get(cache, CACHE_AUTO)
         if tuning not in ("suggested", "autotune"):

@@ -102,15 +102,6 @@ __device__ __forceinline__ cx to_cx(const cx& a) { return a; }
 __device__ __forceinline__ real cabs2(cx a) { return a.x * a.x + a.y * a.y; }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
-// Lane index, optionally behind a value barrier. With OPAQUE the per-lane
-// addresses a routine derives from it are recomputed at every call instead of
-// being hoisted out of an enclosing loop and kept live across it.
-template <bool OPAQUE>
-__device__ __forceinline__ int call_lane() {
-    int l = lane_id();
-    if (OPAQUE) asm volatile("" : "+v"(l));
-    return l;
-}
 __device__ __forceinline__ int wave_id() { return threadIdx.x >> 6; }
 
 // Value of `v` in lane `src` (src wave-uniform), via v_readlane.
@@ -348,13 +339,12 @@ struct Work {
 // previous pivot column, so no thread reads a value another thread writes in
 // the same step; one barrier per step.
 // ---------------------------------------------------------------------------
-template <bool RELANE = false>
 __device__ __forceinline__ int potrf(Work& w) {
     // Column j belongs to wave j % NW. The owner of column k finishes it
     // (pivot check, scaling) right after its own trailing updates of step k-1
     // and publishes it in vb[k & 1]; all waves then update their own columns.
     // One barrier per step.
-    const int lane = call_lane<RELANE>(), wave = wave_id();
+    const int lane = lane_id(), wave = wave_id();
     auto finish_column = [&](int k) {
         const cx a = (lane >= k && lane < N) ? w.A[k * LDA + lane] : cm(0.0, 0.0);
         const real dkk = bcast(a.x, k);
@@ -403,12 +393,11 @@ __device__ __forceinline__ int potrf(Work& w) {
 // columns, lane = row), then right solve row by row (each wave owns rows,
 // lane = column). Pivot values travel by readlane: no barriers inside.
 // ---------------------------------------------------------------------------
-template <bool RELANE = false>
 __device__ __forceinline__ void trsm_left(Work& w) {
     // X = L^-1 B, column j owned by wave j % NW (lane = row). Step k reads the
     // pivot entry B(k, j) with a broadcast LDS read; a wave's own writes are
     // visible to its later reads, so no barrier is needed between steps.
-    const int lane = call_lane<RELANE>(), wave = wave_id();
+    const int lane = lane_id(), wave = wave_id();
     const int lr = lane < N ? lane : N - 1;
     for (int k = 0; k < N; ++k) {
         const real inv = w.invL[k];
@@ -430,11 +419,10 @@ __device__ __forceinline__ void trsm_left(Work& w) {
     }
 }
 
-template <bool RELANE = false>
 __device__ __forceinline__ void trsm_right(Work& w) {
     // Y L^H = X row by row, row i owned by wave i % NW (lane = column):
     // Y(i,k) = X(i,k) / L(k,k), then X(i,j) -= Y(i,k) conj(L(j,k)), j > k.
-    const int lane = call_lane<RELANE>(), wave = wave_id();
+    const int lane = lane_id(), wave = wave_id();
     const int lr = lane < N ? lane : N - 1;
     for (int k = 0; k < N; ++k) {
         const real inv = w.invL[k];
@@ -464,13 +452,12 @@ __device__ __forceinline__ void trsm_right(Work& w) {
 // ---------------------------------------------------------------------------
 // Householder vector for column k, computed by the wave that owns column k:
 // v -> vb[k & 1] (zeros above row k+1) and B(k+2.., k); e[k], d[k], tau[k].
-template <bool RELANE = false>
 __device__ __forceinline__ void house(Work& w, int k) {
-    const int lane = call_lane<RELANE>();
+    const int lane = lane_id();
     const int m0 = k + 1;
     const cx x = (lane >= m0 && lane < N) ? w.B[k * LDA + lane] : cm(0.0, 0.0);
     const cx alpha = bcast(x, m0);
-    const real xn2 = wave_sum(lane > m0 ? cabs2(x) : 0.0);
+    const real xn2 = wave_sum(lane > m0 ? cabs2(x) : real(0.0));
     cx t = cm(0.0, 0.0);
     real beta = alpha.x;
     cx v = cm(0.0, 0.0);
@@ -495,14 +482,13 @@ __device__ __forceinline__ void house(Work& w, int k) {
     }
 }
 
-template <bool RELANE = false>
 __device__ __forceinline__ void hetrd(Work& w) {
     // Column j belongs to wave j % NW. Step k: every wave forms partial sums
     // of p = M v over its own columns (barrier), combines them into w in its
     // own lanes, applies the rank-2 update to its own columns, and the owner of
     // column k+1 then builds the next Householder vector (barrier).
-    const int lane = call_lane<RELANE>(), wave = wave_id();
-    if (wave == 0) house<RELANE>(w, 0);
+    const int lane = lane_id(), wave = wave_id();
+    if (wave == 0) house(w, 0);
     __syncthreads();
     for (int k = 0; k < N - 1; ++k) {
         const int m0 = k + 1;
@@ -554,7 +540,7 @@ __device__ __forceinline__ void hetrd(Work& w) {
                     w.B[j * LDA + lane] = a[c] - cmulc(vl, wj[c]) - cmulc(wi, vj[c]);
             }
         }
-        if (k + 1 < N - 1 && wave == (k + 1) % NW) house<RELANE>(w, k + 1);
+        if (k + 1 < N - 1 && wave == (k + 1) % NW) house(w, k + 1);
         __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -654,7 +640,7 @@ __device__ __forceinline__ int ascending_rank(real dreg) {
 // ascending eigenvalues to W_out and leaves Z (columns in ascending order) in
 // `z` (lane r = row r) plus its rank permutation in `rank`.
 __device__ __forceinline__ int tridiag_eig_wave(const real* dsrc, const real* esrc, real* __restrict__ W_out,
-                                real (&z)[N], int& rank, int n_out = N) {
+                                real (&z)[N], int& rank) {
     const int lane = lane_id();
     real dreg = (lane < N) ? dsrc[lane] : 0.0;
     real ereg = (lane < N - 1) ? esrc[lane] : 0.0;
@@ -662,7 +648,7 @@ __device__ __forceinline__ int tridiag_eig_wave(const real* dsrc, const real* es
     for (int j = 0; j < N; ++j) z[j] = (lane == j) ? 1.0 : 0.0;
     const int info = ql_wave(dreg, ereg, z);
     rank = ascending_rank(lane < N ? dreg : __builtin_inf());
-    if (lane < N && rank < n_out) W_out[rank] = dreg;
+    if (lane < N) W_out[rank] = dreg;
     return info;
 }
 
@@ -991,19 +977,10 @@ static_assert(ZSLICES >= 1, "work area must hold one slice");
 // Back half of every wave's matrix: Z -> V = Q Z (-> U = L^-H V when `with_l`)
 // -> Ub. Waves whose slices do not fit at once take turns; every wave passes
 // the same barriers.
-__device__ __forceinline__ void store_columns_n(cx* __restrict__ Ub, const cx (&b)[N], int n) {
-    const int lane = lane_id();
-    if (lane < n) {
-#pragma unroll
-        for (int i = 0; i < N; ++i)
-            if (i < n) Ub[lane * n + i] = b[i];
-    }
-}
 
 __device__ __forceinline__ void back_half_waves(Work& w, bool active, const real (&z)[N], int rank,
                                                 cx* __restrict__ Ub, const cx* __restrict__ Lb,
-                                                bool with_l, cx* __restrict__ Uout = nullptr,
-                                                int n_out = N) {
+                                                bool with_l) {
     const int wave = wave_id();
     char* base = reinterpret_cast<char*>(&w);
     for (int r = 0; r < (NW + ZSLICES - 1) / ZSLICES; ++r) {
@@ -1024,8 +1001,7 @@ __device__ __forceinline__ void back_half_waves(Work& w, bool active, const real
                 wave_lds_sync();
                 backtrsm_wave(sl, b);
             }
-            if (Uout) store_columns_n(Uout, b, n_out);
-            else store_columns(Ub, b);
+            store_columns(Ub, b);
         }
         __syncthreads();
     }
@@ -1224,112 +1200,6 @@ geneig_fused(const cx* __restrict__ H, const cx* __restrict__ S, real* __restric
 }
 
 
-// ---------------------------------------------------------------------------
-// Uneven batches. geneig_varlen is compiled for a bucket order N and solves
-// every matrix of order n <= N in its bucket: the pencil is embedded as
-// (H (+) pad I, S (+) I) with pad far above any eigenvalue of the pencil, so
-// the padded problem decouples exactly (the reflectors and rotations never
-// mix the two blocks) and its n smallest eigenpairs are the original ones.
-// Workgroup g takes entries [g NW, g NW + NW) of the bucket's work list (ids
-// sorted by the host); matrices are packed back to back (matrix id at element
-// offset moff[id], eigenvalues at woff[id]); the factor stash lives in a
-// per-workgroup scratch slot. (A persistent loop over a work counter was
-// measured to spill: the compiler hoists every loop-invariant per-lane
-// address across the whole body.)
-// ---------------------------------------------------------------------------
-namespace gk {
-constexpr double VARLEN_PAD = 0x1p600;
-
-__device__ __forceinline__ void load_pair_padded(Work& w, const cx* __restrict__ Hb,
-                                                 const cx* __restrict__ Sb, int n) {
-    for (int e = threadIdx.x; e < N * N; e += NT) {
-        const int i = e % N, j = e / N;
-        const bool in = i < n && j < n;
-        w.A[j * LDA + i] = in ? Sb[j * n + i] : cm(i == j ? 1.0 : 0.0, 0.0);
-        w.B[j * LDA + i] = in ? Hb[j * n + i] : cm(i == j ? VARLEN_PAD : 0.0, 0.0);
-    }
-    __syncthreads();
-}
-
-__device__ __forceinline__ void fail_store_n(int code, int n, real* __restrict__ Wb,
-                                             cx* __restrict__ Ub, int* __restrict__ info_b) {
-    for (int e = threadIdx.x; e < n * n; e += NT) Ub[e] = cm(0.0, 0.0);
-    for (int e = threadIdx.x; e < n; e += NT) Wb[e] = __builtin_nan("");
-    if (threadIdx.x == 0) *info_b = code;
-}
-}  // namespace gk
-
-extern "C" __global__ void __launch_bounds__(GENEIG_NT)
-geneig_varlen(const cx* __restrict__ H, const cx* __restrict__ S, real* __restrict__ W,
-              cx* __restrict__ U, int* __restrict__ info, const int* __restrict__ ids,
-              const long long* __restrict__ moff, const long long* __restrict__ woff,
-              const int* __restrict__ nn, int count, cx* __restrict__ scratch) {
-    using namespace gk;
-    __shared__ Work w;
-    __shared__ Staged st;
-    __shared__ int qid[NW];
-    __shared__ int qn[NW];
-    const int lane = lane_id(), wave = wave_id();
-    cx* slot0 = scratch + (size_t)blockIdx.x * NW * N * N;
-    {
-        const int base = blockIdx.x * NW;
-        if (base >= count) return;
-        if (threadIdx.x < NW) {
-            const int idx = base + threadIdx.x;
-            const int id = idx < count ? ids[idx] : -1;
-            qid[threadIdx.x] = id;
-            qn[threadIdx.x] = id >= 0 ? nn[id] : 0;
-        }
-        __syncthreads();
-        for (int q = 0; q < NW; ++q) {
-            const int id = qid[q];
-            if (id < 0) {
-                if (threadIdx.x == 0) st.status[q] = 1;
-                continue;
-            }
-            const int n = qn[q];
-            const long long mo = moff[id];
-            load_pair_padded(w, H + mo, S + mo, n);
-            const int pinfo = potrf<true>(w);
-            if (pinfo) {
-                fail_store_n(n + pinfo, n, W + woff[id], U + mo, info + id);
-                if (threadIdx.x == 0) st.status[q] = 1;
-                __syncthreads();
-                continue;
-            }
-            trsm_left<true>(w);
-            __syncthreads();
-            trsm_right<true>(w);
-            __syncthreads();
-            hetrd<true>(w);
-            stash_factors(w, slot0 + (size_t)q * N * N);
-            if (threadIdx.x < N) {
-                st.d[q][threadIdx.x] = w.d[threadIdx.x];
-                st.e[q][threadIdx.x] = w.e[threadIdx.x];
-            }
-            if (threadIdx.x == 0) st.status[q] = 0;
-            __syncthreads();
-        }
-        const int id = qid[wave];
-        const int n = qn[wave];
-        const bool active = id >= 0 && st.status[wave] == 0;
-        real z[N];
-        int rank = 0;
-        if (active) {
-            const int qi = tridiag_eig_wave(st.d[wave], st.e[wave], W + woff[id], z, rank, n);
-            if (lane == 0) st.qinfo[wave] = qi;
-        }
-        __syncthreads();
-        {
-            // Back half into the scratch slot (N x N), then copy the leading
-            // n x n block out.
-            cx* slot = slot0 + (size_t)wave * N * N;
-            back_half_waves(w, active, z, rank, slot, slot, true, U + (active ? moff[id] : 0), n);
-            if (active && lane == 0) info[id] = st.qinfo[wave];
-        }
-        __syncthreads();
-    }
-}
 
 #endif  // !GENEIG_FP32
 

@@ -1748,20 +1748,24 @@ geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, int* __restrict__ inf
 }
 #endif  // !GENEIG_FP32
 
+#endif  // GENEIG_LG_PIPELINE
+
 #ifndef GENEIG_FP32
-// Uneven batches, buckets above 64: the bucket's matrices (work list ids,
-// packed back to back at element offsets moff, orders nn) are embedded as
-// (H (+) pad I, S (+) I) into a contiguous batch of order N for the staged
-// pipeline; pad is far above any eigenvalue of the pencils, so the padded
-// problem decouples exactly and its n smallest eigenpairs are the original
-// ones. The scatter returns the leading n eigenvalues, the leading n x n
-// block of U and zhegv's info for order n (a failing leading minor k lies in
-// the original block: N + k becomes n + k).
+// Uneven batches (every bucket, in every FP64 module): the bucket's matrices
+// (work list ids, packed back to back at element offsets moff, orders nn) are
+// embedded as (H (+) pad I, S (+) I) into a contiguous batch of order N for
+// the bucket's FP64 split solver; pad is far above any eigenvalue of the
+// pencils, so the padded problem decouples exactly (the Cholesky, the
+// reflectors and the QL rotations never mix the two blocks) and its n
+// smallest eigenpairs are the original ones. The scatter returns the leading
+// n eigenvalues, the leading n x n block of U and zhegv's info for order n (a
+// failing leading minor k lies in the original block: N + k becomes n + k).
+// One workgroup of 256 threads per matrix.
 namespace gk {
 constexpr double LG_PAD = 0x1p600;
 }
 
-extern "C" __global__ void __launch_bounds__(gk::LNT)
+extern "C" __global__ void __launch_bounds__(256)
 geneig_lg_gather(const cx* __restrict__ H, const cx* __restrict__ S, const int* __restrict__ ids,
                  const long long* __restrict__ moff, const int* __restrict__ nn, int count, cx* __restrict__ Hp,
                  cx* __restrict__ Sp) {
@@ -1773,7 +1777,7 @@ geneig_lg_gather(const cx* __restrict__ H, const cx* __restrict__ S, const int* 
     const long long mo = moff[id];
     cx* hb = Hp + (size_t)b * NN;
     cx* sb = Sp + (size_t)b * NN;
-    for (int e = threadIdx.x; e < (int)NN; e += LNT) {
+    for (int e = threadIdx.x; e < (int)NN; e += (int)blockDim.x) {
         const int i = e % N, j = e / N;
         const bool in = i < n && j < n;
         sb[e] = in ? S[mo + (long long)j * n + i] : cm(i == j ? 1.0 : 0.0, 0.0);
@@ -1781,7 +1785,7 @@ geneig_lg_gather(const cx* __restrict__ H, const cx* __restrict__ S, const int* 
     }
 }
 
-extern "C" __global__ void __launch_bounds__(gk::LNT)
+extern "C" __global__ void __launch_bounds__(256)
 geneig_lg_scatter(const real* __restrict__ Wp, const cx* __restrict__ Up, const int* __restrict__ Ip,
                   const int* __restrict__ ids, const long long* __restrict__ moff,
                   const long long* __restrict__ woff, const int* __restrict__ nn, int count, real* __restrict__ W,
@@ -1793,12 +1797,11 @@ geneig_lg_scatter(const real* __restrict__ Wp, const cx* __restrict__ Up, const 
     const int n = nn[id];
     const long long mo = moff[id], wo = woff[id];
     const cx* ub = Up + (size_t)b * NN;
-    for (int e = threadIdx.x; e < n * n; e += LNT) U[mo + e] = ub[(size_t)(e / n) * N + e % n];
-    for (int e = threadIdx.x; e < n; e += LNT) W[wo + e] = Wp[(size_t)b * N + e];
+    for (int e = threadIdx.x; e < n * n; e += (int)blockDim.x) U[mo + e] = ub[(size_t)(e / n) * N + e % n];
+    for (int e = threadIdx.x; e < n; e += (int)blockDim.x) W[wo + e] = Wp[(size_t)b * N + e];
     if (threadIdx.x == 0) {
         const int ip = Ip[b];
         info[id] = (ip > N) ? n + (ip - N) : ip;
     }
 }
 #endif  // !GENEIG_FP32
-#endif  // GENEIG_LG_PIPELINE

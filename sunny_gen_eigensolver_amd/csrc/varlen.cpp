@@ -41,32 +41,19 @@ VarlenSolver::VarlenSolver(int max_n, int device_id, const std::vector<int>& buc
     if (device_id < 0 || device_id >= count)
         throw std::invalid_argument("device_id " + std::to_string(device_id) + " out of range");
     SG_HIP(hipSetDevice(device_id));
-    const std::string arch = device_arch(device_id);
     for (int o : orders_) {
         Bucket b;
         b.order = o;
-        b.block_dim = Solver::suggested_block_dim(o);   // same warm code objects
-        std::unique_ptr<Solver> tuned;
-        if (tuning == Tuning::Autotune && o <= Solver::kMaxLdsOrder) {
-            // The fused solver's sweep for this order: workgroup size, or the
-            // HBM-staged pipeline (then this bucket takes the staged route).
-            tuned = std::make_unique<Solver>(o, device_id, 0, Tuning::Autotune, cache, Mode::Unified);
-            b.block_dim = tuned->block_dim_x();
-        }
-        b.nw = b.block_dim / 64;
-        if (o > Solver::kMaxLdsOrder || (tuned && tuned->staged())) {
-            b.nw = 1;
-            b.large = (tuned && tuned->staged())
-                          ? std::move(tuned)
-                          : std::make_unique<Solver>(o, device_id, 0, Tuning::Suggested, cache, Mode::Unified);
-            b.gather = b.large->function("geneig_lg_gather");
-            b.scatter = b.large->function("geneig_lg_scatter");
-        } else {
-            CacheLayer layer;
-            double ms = 0.0;
-            b.module = acquire_module(ModuleKey{o, b.block_dim, arch}, cache, &layer, &ms);
-            b.fn = b.module->get("geneig_varlen");
-        }
+        // The FP64 split mode at the bucket's order: with Tuning::Autotune its
+        // own timed sweep (workgroup size; LDS-resident or staged from order 48).
+        b.solver = std::make_unique<Solver>(o, device_id, 0, tuning, cache, Mode::Hybrid);
+        b.nw = b.solver->staged() ? 1 : b.solver->block_dim_x() / 64;
+        b.gather = b.solver->function("geneig_lg_gather");
+        b.scatter = b.solver->function("geneig_lg_scatter");
+        const size_t per = 3 * (size_t)o * o * 16;
+        b.round = b.solver->staged()
+                      ? kLargeRound
+                      : (int)std::min<size_t>(kMaxRound, std::max<size_t>(kLargeRound, kRoundBytes / per));
         buckets_.push_back(std::move(b));
     }
     SG_HIP(hipEventCreateWithFlags(&staged_, hipEventDisableTiming));
@@ -89,7 +76,6 @@ VarlenSolver::~VarlenSolver() {
     if (done_) (void)hipEventDestroy(done_);
     if (host_desc_) (void)hipHostFree(host_desc_);
     if (dev_desc_) (void)hipFree(dev_desc_);
-    if (scratch_) (void)hipFree(scratch_);
     if (pad_) (void)hipFree(pad_);
     buckets_.clear();
     (void)hipSetDevice(cur);
@@ -171,25 +157,26 @@ void VarlenSolver::launch(const void* H, const void* S, double* W, void* U, int*
     }
 
     ensure_capacity_(count);
-    size_t scratch_need = 0, pad_need = 0;
+    size_t pad_need = 0;
     bool grew = false;
-    if (large_reserved_.size() != (size_t)nb) large_reserved_.assign(nb, 0);
+    if (reserved_.size() != (size_t)nb) reserved_.assign(nb, 0);
     for (int b = 0; b < nb; ++b) {
         if (!per_bucket[b]) continue;
         const Bucket& B = buckets_[b];
         const size_t no = (size_t)B.order;
-        if (B.large) {
-            const size_t r = std::min<size_t>(per_bucket[b], kLargeRound);
-            pad_need = std::max(pad_need, 3 * al256(r * no * no * 16) + al256(r * no * 8) + al256(r * 4));
-            if (r > large_reserved_[b]) {
-                if (large_reserved_[b]) grew = true;      // its workspace may move
-                B.large->reserve(r);
-                large_reserved_[b] = r;
-            }
-            continue;
+        // Room for at least kMinReserve matrices (at most a round), grown by
+        // doubling: a later call with a few more pencils in a bucket then
+        // seldom moves a workspace that captured graphs hold.
+        const size_t round = (size_t)B.round;
+        size_t r = std::min<size_t>(per_bucket[b], round);
+        if (r > reserved_[b]) r = std::min(round, std::max({r, 2 * reserved_[b], (size_t)kMinReserve}));
+        else r = reserved_[b];
+        pad_need = std::max(pad_need, 3 * al256(r * no * no * 16) + al256(r * no * 8) + al256(r * 4));
+        if (r > reserved_[b]) {
+            if (reserved_[b]) grew = true;      // its workspace may move
+            B.solver->reserve(r);
+            reserved_[b] = r;
         }
-        const size_t wgs = std::min<size_t>((per_bucket[b] + B.nw - 1) / B.nw, kMaxGrid);
-        scratch_need = std::max(scratch_need, wgs * B.nw * no * no * 16);
     }
     if (pad_need > pad_bytes_) {
         SG_HIP(hipEventSynchronize(done_));
@@ -198,15 +185,6 @@ void VarlenSolver::launch(const void* H, const void* S, double* W, void* U, int*
         pad_bytes_ = 0;
         SG_HIP(hipMalloc(&pad_, pad_need));
         pad_bytes_ = pad_need;
-        grew = true;
-    }
-    if (scratch_need > scratch_bytes_) {
-        SG_HIP(hipEventSynchronize(done_));
-        if (scratch_) SG_HIP(hipFree(scratch_));
-        scratch_ = nullptr;
-        scratch_bytes_ = 0;
-        SG_HIP(hipMalloc(&scratch_, scratch_need));
-        scratch_bytes_ = scratch_need;
         grew = true;
     }
     if (grew) clear_graphs_();               // captured grids hold the old areas
@@ -290,38 +268,27 @@ void VarlenSolver::enqueue_(const void* H, const void* S, double* W, void* U, in
     const int64_t* d_woff = d_moff + count;
     const int* d_ids = reinterpret_cast<const int*>(d_woff + count);
     const int* d_nn = d_ids + count;
-    void* scratch = scratch_;
-    // Largest orders first: their grids are the longest, the small ones fill
-    // in behind them.
+    // Largest orders first, one round after another on the caller's stream;
+    // every bucket has its own solver (and so its own workspace), and the
+    // rounds share the padded staging area in stream order.
     for (int b = nb - 1; b >= 0; --b) {
         const Bucket& B = buckets_[b];
-        if (B.large) {
-            const size_t no = (size_t)B.order;
-            for (int off = 0; off < per_bucket[b]; off += kLargeRound) {
-                int cnt = std::min(per_bucket[b] - off, kLargeRound);
-                const size_t r = (size_t)cnt;
-                char* base = static_cast<char*>(pad_);
-                void* Hp = base;
-                void* Sp = base + al256(r * no * no * 16);
-                void* Up = base + 2 * al256(r * no * no * 16);
-                double* Wp = reinterpret_cast<double*>(base + 3 * al256(r * no * no * 16));
-                int* Ip = reinterpret_cast<int*>(base + 3 * al256(r * no * no * 16) + al256(r * no * 8));
-                const int* list = d_ids + start[b] + off;
-                void* ga[] = {&H, &S, &list, &d_moff, &d_nn, &cnt, &Hp, &Sp};
-                SG_HIP(hipModuleLaunchKernel(B.gather, (unsigned)cnt, 1, 1, 256, 1, 1, 0, stream, ga, nullptr));
-                B.large->launch_unordered(Hp, Sp, Wp, Up, Ip, cnt, stream);
-                void* sa[] = {&Wp, &Up, &Ip, &list, &d_moff, &d_woff, &d_nn, &cnt, &W, &U, &info};
-                SG_HIP(hipModuleLaunchKernel(B.scatter, (unsigned)cnt, 1, 1, 256, 1, 1, 0, stream, sa, nullptr));
-            }
-            continue;
-        }
-        for (int off = 0; off < per_bucket[b]; off += kMaxGrid * B.nw) {
-            int cnt = std::min(per_bucket[b] - off, kMaxGrid * B.nw);
+        const size_t no = (size_t)B.order;
+        for (int off = 0; off < per_bucket[b]; off += B.round) {
+            int cnt = std::min(per_bucket[b] - off, B.round);
+            const size_t r = (size_t)cnt;
+            char* base = static_cast<char*>(pad_);
+            void* Hp = base;
+            void* Sp = base + al256(r * no * no * 16);
+            void* Up = base + 2 * al256(r * no * no * 16);
+            double* Wp = reinterpret_cast<double*>(base + 3 * al256(r * no * no * 16));
+            int* Ip = reinterpret_cast<int*>(base + 3 * al256(r * no * no * 16) + al256(r * no * 8));
             const int* list = d_ids + start[b] + off;
-            void* args[] = {&H, &S, &W, &U, &info, &list, &d_moff, &d_woff, &d_nn, &cnt, &scratch};
-            const unsigned grid = (unsigned)((cnt + B.nw - 1) / B.nw);
-            SG_HIP(hipModuleLaunchKernel(B.fn, grid, 1, 1, (unsigned)B.block_dim, 1, 1, 0, stream,
-                                         args, nullptr));
+            void* ga[] = {&H, &S, &list, &d_moff, &d_nn, &cnt, &Hp, &Sp};
+            SG_HIP(hipModuleLaunchKernel(B.gather, (unsigned)cnt, 1, 1, 256, 1, 1, 0, stream, ga, nullptr));
+            B.solver->launch_unordered(Hp, Sp, Wp, Up, Ip, cnt, stream);
+            void* sa[] = {&Wp, &Up, &Ip, &list, &d_moff, &d_woff, &d_nn, &cnt, &W, &U, &info};
+            SG_HIP(hipModuleLaunchKernel(B.scatter, (unsigned)cnt, 1, 1, 256, 1, 1, 0, stream, sa, nullptr));
         }
     }
 }

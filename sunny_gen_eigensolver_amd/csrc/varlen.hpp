@@ -1,10 +1,10 @@
 // Uneven batches: matrices of different orders solved by one call. Every
 // matrix goes to the smallest bucket order >= its own and is embedded in the
 // bucket's order as (H (+) pad I, S (+) I). The host sorts the work list by
-// bucket, so each bucket reads a contiguous run of it. Buckets up to 64 are
-// one grid of the LDS-resident geneig_varlen kernel each; a bucket above 64
-// gathers its pencils into a padded contiguous batch, runs the staged FP64
-// kernels on it and scatters the results back (in rounds of kLargeRound).
+// bucket, so each bucket reads a contiguous run of it. Each bucket gathers its
+// pencils into a padded contiguous batch, runs its FP64 split-mode solver on
+// it (the matrix-core pipeline up to order 64, the staged kernels above) and
+// scatters the results back, in rounds.
 //
 // No counterpart in the reference (its solvers take one order per handle,
 // src/unified/unified_solver.hpp:81-217).
@@ -25,9 +25,9 @@ class VarlenSolver {
 public:
     // Buckets: ascending orders in [1, 192]; empty = multiples of 4 up to 64,
     // then of 8 up to 128, then of 16, ending with max_n itself. With
-    // Tuning::Autotune every bucket up to 64 takes the workgroup size (and so
-    // the matrices per workgroup) that the fused solver's timed sweep picks for
-    // its order (persisted per order and architecture); Suggested takes the
+    // Tuning::Autotune every bucket takes what the split-mode solver's timed
+    // sweep picks for its order (workgroup size, and from order 48 the staged
+    // pipeline; persisted per order and architecture); Suggested takes the
     // fixed rule. Buckets above 64 run the staged kernels (one size).
     VarlenSolver(int max_n, int device_id, const std::vector<int>& buckets, CacheMode cache,
                  Tuning tuning = Tuning::Suggested);
@@ -40,8 +40,8 @@ public:
     // its status at info[k]. Asynchronous on `stream`; `n` is read before
     // return. Calls are ordered after the handle's previous call whatever the
     // stream. The first call with given buffers and orders captures every
-    // grid of the call (the LDS-resident buckets, and the staged buckets'
-    // gather, solve and scatter) into one hipGraph with its own copy of the
+    // grid of the call (each bucket's gather, solve and scatter rounds) into
+    // one hipGraph with its own copy of the
     // work-list descriptors; a later call with the same buffers and orders
     // (compared by count and a 64-bit hash of the orders) replays it: one
     // graph launch, no host sort, no upload. SUNNY_GENEIG_GRAPH=0 at creation
@@ -54,24 +54,27 @@ public:
 
     int max_n() const { return max_n_; }
     const std::vector<int>& buckets() const { return orders_; }
-    // Matrices per workgroup of each bucket's grid.
+    // Matrices per workgroup of each bucket's QL grids (one matrix per wave;
+    // 1 for a bucket on the staged kernels).
     int bucket_waves(int i) const { return buckets_[i].nw; }
 
-    // Workgroups per grid launch; longer runs are split into several grids
-    // that reuse one scratch area.
-    static constexpr int kMaxGrid = 4096;
-    // Matrices per round of a bucket above 64.
+    // Matrices per round of a bucket on the staged kernels.
     static constexpr int kLargeRound = 4096;
+    // Rounds of the other buckets: as many matrices as keep the padded H, S
+    // and U of a round within kRoundBytes, at least kLargeRound and at most
+    // kMaxRound.
+    static constexpr size_t kRoundBytes = size_t(4) << 30;
+    static constexpr int kMaxRound = 65536;
+    // Matrices a bucket's workspace holds at least once the bucket is used.
+    static constexpr int kMinReserve = 256;
 
 private:
     struct Bucket {
         int order = 0;
-        int block_dim = 0;
         int nw = 0;
-        std::shared_ptr<LoadedModule> module;
-        hipFunction_t fn = nullptr;
-        // Buckets above 64: the staged FP64 solver and its gather / scatter.
-        std::unique_ptr<Solver> large;
+        int round = 0;                      // matrices per round
+        // The bucket's FP64 split-mode solver and its gather / scatter.
+        std::unique_ptr<Solver> solver;
         hipFunction_t gather = nullptr, scatter = nullptr;
     };
     void ensure_capacity_(int count);
@@ -95,7 +98,7 @@ private:
     hipStream_t cap_stream_ = nullptr;
     std::vector<GraphEntry> graph_cache_;
     long long graph_captures_ = 0, graph_replays_ = 0;
-    std::vector<size_t> large_reserved_;   // matrices each staged bucket's solver holds
+    std::vector<size_t> reserved_;         // matrices each bucket's solver holds
     void clear_graphs_();
 
     int max_n_ = 0;
@@ -110,10 +113,7 @@ private:
     hipEvent_t staged_ = nullptr;   // last upload from host_desc_ has finished
     bool staged_pending_ = false;
     hipEvent_t done_ = nullptr;     // last launch's grids have finished
-    // Per-workgroup factor stash, sized for the largest bucket's grid.
-    void* scratch_ = nullptr;
-    size_t scratch_bytes_ = 0;
-    // Padded batch of a large bucket's round: H, S, U, W, info.
+    // Padded batch of a bucket's round: H, S, U, W, info.
     void* pad_ = nullptr;
     size_t pad_bytes_ = 0;
 };

@@ -1,15 +1,15 @@
 """Code-generation guard for the gfx950 kernels (CPU only: hipcc emits the
 assembly, nothing runs).
 
-The uneven-batch kernel solves several matrices per workgroup in a loop. At
-some orders the compiler once kept per-lane LDS addresses live across that
-loop and saved them into accumulation registers (v_accvgpr_write) in a basic
-block whose exec mask was still narrowed by the `if (lane == 0)` region that
-had just ended; the matching restore then ran with every lane active, so the
-second matrix of a workgroup was factored from wrong addresses (zhegv INFO
-n + 2 on an HPD S). The kernels now prevent that live range, and this test
-checks every order the package ships warm for the pattern: a save that is
-still unrestored when `s_or_b64 exec, exec, ...` widens the mask."""
+A kernel that solves several matrices per workgroup in a loop (the fused
+kernel does) once had, at some orders, per-lane LDS addresses kept live
+across that loop and saved into accumulation registers (v_accvgpr_write) in a
+basic block whose exec mask was still narrowed by the `if (lane == 0)` region
+that had just ended; the matching restore then ran with every lane active, so
+the second matrix of a workgroup was factored from wrong addresses (zhegv INFO
+n + 2 on an HPD S). This test checks every order the package ships warm for
+the pattern: a save that is still unrestored when `s_or_b64 exec, exec, ...`
+widens the mask."""
 import os
 import re
 import subprocess
@@ -19,7 +19,7 @@ import pytest
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(REPO, "sunny_gen_eigensolver_amd", "csrc", "kernels", "geneig_kernels.hip")
-KERNELS = ("geneig_fused", "geneig_varlen", "geneig_mx_front", "geneig_split_ql", "geneig_ql_full", "geneig_mx_back",
+KERNELS = ("geneig_fused", "geneig_mx_front", "geneig_split_ql", "geneig_ql_full", "geneig_mx_back",
            "geneig_mx_backtri")
 ORDERS = sorted(set(range(4, 65, 4)) | {27, 54})
 

@@ -3,8 +3,10 @@ different orders in one call, each checked against FP64 LAPACK zhegv on the
 same matrix with the fixed-order bars (eig_rel and phase < 1e-10, residual
 and S-orthogonality <= 50 N eps), plus zhegv's INFO codes for a non-HPD S in
 the mix, and the alone-versus-mixed property: a matrix's result does not
-depend on what else is in the call (bit-identical, since every matrix is
-solved by the same bucket kernel whatever its neighbours)."""
+depend on what else is in the call (bit-identical: every matrix is solved by
+its own workgroups of the bucket's kernels whatever its neighbours, and the
+calls here stay below the batch size at which the QL switches to its
+rotation log)."""
 import numpy as np
 import pytest
 
@@ -122,20 +124,20 @@ def test_non_hpd_in_mix(vs64):
                   [U[k] for k in good], info[good], [orders[k] for k in good], "varlen beside non-HPD")
 
 
-def test_multiple_grids_per_bucket():
-    """More matrices than one grid of a bucket holds (4096 workgroups x 1
-    matrix at order 4): the run is split over several grids sharing one
-    scratch area."""
+def test_multiple_rounds_per_bucket():
+    """More matrices than one round of a bucket holds (65536 at order 4): the
+    run is split over several gather / solve / scatter rounds sharing one
+    padded staging area."""
     import sunny_gen_eigensolver_amd as sg
     n = 4
-    count = 4096 + 37
+    count = 65536 + 37
     H, S = O.gen_reference(n, count, 31337, "random")
     orders = [3 if k % 5 == 0 else 4 for k in range(count)]
     Hs = [H[k][:o, :o].copy() if o < n else H[k] for k, o in enumerate(orders)]
     Ss = [S[k][:o, :o].copy() if o < n else S[k] for k, o in enumerate(orders)]
     with sg.VarlenSolver(8, buckets=[4, 8]) as vs:
         W, U, info = _solve(vs, Hs, Ss, orders)
-    _check_groups(Hs, Ss, W, U, info, orders, "varlen multi-grid")
+    _check_groups(Hs, Ss, W, U, info, orders, "varlen multi-round")
 
 
 def test_rejects_order_above_max():
@@ -151,9 +153,10 @@ def test_rejects_order_above_max():
 
 @pytest.mark.parametrize("bucket", list(range(4, 65, 4)))
 def test_every_default_bucket_every_slot(vs64, bucket):
-    """Each default bucket with its workgroups full: every wave slot holds a
-    matrix of the bucket's own order or just below it (padded), so the second
-    and later matrices of a workgroup are covered at every order."""
+    """Each default bucket with the workgroups of its QL grids full (one
+    matrix per wave): every wave slot holds a matrix of the bucket's own order
+    or just below it (padded), so the second and later matrices of a
+    workgroup are covered at every order."""
     import sunny_gen_eigensolver_amd as sg
     nw = (64 if bucket <= 16 else 128 if bucket <= 32 else 256) // 64
     lo = max(1, bucket - 3)
@@ -236,17 +239,18 @@ def test_large_bucket_non_hpd(vs128):
 
 
 def test_autotuned_buckets_vs_lapack():
-    """tuning="autotune": every bucket up to 64 takes the workgroup size that
-    the fused solver's timed sweep picks for its order (the same choice as
-    Solver(n, tuning="autotune"), read back from the persisted table), and
-    the results stay within the parity bars. A bucket whose sweep picked the
-    HBM-staged kernels (orders from 48) runs one matrix per workgroup."""
+    """tuning="autotune": every bucket takes what the split-mode solver's
+    timed sweep picks for its order (the same choice as
+    Solver(n, tuning="autotune", kernel="hybrid"), read back from the
+    persisted table), and the results stay within the parity bars. A bucket
+    whose sweep picked the HBM-staged kernels (orders from 48) reports one
+    matrix per workgroup."""
     import sunny_gen_eigensolver_amd as sg
     buckets = [16, 24, 32, 48]
     with sg.VarlenSolver(48, buckets=buckets, tuning="autotune") as vs:
         assert vs.buckets == buckets
         for o, waves in zip(buckets, vs.bucket_waves):
-            s = sg.Solver(o, tuning="autotune")
+            s = sg.Solver(o, tuning="autotune", kernel="hybrid")
             want = 1 if s.staged else s.block_dim_x // 64
             assert waves == want, (o, waves, s.block_dim_x, s.staged)
             s.close()
