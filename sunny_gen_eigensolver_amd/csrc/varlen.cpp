@@ -58,6 +58,11 @@ VarlenSolver::VarlenSolver(int max_n, int device_id, const std::vector<int>& buc
     }
     SG_HIP(hipEventCreateWithFlags(&staged_, hipEventDisableTiming));
     SG_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
+    SG_HIP(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+    for (int k = 0; k < kBranches; ++k) {
+        SG_HIP(hipStreamCreateWithFlags(&branch_[k], hipStreamNonBlocking));
+        SG_HIP(hipEventCreateWithFlags(&join_[k], hipEventDisableTiming));
+    }
     if (const char* v = std::getenv("SUNNY_GENEIG_GRAPH")) graphs_ = std::atoi(v) != 0;
 }
 
@@ -72,6 +77,12 @@ VarlenSolver::~VarlenSolver() {
     }
     graph_cache_.clear();
     if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
+    for (int k = 0; k < kBranches; ++k) {
+        if (branch_[k]) (void)hipStreamSynchronize(branch_[k]);
+        if (branch_[k]) (void)hipStreamDestroy(branch_[k]);
+        if (join_[k]) (void)hipEventDestroy(join_[k]);
+    }
+    if (fork_) (void)hipEventDestroy(fork_);
     if (staged_) (void)hipEventDestroy(staged_);
     if (done_) (void)hipEventDestroy(done_);
     if (host_desc_) (void)hipHostFree(host_desc_);
@@ -183,7 +194,7 @@ void VarlenSolver::launch(const void* H, const void* S, double* W, void* U, int*
         if (pad_) SG_HIP(hipFree(pad_));
         pad_ = nullptr;
         pad_bytes_ = 0;
-        SG_HIP(hipMalloc(&pad_, pad_need));
+        SG_HIP(hipMalloc(&pad_, pad_need * kBranches));
         pad_bytes_ = pad_need;
         grew = true;
     }
@@ -268,16 +279,24 @@ void VarlenSolver::enqueue_(const void* H, const void* S, double* W, void* U, in
     const int64_t* d_woff = d_moff + count;
     const int* d_ids = reinterpret_cast<const int*>(d_woff + count);
     const int* d_nn = d_ids + count;
-    // Largest orders first, one round after another on the caller's stream;
-    // every bucket has its own solver (and so its own workspace), and the
-    // rounds share the padded staging area in stream order.
+    // Buckets, largest orders first, each to the branch with the least work
+    // so far (pencils x order^3); a branch runs its buckets' rounds one after
+    // another through its own padded staging area. Every bucket has its own
+    // solver, and so its own workspace.
+    SG_HIP(hipEventRecord(fork_, stream));
+    for (int k = 0; k < kBranches; ++k) SG_HIP(hipStreamWaitEvent(branch_[k], fork_, 0));
+    double load[kBranches] = {};
     for (int b = nb - 1; b >= 0; --b) {
+        if (!per_bucket[b]) continue;
         const Bucket& B = buckets_[b];
         const size_t no = (size_t)B.order;
+        const int k = (int)(std::min_element(load, load + kBranches) - load);
+        load[k] += (double)per_bucket[b] * (double)(no * no * no);
+        hipStream_t bs = branch_[k];
         for (int off = 0; off < per_bucket[b]; off += B.round) {
             int cnt = std::min(per_bucket[b] - off, B.round);
             const size_t r = (size_t)cnt;
-            char* base = static_cast<char*>(pad_);
+            char* base = static_cast<char*>(pad_) + (size_t)k * pad_bytes_;
             void* Hp = base;
             void* Sp = base + al256(r * no * no * 16);
             void* Up = base + 2 * al256(r * no * no * 16);
@@ -285,11 +304,15 @@ void VarlenSolver::enqueue_(const void* H, const void* S, double* W, void* U, in
             int* Ip = reinterpret_cast<int*>(base + 3 * al256(r * no * no * 16) + al256(r * no * 8));
             const int* list = d_ids + start[b] + off;
             void* ga[] = {&H, &S, &list, &d_moff, &d_nn, &cnt, &Hp, &Sp};
-            SG_HIP(hipModuleLaunchKernel(B.gather, (unsigned)cnt, 1, 1, 256, 1, 1, 0, stream, ga, nullptr));
-            B.solver->launch_unordered(Hp, Sp, Wp, Up, Ip, cnt, stream);
+            SG_HIP(hipModuleLaunchKernel(B.gather, (unsigned)cnt, 1, 1, 256, 1, 1, 0, bs, ga, nullptr));
+            B.solver->launch_unordered(Hp, Sp, Wp, Up, Ip, cnt, bs);
             void* sa[] = {&Wp, &Up, &Ip, &list, &d_moff, &d_woff, &d_nn, &cnt, &W, &U, &info};
-            SG_HIP(hipModuleLaunchKernel(B.scatter, (unsigned)cnt, 1, 1, 256, 1, 1, 0, stream, sa, nullptr));
+            SG_HIP(hipModuleLaunchKernel(B.scatter, (unsigned)cnt, 1, 1, 256, 1, 1, 0, bs, sa, nullptr));
         }
+    }
+    for (int k = 0; k < kBranches; ++k) {
+        SG_HIP(hipEventRecord(join_[k], branch_[k]));
+        SG_HIP(hipStreamWaitEvent(stream, join_[k], 0));
     }
 }
 

@@ -63,7 +63,11 @@ public:
     // Rounds of the other buckets: as many matrices as keep the padded H, S
     // and U of a round within kRoundBytes, at least kLargeRound and at most
     // kMaxRound.
-    static constexpr size_t kRoundBytes = size_t(4) << 30;
+    static constexpr size_t kRoundBytes = size_t(1) << 30;
+    // Buckets run on this many branches of the call (streams forked from and
+    // joined back into the caller's), each with its own padded staging area,
+    // so that one bucket's kernels fill the CUs another's leave idle.
+    static constexpr int kBranches = 4;
     static constexpr int kMaxRound = 65536;
     // Matrices a bucket's workspace holds at least once the bucket is used.
     static constexpr int kMinReserve = 256;
@@ -113,7 +117,12 @@ private:
     hipEvent_t staged_ = nullptr;   // last upload from host_desc_ has finished
     bool staged_pending_ = false;
     hipEvent_t done_ = nullptr;     // last launch's grids have finished
-    // Padded batch of a bucket's round: H, S, U, W, info.
+    // Branch streams (default priority) and their fork / join events.
+    hipStream_t branch_[kBranches] = {};
+    hipEvent_t fork_ = nullptr;
+    hipEvent_t join_[kBranches] = {};
+    // Padded batches of the branches' rounds: H, S, U, W, info; pad_bytes_
+    // per branch.
     void* pad_ = nullptr;
     size_t pad_bytes_ = 0;
 };
