@@ -307,6 +307,96 @@ __device__ __forceinline__ ctile c3_final(const ctile3& c) {
     return t;
 }
 
+// The 16 x 16 triangular system L(ib, ib)^H x = acc of block row ib, in
+// place in the FP64 tile layout (row r of the tile: element r / 4 of lane
+// group r % 4), rows from the last: u_r from its owner lane of the same
+// column, then the rows above it updated. Rows at or beyond N are left alone.
+template <class FL>
+__device__ __forceinline__ void backsub_diag(ctile& acc, FL fL, int ib) {
+    const int l = lane_id();
+    const int c16 = l & 15, g4 = l >> 4;
+#pragma unroll
+    for (int r = 15; r >= 0; --r) {
+        const int rg = ib * 16 + r;
+        if (rg >= N) continue;
+        const int own = c16 + 16 * (r % 4);
+        const real ur = __shfl(acc.re[r / 4], own, 64);
+        const real ui = __shfl(acc.im[r / 4], own, 64);
+        const real inv = real(1.0) / fL(rg, rg).x;
+        const cx u = cm(ur * inv, ui * inv);
+        if (g4 == r % 4) {
+            acc.re[r / 4] = u.x;
+            acc.im[r / 4] = u.y;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = 4 * v + g4;              // row of the tile in this element
+            if (4 * v < r) {
+                const bool upd = i < r;
+                const cx lr = fL(rg, ib * 16 + (upd ? i : 0));
+                const cx t = cconjmul(lr, u);
+                if (upd) {
+                    acc.re[v] -= t.x;
+                    acc.im[v] -= t.y;
+                }
+            }
+        }
+    }
+}
+
+// U(:, this wave's 16-column block cb) = L^-H V, L lower triangular, by
+// block back substitution from the last 16-row block: row tile ib is
+// V(ib) - sum_{jb > ib} L(jb, ib)^H U(jb) on the matrix cores (the finished
+// tiles U(jb) are the B operand, straight from registers), then the 16 x 16
+// triangular system L(ib, ib)^H U(ib) = that, solved row by row in the tile
+// layout: row r of the tile is element r / 4 of lane group r % 4, so each
+// step takes u_r from its owner lane of the same column and updates the rows
+// above it. No explicit inverse of L is formed (with an ill-conditioned S a
+// product with W = L^-1 doubles the residual that the substitution leaves,
+// as zhegv's back-transform does). NBLK: 16-blocks per dimension; KE: the
+// k-steps that touch rows below N. fL(i, k): L(i, k) for i >= k, k < N.
+// fV(i, j): V(i, j), zero outside N.
+template <int NBLK, int KE, class FL, class FV>
+__device__ __forceinline__ void backsub_lh(ctile (&Ut)[NBLK], FL fL, FV fV, int cb) {
+    static_assert(sizeof(real) == 8, "the FP64 tile layout: row 4 e + g of element e, lane group g");
+    const int l = lane_id();
+    const int c16 = l & 15, g4 = l >> 4;
+    const int j = cb * 16 + c16;
+#pragma unroll
+    for (int ib = NBLK - 1; ib >= 0; --ib) {
+        ctile acc;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const cx x = fV(ib * 16 + 4 * v + g4, j);
+            acc.re[v] = x.x;
+            acc.im[v] = x.y;
+        }
+        if (ib < NBLK - 1) {
+            // sum_{k in blocks jb > ib} conj(L(k, 16 ib + i)) U(k, j).
+            ctile3 a3;
+            c3_zero(a3);
+#pragma unroll
+            for (int q = ib + 1; q < NBLK; ++q) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    if (16 * q + 4 * v >= KE) continue;
+                    const cx b = tile_get(Ut[q], v);
+                    const real bs = b.x + b.y;
+                    const int k = 16 * q + 4 * v + g4;
+                    const int i = ib * 16 + c16;
+                    const cx a = (k < N && i < N) ? cconj(fL(k, i)) : cm(0.0, 0.0);
+                    cmfma3(a3, a, b, bs);
+                }
+            }
+            const ctile sm = c3_final(a3);
+            acc.re = acc.re - sm.re;
+            acc.im = acc.im - sm.im;
+        }
+        backsub_diag(acc, fL, ib);
+        Ut[ib] = acc;
+    }
+}
+
 }  // namespace gk
 
 // The staged kernels (orders above 64; their Cholesky for every order).
@@ -1467,11 +1557,12 @@ geneig_ql_full(real* __restrict__ W, const real* __restrict__ E, real* __restric
 //   geneig_mx_back    (FP32 module): V = Q Z into U32.
 //   geneig_mx_refine  (FP64 module): Ogita-Aishima refinement of (M, V) in
 //                     the Cholesky frame on the matrix cores (M from Mp),
-//                     Rayleigh-Ritz on clusters, then U = W^H V; W, U out.
+//                     Rayleigh-Ritz on clusters, then U = L^-H V by back
+//                     substitution with L (packed); W, U out.
 // The FP64 split mode runs the same stages in the FP64 module, without the
 // refinement: geneig_lg_chol, geneig_mx_formm, geneig_mx_front (T to W and E,
 // the stash in U), geneig_ql_record + geneig_split_ql (or geneig_ql_full),
-// geneig_mx_back (V = Q Z in U), then geneig_mx_backtri (U = W^H V in place).
+// geneig_mx_back (V = Q Z in U), then geneig_mx_backtri (U = L^-H V in place).
 // ---------------------------------------------------------------------------
 
 // Built in both precisions: FP32 for the mixed mode, FP64 for the split mode.
@@ -1491,7 +1582,7 @@ struct WorkF {
     real e[N];
 };
 
-// Stash for the FP32 back half: tau on the superdiagonal and the reflector
+// Stash for the back half: tau on the superdiagonal and the reflector
 // entries from the owning waves' registers (column k, rows >= k+2, go to
 // (k, i)); the lower triangle is left alone (nothing reads it).
 __device__ __forceinline__ void stash_refl_reg(const WorkF& w, const cx (&m)[CPW], cx* __restrict__ Ub) {
@@ -1862,27 +1953,28 @@ geneig_mx_formm_diag(const real* __restrict__ Hd, long long hd_stride, const cx*
     store_m_packed(Mp + (size_t)b * NPK, M, j, jin, g4);
 }
 
-// U = W^H V of the FP64 split mode, in place (V arrives in U): V and W's
-// lower triangle staged in LDS, then each wave's 16-column block on the
-// matrix cores (W^H is upper triangular: row tile rt takes k >= 16 rt).
-// Matrices whose Cholesky failed keep the outputs it wrote.
+// U = L^-H V of the FP64 split mode, in place (V arrives in U): V and L's
+// packed lower triangle staged in LDS, then each wave's 16-column block by
+// block back substitution (backsub_lh). Matrices whose Cholesky failed keep
+// the outputs it wrote.
 extern "C" __global__ void __launch_bounds__(gk::REF_NT, 2)
-geneig_mx_backtri(const cx* __restrict__ Wd, cx* __restrict__ U, const int* __restrict__ info, int batch) {
+geneig_mx_backtri(const cx* __restrict__ Lp, cx* __restrict__ U, const int* __restrict__ info, int batch) {
     using namespace gk;
     __shared__ cx Vs[N * LDR];
-    __shared__ cx Ws[NPK];
+    __shared__ cx Ls[NPK];
     const int b = blockIdx.x;
     if (b >= batch || info[b] > N) return;
     const int l = lane_id(), cb = wave_id();
     const int g4 = l >> 4;
     const int j = cb * 16 + (l & 15);          // this lane's column
     cx* Ub = U + (size_t)b * N * N;
+    const cx* Lb = Lp + (size_t)b * NPK;
     for (int e = threadIdx.x; e < N * N; e += REF_NT) Vs[(e / N) * LDR + e % N] = Ub[e];
-    stage_w(Ws, w_of(Wd, b), threadIdx.x, REF_NT);
+    for (int e = threadIdx.x; e < NPK; e += REF_NT) Ls[e] = Lb[e];
     __syncthreads();
     ctile Uo[NB];
-    cgemm<true>(Uo, [&](int i, int k) { return cconj(lelem(Ws, k, i)); },
-                [&](int k, int jj) { return elem(Vs, LDR, k, jj); }, cb);
+    backsub_lh<NB, KEND>(Uo, [&](int i, int k) { return Ls[pk(k) + i - k]; },
+               [&](int i, int jj) { return elem(Vs, LDR, i, jj); }, cb);
 #pragma unroll
     for (int rt = 0; rt < NB; ++rt) {
 #pragma unroll
@@ -1898,14 +1990,15 @@ geneig_mx_backtri(const cx* __restrict__ Wd, cx* __restrict__ U, const int* __re
 // (T = V^H M V, G = V^H V, lambda_j = T_jj / G_jj, V <- V + V E with E as
 // described at its computation), Rayleigh-Ritz on every cluster (a run of
 // eigenvalues with adjacent gaps below delta: one exact 2 x 2 step for a
-// pair, Jacobi sweeps over all its pairs otherwise), then U = W^H V. In step
-// it (from 0), delta = 2^it delta_rel max |lambda|. U^H S U = V^H V up to the
-// accuracy of W, as in the FP64 modes. Convergence test: the error left
+// pair, Jacobi sweeps over all its pairs otherwise), then U = L^-H V by
+// back substitution with the packed L (Lp). In step it (from 0),
+// delta = 2^it delta_rel max |lambda|. U^H S U = V^H V up to the accuracy of
+// the substitution, as in the FP64 modes. Convergence test: the error left
 // after a step is of the order of the square of its correction, so a last
 // step whose correction E has an entry above `tol` marks the matrix
 // info = N (a value zhegv's INFO never takes), with W and U as computed.
 extern "C" __global__ void __launch_bounds__(gk::REF_NT, 2)
-geneig_mx_refine(const cx* __restrict__ Mp, const cx* __restrict__ Wd, real* __restrict__ W,
+geneig_mx_refine(const cx* __restrict__ Mp, const cx* __restrict__ Lp, real* __restrict__ W,
                  cx* __restrict__ U, int* __restrict__ info, const gk::cf* __restrict__ V32,
                  const float* __restrict__ W32, int batch, int iters, real delta_rel, real tol) {
     using namespace gk;
@@ -1917,11 +2010,10 @@ geneig_mx_refine(const cx* __restrict__ Mp, const cx* __restrict__ Wd, real* __r
     const int j = cb * 16 + (l & 15);          // this lane's column
     const bool jin = j < N;
     const size_t off = (size_t)b * N * N;
-    const cx* Wb = w_of(Wd, b);                     // W = L^-1
+    const cx* Lb = Lp + (size_t)b * NPK;            // L, packed by columns
     auto fM = [&](int i, int k) { return melem(r.M, i, k); };
     auto fV = [&](int i, int k) { return elem(r.U, LDR, i, k); };
     auto fVh = [&](int i, int k) { return cconj(elem(r.U, LDR, k, i)); };
-    auto fW = [&](int i, int k) { return lelem(r.M, i, k); };
     {
         const cx* Mb = Mp + (size_t)b * NPK;
         for (int e = threadIdx.x; e < NPK; e += REF_NT) r.M[e] = Mb[e];
@@ -2043,11 +2135,12 @@ geneig_mx_refine(const cx* __restrict__ Mp, const cx* __restrict__ Wd, real* __r
         for (int q = 1; q < NB; ++q) m = fmax(m, r.emax[q]);
         if (!(m <= tol)) info[b] = N;
     }
-    // U = W^H V, W's lower triangle staged again over M (no longer needed).
-    stage_w(r.M, Wb, threadIdx.x, REF_NT);
+    // U = L^-H V by block back substitution, L's packed lower triangle staged
+    // over M (no longer needed).
+    for (int e = threadIdx.x; e < NPK; e += REF_NT) r.M[e] = Lb[e];
     __syncthreads();
     ctile Uo[NB];
-    cgemm<true>(Uo, [&](int i, int k) { return cconj(fW(k, i)); }, fV, cb);
+    backsub_lh<NB, KEND>(Uo, [&](int i, int k) { return r.M[pk(k) + i - k]; }, fV, cb);
     cx* Ub = U + off;
 #pragma unroll
     for (int rt = 0; rt < NB; ++rt) {

@@ -24,7 +24,8 @@
 //                           of Z (in LDS, or in global memory for the rows that
 //                           do not fit), then sorts.
 //   geneig_lg_back    P     V = Q Z, blocks of 8 columns per wave in registers.
-//   geneig_lg_backtri FP64  U = W^H V on the matrix cores.
+//   geneig_lg_backtri FP64  U = L^-H V by block back substitution (the
+//                           off-diagonal blocks on the matrix cores).
 // P is the module precision: FP64 for the FP64 modes; FP32 in the mixed mode,
 // which then refines (M, V) in FP64 (geneig_lg_refine) instead of
 // geneig_lg_backtri.
@@ -62,7 +63,8 @@ constexpr size_t WS_V = 48 * NN;              // V (complex, module precision; F
 constexpr size_t WS_R = 64 * NN;              // reflectors: v_k(i) at (i, k), i >= k+2; tau_k at (k+1, k)
 constexpr size_t WS_Z = 80 * NN;              // Z, columns in ascending eigenvalue order (real)
 constexpr size_t WS_D = 88 * NN;              // d[N], e[N], ascending eigenvalues[N] (real)
-constexpr size_t WS_Q = (88 * NN + 24 * (size_t)N + 255) / 256 * 256;   // QL rotation log
+constexpr size_t WS_L = (88 * NN + 24 * (size_t)N + 255) / 256 * 256;   // L packed by columns (complex128)
+constexpr size_t WS_Q = (WS_L + 8 * (size_t)N * (N + 1) + 255) / 256 * 256;   // QL rotation log
 // The log: header (sweeps, overflow flag), the window (l, m) of every sweep,
 // the ascending rank of every eigenvalue, then the rotations (c_i, s_i) of all
 // sweeps back to back. A matrix whose QL needs more sweeps or rotations than
@@ -690,7 +692,9 @@ using gk::cx;
 using gk::real;
 
 #ifndef GENEIG_FP32
-// S = L L^H and W = L^-1 in the workspace (upper triangle zero). On a
+// S = L L^H and W = L^-1 in the workspace (upper triangle zero), and L
+// packed by columns: in the staged pipeline to the workspace (WS_L), else to
+// Lout when it is not NULL. On a
 // non-positive pivot k (0-based) of a leading minor: info = N + k + 1, W NaN,
 // U zero (LAPACK zhegv's convention), and the later stages skip the matrix.
 // For small orders (while three workgroups still fit a CU), L and then W live
@@ -713,7 +717,7 @@ extern "C" __device__ const int geneig_w_packed = gk::W_PACKED_OUT ? 1 : 0;
 
 extern "C" __global__ void __launch_bounds__(gk::LNT, 3)
 geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ U, int* __restrict__ info,
-               char* __restrict__ ws, long long wss, int batch) {
+               char* __restrict__ ws, long long wss, cx* __restrict__ Lout, int batch) {
     using namespace gk;
     __shared__ CholWork cw;
     __shared__ cx Lp[CHOL_LDS ? LPK : 1];
@@ -827,6 +831,23 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
         }
         __syncthreads();
     }
+    // L itself, packed by columns, for a back-transform by substitution,
+    // before W overwrites it: to the workspace in the staged pipeline, else
+    // to Lout when the caller asks for it.
+    cx* Lo = nullptr;
+    if constexpr (GENEIG_LG_PIPELINE) Lo = reinterpret_cast<cx*>(ws_of(ws, wss, b) + WS_L);
+    else if (Lout) Lo = Lout + (size_t)b * LPK;
+    if (Lo) {
+        if constexpr (CHOL_LDS) {
+            for (int e = tid; e < LPK; e += LNT) Lo[e] = Lp[e];
+        } else {
+            for (int e = tid; e < (int)NN; e += LNT) {
+                const int i = e % N, j = e / N;
+                if (i >= j) Lo[lpk(j) + i - j] = Lg[e];
+            }
+        }
+        __syncthreads();
+    }
     // W = L^-1 in place, block row ib after block row ib-1: with X = W,
     // X(ib, j) = L_ii^-1 (delta - sum_{k < r0} L(ib, k) X(k, j)).
     for (int ib = 0; ib < LNB; ++ib) {
@@ -933,33 +954,78 @@ geneig_lg_formm(const cx* __restrict__ H, const int* __restrict__ info, char* __
     }
 }
 
-// U = W^H V (V in FP64 from the workspace); W^H is upper triangular, so row
-// tile rt needs k >= 16 rt only.
+// U = L^-H V (V in FP64 from the workspace) by block back substitution with
+// L's packed lower triangle from the workspace (no explicit inverse in the
+// back-transform): a wave takes one 16-column block of U at a time, row tile
+// by row tile from the last, keeping the finished tiles in its LDS slice
+// (N x 16) as the B operand of the blocks above, and storing each tile to U
+// as it completes. BS_SLICES waves work (the column blocks spread evenly over
+// them): as many as leave two workgroups per CU, or, where fewer than two
+// slices would then fit, as many as fit one workgroup.
+namespace gk {
+constexpr int BS_SLICE_BYTES = N * 16 * 16;
+constexpr int BS_FIT2 = (78 * 1024) / BS_SLICE_BYTES;
+constexpr int BS_FIT1 = (150 * 1024) / BS_SLICE_BYTES;
+constexpr int BS_SLICES = (BS_FIT2 >= 2 ? BS_FIT2 : BS_FIT1) < LNW ? (BS_FIT2 >= 2 ? BS_FIT2 : BS_FIT1) : LNW;
+static_assert(BS_SLICES >= 1, "one back-substitution slice must fit");
+}  // namespace gk
+
 extern "C" __global__ void __launch_bounds__(gk::LNT)
 geneig_lg_backtri(cx* __restrict__ U, const int* __restrict__ info, char* __restrict__ ws, long long wss,
                   int batch) {
     using namespace gk;
+    __shared__ cx bsl[BS_SLICES][N * 16];
     const int b = blockIdx.x;
     if (b >= batch || info[b] > N) return;
     const int wave = wave_id(), lane = lane_id();
+    const int c16 = lane & 15, g4 = lane >> 4;
     char* wsb = ws_of(ws, wss, b);
-    const cx* Wi = reinterpret_cast<const cx*>(wsb + WS_WI);
+    const cx* Lb = reinterpret_cast<const cx*>(wsb + WS_L);
     const cx* V = reinterpret_cast<const cx*>(wsb + WS_V);
     cx* Ub = U + (size_t)b * NN;
-    constexpr int NRG = (LNB + 3) / 4;
-    for (int t = wave; t < NRG * LNB; t += LNW) {
-        const int rt0 = 4 * (t / LNB), ct = t % LNB;
-        ctile acc[4];
-        gemm_rg<4>(acc, [&](int i, int k) { return cconj(gel(Wi, k, i)); }, [&](int k, int j) { return gel(V, k, j); },
-                   rt0, ct, 16 * rt0, LKEND);
-        const int j = ct * 16 + (lane & 15);
+    auto fL = [&](int i, int k) { return Lb[lpk(k) + i - k]; };
+    {
+        if (wave < BS_SLICES) {
+            cx* sl = bsl[wave];
+            for (int ct = wave; ct < LNB; ct += BS_SLICES) {
+                const int j = ct * 16 + c16;
+                for (int ib = LNB - 1; ib >= 0; --ib) {
+                    ctile acc;
 #pragma unroll
-        for (int g = 0; g < 4; ++g)
+                    for (int v = 0; v < 4; ++v) {
+                        const cx x = gel(V, ib * 16 + 4 * v + g4, j);
+                        acc.re[v] = x.x;
+                        acc.im[v] = x.y;
+                    }
+                    if (ib < LNB - 1) {
+                        // sum_{k in blocks above} conj(L(k, 16 ib + i)) U(k, j).
+                        ctile3 a3;
+                        c3_zero(a3);
+                        const int i = ib * 16 + c16;
+                        for (int k0 = 16 * (ib + 1); k0 < LKEND; k0 += 4) {
+                            const int k = k0 + g4;
+                            const cx bu = (k < N) ? sl[c16 * N + k] : cm(0.0, 0.0);
+                            const cx a = (k < N && i < N) ? cconj(fL(k, i)) : cm(0.0, 0.0);
+                            cmfma3(a3, a, bu, bu.x + bu.y);
+                        }
+                        const ctile sm = c3_final(a3);
+                        acc.re = acc.re - sm.re;
+                        acc.im = acc.im - sm.im;
+                    }
+                    backsub_diag(acc, fL, ib);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = tile_row(rt0 + g, v);
-                if (i < N && j < N) Ub[(size_t)j * N + i] = tile_get(acc[g], v);
+                    for (int v = 0; v < 4; ++v) {
+                        const int row = ib * 16 + 4 * v + g4;
+                        if (row < N) {
+                            const cx u = cm(acc.re[v], acc.im[v]);
+                            sl[c16 * N + row] = u;
+                            if (j < N) Ub[(size_t)j * N + row] = u;
+                        }
+                    }
+                    wave_lds_sync();
+                }
             }
+        }
     }
 }
 #endif  // GENEIG_LG_PIPELINE

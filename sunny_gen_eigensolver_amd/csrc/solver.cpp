@@ -116,19 +116,20 @@ namespace {
 // and off-diagonals, Z (FP32), M's lower triangle packed (complex128) and M
 // in full (complex64).
 struct MixedLayout {
-    size_t wd, v32, w32, e32, z32, mp, total;
+    size_t wd, lp, v32, w32, e32, z32, mp, total;
 };
 // The FP64 split mode's workspace: W = L^-1 (room for the full matrix;
-// packed where geneig_lg_chol packs it), M (packed complex128), the
+// packed where geneig_lg_chol packs it), L and M (packed complex128), the
 // tridiagonal's off-diagonal and Z (float64).
 struct HybridLayout {
-    size_t wd, mp, e, z, total;
+    size_t wd, lp, mp, e, z, total;
 };
 HybridLayout hybrid_layout(size_t n, size_t B) {
     auto al = [](size_t x) { return (x + 255) / 256 * 256; };
     HybridLayout h;
     h.wd = 0;
-    h.mp = al(h.wd + B * n * n * 16);
+    h.lp = al(h.wd + B * n * n * 16);
+    h.mp = al(h.lp + B * n * (n + 1) / 2 * 16);
     h.e = al(h.mp + B * n * (n + 1) / 2 * 16);
     h.z = al(h.e + B * n * 8);
     h.total = al(h.z + B * n * n * 8);
@@ -139,7 +140,8 @@ MixedLayout mixed_layout(size_t n, size_t B) {
     auto al = [](size_t x) { return (x + 255) / 256 * 256; };
     MixedLayout m;
     m.wd = 0;
-    m.v32 = al(m.wd + B * n * n * 16);
+    m.lp = al(m.wd + B * n * n * 16);
+    m.v32 = al(m.lp + B * n * (n + 1) / 2 * 16);
     m.w32 = al(m.v32 + B * n * n * 8);
     m.e32 = al(m.w32 + B * n * 4);
     m.z32 = al(m.e32 + B * n * 4);
@@ -296,7 +298,7 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
                                      std::to_string(ql_log_bytes(n_)) + " B");
     }
     if (mode_ == Mode::Hybrid || mode_ == Mode::Mixed) {
-        // The FP64 stages both modes share: Cholesky and W = L^-1, M = W H W^H.
+        // The FP64 stages both modes share: Cholesky, L and W = L^-1, M = W H W^H.
         k_mx_chol_ = module_->get("geneig_lg_chol");
         k_mx_formm_ = module_->get("geneig_mx_formm");
         k_mx_formm_diag_ = module_->get("geneig_mx_formm_diag");
@@ -692,22 +694,23 @@ void Solver::ensure_workspace_(size_t batch) {
         lbuf_ = nullptr;
         lbuf_batch_ = 0;
     }
-    // Per matrix: W = L^-1, packed M, the tridiagonal's off-diagonal and Z (hybrid_layout).
+    // Per matrix: W = L^-1, packed L and M, the tridiagonal's off-diagonal and Z (hybrid_layout).
     SG_HIP(hipMalloc(&lbuf_, hybrid_layout((size_t)n_, batch).total));
     lbuf_batch_ = batch;
 }
 
 // Split mode (reference: hybrid_solver.cpp:295-393), FP64 throughout: the
-// reduction (Cholesky and W = L^-1, M = W H W^H on the matrix cores, the
-// tridiagonalisation of M), QL, then the back transform (V = Q Z and
-// U = W^H V on the matrix cores); each kernel at the occupancy its own
-// footprint allows.
+// reduction (Cholesky, L and W = L^-1, M = W H W^H on the matrix cores, the
+// tridiagonalisation of M), QL, then the back transform (V = Q Z on the
+// matrix cores, U = L^-H V by block back substitution); each kernel at the
+// occupancy its own footprint allows.
 void Solver::launch_hybrid_(const void* H, const void* S, double* W, void* U, int* info, int batch,
                             hipStream_t stream, const Workspace& ws) {
     const size_t n = (size_t)n_, B = (size_t)batch;
     const HybridLayout hl = hybrid_layout(n, B);
     char* base = static_cast<char*>(ws.split);
     void* Wd = base + hl.wd;
+    void* Lp = base + hl.lp;
     void* Mp = base + hl.mp;
     double* E = reinterpret_cast<double*>(base + hl.e);
     double* Zw = reinterpret_cast<double*>(base + hl.z);
@@ -717,7 +720,7 @@ void Solver::launch_hybrid_(const void* H, const void* S, double* W, void* U, in
     // One wave per 16-row (or 16-column) tile for the matrix-core kernels.
     const unsigned bd_tiles = (unsigned)(64 * ((n_ + 15) / 16));
     long long wstride = (long long)((w_packed_ ? n * (n + 1) / 2 : n * n) * 16);
-    void* a1[] = {&S, &W, &U, &info, &Wd, &wstride, &batch};
+    void* a1[] = {&S, &W, &U, &info, &Wd, &wstride, &Lp, &batch};
     launch_(k_mx_chol_, grid_b, bd_tiles, stream, a1);
     mark_(stream);
     if (ws.hdiag && !ws.hfull) {
@@ -753,7 +756,7 @@ void Solver::launch_hybrid_(const void* H, const void* S, double* W, void* U, in
     void* a4[] = {&U, &Zw, &info, &batch};
     launch_(k_back64_, grid_b, bd_tiles, stream, a4);
     mark_(stream);
-    void* a5[] = {&Wd, &U, &info, &batch};
+    void* a5[] = {&Lp, &U, &info, &batch};
     launch_(k_backtri_, grid_b, bd_tiles, stream, a5);
     mark_(stream);
 }
@@ -767,15 +770,16 @@ void Solver::ensure_mixed_workspace_(size_t batch) {
         mbuf_ = nullptr;
         mbuf_batch_ = 0;
     }
-    // complex128 W = L^-1 (room for the full matrix), complex64 stash / V,
-    // float eigenvalues, off-diagonals and Z, M (packed complex128).
+    // complex128 W = L^-1 (room for the full matrix) and L (packed),
+    // complex64 stash / V, float eigenvalues, off-diagonals and Z, M (packed
+    // complex128).
     SG_HIP(hipMalloc(&mbuf_, mixed_layout((size_t)n_, batch).total));
     mbuf_batch_ = batch;
 }
 
-// Mixed precision: FP64 Cholesky and L^-1, FP32 reduction / tridiagonalisation
-// / QL / V = Q Z (the split kernels built in FP32), FP64 refinement of (M, V)
-// in the Cholesky frame and U = W^H V on the matrix cores.
+// Mixed precision: FP64 Cholesky, L and L^-1, FP32 tridiagonalisation / QL /
+// V = Q Z (the split kernels built in FP32), FP64 refinement of (M, V) in the
+// Cholesky frame on the matrix cores, then U = L^-H V by back substitution.
 void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int* info, int batch,
                            hipStream_t stream, const Workspace& ws) {
     const size_t n = (size_t)n_, B = (size_t)batch;
@@ -783,6 +787,7 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
     const MixedLayout ml = mixed_layout(n, B);
     char* base = static_cast<char*>(ws.mixed);
     void* Wd = base + ml.wd;
+    void* Lp = base + ml.lp;
     void* V32 = base + ml.v32;
     void* W32 = base + ml.w32;
     void* E32 = base + ml.e32;
@@ -796,7 +801,7 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
     // workgroup per matrix of one wave per 16-row tile (the kernel's size for
     // orders up to 64).
     long long wstride = (long long)((w_packed_ ? n * (n + 1) / 2 : n * n) * 16);
-    void* a1[] = {&S, &W, &U, &info, &Wd, &wstride, &batch};
+    void* a1[] = {&S, &W, &U, &info, &Wd, &wstride, &Lp, &batch};
     const unsigned bd_chol = (unsigned)(64 * ((n_ + 15) / 16));
     launch_(k_mx_chol_, grid_b, bd_chol, stream, a1);
     mark_(stream);
@@ -839,7 +844,7 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
     int iters = refine_iters_;
     double delta = refine_delta_;
     double tol = refine_tol_;
-    void* a5[] = {&Mp, &Wd, &W, &U, &info, &V32, &W32, &batch, &iters, &delta, &tol};
+    void* a5[] = {&Mp, &Lp, &W, &U, &info, &V32, &W32, &batch, &iters, &delta, &tol};
     // The refinement runs one wave per 16-column block of the matrix.
     launch_(k_refine_, grid_b, bd_ref, stream, a5);
     mark_(stream);
@@ -850,7 +855,8 @@ size_t Solver::large_matrix_bytes(int n_) {
     // QL rotation log Q_*); bind_module_ checks it against the module's own.
     const size_t n = (size_t)n_;
     auto al = [](size_t x, size_t a) { return (x + a - 1) / a * a; };
-    const size_t wsq = al(88 * n * n + 24 * n, 256);
+    const size_t wsl = al(88 * n * n + 24 * n, 256);           // L packed (complex128)
+    const size_t wsq = al(wsl + 8 * n * (n + 1), 256);
     const size_t q_rank = 16 + 8 * (8 * n);
     const size_t q_rot = al(q_rank + 4 * n, 16);
     const size_t q_bytes = q_rot + 16 * (2 * n * n);
@@ -931,7 +937,8 @@ void Solver::launch_large_(const void* H, const void* S, double* W, void* U, int
         auto mark = [&]() {
             if (marks) mark_(stream);
         };
-        void* a1[] = {&Sc, &Wc, &Uc, &Ic, &ws, (void*)&wss, &cnt};
+        void* no_l = nullptr;     // the staged back-transform reads W
+        void* a1[] = {&Sc, &Wc, &Uc, &Ic, &ws, (void*)&wss, &no_l, &cnt};
         launch_(k_lg_chol_, (unsigned)cnt, bd, stream, a1);
         mark();
         void* a2[] = {&Hc, &Ic, &ws, (void*)&wss, &cnt};

@@ -151,8 +151,8 @@ private:
     hipFunction_t k_qlrec_ = nullptr;
     void* qlog_ = nullptr;
     size_t qlog_batch_ = 0;
-    // Split-mode workspace: W = L^-1, packed M, tridiagonal off-diagonals and
-    // Z, one set per matrix.
+    // Split-mode workspace: W = L^-1, packed L and M, tridiagonal
+    // off-diagonals and Z, one set per matrix.
     void* lbuf_ = nullptr;
     size_t lbuf_batch_ = 0;
     // Mixed mode: the FP32 module and the FP32 hand-offs between its stages
