@@ -46,8 +46,9 @@ def flops_per_solve(n: int) -> float:
 def refine_flops_per_solve(n: int, iters: int) -> float:
     """Algorithmic complex multiply-adds of the refinement kernel, 8 real
     flops each: per step X = M V, T = V^H X, G = V^H V and V E (4 N^3), then
-    U = W^H V with W = L^-1 lower triangular (N^3 / 2). (M = W H W^H is formed
-    by its own kernel beforehand.)"""
+    U = L^-H V by back substitution with the Cholesky factor (N^3 / 2, the
+    count of a triangular product). (M = W H W^H, W = L^-1, is formed by its
+    own kernel beforehand.)"""
     return 8.0 * (0.5 + 4 * iters) * n ** 3
 
 
@@ -353,7 +354,7 @@ def main():
                   "achieved_TFLOPs": B * rf / (refine_ms * 1e-3) / 1e12,
                   "frac_of_fp64_mfma_peak": B * rf / (refine_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
                   "note": "the refinement's complex products (per step M V, V^H X, V^H V, V E; then "
-                          "U = W^H V at its triangular size), 8 real flops per complex multiply-add, "
+                          "U = L^-H V by back substitution, N^3/2), 8 real flops per complex multiply-add, "
                           "over the kernel's launch time"}
     dom_name, dom_ms = max(st, key=lambda kv: kv[1])
 

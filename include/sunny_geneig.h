@@ -183,7 +183,7 @@ sunny_geneig_status_t sunny_geneig_launch_hdiag(
  * runs on (set up when a handle is created there) until the process ends; the
  * call allocates nothing and captures no graph. A batch whose workspace does
  * not fit returns INVALID_ARG with the limit in the message (at N = 54: about
- * 1,260 matrices per call in the mixed mode, 1,420 in the split mode; the
+ * 1,030 matrices per call in the mixed mode, 1,130 in the split mode; the
  * fused mode needs no workspace; orders above 64 run in rounds that fit).
  * Calls on one GPU share its array and are serialised on the device through
  * an event, whatever stream or handle they use. */

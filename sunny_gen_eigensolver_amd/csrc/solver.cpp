@@ -112,9 +112,10 @@ int Solver::suggested_block_dim(int n) {
 
 namespace {
 // Mixed-mode workspace for a batch of B matrices of order n (byte offsets):
-// W = L^-1 (complex128, full), the complex64 stash / V, the FP32 eigenvalues
-// and off-diagonals, Z (FP32), M's lower triangle packed (complex128) and M
-// in full (complex64).
+// W = L^-1 (complex128; room for the full matrix, packed where geneig_lg_chol
+// packs it), L packed (complex128), the complex64 stash / V, the FP32
+// eigenvalues and off-diagonals, Z (FP32) and M's lower triangle packed
+// (complex128).
 struct MixedLayout {
     size_t wd, lp, v32, w32, e32, z32, mp, total;
 };
