@@ -1299,8 +1299,9 @@ geneig_fused(const cx* __restrict__ H, const cx* __restrict__ S, real* __restric
 // geneig_ql_record runs it for 64 matrices at once, one per lane (its d and e
 // in LDS, element i of lane l at [i][l], so a lane always hits its own bank
 // whatever index it is at), as a per-lane state machine that advances every
-// lane by one step per iteration: a deflation scan of four off-diagonals, or
-// one rotation. Each matrix's sweeps (l, m), rotations, ascending ranks and
+// lane by one rotation per iteration; a lane whose sweep ends picks its next
+// window from a bit mask of negligible off-diagonals in the same iteration
+// (no scan of the off-diagonals). Each matrix's sweeps (l, m), rotations, ascending ranks and
 // eigenvalues go to its log; geneig_split_ql then rotates the register rows
 // of Z with them. A matrix whose QL outgrows its log (or the caller's sweep
 // cap, at most the log's) is redone by geneig_ql_full instead.
@@ -1339,46 +1340,72 @@ geneig_ql_record(const real* __restrict__ W, const real* __restrict__ E, const i
     real* lam = reinterpret_cast<real*>(lg + QG_LAM);
     real* rot = reinterpret_cast<real*>(lg + QG_ROT);
     const real eps = REAL_HALF_ULP;
-    enum { SCAN = 0, SWEEP = 1, DONE = 2 };
-    int phase = mine ? SCAN : DONE;
-    int l = 0, sc = 0, m = 0, i = 0, iter = 0, nsw = 0, nrot = 0, unconverged = 0, overflow = 0;
+    // Bit k of small: off-diagonal k is negligible against its diagonals
+    // (|e_k| <= eps (|d_k| + |d_k+1|)); bits N-1 and up are set. A sweep over
+    // [l, m] changes only e_l .. e_m and d_l .. d_m, and each rotation leaves
+    // its e_(i+1), d_(i+1) and d_(i+2) final for the sweep, so the rotations
+    // re-test exactly the bits the sweep can change and the choice of the
+    // next window needs no scan: l moves to the lowest clear bit at or above
+    // it (the deflations), m is the lowest set bit above that.
+    unsigned long long small = ~0ull << (N - 1);
+    for (int k = 0; k < N - 1; ++k)
+        if (fabs(es[k][lane]) <= eps * (fabs(ds[k][lane]) + fabs(ds[k + 1][lane]))) small |= 1ull << k;
+    enum { START = 0, SWEEP = 1, DONE = 2 };
+    int phase = mine ? START : DONE;
+    int l = 0, m = 0, i = 0, iter = 0, nsw = 0, nrot = 0, unconverged = 0, overflow = 0;
     real s = real(1.0), c = real(1.0), p = real(0.0), g = real(0.0), d_up = real(0.0), d_l = real(0.0);
+    real d_prev = real(0.0);                            // d_(i+2) of the current rotation
     while (__ballot(phase != DONE)) {
-        if (phase == SCAN) {
+        if (phase == SWEEP) {
+            real e_next, d_next;
+            ql_rotation(es[i][lane], ds[i][lane], s, c, g, p, d_up, e_next, d_next);
+            es[i + 1][lane] = e_next;
+            ds[i + 1][lane] = d_next;
+            rot[2 * (nrot + i - l)] = c;
+            rot[2 * (nrot + i - l) + 1] = s;
+            if (i + 1 < m) {
+                const unsigned long long bit = 1ull << (i + 1);
+                small = (fabs(e_next) <= eps * (fabs(d_next) + fabs(d_prev))) ? (small | bit) : (small & ~bit);
+            }
+            d_prev = d_next;
+            if (--i < l) {
+                const real dl_new = d_l - p;
+                ds[l][lane] = dl_new;
+                es[l][lane] = g;
+                es[m][lane] = real(0.0);
+                const unsigned long long bit = 1ull << l;
+                small = (fabs(g) <= eps * (fabs(dl_new) + fabs(d_prev))) ? (small | bit) : (small & ~bit);
+                small |= 1ull << m;
+                nrot += m - l;
+                phase = START;
+            }
+        }
+        if (phase == START) {
+            // Deflate: l moves past every negligible off-diagonal at l.
+            const unsigned long long open = ~small & (~0ull << l);
+            const int l_new = open ? __builtin_ctzll(open) : N;
+            if (l_new != l) {
+                l = l_new;
+                iter = 0;
+            }
             if (l >= N) {
                 phase = DONE;
             } else {
-                // First small off-diagonal at or after sc (up to four tested).
-                int hit = -1;
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int k = sc + u;
-                    if (hit < 0 && k < N) {
-                        const bool small = (k >= N - 1) ||
-                                           fabs(es[k][lane]) <= eps * (fabs(ds[k][lane]) + fabs(ds[k + 1 < N ? k + 1 : k][lane]));
-                        if (small) hit = k;
-                    }
-                }
-                if (hit < 0) {
-                    sc += 4;
-                } else if (hit == l) {
-                    ++l;
-                    iter = 0;
-                    sc = l;
-                } else if (iter == QL_MAXIT) {
+                m = __builtin_ctzll(small & (~0ull << l));
+                if (iter == QL_MAXIT) {
                     for (int k = l; k < N - 1; ++k) unconverged += (es[k][lane] != real(0.0)) ? 1 : 0;
                     phase = DONE;
-                } else if (nsw >= cap || nrot + (hit - l) > QR_CAP) {
+                } else if (nsw >= cap || nrot + (m - l) > QR_CAP) {
                     overflow = 1;
                     phase = DONE;
                 } else {
                     ++iter;
-                    m = hit;
                     d_l = ds[l][lane];
                     const real e_l = es[l][lane];
                     g = (ds[l + 1][lane] - d_l) / (real(2.0) * e_l);
                     const real r = sqrt_r(g * g + real(1.0));
                     d_up = ds[m][lane];
+                    d_prev = d_up;
                     g = d_up - d_l + e_l / (g + copysign(r, g));
                     s = real(1.0);
                     c = real(1.0);
@@ -1389,21 +1416,6 @@ geneig_ql_record(const real* __restrict__ W, const real* __restrict__ E, const i
                     ++nsw;
                     phase = SWEEP;
                 }
-            }
-        } else if (phase == SWEEP) {
-            real e_next, d_next;
-            ql_rotation(es[i][lane], ds[i][lane], s, c, g, p, d_up, e_next, d_next);
-            es[i + 1][lane] = e_next;
-            ds[i + 1][lane] = d_next;
-            rot[2 * (nrot + i - l)] = c;
-            rot[2 * (nrot + i - l) + 1] = s;
-            if (--i < l) {
-                ds[l][lane] = d_l - p;
-                es[l][lane] = g;
-                es[m][lane] = real(0.0);
-                nrot += m - l;
-                sc = l;
-                phase = SCAN;
             }
         }
     }
