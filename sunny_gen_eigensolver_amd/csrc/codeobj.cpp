@@ -51,6 +51,18 @@ hipFunction_t LoadedModule::get(const char* name) {
     return f;
 }
 
+hipFunction_t LoadedModule::find(const char* name) {
+    auto it = functions.find(name);
+    if (it != functions.end()) return it->second;
+    hipFunction_t f = nullptr;
+    if (hipModuleGetFunction(&f, module, name) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    functions[name] = f;
+    return f;
+}
+
 namespace {
 
 uint64_t fnv1a(const void* data, size_t len, uint64_t h = 0xcbf29ce484222325ULL) {

@@ -58,6 +58,9 @@ struct LoadedModule {
     std::map<std::string, hipFunction_t> functions;
     ~LoadedModule();
     hipFunction_t get(const char* name);
+    // nullptr where the module has no such kernel (built only for some
+    // orders or precisions).
+    hipFunction_t find(const char* name);
 };
 
 enum class CacheLayer { InProcess, Disk, Compile };

@@ -397,6 +397,136 @@ __device__ __forceinline__ void backsub_lh(ctile (&Ut)[NBLK], FL fL, FV fV, int 
     }
 }
 
+// A per-lane set of N bits in registers (words selected by unrolled
+// compares, so no index is dynamic).
+struct LaneBits {
+    static constexpr int NW = (N + 63) / 64;
+    unsigned long long w[NW];
+    __device__ __forceinline__ void assign(int k, bool v) {
+#pragma unroll
+        for (int q = 0; q < NW; ++q)
+            if (q == (k >> 6)) {
+                const unsigned long long bit = 1ull << (k & 63);
+                w[q] = v ? (w[q] | bit) : (w[q] & ~bit);
+            }
+    }
+    // Lowest k >= from whose bit equals v; NW * 64 if there is none.
+    __device__ __forceinline__ int find(int from, bool v) const {
+        int r = NW * 64;
+#pragma unroll
+        for (int q = NW - 1; q >= 0; --q) {
+            unsigned long long x = v ? w[q] : ~w[q];
+            const int sh = from - 64 * q;
+            if (sh >= 64) x = 0;
+            else if (sh > 0) x &= ~0ull << sh;
+            if (x) r = 64 * q + __builtin_ctzll(x);
+        }
+        return r;
+    }
+};
+
+// Implicit QL recurrence for 64 matrices at once, one per lane of the calling
+// wave, with the rotations logged (the first half of the split QL; method:
+// Bowdler, Martin, Reinsch & Wilkinson, Handbook II/3). d and e of lane l's
+// matrix sit in LDS at ds[i * 64 + l] and es[i * 64 + l], so a lane always
+// hits its own bank whatever index it is at; d is left holding the
+// eigenvalues. The lanes advance as per-lane state machines, every lane by
+// one rotation per iteration; a lane whose sweep ends picks its next window
+// in the same iteration from a bit mask of negligible off-diagonals (no scan
+// of the off-diagonals). Sweep k's window (l, m) goes to sw[2k], sw[2k + 1]
+// and its rotations (c, s) to rot[2 (r + j)], rot[2 (r + j) + 1] (r = the
+// rotations of the sweeps before it, j = i - l). Returns through nsw the
+// sweeps, through unconverged the off-diagonals left after QL_MAXIT sweeps at
+// one l (LAPACK's INFO), and through overflow whether the log's capacity
+// (cap sweeps, rot_cap rotations) was reached; lanes with active false do
+// nothing but take part in the wave's loop.
+__device__ __forceinline__ void ql_lanes_logged(real* ds, real* es, bool active, int cap, int rot_cap, int* sw,
+                                                real* rot, int& nsw, int& unconverged, int& overflow) {
+    const int lane = lane_id();
+    const real eps = REAL_HALF_ULP;
+    // Bit k of small: off-diagonal k is negligible against its diagonals
+    // (|e_k| <= eps (|d_k| + |d_k+1|)); bits N-1 and up are set. A sweep over
+    // [l, m] changes only e_l .. e_m and d_l .. d_m, and each rotation leaves
+    // its e_(i+1), d_(i+1) and d_(i+2) final for the sweep, so the rotations
+    // re-test exactly the bits the sweep can change and the choice of the
+    // next window needs no scan: l moves to the lowest clear bit at or above
+    // it (the deflations), m is the lowest set bit above that.
+    LaneBits small;
+#pragma unroll
+    for (int q = 0; q < LaneBits::NW; ++q) small.w[q] = 0;
+    for (int k = N - 1; k < 64 * LaneBits::NW; ++k) small.assign(k, true);
+    for (int k = 0; k < N - 1; ++k)
+        small.assign(k, fabs(es[k * 64 + lane]) <= eps * (fabs(ds[k * 64 + lane]) + fabs(ds[(k + 1) * 64 + lane])));
+    enum { START = 0, SWEEP = 1, DONE = 2 };
+    int phase = active ? START : DONE;
+    int l = 0, m = 0, i = 0, iter = 0, nrot = 0;
+    nsw = 0;
+    unconverged = 0;
+    overflow = 0;
+    real s = real(1.0), c = real(1.0), p = real(0.0), g = real(0.0), d_up = real(0.0), d_l = real(0.0);
+    real d_prev = real(0.0);                            // d_(i+2) of the current rotation
+    while (__ballot(phase != DONE)) {
+        if (phase == SWEEP) {
+            real e_next, d_next;
+            ql_rotation(es[i * 64 + lane], ds[i * 64 + lane], s, c, g, p, d_up, e_next, d_next);
+            es[(i + 1) * 64 + lane] = e_next;
+            ds[(i + 1) * 64 + lane] = d_next;
+            rot[2 * (nrot + i - l)] = c;
+            rot[2 * (nrot + i - l) + 1] = s;
+            if (i + 1 < m) small.assign(i + 1, fabs(e_next) <= eps * (fabs(d_next) + fabs(d_prev)));
+            d_prev = d_next;
+            if (--i < l) {
+                const real dl_new = d_l - p;
+                ds[l * 64 + lane] = dl_new;
+                es[l * 64 + lane] = g;
+                es[m * 64 + lane] = real(0.0);
+                small.assign(l, fabs(g) <= eps * (fabs(dl_new) + fabs(d_prev)));
+                small.assign(m, true);
+                nrot += m - l;
+                phase = START;
+            }
+        }
+        if (phase == START) {
+            // Deflate: l moves past every negligible off-diagonal at l.
+            const int l_open = small.find(l, false);
+            const int l_new = l_open < N ? l_open : N;
+            if (l_new != l) {
+                l = l_new;
+                iter = 0;
+            }
+            if (l >= N) {
+                phase = DONE;
+            } else {
+                m = small.find(l, true);
+                if (iter == QL_MAXIT) {
+                    for (int k = l; k < N - 1; ++k) unconverged += (es[k * 64 + lane] != real(0.0)) ? 1 : 0;
+                    phase = DONE;
+                } else if (nsw >= cap || nrot + (m - l) > rot_cap) {
+                    overflow = 1;
+                    phase = DONE;
+                } else {
+                    ++iter;
+                    d_l = ds[l * 64 + lane];
+                    const real e_l = es[l * 64 + lane];
+                    g = (ds[(l + 1) * 64 + lane] - d_l) / (real(2.0) * e_l);
+                    const real r = sqrt_r(g * g + real(1.0));
+                    d_up = ds[m * 64 + lane];
+                    d_prev = d_up;
+                    g = d_up - d_l + e_l / (g + copysign(r, g));
+                    s = real(1.0);
+                    c = real(1.0);
+                    p = real(0.0);
+                    i = m - 1;
+                    sw[2 * nsw] = l;
+                    sw[2 * nsw + 1] = m;
+                    ++nsw;
+                    phase = SWEEP;
+                }
+            }
+        }
+    }
+}
+
 }  // namespace gk
 
 // The staged kernels (orders above 64; their Cholesky for every order).
@@ -1339,86 +1469,8 @@ geneig_ql_record(const real* __restrict__ W, const real* __restrict__ E, const i
     int* rk = reinterpret_cast<int*>(lg + QG_RANK);
     real* lam = reinterpret_cast<real*>(lg + QG_LAM);
     real* rot = reinterpret_cast<real*>(lg + QG_ROT);
-    const real eps = REAL_HALF_ULP;
-    // Bit k of small: off-diagonal k is negligible against its diagonals
-    // (|e_k| <= eps (|d_k| + |d_k+1|)); bits N-1 and up are set. A sweep over
-    // [l, m] changes only e_l .. e_m and d_l .. d_m, and each rotation leaves
-    // its e_(i+1), d_(i+1) and d_(i+2) final for the sweep, so the rotations
-    // re-test exactly the bits the sweep can change and the choice of the
-    // next window needs no scan: l moves to the lowest clear bit at or above
-    // it (the deflations), m is the lowest set bit above that.
-    unsigned long long small = ~0ull << (N - 1);
-    for (int k = 0; k < N - 1; ++k)
-        if (fabs(es[k][lane]) <= eps * (fabs(ds[k][lane]) + fabs(ds[k + 1][lane]))) small |= 1ull << k;
-    enum { START = 0, SWEEP = 1, DONE = 2 };
-    int phase = mine ? START : DONE;
-    int l = 0, m = 0, i = 0, iter = 0, nsw = 0, nrot = 0, unconverged = 0, overflow = 0;
-    real s = real(1.0), c = real(1.0), p = real(0.0), g = real(0.0), d_up = real(0.0), d_l = real(0.0);
-    real d_prev = real(0.0);                            // d_(i+2) of the current rotation
-    while (__ballot(phase != DONE)) {
-        if (phase == SWEEP) {
-            real e_next, d_next;
-            ql_rotation(es[i][lane], ds[i][lane], s, c, g, p, d_up, e_next, d_next);
-            es[i + 1][lane] = e_next;
-            ds[i + 1][lane] = d_next;
-            rot[2 * (nrot + i - l)] = c;
-            rot[2 * (nrot + i - l) + 1] = s;
-            if (i + 1 < m) {
-                const unsigned long long bit = 1ull << (i + 1);
-                small = (fabs(e_next) <= eps * (fabs(d_next) + fabs(d_prev))) ? (small | bit) : (small & ~bit);
-            }
-            d_prev = d_next;
-            if (--i < l) {
-                const real dl_new = d_l - p;
-                ds[l][lane] = dl_new;
-                es[l][lane] = g;
-                es[m][lane] = real(0.0);
-                const unsigned long long bit = 1ull << l;
-                small = (fabs(g) <= eps * (fabs(dl_new) + fabs(d_prev))) ? (small | bit) : (small & ~bit);
-                small |= 1ull << m;
-                nrot += m - l;
-                phase = START;
-            }
-        }
-        if (phase == START) {
-            // Deflate: l moves past every negligible off-diagonal at l.
-            const unsigned long long open = ~small & (~0ull << l);
-            const int l_new = open ? __builtin_ctzll(open) : N;
-            if (l_new != l) {
-                l = l_new;
-                iter = 0;
-            }
-            if (l >= N) {
-                phase = DONE;
-            } else {
-                m = __builtin_ctzll(small & (~0ull << l));
-                if (iter == QL_MAXIT) {
-                    for (int k = l; k < N - 1; ++k) unconverged += (es[k][lane] != real(0.0)) ? 1 : 0;
-                    phase = DONE;
-                } else if (nsw >= cap || nrot + (m - l) > QR_CAP) {
-                    overflow = 1;
-                    phase = DONE;
-                } else {
-                    ++iter;
-                    d_l = ds[l][lane];
-                    const real e_l = es[l][lane];
-                    g = (ds[l + 1][lane] - d_l) / (real(2.0) * e_l);
-                    const real r = sqrt_r(g * g + real(1.0));
-                    d_up = ds[m][lane];
-                    d_prev = d_up;
-                    g = d_up - d_l + e_l / (g + copysign(r, g));
-                    s = real(1.0);
-                    c = real(1.0);
-                    p = real(0.0);
-                    i = m - 1;
-                    sw[2 * nsw] = l;
-                    sw[2 * nsw + 1] = m;
-                    ++nsw;
-                    phase = SWEEP;
-                }
-            }
-        }
-    }
+    int nsw, unconverged, overflow;
+    ql_lanes_logged(&ds[0][0], &es[0][0], mine, cap, QR_CAP, sw, rot, nsw, unconverged, overflow);
     if (!mine) return;
     if (!overflow) {
         // Ascending ranks (ties by index) and the eigenvalues.

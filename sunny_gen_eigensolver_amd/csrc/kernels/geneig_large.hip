@@ -1253,6 +1253,63 @@ geneig_lg_qlr(real* __restrict__ wout, long long wout_stride, int* __restrict__ 
     }
 }
 
+// QL, first half, for large batches: 64 matrices per wave, one per lane
+// (ql_lanes_logged), instead of one wave per matrix. The one-wave form spends
+// the whole wave on one scalar recurrence, so its time grows with the batch;
+// here every lane runs its own and the time is about one lane's recurrence,
+// whatever the batch up to the waves the CUs hold at once (measured at
+// N = 96: 5.6 ms in FP64, 4.3 ms in FP32, against 12.3 and 6.0 ms for the
+// one-wave form at 16,384 matrices and 4.2 / 2.1 ms at 4,096). Same inputs,
+// log, outputs and info as geneig_lg_qlr. d and e of the wave's matrices sit
+// in LDS (2 N 64 reals: FP32 at every staged order, FP64 up to order 128).
+#if defined(GENEIG_FP32) || GENEIG_N <= 128
+extern "C" __global__ void __launch_bounds__(64)
+geneig_lg_qlr_lanes(real* __restrict__ wout, long long wout_stride, int* __restrict__ info, char* __restrict__ ws,
+                    long long wss, int batch, int sweep_cap) {
+    using namespace gk;
+    const int cap = sweep_cap < QL_SW_CAP ? sweep_cap : QL_SW_CAP;
+    __shared__ real ds[N * 64];
+    __shared__ real es[N * 64];
+    const int lane = lane_id();
+    const int b = blockIdx.x * 64 + lane;
+    const bool mine = b < batch && info[b] <= N;
+    char* wsb = ws_of(ws, wss, mine ? b : 0);
+    const real* dsrc = reinterpret_cast<const real*>(wsb + WS_D);
+    const real* esrc = dsrc + N;
+    for (int i = 0; i < N; ++i) {
+        ds[i * 64 + lane] = mine ? dsrc[i] : real(0.0);
+        es[i * 64 + lane] = (mine && i < N - 1) ? esrc[i] : real(0.0);
+    }
+    char* qb = wsb + WS_Q;
+    int* hdr = reinterpret_cast<int*>(qb);
+    int* sw = reinterpret_cast<int*>(qb + Q_SW);
+    int* rk = reinterpret_cast<int*>(qb + Q_RANK);
+    real* rot = reinterpret_cast<real*>(qb + Q_ROT);
+    int nsw, unconverged, overflow;
+    ql_lanes_logged(ds, es, mine, cap, QL_ROT_CAP, sw, rot, nsw, unconverged, overflow);
+    if (!mine) return;
+    if (overflow) {
+        hdr[0] = 0;
+        hdr[1] = 1;
+        return;
+    }
+    // Ascending ranks (ties by index) and the eigenvalues.
+    for (int k = 0; k < N; ++k) {
+        const real dk = ds[k * 64 + lane];
+        int rank = 0;
+        for (int j = 0; j < N; ++j) {
+            const real dj = ds[j * 64 + lane];
+            rank += (dj < dk || (dj == dk && j < k)) ? 1 : 0;
+        }
+        rk[k] = rank;
+        wout[(size_t)b * (size_t)wout_stride + rank] = dk;
+    }
+    hdr[0] = nsw;
+    hdr[1] = 0;
+    if (unconverged) info[b] = unconverged;
+}
+#endif
+
 // Rows of Z in registers: every staged order in FP32, up to order 128 in FP64
 // (a row is N registers in FP32, 2N in FP64; above order 104 the compiler
 // keeps part of it in the accumulation registers, one wave per SIMD).

@@ -161,7 +161,10 @@ Solver::Solver(int n, int device_id, int bpb_request, Tuning tuning, CacheMode c
     large_ = n > kMaxLdsOrder;
     if (const char* v = std::getenv("SUNNY_GENEIG_GRAPH")) graphs_ = std::atoi(v) != 0;
     if (const char* v = std::getenv("SUNNY_GENEIG_QL_SWEEP_CAP")) ql_sweep_cap_ = std::max(0, std::atoi(v));
+    ql_log_min_batch_ = ql_log_min_batch_default(mode);
     if (const char* v = std::getenv("SUNNY_GENEIG_QL_LOG_MIN_BATCH")) ql_log_min_batch_ = std::max(0, std::atoi(v));
+    lanes_min_batch_ = mode == Mode::Mixed ? kLanesMinBatch32 : kLanesMinBatch64;
+    if (const char* v = std::getenv("SUNNY_GENEIG_QL_LANES_MIN_BATCH")) lanes_min_batch_ = std::max(0, std::atoi(v));
     // One matrix per wave: a workgroup of k waves solves k matrices, and its
     // LDS work area and register budget allow k <= 4. The reference returns
     // NULL for an explicit request it cannot build (unified_solver.cpp:1021-1028);
@@ -241,6 +244,7 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
             acquire_ms_ += ms32;
             k_lg_hetrd_ = module32_->get("geneig_lg_hetrd");
             k_lg_ql_ = module32_->get("geneig_lg_qlr");
+            k_lg_qll_ = module32_->find("geneig_lg_qlr_lanes");
             k_lg_qlz_ = module32_->get("geneig_lg_qlz");
             k_lg_qlzi_ = module32_->get("geneig_lg_qlz_inline");
             k_lg_back_ = module32_->get("geneig_lg_back");
@@ -251,6 +255,7 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
         } else {
             k_lg_hetrd_ = module_->get("geneig_lg_hetrd");
             k_lg_ql_ = module_->get("geneig_lg_qlr");
+            k_lg_qll_ = module_->find("geneig_lg_qlr_lanes");
             k_lg_qlz_ = module_->get("geneig_lg_qlz");
             k_lg_qlzi_ = module_->get("geneig_lg_qlz_inline");
             k_lg_back_ = module_->get("geneig_lg_back");
@@ -957,7 +962,10 @@ void Solver::launch_large_(const void* H, const void* S, double* W, void* U, int
         }
         int cap = ql_sweep_cap_;
         void* a4[] = {&wout, &wstride, &Ic, &ws, (void*)&wss, &cnt, &cap};
-        launch_(k_lg_ql_, (unsigned)cnt, 64, stream, a4);
+        if (k_lg_qll_ && cnt >= lanes_min_batch_)
+            launch_(k_lg_qll_, (unsigned)((cnt + 63) / 64), 64, stream, a4);
+        else
+            launch_(k_lg_ql_, (unsigned)cnt, 64, stream, a4);
         mark();
         // Matrices whose rotation log overflowed (none, usually: each
         // workgroup returns after one header read), then every other one.
@@ -985,7 +993,7 @@ size_t Solver::workspace_bytes(Mode mode, int n_, size_t batch) {
     if (n_ > kMaxLdsOrder) return std::min(batch, large_chunk(n_)) * large_matrix_bytes(n_);
     const size_t n = (size_t)n_;
     // The QL rotation logs only for batches that run QL through them.
-    const size_t qlog = batch >= (size_t)kQlLogMinBatch ? batch * ql_log_bytes(n_) : 0;
+    const size_t qlog = batch >= (size_t)ql_log_min_batch_default(mode) ? batch * ql_log_bytes(n_) : 0;
     if (mode == Mode::Hybrid) return hybrid_layout(n, batch).total + qlog;
     if (mode == Mode::Mixed) return mixed_layout(n, batch).total + qlog;
     return 0;

@@ -173,6 +173,16 @@ private:
     // for up to large_chunk(n) matrices at a time.
     bool large_ = false;     // staged kernels: order above 64, or staged_
     bool staged_ = false;    // staged kernels chosen at an order up to 64
+    // k_lg_qll_: the lane-parallel QL recorder (nullptr where the order's
+    // d and e do not fit LDS), used from lanes_min_batch_ matrices per launch:
+    // the measured crossovers with the one-wave form at orders 96 and 128
+    // (about 7,000 matrices in FP64, 11,000 in FP32; its time is nearly
+    // constant, the one-wave form's grows with the batch).
+    // SUNNY_GENEIG_QL_LANES_MIN_BATCH overrides it.
+    hipFunction_t k_lg_qll_ = nullptr;
+    static constexpr int kLanesMinBatch64 = 7168;
+    static constexpr int kLanesMinBatch32 = 12288;
+    int lanes_min_batch_ = kLanesMinBatch64;
     hipFunction_t k_lg_chol_ = nullptr, k_lg_formm_ = nullptr, k_lg_hetrd_ = nullptr, k_lg_ql_ = nullptr,
                   k_lg_qlz_ = nullptr, k_lg_qlzi_ = nullptr, k_lg_back_ = nullptr, k_lg_backtri_ = nullptr, k_lg_refine_ = nullptr;
     unsigned lg_back_threads_ = 256;   // workgroup size of geneig_lg_back (its launch bound)
@@ -192,11 +202,16 @@ private:
     int ql_sweep_cap_ = 1 << 30;
     // Batches from this size on run QL through the log (recorder + applier);
     // smaller ones run the no-log kernel for every matrix, since the recorder
-    // costs one wave's whole recurrence whatever the batch (2.7 ms at N = 54).
-    // Measured crossover about 40k matrices at N = 24 and 54;
-    // SUNNY_GENEIG_QL_LOG_MIN_BATCH overrides it.
-    static constexpr int kQlLogMinBatch = 45056;
-    int ql_log_min_batch_ = kQlLogMinBatch;
+    // costs about one lane's whole recurrence whatever the batch (N = 54:
+    // 1.9 ms in FP64, 1.3 ms in FP32). Measured crossovers at orders 24..64:
+    // about 6,000 matrices in FP64 (split mode), 10,000-12,000 in FP32
+    // (mixed mode). SUNNY_GENEIG_QL_LOG_MIN_BATCH overrides it.
+    static constexpr int kQlLogMinBatch64 = 6144;
+    static constexpr int kQlLogMinBatch32 = 12288;
+    static int ql_log_min_batch_default(Mode mode) {
+        return mode == Mode::Mixed ? kQlLogMinBatch32 : kQlLogMinBatch64;
+    }
+    int ql_log_min_batch_ = kQlLogMinBatch64;
     bool last_ql_log_ = true;               // QL path of the last split / mixed launch
 
     // Multi-kernel launches (split, mixed, staged) are captured once per
