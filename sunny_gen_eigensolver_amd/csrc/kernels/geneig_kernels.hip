@@ -1285,17 +1285,14 @@ __device__ __forceinline__ void hetrd_reg(WK& w, cx (&m)[CPW]) {
         const cx t = w.tau[k];
         if (t.x != real(0.0) || t.y != real(0.0)) {
             const cx vl = (lane < N) ? w.u.h.vb[p][lr] : cm(0.0, 0.0);
-            cx vj[CPW];
-#pragma unroll
-            for (int c = 0; c < CPW; ++c) {
-                const int j = wave + NW * c;
-                vj[c] = w.u.h.vb[p][j < N ? j : N - 1];
-            }
+            // v_j is read from LDS (a broadcast) where it is used, here and
+            // in the update below, instead of being held for the whole step:
+            // CPW complex registers fewer (vb[p] stays valid until step k + 2).
             cx acc = cm(0.0, 0.0);
 #pragma unroll
             for (int c = 0; c < CPW; ++c) {
                 const int j = wave + NW * c;
-                const cx prod = cmul(m[c], vj[c]);
+                const cx prod = cmul(m[c], w.u.h.vb[p][j < N ? j : N - 1]);
                 if (j >= m0 && j < N) acc = acc + prod;
             }
             if (lane < N) w.u.h.part[wave * N + lane] = acc;
@@ -1318,14 +1315,21 @@ __device__ __forceinline__ void hetrd_reg(WK& w, cx (&m)[CPW]) {
             for (int c = 0; c < CPW; ++c) {
                 const int j = wave + NW * c;
                 const cx wj = w.u.h.wb[wave][j < N ? j : N - 1];
-                const cx na = m[c] - cmulc(vl, wj) - cmulc(wi, vj[c]);
+                const cx na = m[c] - cmulc(vl, wj) - cmulc(wi, w.u.h.vb[p][j < N ? j : N - 1]);
                 if (j >= m0 && j < N && lane >= m0 && lane < N) m[c] = na;
             }
         }
         if (k + 1 < N - 1 && wave == (k + 1) % NW) {
+            // The owned column k + 1 through one copy of house_reg (selected
+            // and written back by wave-uniform compares; the value barriers
+            // keep the selects from becoming a scratch-backed array index).
+            const int cs = (k + 1) / NW;
+            cx col = m[0];
 #pragma unroll
-            for (int c = 0; c < CPW; ++c)
-                if (wave + NW * c == k + 1) house_reg(w, k + 1, m[c]);
+            for (int c = 1; c < CPW; ++c) col = (c == cs) ? opq(m[c]) : col;
+            house_reg(w, k + 1, col);
+#pragma unroll
+            for (int c = 0; c < CPW; ++c) m[c] = (c == cs) ? opq(col) : m[c];
         }
         __syncthreads();
     }
@@ -1660,15 +1664,30 @@ __device__ __forceinline__ void stash_refl_reg(const WorkF& w, const cx (&m)[CPW
 }
 }  // namespace gk
 
-extern "C" __global__ void __launch_bounds__(GENEIG_NT, 2)
+// FP32: M in full (rounded) in LDS; FP64: its packed lower triangle. Both
+// only until the owning waves hold their columns, so the area is shared with
+// the tridiagonalisation's buffers.
+namespace gk {
+constexpr bool FRONT_MS_FULL = sizeof(real) == 4;
+union FrontLds {
+    WorkF w;
+    cx ms[FRONT_MS_FULL ? N * LDA : NPK];
+};
+// Waves per SIMD the front half is compiled for. FP32: five (66 VGPRs; with
+// the shared LDS area, five workgroups fit a CU; N = 54: 6.5 -> 5.7 ms).
+// FP64: three (154 VGPRs); at four the compiler spills 45 VGPRs and the
+// kernel took 23 instead of 12.6 ms.
+constexpr int FRONT_WAVES_PER_EU = sizeof(real) == 4 ? 5 : 3;
+}  // namespace gk
+
+extern "C" __global__ void __launch_bounds__(GENEIG_NT) __attribute__((amdgpu_waves_per_eu(gk::FRONT_WAVES_PER_EU)))
 geneig_mx_front(const gk::cd* __restrict__ Mp, real* __restrict__ W32, cx* __restrict__ U32,
                 const int* __restrict__ info, real* __restrict__ E32, int batch) {
     using namespace gk;
-    __shared__ WorkF w;
-    // FP32: M in full (rounded); FP64: its packed lower triangle (half the
-    // LDS, so that two workgroups still share a CU).
-    constexpr bool MS_FULL = sizeof(real) == 4;
-    __shared__ cx ms[MS_FULL ? N * LDA : NPK];
+    __shared__ FrontLds lds;
+    WorkF& w = lds.w;
+    cx* ms = lds.ms;
+    constexpr bool MS_FULL = FRONT_MS_FULL;
     const int b = blockIdx.x;
     if (b >= batch || info[b] != 0) return;
     const size_t off = (size_t)b * N * N;
@@ -1705,6 +1724,7 @@ geneig_mx_front(const gk::cd* __restrict__ Mp, real* __restrict__ W32, cx* __res
             m[c] = !in ? cm(real(0.0), real(0.0)) : (lo ? v : cconj(v));
         }
     }
+    __syncthreads();    // ms is read; its area now holds w
     hetrd_reg(w, m);
     stash_refl_reg(w, m, U32 + off);
     if (threadIdx.x < N) {
