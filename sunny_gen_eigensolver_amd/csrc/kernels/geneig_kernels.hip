@@ -1664,19 +1664,19 @@ __device__ __forceinline__ void stash_refl_reg(const WorkF& w, const cx (&m)[CPW
 }
 }  // namespace gk
 
-// FP32: M in full (rounded) in LDS; FP64: its packed lower triangle. Both
+// M's packed lower triangle in LDS (rounded to complex64 in the FP32 build),
 // only until the owning waves hold their columns, so the area is shared with
 // the tridiagonalisation's buffers.
 namespace gk {
-constexpr bool FRONT_MS_FULL = sizeof(real) == 4;
 union FrontLds {
     WorkF w;
-    cx ms[FRONT_MS_FULL ? N * LDA : NPK];
+    cx ms[NPK];
 };
 // Waves per SIMD the front half is compiled for. FP32: five (66 VGPRs; with
-// the shared LDS area, five workgroups fit a CU; N = 54: 6.5 -> 5.7 ms).
-// FP64: three (154 VGPRs); at four the compiler spills 45 VGPRs and the
-// kernel took 23 instead of 12.6 ms.
+// the packed, shared LDS area (12 KB at N = 54) the registers, not the LDS,
+// bound the workgroups per CU: N = 54 6.5 -> 5.2 ms). FP64: three (154
+// VGPRs); at four the compiler spills 45 VGPRs and the kernel took 23
+// instead of 12.6 ms.
 constexpr int FRONT_WAVES_PER_EU = sizeof(real) == 4 ? 5 : 3;
 }  // namespace gk
 
@@ -1687,7 +1687,6 @@ geneig_mx_front(const gk::cd* __restrict__ Mp, real* __restrict__ W32, cx* __res
     __shared__ FrontLds lds;
     WorkF& w = lds.w;
     cx* ms = lds.ms;
-    constexpr bool MS_FULL = FRONT_MS_FULL;
     const int b = blockIdx.x;
     if (b >= batch || info[b] != 0) return;
     const size_t off = (size_t)b * N * N;
@@ -1701,12 +1700,7 @@ geneig_mx_front(const gk::cd* __restrict__ Mp, real* __restrict__ W32, cx* __res
             const int i = j + lane;
             if (i < N) {
                 const cx v = to_cx(Mb[pk(j) + lane]);
-                if constexpr (MS_FULL) {
-                    ms[j * LDA + i] = (i == j) ? cm(v.x, real(0.0)) : v;
-                    if (i != j) ms[i * LDA + j] = cconj(v);
-                } else {
-                    ms[pk(j) + lane] = (i == j) ? cm(v.x, real(0.0)) : v;
-                }
+                ms[pk(j) + lane] = (i == j) ? cm(v.x, real(0.0)) : v;
             }
         }
         __syncthreads();
@@ -1716,13 +1710,9 @@ geneig_mx_front(const gk::cd* __restrict__ Mp, real* __restrict__ W32, cx* __res
     for (int c = 0; c < CPW; ++c) {
         const int j = wave + NW * c;
         const bool in = lane < N && j < N;
-        if constexpr (MS_FULL) {
-            m[c] = in ? ms[j * LDA + lane] : cm(real(0.0), real(0.0));
-        } else {
-            const bool lo = lane >= j;
-            const cx v = ms[in ? (lo ? pk(j) + lane - j : pk(lane) + j - lane) : 0];
-            m[c] = !in ? cm(real(0.0), real(0.0)) : (lo ? v : cconj(v));
-        }
+        const bool lo = lane >= j;
+        const cx v = ms[in ? (lo ? pk(j) + lane - j : pk(lane) + j - lane) : 0];
+        m[c] = !in ? cm(real(0.0), real(0.0)) : (lo ? v : cconj(v));
     }
     __syncthreads();    // ms is read; its area now holds w
     hetrd_reg(w, m);
