@@ -175,6 +175,17 @@ __device__ __forceinline__ double rsqrt_nr(double t) {
 }
 #endif
 
+// 1/x for the QL shift: v_rcp_f32 (about one ulp); in FP64 v_rcp_f64 and one
+// Newton step.
+#ifdef GENEIG_FP32
+__device__ __forceinline__ float shift_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+#else
+__device__ __forceinline__ double shift_rcp(double x) {
+    const double y = __builtin_amdgcn_rcp(x);
+    return fma(fma(-x, y, 1.0), y, y);
+}
+#endif
+
 // One rotation of an implicit QL sweep (Handbook II/3, tql2's inner loop)
 // on the pair (d_i, e_i) with the sweep's running (s, c, g, p) and d_up (the
 // original d[i+1]); returns the new e[i+1] and d[i+1] through e_next and
@@ -425,12 +436,15 @@ struct LaneBits {
     }
 };
 
-// Implicit QL recurrence for 64 matrices at once, one per lane of the calling
-// wave, with the rotations logged (the first half of the split QL; method:
-// Bowdler, Martin, Reinsch & Wilkinson, Handbook II/3). d and e of lane l's
-// matrix sit in LDS at ds[i * 64 + l] and es[i * 64 + l], so a lane always
-// hits its own bank whatever index it is at; d is left holding the
-// eigenvalues. The lanes advance as per-lane state machines, every lane by
+// Implicit QL recurrence for LANES matrices at once, one per lane of the
+// calling wave (lanes LANES and up pass active = false), with the rotations
+// logged (the first half of the split QL; method: Bowdler, Martin, Reinsch &
+// Wilkinson, Handbook II/3). d and e of lane l's matrix sit in LDS at
+// ds[i * LANES + l] and es[i * LANES + l]: two lanes at any indices never
+// share a bank (LANES a power of two up to 64); d is left holding the
+// eigenvalues. The recurrence is one dependent chain per lane, so the wave's
+// speed is its latency; fewer matrices per wave (more waves, each with less
+// LDS) let the SIMDs interleave waves. The lanes advance as per-lane state machines, every lane by
 // one rotation per iteration; a lane whose sweep ends picks its next window
 // in the same iteration from a bit mask of negligible off-diagonals (no scan
 // of the off-diagonals). Sweep k's window (l, m) goes to sw[2k], sw[2k + 1]
@@ -440,9 +454,11 @@ struct LaneBits {
 // one l (LAPACK's INFO), and through overflow whether the log's capacity
 // (cap sweeps, rot_cap rotations) was reached; lanes with active false do
 // nothing but take part in the wave's loop.
+template <int LANES>
 __device__ __forceinline__ void ql_lanes_logged(real* ds, real* es, bool active, int cap, int rot_cap, int* sw,
                                                 real* rot, int& nsw, int& unconverged, int& overflow) {
-    const int lane = lane_id();
+    static_assert(LANES >= 1 && LANES <= 64 && (LANES & (LANES - 1)) == 0, "lanes: a power of two up to 64");
+    const int lane = lane_id() & (LANES - 1);
     const real eps = REAL_HALF_ULP;
     // Bit k of small: off-diagonal k is negligible against its diagonals
     // (|e_k| <= eps (|d_k| + |d_k+1|)); bits N-1 and up are set. A sweep over
@@ -456,7 +472,7 @@ __device__ __forceinline__ void ql_lanes_logged(real* ds, real* es, bool active,
     for (int q = 0; q < LaneBits::NW; ++q) small.w[q] = 0;
     for (int k = N - 1; k < 64 * LaneBits::NW; ++k) small.assign(k, true);
     for (int k = 0; k < N - 1; ++k)
-        small.assign(k, fabs(es[k * 64 + lane]) <= eps * (fabs(ds[k * 64 + lane]) + fabs(ds[(k + 1) * 64 + lane])));
+        small.assign(k, fabs(es[k * LANES + lane]) <= eps * (fabs(ds[k * LANES + lane]) + fabs(ds[(k + 1) * LANES + lane])));
     enum { START = 0, SWEEP = 1, DONE = 2 };
     int phase = active ? START : DONE;
     int l = 0, m = 0, i = 0, iter = 0, nrot = 0;
@@ -465,28 +481,46 @@ __device__ __forceinline__ void ql_lanes_logged(real* ds, real* es, bool active,
     overflow = 0;
     real s = real(1.0), c = real(1.0), p = real(0.0), g = real(0.0), d_up = real(0.0), d_l = real(0.0);
     real d_prev = real(0.0);                            // d_(i+2) of the current rotation
+    real e_cur = real(0.0), d_cur = real(0.0);          // (e_i, d_i), fetched ahead
+    int step = 0;
     while (__ballot(phase != DONE)) {
+        // Lanes between sweeps are served together, on every fourth
+        // iteration (or at once when no lane is sweeping): the wave pays for
+        // the window choice and shift a quarter as often, a lane waits at
+        // most three iterations per sweep.
+        const bool serve = (step++ & 7) == 0 || __ballot(phase == SWEEP) == 0;
         if (phase == SWEEP) {
-            real e_next, d_next;
-            ql_rotation(es[i * 64 + lane], ds[i * 64 + lane], s, c, g, p, d_up, e_next, d_next);
-            es[(i + 1) * 64 + lane] = e_next;
-            ds[(i + 1) * 64 + lane] = d_next;
+            real e_next, d_next, d_i = d_up;
+            ql_rotation(e_cur, d_cur, s, c, g, p, d_i, e_next, d_next);
+            es[(i + 1) * LANES + lane] = e_next;
+            ds[(i + 1) * LANES + lane] = d_next;
             rot[2 * (nrot + i - l)] = c;
             rot[2 * (nrot + i - l) + 1] = s;
             if (i + 1 < m) small.assign(i + 1, fabs(e_next) <= eps * (fabs(d_next) + fabs(d_prev)));
             d_prev = d_next;
+            // The next rotation's operands, fetched now so that their latency
+            // passes during the rest of the iteration: e_(i-1), d_(i-1) and
+            // d_up = the original d_i (this sweep writes d_i only in its
+            // next rotation). Unconditional, and after this rotation's last
+            // use of the current ones (the scheduling barrier), so the loads
+            // land in e_cur / d_cur / d_up themselves.
+            __builtin_amdgcn_sched_barrier(0);
+            const int ip = i > 0 ? i - 1 : 0;
+            e_cur = es[ip * LANES + lane];
+            d_cur = ds[ip * LANES + lane];
+            d_up = ds[i * LANES + lane];
             if (--i < l) {
                 const real dl_new = d_l - p;
-                ds[l * 64 + lane] = dl_new;
-                es[l * 64 + lane] = g;
-                es[m * 64 + lane] = real(0.0);
+                ds[l * LANES + lane] = dl_new;
+                es[l * LANES + lane] = g;
+                es[m * LANES + lane] = real(0.0);
                 small.assign(l, fabs(g) <= eps * (fabs(dl_new) + fabs(d_prev)));
                 small.assign(m, true);
                 nrot += m - l;
                 phase = START;
             }
         }
-        if (phase == START) {
+        if (phase == START && serve) {
             // Deflate: l moves past every negligible off-diagonal at l.
             const int l_open = small.find(l, false);
             const int l_new = l_open < N ? l_open : N;
@@ -499,20 +533,29 @@ __device__ __forceinline__ void ql_lanes_logged(real* ds, real* es, bool active,
             } else {
                 m = small.find(l, true);
                 if (iter == QL_MAXIT) {
-                    for (int k = l; k < N - 1; ++k) unconverged += (es[k * 64 + lane] != real(0.0)) ? 1 : 0;
+                    for (int k = l; k < N - 1; ++k) unconverged += (es[k * LANES + lane] != real(0.0)) ? 1 : 0;
                     phase = DONE;
                 } else if (nsw >= cap || nrot + (m - l) > rot_cap) {
                     overflow = 1;
                     phase = DONE;
                 } else {
                     ++iter;
-                    d_l = ds[l * 64 + lane];
-                    const real e_l = es[l * 64 + lane];
-                    g = (ds[(l + 1) * 64 + lane] - d_l) / (real(2.0) * e_l);
-                    const real r = sqrt_r(g * g + real(1.0));
-                    d_up = ds[m * 64 + lane];
+                    d_l = ds[l * LANES + lane];
+                    const real e_l = es[l * LANES + lane];
+                    // The shift (Wilkinson's, from the leading 2 x 2 block)
+                    // through reciprocals instead of correctly rounded
+                    // divisions and square root: it only steers convergence,
+                    // and every rotation stays exactly orthogonal whatever
+                    // its value. e_l is not negligible here, so |g| < 1 / (2
+                    // eps) and g^2 + 1 stays finite.
+                    g = (ds[(l + 1) * LANES + lane] - d_l) * shift_rcp(real(2.0) * e_l);
+                    const real t = fma(g, g, real(1.0));
+                    const real r = t * rsqrt_nr(t);
+                    d_up = ds[m * LANES + lane];
+                    e_cur = es[(m - 1) * LANES + lane];
+                    d_cur = ds[(m - 1) * LANES + lane];
                     d_prev = d_up;
-                    g = d_up - d_l + e_l / (g + copysign(r, g));
+                    g = d_up - d_l + e_l * shift_rcp(g + copysign(r, g));
                     s = real(1.0);
                     c = real(1.0);
                     p = real(0.0);
@@ -1442,6 +1485,11 @@ geneig_fused(const cx* __restrict__ H, const cx* __restrict__ S, real* __restric
 // ---------------------------------------------------------------------------
 namespace gk {
 constexpr int QS_CAP = 8 * N;                       // sweeps per matrix
+// Recorder matrices per wave: 64. At these orders a large batch keeps every
+// SIMD busy with one such wave; 16 per wave (four waves per SIMD) was slower
+// (N = 54: FP32 recorder 1.66 -> 1.94 ms): each wave issues the whole
+// per-iteration instruction stream for a quarter of the matrices.
+constexpr int QL_REC_LANES = 64;
 constexpr int QR_CAP = 2 * N * N;                   // rotations per matrix
 constexpr size_t QG_SW = 16;                        // int[2 QS_CAP]: (l, m) per sweep
 constexpr size_t QG_RANK = QG_SW + 8 * (size_t)QS_CAP;                     // int[N]
@@ -1453,19 +1501,25 @@ constexpr size_t QG_BYTES = (QG_ROT + 16 * (size_t)QR_CAP + 255) / 256 * 256;
 // Bytes of one matrix's QL log, for the host's consistency check.
 extern "C" __device__ const unsigned long long geneig_ql_log_bytes = gk::QG_BYTES;
 
+// Matrices per wave of the recorder (one per lane; the other lanes idle).
+extern "C" __device__ const int geneig_ql_record_lanes = gk::QL_REC_LANES;
+
 extern "C" __global__ void __launch_bounds__(64)
 geneig_ql_record(const real* __restrict__ W, const real* __restrict__ E, const int* __restrict__ info,
                  char* __restrict__ qlog, long long log_stride, int batch, int sweep_cap) {
     using namespace gk;
+    constexpr int LN = QL_REC_LANES;
     const int cap = sweep_cap < QS_CAP ? sweep_cap : QS_CAP;
-    __shared__ real ds[N][64];
-    __shared__ real es[N][64];
+    __shared__ real ds[N][LN];
+    __shared__ real es[N][LN];
     const int lane = lane_id();
-    const int b = blockIdx.x * 64 + lane;
-    const bool mine = b < batch && info[b] == 0;
-    for (int i = 0; i < N; ++i) {
-        ds[i][lane] = mine ? W[(size_t)b * N + i] : real(0.0);
-        es[i][lane] = (mine && i < N - 1) ? E[(size_t)b * N + i] : real(0.0);
+    const int b = blockIdx.x * LN + lane;
+    const bool mine = lane < LN && b < batch && info[b] == 0;
+    if (lane < LN) {
+        for (int i = 0; i < N; ++i) {
+            ds[i][lane] = mine ? W[(size_t)b * N + i] : real(0.0);
+            es[i][lane] = (mine && i < N - 1) ? E[(size_t)b * N + i] : real(0.0);
+        }
     }
     char* lg = qlog + (size_t)(mine ? b : 0) * (size_t)log_stride;
     int* hdr = reinterpret_cast<int*>(lg);
@@ -1474,7 +1528,7 @@ geneig_ql_record(const real* __restrict__ W, const real* __restrict__ E, const i
     real* lam = reinterpret_cast<real*>(lg + QG_LAM);
     real* rot = reinterpret_cast<real*>(lg + QG_ROT);
     int nsw, unconverged, overflow;
-    ql_lanes_logged(&ds[0][0], &es[0][0], mine, cap, QR_CAP, sw, rot, nsw, unconverged, overflow);
+    ql_lanes_logged<LN>(&ds[0][0], &es[0][0], mine, cap, QR_CAP, sw, rot, nsw, unconverged, overflow);
     if (!mine) return;
     if (!overflow) {
         // Ascending ranks (ties by index) and the eigenvalues.

@@ -1258,27 +1258,40 @@ geneig_lg_qlr(real* __restrict__ wout, long long wout_stride, int* __restrict__ 
 // the whole wave on one scalar recurrence, so its time grows with the batch;
 // here every lane runs its own and the time is about one lane's recurrence,
 // whatever the batch up to the waves the CUs hold at once (measured at
-// N = 96: 5.6 ms in FP64, 4.3 ms in FP32, against 12.3 and 6.0 ms for the
-// one-wave form at 16,384 matrices and 4.2 / 2.1 ms at 4,096). Same inputs,
-// log, outputs and info as geneig_lg_qlr. d and e of the wave's matrices sit
-// in LDS (2 N 64 reals: FP32 at every staged order, FP64 up to order 128).
-#if defined(GENEIG_FP32) || GENEIG_N <= 128
+// N = 96: 5.3 ms in FP64, 3.7 ms in FP32, against 12.2 and 6.0 ms for the
+// one-wave form at 16,384 matrices, and 3.9 / 3.1 ms against 4.2 / 2.1 ms at
+// 4,096). Same inputs, log, outputs and info as geneig_lg_qlr. d and e of
+// the wave's matrices sit in LDS (2 N LG_QLR_LANES reals), orders up to 128.
+#if GENEIG_N <= 128
+namespace gk {
+// Matrices per wave (one per lane; the other lanes idle): 16, so that a
+// batch of a few thousand matrices still spreads over every SIMD (N = 96,
+// 1,024 matrices: 4.1 ms against 6.0 ms with 64 per wave). Above order 128
+// the batches are too small for either form to beat one wave per matrix
+// (N = 192 FP32, 2,048 matrices: 17.4 ms with 4 per wave against 5.9 ms).
+constexpr int LG_QLR_LANES = 16;
+}  // namespace gk
+extern "C" __device__ const int geneig_lg_qlr_lanes_count = gk::LG_QLR_LANES;
+
 extern "C" __global__ void __launch_bounds__(64)
 geneig_lg_qlr_lanes(real* __restrict__ wout, long long wout_stride, int* __restrict__ info, char* __restrict__ ws,
                     long long wss, int batch, int sweep_cap) {
     using namespace gk;
     const int cap = sweep_cap < QL_SW_CAP ? sweep_cap : QL_SW_CAP;
-    __shared__ real ds[N * 64];
-    __shared__ real es[N * 64];
+    constexpr int LN = LG_QLR_LANES;
+    __shared__ real ds[N * LN];
+    __shared__ real es[N * LN];
     const int lane = lane_id();
-    const int b = blockIdx.x * 64 + lane;
-    const bool mine = b < batch && info[b] <= N;
+    const int b = blockIdx.x * LN + lane;
+    const bool mine = lane < LN && b < batch && info[b] <= N;
     char* wsb = ws_of(ws, wss, mine ? b : 0);
     const real* dsrc = reinterpret_cast<const real*>(wsb + WS_D);
     const real* esrc = dsrc + N;
-    for (int i = 0; i < N; ++i) {
-        ds[i * 64 + lane] = mine ? dsrc[i] : real(0.0);
-        es[i * 64 + lane] = (mine && i < N - 1) ? esrc[i] : real(0.0);
+    if (lane < LN) {
+        for (int i = 0; i < N; ++i) {
+            ds[i * LN + lane] = mine ? dsrc[i] : real(0.0);
+            es[i * LN + lane] = (mine && i < N - 1) ? esrc[i] : real(0.0);
+        }
     }
     char* qb = wsb + WS_Q;
     int* hdr = reinterpret_cast<int*>(qb);
@@ -1286,7 +1299,7 @@ geneig_lg_qlr_lanes(real* __restrict__ wout, long long wout_stride, int* __restr
     int* rk = reinterpret_cast<int*>(qb + Q_RANK);
     real* rot = reinterpret_cast<real*>(qb + Q_ROT);
     int nsw, unconverged, overflow;
-    ql_lanes_logged(ds, es, mine, cap, QL_ROT_CAP, sw, rot, nsw, unconverged, overflow);
+    ql_lanes_logged<LN>(ds, es, mine, cap, QL_ROT_CAP, sw, rot, nsw, unconverged, overflow);
     if (!mine) return;
     if (overflow) {
         hdr[0] = 0;
@@ -1295,10 +1308,10 @@ geneig_lg_qlr_lanes(real* __restrict__ wout, long long wout_stride, int* __restr
     }
     // Ascending ranks (ties by index) and the eigenvalues.
     for (int k = 0; k < N; ++k) {
-        const real dk = ds[k * 64 + lane];
+        const real dk = ds[k * LN + lane];
         int rank = 0;
         for (int j = 0; j < N; ++j) {
-            const real dj = ds[j * 64 + lane];
+            const real dj = ds[j * LN + lane];
             rank += (dj < dk || (dj == dk && j < k)) ? 1 : 0;
         }
         rk[k] = rank;

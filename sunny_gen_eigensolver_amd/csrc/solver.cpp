@@ -152,6 +152,19 @@ MixedLayout mixed_layout(size_t n, size_t B) {
 }
 }  // namespace
 
+namespace {
+// An int the module states (a kernel's launch shape).
+int module_int(LoadedModule& m, const char* name) {
+    hipDeviceptr_t sym = nullptr;
+    size_t bytes = 0;
+    int v = 0;
+    SG_HIP(hipModuleGetGlobal(&sym, &bytes, m.module, name));
+    SG_HIP(hipMemcpyDtoH(&v, sym, sizeof(v)));
+    if (v <= 0 || v > 1024) throw std::runtime_error(std::string("module states ") + name + " = " + std::to_string(v));
+    return v;
+}
+}  // namespace
+
 Solver::Solver(int n, int device_id, int bpb_request, Tuning tuning, CacheMode cache, Mode mode)
     : n_(n), device_(device_id), mode_(mode) {
     if (n < 1) throw std::invalid_argument("matrix size must be >= 1");
@@ -245,6 +258,7 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
             k_lg_hetrd_ = module32_->get("geneig_lg_hetrd");
             k_lg_ql_ = module32_->get("geneig_lg_qlr");
             k_lg_qll_ = module32_->find("geneig_lg_qlr_lanes");
+            if (k_lg_qll_) lg_qll_lanes_ = module_int(*module32_, "geneig_lg_qlr_lanes_count");
             k_lg_qlz_ = module32_->get("geneig_lg_qlz");
             k_lg_qlzi_ = module32_->get("geneig_lg_qlz_inline");
             k_lg_back_ = module32_->get("geneig_lg_back");
@@ -256,6 +270,7 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
             k_lg_hetrd_ = module_->get("geneig_lg_hetrd");
             k_lg_ql_ = module_->get("geneig_lg_qlr");
             k_lg_qll_ = module_->find("geneig_lg_qlr_lanes");
+            if (k_lg_qll_) lg_qll_lanes_ = module_int(*module_, "geneig_lg_qlr_lanes_count");
             k_lg_qlz_ = module_->get("geneig_lg_qlz");
             k_lg_qlzi_ = module_->get("geneig_lg_qlz_inline");
             k_lg_back_ = module_->get("geneig_lg_back");
@@ -293,6 +308,7 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
     k_heev_ = module_->get("geneig_split_ql");
     k_qlfull_ = module_->get("geneig_ql_full");
     k_qlrec_ = module_->get("geneig_ql_record");
+    ql_rec_lanes_ = module_int(*module_, "geneig_ql_record_lanes");
     {
         hipDeviceptr_t sym = nullptr;
         size_t sym_bytes = 0;
@@ -333,6 +349,7 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
         k_ql32_ = module32_->get("geneig_split_ql");
         k_qlfull32_ = module32_->get("geneig_ql_full");
         k_qlrec_ = module32_->get("geneig_ql_record");
+        ql_rec_lanes_ = module_int(*module32_, "geneig_ql_record_lanes");
         k_back32_ = module32_->get("geneig_mx_back");
         if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_ITERS")) refine_iters_ = std::max(0, std::atoi(v));
         if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_DELTA")) refine_delta_ = std::atof(v);
@@ -750,7 +767,7 @@ void Solver::launch_hybrid_(const void* H, const void* S, double* W, void* U, in
     last_ql_log_ = !all;
     if (!all) {
         void* ar[] = {&W, &E, &info, &qlog, &qstride, &batch, &cap};
-        launch_(k_qlrec_, (unsigned)((batch + 63) / 64), 64, stream, ar);
+        launch_(k_qlrec_, (unsigned)((batch + ql_rec_lanes_ - 1) / ql_rec_lanes_), 64, stream, ar);
         mark_(stream);
         void* a3[] = {&W, &E, &Zw, &info, &qlog, &qstride, &batch};
         launch_(k_heev_, grid_w, bd, stream, a3);
@@ -834,7 +851,7 @@ void Solver::launch_mixed_(const void* H, const void* S, double* W, void* U, int
     last_ql_log_ = !all;
     if (!all) {
         void* ar[] = {&W32, &E32, &info, &qlog, &qstride, &batch, &cap};
-        launch_(k_qlrec_, (unsigned)((batch + 63) / 64), 64, stream, ar);
+        launch_(k_qlrec_, (unsigned)((batch + ql_rec_lanes_ - 1) / ql_rec_lanes_), 64, stream, ar);
         mark_(stream);
         void* a3[] = {&W32, &E32, &Z32, &info, &qlog, &qstride, &batch};
         launch_(k_ql32_, grid_w, bd, stream, a3);
@@ -963,7 +980,7 @@ void Solver::launch_large_(const void* H, const void* S, double* W, void* U, int
         int cap = ql_sweep_cap_;
         void* a4[] = {&wout, &wstride, &Ic, &ws, (void*)&wss, &cnt, &cap};
         if (k_lg_qll_ && cnt >= lanes_min_batch_)
-            launch_(k_lg_qll_, (unsigned)((cnt + 63) / 64), 64, stream, a4);
+            launch_(k_lg_qll_, (unsigned)((cnt + lg_qll_lanes_ - 1) / lg_qll_lanes_), 64, stream, a4);
         else
             launch_(k_lg_ql_, (unsigned)cnt, 64, stream, a4);
         mark();

@@ -173,15 +173,17 @@ private:
     // for up to large_chunk(n) matrices at a time.
     bool large_ = false;     // staged kernels: order above 64, or staged_
     bool staged_ = false;    // staged kernels chosen at an order up to 64
-    // k_lg_qll_: the lane-parallel QL recorder (nullptr where the order's
-    // d and e do not fit LDS), used from lanes_min_batch_ matrices per launch:
-    // the measured crossovers with the one-wave form at orders 96 and 128
-    // (about 7,000 matrices in FP64, 11,000 in FP32; its time is nearly
-    // constant, the one-wave form's grows with the batch).
-    // SUNNY_GENEIG_QL_LANES_MIN_BATCH overrides it.
+    // k_lg_qll_: the lane-parallel QL recorder (orders up to 128; nullptr
+    // above), used from lanes_min_batch_ matrices per launch: the measured
+    // crossovers with the one-wave form at orders 96 and 128 (about 3,500
+    // matrices in FP64, 9,000 in FP32; its time is nearly constant, the
+    // one-wave form's grows with the batch). SUNNY_GENEIG_QL_LANES_MIN_BATCH
+    // overrides it.
     hipFunction_t k_lg_qll_ = nullptr;
-    static constexpr int kLanesMinBatch64 = 7168;
-    static constexpr int kLanesMinBatch32 = 12288;
+    int lg_qll_lanes_ = 64;       // its matrices per wave (the module states it)
+    int ql_rec_lanes_ = 64;       // geneig_ql_record's matrices per wave
+    static constexpr int kLanesMinBatch64 = 3584;
+    static constexpr int kLanesMinBatch32 = 9216;
     int lanes_min_batch_ = kLanesMinBatch64;
     hipFunction_t k_lg_chol_ = nullptr, k_lg_formm_ = nullptr, k_lg_hetrd_ = nullptr, k_lg_ql_ = nullptr,
                   k_lg_qlz_ = nullptr, k_lg_qlzi_ = nullptr, k_lg_back_ = nullptr, k_lg_backtri_ = nullptr, k_lg_refine_ = nullptr;

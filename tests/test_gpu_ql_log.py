@@ -13,13 +13,12 @@ matrix down the no-log path, a cap of about the typical sweep count splits a
 batch between the two. Results are checked against zhegv with the usual
 bars.
 
-At the staged orders the recurrence runs one wave per matrix below
-SUNNY_GENEIG_QL_LANES_MIN_BATCH matrices per launch and 64 matrices per wave
-(one per lane) from there, where the order's d and e fit LDS (FP32 at every
-staged order, FP64 up to 128); here the minimum is set to 0 to force the lane
-form on small batches, with partial last waves, non-HPD pencils beside HPD
-ones and the sweep cap splitting the batch between the log and the inline
-path."""
+At the staged orders up to 128 the recurrence runs one wave per matrix below
+SUNNY_GENEIG_QL_LANES_MIN_BATCH matrices per launch and 16 matrices per wave
+(one per lane) from there; here the minimum is set to 0 to force the lane
+form on small batches (order 150 keeps the one-wave form), with partial last
+waves, non-HPD pencils beside HPD ones and the sweep cap splitting the batch
+between the log and the inline path."""
 import numpy as np
 import pytest
 
