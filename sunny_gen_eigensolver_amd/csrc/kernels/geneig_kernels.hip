@@ -87,6 +87,13 @@ __device__ __forceinline__ cx cconjmul(cx a, cx b) {
     return cm(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
 }
 __device__ __forceinline__ cx cconj(cx a) { return cm(a.x, -a.y); }
+// acc + a * b and m - a * conj(b) as four FMAs each.
+__device__ __forceinline__ cx cfma(cx a, cx b, cx acc) {
+    return cm(fma(a.x, b.x, fma(-a.y, b.y, acc.x)), fma(a.x, b.y, fma(a.y, b.x, acc.y)));
+}
+__device__ __forceinline__ cx cfms_conj(cx a, cx b, cx m) {
+    return cm(fma(-a.x, b.x, fma(-a.y, b.y, m.x)), fma(-a.y, b.x, fma(a.x, b.y, m.y)));
+}
 
 // Complex numbers as stored in global memory by the mixed-precision path:
 // complex128 inputs and outputs, complex64 hand-offs between its stages.
@@ -1327,16 +1334,18 @@ __device__ __forceinline__ void hetrd_reg(WK& w, cx (&m)[CPW]) {
         const int p = k & 1;
         const cx t = w.tau[k];
         if (t.x != real(0.0) || t.y != real(0.0)) {
+            // v is zero above row m0 and outside N, so the rows above the
+            // active block add nothing and (with w zeroed there) take no
+            // update; finished and padding columns are skipped by wave-uniform
+            // branches. v_j is read from LDS (a broadcast) where it is used,
+            // here and in the update below, instead of being held for the
+            // whole step (vb[p] stays valid until step k + 2).
             const cx vl = (lane < N) ? w.u.h.vb[p][lr] : cm(0.0, 0.0);
-            // v_j is read from LDS (a broadcast) where it is used, here and
-            // in the update below, instead of being held for the whole step:
-            // CPW complex registers fewer (vb[p] stays valid until step k + 2).
             cx acc = cm(0.0, 0.0);
 #pragma unroll
             for (int c = 0; c < CPW; ++c) {
                 const int j = wave + NW * c;
-                const cx prod = cmul(m[c], w.u.h.vb[p][j < N ? j : N - 1]);
-                if (j >= m0 && j < N) acc = acc + prod;
+                if (j >= m0 && j < N) acc = cfma(m[c], w.u.h.vb[p][j], acc);
             }
             if (lane < N) w.u.h.part[wave * N + lane] = acc;
             __syncthreads();
@@ -1349,17 +1358,17 @@ __device__ __forceinline__ void hetrd_reg(WK& w, cx (&m)[CPW]) {
             const cx pvl = cconjmul(pi, vl);
             const cx pv = cm(wave_sum(pvl.x), wave_sum(pvl.y));
             const cx cc = cmul(t, pv) * -0.5;
-            const cx wi = pi + cmul(cc, vl);
-            if (lane < N) w.u.h.wb[wave][lane] = wi;
+            const cx wall = pi + cmul(cc, vl);
+            if (lane < N) w.u.h.wb[wave][lane] = wall;
+            const cx wi = (lane >= m0 && lane < N) ? wall : cm(0.0, 0.0);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
             for (int c = 0; c < CPW; ++c) {
                 const int j = wave + NW * c;
-                const cx wj = w.u.h.wb[wave][j < N ? j : N - 1];
-                const cx na = m[c] - cmulc(vl, wj) - cmulc(wi, w.u.h.vb[p][j < N ? j : N - 1]);
-                if (j >= m0 && j < N && lane >= m0 && lane < N) m[c] = na;
+                if (j >= m0 && j < N)
+                    m[c] = cfms_conj(wi, w.u.h.vb[p][j], cfms_conj(vl, w.u.h.wb[wave][j], m[c]));
             }
         }
         if (k + 1 < N - 1 && wave == (k + 1) % NW) {
@@ -1726,12 +1735,11 @@ union FrontLds {
     WorkF w;
     cx ms[NPK];
 };
-// Waves per SIMD the front half is compiled for. FP32: five (66 VGPRs; with
-// the packed, shared LDS area (12 KB at N = 54) the registers, not the LDS,
-// bound the workgroups per CU: N = 54 6.5 -> 5.2 ms). FP64: three (154
-// VGPRs); at four the compiler spills 45 VGPRs and the kernel took 23
-// instead of 12.6 ms.
-constexpr int FRONT_WAVES_PER_EU = sizeof(real) == 4 ? 5 : 3;
+// Waves per SIMD the front half is compiled for: five in FP32 (67 VGPRs at
+// N = 54; with the packed, shared LDS area (12 KB) the registers, not the
+// LDS, bound the workgroups per CU), four in FP64 (114 VGPRs; before the
+// FMA-form updates and column skips it needed 154, three waves per SIMD).
+constexpr int FRONT_WAVES_PER_EU = sizeof(real) == 4 ? 5 : 4;
 }  // namespace gk
 
 extern "C" __global__ void __launch_bounds__(GENEIG_NT) __attribute__((amdgpu_waves_per_eu(gk::FRONT_WAVES_PER_EU)))

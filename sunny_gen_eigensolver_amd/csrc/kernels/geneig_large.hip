@@ -682,6 +682,7 @@ constexpr int YLD = 17;
 struct CholWork {
     cx pan[(16 * PLD > YLD * 16 * LNB) ? 16 * PLD : YLD * 16 * LNB];   // panel; trtri: Y, then X
     cx lii[256];              // trtri: diagonal block of L
+    real liinv[16];           // trtri: 1 / L(r, r) of that block
     int fail;                 // pivot failure of the panel (orders up to 64)
 };
 #endif  // !GENEIG_FP32
@@ -779,8 +780,10 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
                         if (!(piv > real(0.0))) {
                             fail = j0 + c + 1;
                         } else {
-                            const real sq = sqrt_r(piv);
-                            const real ri = real(1.0) / sq;
+                            // 1 / sqrt(piv) to about an ulp (v_rsq_f64 and a
+                            // third-order step), sqrt(piv) = piv / sqrt(piv).
+                            const real ri = rsqrt_nr(piv);
+                            const real sq = piv * ri;
                             const cx lc = (t == c) ? cm(sq, 0.0) : r[c] * ri;
                             r[c] = lc;
 #pragma unroll
@@ -857,6 +860,7 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
             const int rr = e & 15, c = e >> 4;
             cw.lii[e] = (rr < rw && c < rw) ? fL(r0 + rr, r0 + c) : cm(0.0, 0.0);
         }
+        if (tid < rw) cw.liinv[tid] = real(1.0) / fL(r0 + tid, r0 + tid).x;
         for (int ct = wave; ct < ib; ct += LNW) {
             ctile acc[1];
             gemm_rg<1>(acc, fL, fL, ib, ct, 16 * ct, r0);
@@ -876,7 +880,7 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
             for (int rr = 0; rr < rw; ++rr) {
                 cx s = (j < r0) ? cm(0.0, 0.0) - xj[rr] : cm((rr == j - r0) ? 1.0 : 0.0, 0.0);
                 for (int ss = 0; ss < rr; ++ss) s = s - cmul(cw.lii[ss * 16 + rr], xj[ss]);
-                xj[rr] = s * (real(1.0) / cw.lii[rr * 16 + rr].x);
+                xj[rr] = s * cw.liinv[rr];
             }
             for (int rr = 0; rr < rw; ++rr)
                 if (CHOL_LDS ? r0 + rr >= j : true) setL(r0 + rr, j, xj[rr]);
