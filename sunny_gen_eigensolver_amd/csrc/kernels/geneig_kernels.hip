@@ -182,12 +182,13 @@ __device__ __forceinline__ double rsqrt_nr(double t) {
 }
 #endif
 
-// 1/x for the QL shift: v_rcp_f32 (about one ulp); in FP64 v_rcp_f64 and one
-// Newton step.
+// 1/x for the QL shift and the refinement's gap divisions: v_rcp_f32 (about
+// one ulp); in FP64 v_rcp_f64 and one Newton step (relative error of order
+// 2^-48, far below what either use needs).
 #ifdef GENEIG_FP32
-__device__ __forceinline__ float shift_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float rcp_nr(float x) { return __builtin_amdgcn_rcpf(x); }
 #else
-__device__ __forceinline__ double shift_rcp(double x) {
+__device__ __forceinline__ double rcp_nr(double x) {
     const double y = __builtin_amdgcn_rcp(x);
     return fma(fma(-x, y, 1.0), y, y);
 }
@@ -555,14 +556,14 @@ __device__ __forceinline__ void ql_lanes_logged(real* ds, real* es, bool active,
                     // and every rotation stays exactly orthogonal whatever
                     // its value. e_l is not negligible here, so |g| < 1 / (2
                     // eps) and g^2 + 1 stays finite.
-                    g = (ds[(l + 1) * LANES + lane] - d_l) * shift_rcp(real(2.0) * e_l);
+                    g = (ds[(l + 1) * LANES + lane] - d_l) * rcp_nr(real(2.0) * e_l);
                     const real t = fma(g, g, real(1.0));
                     const real r = t * rsqrt_nr(t);
                     d_up = ds[m * LANES + lane];
                     e_cur = es[(m - 1) * LANES + lane];
                     d_cur = ds[(m - 1) * LANES + lane];
                     d_prev = d_up;
-                    g = d_up - d_l + e_l * shift_rcp(g + copysign(r, g));
+                    g = d_up - d_l + e_l * rcp_nr(g + copysign(r, g));
                     s = real(1.0);
                     c = real(1.0);
                     p = real(0.0);
@@ -2196,7 +2197,7 @@ geneig_mx_refine(const cx* __restrict__ Mp, const cx* __restrict__ Lp, real* __r
                 cx e = g * real(-0.5);
                 const real gap = lj - li;
                 if (i == j) e = cm(real(0.5) * (real(1.0) - g.x), 0.0);
-                else if (!(fabs(gap) < delta)) e = e + (t - g * (real(0.5) * (li + lj))) * (real(1.0) / gap);
+                else if (!(fabs(gap) < delta)) e = e + (t - g * (real(0.5) * (li + lj))) * rcp_nr(gap);
                 if (i >= N || !jin) e = cm(0.0, 0.0);
                 if (it == iters - 1) emax = fmax(emax, fmax(fabs(e.x), fabs(e.y)));
                 T[rt].re[v] = e.x;

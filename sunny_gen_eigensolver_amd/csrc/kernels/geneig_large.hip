@@ -1796,7 +1796,7 @@ geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, int* __restrict__ inf
                         cx e = gv * real(-0.5);
                         const real gap = lj - li;
                         if (i == j) e = cm(real(0.5) * (real(1.0) - gv.x), 0.0);
-                        else if (!(fabs(gap) < delta)) e = e + (tv - gv * (real(0.5) * (li + lj))) * (real(1.0) / gap);
+                        else if (!(fabs(gap) < delta)) e = e + (tv - gv * (real(0.5) * (li + lj))) * rcp_nr(gap);
                         if (it == iters - 1) emax = fmax(emax, fmax(fabs(e.x), fabs(e.y)));
                         E[(size_t)j * N + i] = e;
                     }
