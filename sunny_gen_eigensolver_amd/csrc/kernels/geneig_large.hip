@@ -320,11 +320,21 @@ struct HtWork {
 
 __device__ __forceinline__ int pkl(int c) { return c * NL - (c * (c - 1)) / 2; }
 
+// The matrix-vector product by row sums with mirrored reads (orders whose
+// rows fit two per lane; the LDS-resident steps only): it replaces a wave
+// reduction per column by one more LDS read per row (N = 96 FP64: 57.1 ->
+// 53.0 ms per batch). At N = 192 it was slower (94.5 -> 105.1 ms FP64,
+// where the first 94 columns are reduced in global memory, and 64.7 -> 65.7
+// ms FP32).
+constexpr bool ROW_SUMS = ND <= 2;
+
 struct HtLds {
+    static constexpr bool in_lds = true;
     cx* P;
     __device__ __forceinline__ cx& at(int i, int j) const { return P[pkl(j - K0) + i - j]; }
 };
 struct HtGlb {
+    static constexpr bool in_lds = false;
     cx* G;
     __device__ __forceinline__ cx& at(int i, int j) const { return G[(size_t)j * N + i]; }
 };
@@ -393,20 +403,42 @@ __device__ __forceinline__ void ht_step(HtWork& w, const ST& A, int k, cx* __res
             vr[q] = (r < N) ? w.vb[p][r] : cm(0.0, 0.0);
             acc[q] = cm(0.0, 0.0);
         }
-        for (int j = jfirst; j < N; j += LNW) {
-            const cx vj = w.vb[p][j];
-            cx s = cm(0.0, 0.0);
+        if constexpr (ROW_SUMS && ST::in_lds) {
+            // p = M v over the active block by row sums only: column j
+            // (owned by this wave) adds M(r, j) v_j to every active row r,
+            // M(r, j) read from the stored lower triangle below the diagonal
+            // and as the conjugate of M(j, r) above it (no reduction across
+            // lanes per column).
+            for (int j = jfirst; j < N; j += LNW) {
+                const cx vj = w.vb[p][j];
 #pragma unroll
-            for (int q = 0; q < ND; ++q) {
-                const int r = 64 * q + lane;
-                if (r >= j && r < N) {
-                    const cx a = A.at(r, j);
-                    acc[q] = acc[q] + cmul(a, vj);
-                    if (r > j) s = s + cconjmul(a, vr[q]);
+                for (int q = 0; q < ND; ++q) {
+                    const int r = 64 * q + lane;
+                    if (r >= m0 && r < N) {
+                        const cx a = (r >= j) ? A.at(r, j) : cconj(A.at(j, r));
+                        acc[q] = cfma(a, vj, acc[q]);
+                    }
                 }
             }
-            s = wave_sum_cx(s);
-            if (lane == 0) w.pc[j] = s;
+        } else {
+            // Each stored element once: row sums below the diagonal per
+            // lane, and the column sums of its mirror by a wave reduction
+            // per column (w.pc).
+            for (int j = jfirst; j < N; j += LNW) {
+                const cx vj = w.vb[p][j];
+                cx s = cm(0.0, 0.0);
+#pragma unroll
+                for (int q = 0; q < ND; ++q) {
+                    const int r = 64 * q + lane;
+                    if (r >= j && r < N) {
+                        const cx a = A.at(r, j);
+                        acc[q] = cfma(a, vj, acc[q]);
+                        if (r > j) s = s + cconjmul(a, vr[q]);
+                    }
+                }
+                s = wave_sum_cx(s);
+                if (lane == 0) w.pc[j] = s;
+            }
         }
 #pragma unroll
         for (int q = 0; q < ND; ++q) {
@@ -421,7 +453,7 @@ __device__ __forceinline__ void ht_step(HtWork& w, const ST& A, int k, cx* __res
             const int r = 64 * q + lane;
             cx pr = cm(0.0, 0.0);
             if (r >= m0 && r < N) {
-                pr = w.pc[r];
+                if constexpr (!(ROW_SUMS && ST::in_lds)) pr = w.pc[r];
 #pragma unroll
                 for (int u = 0; u < LNW; ++u) pr = pr + w.part[u][r];
                 pr = cmul(t, pr);
@@ -443,7 +475,7 @@ __device__ __forceinline__ void ht_step(HtWork& w, const ST& A, int k, cx* __res
 #pragma unroll
             for (int q = 0; q < ND; ++q) {
                 const int r = 64 * q + lane;
-                if (r >= j && r < N) A.at(r, j) = A.at(r, j) - cmulc(vr[q], wj) - cmulc(wr[q], vj);
+                if (r >= j && r < N) A.at(r, j) = cfms_conj(wr[q], vj, cfms_conj(vr[q], wj, A.at(r, j)));
             }
         }
     }
