@@ -408,7 +408,9 @@ __device__ __forceinline__ void ht_step(HtWork& w, const ST& A, int k, cx* __res
             // (owned by this wave) adds M(r, j) v_j to every active row r,
             // M(r, j) read from the stored lower triangle below the diagonal
             // and as the conjugate of M(j, r) above it (no reduction across
-            // lanes per column).
+            // lanes per column). Unrolled so that several columns' loads
+            // are in flight at once.
+#pragma unroll 4
             for (int j = jfirst; j < N; j += LNW) {
                 const cx vj = w.vb[p][j];
 #pragma unroll
@@ -470,6 +472,7 @@ __device__ __forceinline__ void ht_step(HtWork& w, const ST& A, int k, cx* __res
             if (r < N && (r % LNW) == wave) w.wb[r] = wr[q];   // entries of this wave's columns
         }
         wave_lds_sync();
+#pragma unroll 4
         for (int j = jfirst; j < N; j += LNW) {
             const cx vj = w.vb[p][j], wj = w.wb[j];
 #pragma unroll
