@@ -282,9 +282,19 @@ __device__ __forceinline__ void back_wy(BackWork& bw, FV vget, FT tget, FZ zget,
 // ---------------------------------------------------------------------------
 // Tridiagonalisation (module precision).
 // ---------------------------------------------------------------------------
+// Waves of the tridiagonalisation's workgroup: eight, twice the other staged
+// kernels', so that each step's column loops are half as long per wave and
+// a CU holds twice the waves to hide their latency; four where the FP32
+// triangle and eight waves' partial sums would not fit LDS together (N = 192).
+#ifdef GENEIG_FP32
+constexpr int HT_NW = ((size_t)N * (N + 1) / 2 + (size_t)(2 + 1 + 8 + 1) * N) * sizeof(cx) + 64 > LDS_CAP ? 4 : 8;
+#else
+constexpr int HT_NW = 8;
+#endif
+constexpr int HT_NT = 64 * HT_NW;
 // Bytes beside the packed triangle: v (two steps), w, per-wave partial sums
 // of A v and the column sums of its upper part.
-constexpr size_t HT_EXTRA = (size_t)(2 + 1 + LNW + 1) * N * sizeof(cx) + 64;
+constexpr size_t HT_EXTRA = (size_t)(2 + 1 + HT_NW + 1) * N * sizeof(cx) + 64;
 // LDS budget of the tridiagonalisation: in FP64 half the CU's LDS, so that
 // two workgroups share a CU; the columns whose packed triangle does not fit
 // are reduced in place in global memory first (K0 below). Measured at batch
@@ -313,7 +323,7 @@ struct HtWork {
     cx P[NL * (NL + 1) / 2];                  // lower triangle of rows/columns K0.., packed by columns
     cx vb[2][N];
     cx wb[N];
-    cx part[LNW][N];
+    cx part[HT_NW][N];
     cx pc[N];
     cx tau[2];
 };
@@ -339,7 +349,7 @@ struct HtGlb {
     __device__ __forceinline__ cx& at(int i, int j) const { return G[(size_t)j * N + i]; }
 };
 
-// Householder vector of column k, by the wave that owns column k (k % LNW):
+// Householder vector of column k, by the wave that owns column k (k % HT_NW):
 // v(k+1) = 1, tau and the real beta as in LAPACK's zlarfg; v to LDS and to
 // the reflector store, d_k, e_k = beta and tau to global memory.
 template <class ST>
@@ -383,7 +393,7 @@ __device__ __forceinline__ void ht_house(HtWork& w, const ST& A, int k, cx* __re
 }
 
 // Step k: A' -= v w^H + w v^H on the trailing rows/columns m0 = k+1.. with
-// w = p - (tau/2)(p^H v) v, p = tau A' v. Column j belongs to wave j % LNW
+// w = p - (tau/2)(p^H v) v, p = tau A' v. Column j belongs to wave j % HT_NW
 // (lane = row, ND rows per lane). The product reads the lower triangle once:
 // A(r, j) v_j goes to row r (per-lane sums, combined across waves in LDS)
 // and conj(A(r, j)) v_r, r > j, to row j (a wave sum). The owner of column
@@ -394,7 +404,7 @@ __device__ __forceinline__ void ht_step(HtWork& w, const ST& A, int k, cx* __res
     const int lane = lane_id(), wave = wave_id();
     const int m0 = k + 1, p = k & 1;
     const cx t = w.tau[p];
-    const int jfirst = m0 + (((wave - m0) % LNW) + LNW) % LNW;
+    const int jfirst = m0 + (((wave - m0) % HT_NW) + HT_NW) % HT_NW;
     if (t.x != real(0.0) || t.y != real(0.0)) {
         cx vr[ND], acc[ND];
 #pragma unroll
@@ -411,7 +421,7 @@ __device__ __forceinline__ void ht_step(HtWork& w, const ST& A, int k, cx* __res
             // lanes per column). Unrolled so that several columns' loads
             // are in flight at once.
 #pragma unroll 4
-            for (int j = jfirst; j < N; j += LNW) {
+            for (int j = jfirst; j < N; j += HT_NW) {
                 const cx vj = w.vb[p][j];
 #pragma unroll
                 for (int q = 0; q < ND; ++q) {
@@ -426,7 +436,7 @@ __device__ __forceinline__ void ht_step(HtWork& w, const ST& A, int k, cx* __res
             // Each stored element once: row sums below the diagonal per
             // lane, and the column sums of its mirror by a wave reduction
             // per column (w.pc).
-            for (int j = jfirst; j < N; j += LNW) {
+            for (int j = jfirst; j < N; j += HT_NW) {
                 const cx vj = w.vb[p][j];
                 cx s = cm(0.0, 0.0);
 #pragma unroll
@@ -457,7 +467,7 @@ __device__ __forceinline__ void ht_step(HtWork& w, const ST& A, int k, cx* __res
             if (r >= m0 && r < N) {
                 if constexpr (!(ROW_SUMS && ST::in_lds)) pr = w.pc[r];
 #pragma unroll
-                for (int u = 0; u < LNW; ++u) pr = pr + w.part[u][r];
+                for (int u = 0; u < HT_NW; ++u) pr = pr + w.part[u][r];
                 pr = cmul(t, pr);
             }
             pv = pv + cconjmul(pr, vr[q]);
@@ -469,11 +479,11 @@ __device__ __forceinline__ void ht_step(HtWork& w, const ST& A, int k, cx* __res
         for (int q = 0; q < ND; ++q) {
             wr[q] = wr[q] + cmul(cc, vr[q]);
             const int r = 64 * q + lane;
-            if (r < N && (r % LNW) == wave) w.wb[r] = wr[q];   // entries of this wave's columns
+            if (r < N && (r % HT_NW) == wave) w.wb[r] = wr[q];   // entries of this wave's columns
         }
         wave_lds_sync();
 #pragma unroll 4
-        for (int j = jfirst; j < N; j += LNW) {
+        for (int j = jfirst; j < N; j += HT_NW) {
             const cx vj = w.vb[p][j], wj = w.wb[j];
 #pragma unroll
             for (int q = 0; q < ND; ++q) {
@@ -482,7 +492,7 @@ __device__ __forceinline__ void ht_step(HtWork& w, const ST& A, int k, cx* __res
             }
         }
     }
-    if (k + 1 < N - 1 && wave == (k + 1) % LNW) ht_house(w, A, k + 1, R, dg, eg);
+    if (k + 1 < N - 1 && wave == (k + 1) % HT_NW) ht_house(w, A, k + 1, R, dg, eg);
     __syncthreads();
 }
 
@@ -1074,9 +1084,12 @@ geneig_lg_backtri(cx* __restrict__ U, const int* __restrict__ info, char* __rest
 // Bytes of one matrix's workspace, for the host's consistency check.
 extern "C" __device__ const unsigned long long geneig_lg_ws_bytes = gk::WS_BYTES;
 
+// Workgroup size of geneig_lg_hetrd, read by the host before launching it.
+extern "C" __device__ const int geneig_lg_hetrd_threads = gk::HT_NT;
+
 // Tridiagonalisation of M (read as FP64, reduced in the module precision):
 // d, e and the reflectors to the workspace.
-extern "C" __global__ void __launch_bounds__(gk::LNT)
+extern "C" __global__ void __launch_bounds__(gk::HT_NT)
 geneig_lg_hetrd(const int* __restrict__ info, char* __restrict__ ws, long long wss, int batch) {
     using namespace gk;
     __shared__ HtWork w;
@@ -1090,7 +1103,7 @@ geneig_lg_hetrd(const int* __restrict__ info, char* __restrict__ ws, long long w
     real* eg = dg + N;
     const HtLds L{w.P};
     auto load_trailing = [&]() {
-        for (int e = tid; e < (int)NN; e += LNT) {
+        for (int e = tid; e < (int)NN; e += HT_NT) {
             const int i = e % N, j = e / N;
             if (j >= K0 && i >= j) w.P[pkl(j - K0) + i - j] = to_cx(Mg[e]);
         }

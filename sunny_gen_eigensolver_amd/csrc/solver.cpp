@@ -256,6 +256,7 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
             module32_ = acquire_module(k32, cache, &l32, &ms32);
             acquire_ms_ += ms32;
             k_lg_hetrd_ = module32_->get("geneig_lg_hetrd");
+            lg_hetrd_threads_ = (unsigned)module_int(*module32_, "geneig_lg_hetrd_threads");
             k_lg_ql_ = module32_->get("geneig_lg_qlr");
             k_lg_qll_ = module32_->find("geneig_lg_qlr_lanes");
             if (k_lg_qll_) lg_qll_lanes_ = module_int(*module32_, "geneig_lg_qlr_lanes_count");
@@ -268,6 +269,7 @@ void Solver::bind_module_(int block_dim, CacheMode cache) {
             if (const char* v = std::getenv("SUNNY_GENEIG_REFINE_TOL")) refine_tol_ = std::atof(v);
         } else {
             k_lg_hetrd_ = module_->get("geneig_lg_hetrd");
+            lg_hetrd_threads_ = (unsigned)module_int(*module_, "geneig_lg_hetrd_threads");
             k_lg_ql_ = module_->get("geneig_lg_qlr");
             k_lg_qll_ = module_->find("geneig_lg_qlr_lanes");
             if (k_lg_qll_) lg_qll_lanes_ = module_int(*module_, "geneig_lg_qlr_lanes_count");
@@ -968,7 +970,7 @@ void Solver::launch_large_(const void* H, const void* S, double* W, void* U, int
         launch_(k_lg_formm_, (unsigned)cnt, bd, stream, a2);
         mark();
         void* a3[] = {&Ic, &ws, (void*)&wss, &cnt};
-        launch_(k_lg_hetrd_, (unsigned)cnt, bd, stream, a3);
+        launch_(k_lg_hetrd_, (unsigned)cnt, lg_hetrd_threads_, stream, a3);
         mark();
         // QL eigenvalues: the caller's W (FP64), or the workspace's FP32 slot.
         void* wout = Wc;

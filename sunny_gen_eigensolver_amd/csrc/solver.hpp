@@ -181,6 +181,7 @@ private:
     // overrides it.
     hipFunction_t k_lg_qll_ = nullptr;
     int lg_qll_lanes_ = 64;       // its matrices per wave (the module states it)
+    unsigned lg_hetrd_threads_ = 256;   // geneig_lg_hetrd's workgroup (the module states it)
     int ql_rec_lanes_ = 64;       // geneig_ql_record's matrices per wave
     static constexpr int kLanesMinBatch64 = 3584;
     static constexpr int kLanesMinBatch32 = 9216;
