@@ -1375,10 +1375,11 @@ geneig_lg_qlr_lanes(real* __restrict__ wout, long long wout_stride, int* __restr
 }
 #endif
 
-// Rows of Z in registers: every staged order in FP32, up to order 128 in FP64
-// (a row is N registers in FP32, 2N in FP64; above order 104 the compiler
-// keeps part of it in the accumulation registers, one wave per SIMD).
-#if defined(GENEIG_FP32) || GENEIG_N <= 128
+// Rows of Z in registers at every staged order (a row is N registers in
+// FP32, 2N in FP64; above order 104 the compiler keeps part of it in the
+// accumulation registers, one wave per SIMD; at N = 192 FP64, 256 VGPRs and
+// 172 AGPRs, no spills).
+#if defined(GENEIG_FP32) || GENEIG_N <= 192
 // QL, second half: Z = product of the logged rotations, one row of Z per
 // thread held in registers, then Z with its columns in ascending order to the
 // workspace. Every sweep's rotations are staged by the wave in its own LDS
