@@ -547,61 +547,6 @@ struct rpair {
     real c, s;
 };
 
-// Apply one sweep's logged rotations (rotation l + j held by lane j % 64 in
-// cur[j / 64], fetched by the wave beforehand with coalesced loads) to a row of
-// Z whose element c sits at z[c * LD], for i = m-1 down to l: each rotation
-// travels to the lanes by v_readlane, so the only dependence between steps is
-// the carried element.
-template <int LD, class ZP>
-__device__ __forceinline__ void ql_apply_reg(ZP z, const rpair (&cur)[ND], int l, int m) {
-    real zc = z[m * LD];
-#pragma unroll
-    for (int q = ND - 1; q >= 0; --q) {
-        const int jtop = m - 1 - l - 64 * q;          // highest rotation of this chunk
-        if (jtop < 0) continue;
-        int jj = jtop > 63 ? 63 : jtop;
-        const int i0 = l + 64 * q;
-        for (; jj >= 3; jj -= 4) {
-            const int i = i0 + jj;
-            const real z0 = z[i * LD], z1 = z[(i - 1) * LD], z2 = z[(i - 2) * LD], z3 = z[(i - 3) * LD];
-            real c = bcast(cur[q].c, jj), s = bcast(cur[q].s, jj);
-            z[(i + 1) * LD] = fma(s, z0, c * zc);
-            zc = fma(c, z0, -(s * zc));
-            c = bcast(cur[q].c, jj - 1);
-            s = bcast(cur[q].s, jj - 1);
-            z[i * LD] = fma(s, z1, c * zc);
-            zc = fma(c, z1, -(s * zc));
-            c = bcast(cur[q].c, jj - 2);
-            s = bcast(cur[q].s, jj - 2);
-            z[(i - 1) * LD] = fma(s, z2, c * zc);
-            zc = fma(c, z2, -(s * zc));
-            c = bcast(cur[q].c, jj - 3);
-            s = bcast(cur[q].s, jj - 3);
-            z[(i - 2) * LD] = fma(s, z3, c * zc);
-            zc = fma(c, z3, -(s * zc));
-        }
-        for (; jj >= 0; --jj) {
-            const int i = i0 + jj;
-            const real zi = z[i * LD];
-            const real c = bcast(cur[q].c, jj), s = bcast(cur[q].s, jj);
-            z[(i + 1) * LD] = fma(s, zi, c * zc);
-            zc = fma(c, zi, -(s * zc));
-        }
-    }
-    z[l * LD] = zc;
-}
-
-// Lane share of one sweep's rotations: rotation l + j (j < cnt) to lane j % 64
-// of register j / 64.
-__device__ __forceinline__ void ql_fetch(rpair (&r)[ND], const rpair* __restrict__ base, int cnt) {
-    const int lane = lane_id();
-#pragma unroll
-    for (int q = 0; q < ND; ++q) {
-        const int j = 64 * q + lane;
-        r[q] = (j < cnt) ? base[j] : rpair{real(1.0), real(0.0)};
-    }
-}
-
 // First m >= l with a negligible off-diagonal e_m (m = N - 1 if none).
 __device__ __forceinline__ int ql_find_m(const real (&dr)[ND], const real (&er)[ND], int l) {
     const int lane = lane_id();
@@ -1378,8 +1323,8 @@ geneig_lg_qlr_lanes(real* __restrict__ wout, long long wout_stride, int* __restr
 // Rows of Z in registers at every staged order (a row is N registers in
 // FP32, 2N in FP64; above order 104 the compiler keeps part of it in the
 // accumulation registers, one wave per SIMD; at N = 192 FP64, 256 VGPRs and
-// 172 AGPRs, no spills).
-#if defined(GENEIG_FP32) || GENEIG_N <= 192
+// 172 AGPRs, no spills). It replaced, above order 128 in FP64, a form with
+// the rows in LDS or global memory (N = 192: 31.0 -> 9 ms).
 // QL, second half: Z = product of the logged rotations, one row of Z per
 // thread held in registers, then Z with its columns in ascending order to the
 // workspace. Every sweep's rotations are staged by the wave in its own LDS
@@ -1461,71 +1406,6 @@ geneig_lg_qlz(real* __restrict__ wout, long long wout_stride, int* __restrict__ 
         for (int c = 0; c < N; ++c) zout[(size_t)rk[c] * N + r] = z[c];
     }
 }
-#else
-// QL, second half: Z = product of the logged rotations, one row per thread
-// (rows in LDS, or in global memory for the rows that do not fit), then Z with
-// its columns in ascending order to the workspace. Matrices whose log
-// overflowed are left to geneig_lg_qlz_inline.
-extern "C" __global__ void __launch_bounds__(gk::QNT)
-geneig_lg_qlz(real* __restrict__ wout, long long wout_stride, int* __restrict__ info, char* __restrict__ ws,
-              long long wss, int batch) {
-    using namespace gk;
-    __shared__ real zs[ZR_LDS > 0 ? ZR_LDS * N : 1];
-    const int b = blockIdx.x;
-    if (b >= batch || info[b] > N) return;
-    char* wsb = ws_of(ws, wss, b);
-    const char* qb = wsb + WS_Q;
-    const int* hdr = reinterpret_cast<const int*>(qb);
-    if (hdr[1]) return;
-    const int* sw = reinterpret_cast<const int*>(qb + Q_SW);
-    const int* rk = reinterpret_cast<const int*>(qb + Q_RANK);
-    const rpair* rot = reinterpret_cast<const rpair*>(qb + Q_ROT);
-    const int lane = lane_id(), wave = wave_id();
-    const int r = wave * 64 + lane;
-    const bool rin = r < N;
-    const bool in_lds = wave < QW_LDS;
-    real* zg = reinterpret_cast<real*>(wsb + WS_X);
-    real* zout = reinterpret_cast<real*>(wsb + WS_Z);
-    if (rin) {
-        if (in_lds) {
-            for (int c = 0; c < N; ++c) zs[c * ZR_LDS + r] = (c == r) ? real(1.0) : real(0.0);
-        } else {
-            for (int c = 0; c < N; ++c) zg[(size_t)c * N + r] = (c == r) ? real(1.0) : real(0.0);
-        }
-    }
-    const int nsw = hdr[0];
-    // The next sweep's window and rotations are fetched while the current one
-    // is applied.
-    int off = 0;
-    int l = nsw > 0 ? sw[0] : 0, m = nsw > 0 ? sw[1] : 0;
-    rpair cur[ND];
-    ql_fetch(cur, rot, m - l);
-    for (int t = 0; t < nsw; ++t) {
-        const int off_n = off + (m - l);
-        const bool last = t + 1 == nsw;
-        const int ln = last ? 0 : sw[2 * t + 2], mn = last ? 0 : sw[2 * t + 3];
-        rpair nxt[ND];
-        ql_fetch(nxt, rot + off_n, mn - ln);
-        if (rin) {
-            if (in_lds) ql_apply_reg<ZR_LDS>(zs + r, cur, l, m);
-            else ql_apply_reg<N>(zg + r, cur, l, m);
-        }
-#pragma unroll
-        for (int q = 0; q < ND; ++q) cur[q] = nxt[q];
-        off = off_n;
-        l = ln;
-        m = mn;
-    }
-    if (rin) {
-        if (in_lds) {
-            for (int c = 0; c < N; ++c) zout[(size_t)rk[c] * N + r] = zs[c * ZR_LDS + r];
-        } else {
-            for (int c = 0; c < N; ++c) zout[(size_t)rk[c] * N + r] = zg[(size_t)c * N + r];
-        }
-    }
-}
-
-#endif  // register rows
 
 // Matrices whose QL rotation log overflowed: QL solved inline (ql_inline),
 // eigenvalues to wout, Z to the workspace; every other matrix returns at once.
