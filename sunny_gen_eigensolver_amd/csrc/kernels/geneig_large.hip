@@ -862,18 +862,27 @@ geneig_lg_chol(const cx* __restrict__ S, real* __restrict__ W, cx* __restrict__ 
             }
         }
         __syncthreads();
-        // Forward substitution with L_ii, thread j = column j, in place in
-        // its 16 entries of pan.
+        // Forward substitution with L_ii, thread j = column j, its 16 entries
+        // in registers (unrolled; the term of the newest entry comes last, so
+        // each entry waits on its predecessor for one multiply-add only).
         const int j = tid;
         if (j < r0 + rw) {
-            cx* xj = cw.pan + j * YLD;
-            for (int rr = 0; rr < rw; ++rr) {
-                cx s = (j < r0) ? cm(0.0, 0.0) - xj[rr] : cm((rr == j - r0) ? 1.0 : 0.0, 0.0);
-                for (int ss = 0; ss < rr; ++ss) s = s - cmul(cw.lii[ss * 16 + rr], xj[ss]);
-                xj[rr] = s * cw.liinv[rr];
+            const cx* xj = cw.pan + j * YLD;
+            cx x[16];
+#pragma unroll
+            for (int rr = 0; rr < 16; ++rr) x[rr] = (rr < rw && j < r0) ? xj[rr] : cm(0.0, 0.0);
+#pragma unroll
+            for (int rr = 0; rr < 16; ++rr) {
+                if (rr < rw) {
+                    cx s = (j < r0) ? cm(0.0, 0.0) - x[rr] : cm((rr == j - r0) ? 1.0 : 0.0, 0.0);
+#pragma unroll
+                    for (int ss = 0; ss < rr; ++ss) s = s - cmul(cw.lii[ss * 16 + rr], x[ss]);
+                    x[rr] = s * cw.liinv[rr];
+                }
             }
-            for (int rr = 0; rr < rw; ++rr)
-                if (CHOL_LDS ? r0 + rr >= j : true) setL(r0 + rr, j, xj[rr]);
+#pragma unroll
+            for (int rr = 0; rr < 16; ++rr)
+                if (rr < rw && (CHOL_LDS ? r0 + rr >= j : true)) setL(r0 + rr, j, x[rr]);
         }
         __syncthreads();
     }
