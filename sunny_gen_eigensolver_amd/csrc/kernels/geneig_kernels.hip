@@ -1837,14 +1837,18 @@ __device__ __forceinline__ cx elem(const cx* M, int ld, int i, int k) {
 
 // acc[rt] (row tile rt, column block cb) = A B, A(i, k) = fa(i, k),
 // B(k, j) = fb(k, j), k < ke (the fetchers return zero outside N). With
-// UPPER_A, A is upper triangular: row tile rt only takes k >= 16 rt.
+// UPPER_A, A is upper triangular: row tile rt only takes k >= 16 rt. Only
+// the groups of row tiles that meet [rlo, rhi] (wave-uniform) are formed;
+// the other tiles are left as they are.
 template <bool UPPER_A = false, class FA, class FB>
-__device__ __forceinline__ void cgemm(ctile (&acc)[NB], FA fa, FB fb, int cb, int ke = KEND) {
+__device__ __forceinline__ void cgemm(ctile (&acc)[NB], FA fa, FB fb, int cb, int ke = KEND, int rlo = 0,
+                                      int rhi = NB - 1) {
     const int l = lane_id();
     const int c16 = l & 15, k4 = l >> 4;
     // Row tiles in groups of RGR, so that only RGR 3M accumulators are live.
 #pragma unroll
     for (int g0 = 0; g0 < NB; g0 += RGR) {
+        if (g0 + RGR - 1 < rlo || g0 > rhi) continue;
         ctile3 a3[RGR];
 #pragma unroll
         for (int g = 0; g < RGR; ++g) c3_zero(a3[g]);
@@ -2161,58 +2165,94 @@ geneig_mx_refine(const cx* __restrict__ Mp, const cx* __restrict__ Lp, real* __r
     real delta = real(0.0);
     real emax = real(0.0);                       // this lane's largest |E| entry, last step
     for (int it = 0; it < iters; ++it) {
-        ctile T[NB], G[NB];
-        {
-            ctile X[NB];
-            cgemm(X, fM, fV, cb);                // X = M V
-            cgemm_reg(T, fVh, X);                // T = V^H X
-        }
-        cgemm(G, fVh, fV, cb);                   // G = V^H V
-        // Diagonal: lambda_j = T_jj / G_jj (the lane holding (j, j) publishes).
+        ctile X[NB], Y[NB];
+        cgemm(X, fM, fV, cb);                    // X = M V
+        // Rayleigh quotients of this wave's columns, lambda_j = v_j^H x_j /
+        // v_j^H v_j (the four lanes of column j hold a quarter of its rows each).
+        real tj = real(0.0), gj = real(0.0);
 #pragma unroll
         for (int rt = 0; rt < NB; ++rt) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = rt * 16 + g4 + 4 * v;
-                if (i == j && jin) r.lam[j] = T[rt].re[v] / G[rt].re[v];
+                const cx vv = elem(r.U, LDR, i, j);
+                tj += cconjmul(vv, tile_get(X[rt], v)).x;
+                gj += cabs2(vv);
             }
         }
+        tj = col_sum(tj);
+        gj = col_sum(gj);
+        const real lj = jin ? tj / gj : real(0.0);
+        if (jin && g4 == 0) r.lam[j] = lj;
         __syncthreads();
         // The threshold doubles every step, so a pair treated as close once
         // stays close (its vectors are then mixed) and ends in Rayleigh-Ritz.
         delta = delta_rel * (real)(1 << it) * wave_max(l < N ? fabs(r.lam[l < N ? l : 0]) : real(0.0));
-        // E = -(G - I) / 2 + A with A anti-Hermitian: A_ij = (T_ij - (lambda_i +
-        // lambda_j) G_ij / 2) / (lambda_j - lambda_i) for separated pairs, 0 for
-        // close ones (equal to Ogita-Aishima's (T_ij - lambda_j G_ij) /
-        // (lambda_j - lambda_i) in exact arithmetic; the split keeps V^H V = I
-        // free of rounding divided by small gaps).
-        const real lj = jin ? r.lam[j] : real(0.0);
+        // Residual R = M V - V diag(lambda) in place of X, then Y = V^H R.
+#pragma unroll
+        for (int rt = 0; rt < NB; ++rt) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = rt * 16 + g4 + 4 * v;
+                const cx rr = tile_get(X[rt], v) - elem(r.U, LDR, i, j) * lj;
+                X[rt].re[v] = rr.x;
+                X[rt].im[v] = rr.y;
+            }
+        }
+        cgemm_reg(Y, fVh, X);
+        // Ogita-Aishima's correction E_ij = (T_ij - lambda_j G_ij) /
+        // (lambda_j - lambda_i) with T = V^H M V and G = V^H V is
+        // Y_ij / (lambda_j - lambda_i): for separated pairs only Y is needed.
+        // Pairs closer than delta take E_ij = -G_ij / 2 and the diagonal
+        // E_jj = (1 - G_jj) / 2, so G is formed only over the row tiles that
+        // hold this block's close pairs (adjacent to the diagonal tile when
+        // the eigenvalues are sorted; found here pair by pair, whatever the order).
+        real plo = real(N), phi = real(-1.0);
+        if (jin) {
+            for (int i = 0; i < N; ++i) {
+                if (i != j && fabs(r.lam[i] - lj) < delta) {
+                    plo = fmin(plo, (real)i);
+                    phi = fmax(phi, (real)i);
+                }
+            }
+        }
+        plo = -wave_max(-plo);
+        phi = wave_max(phi);
+        ctile G[NB];
+#pragma unroll
+        for (int rt = 0; rt < NB; ++rt) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                G[rt].re[v] = real(0.0);
+                G[rt].im[v] = real(0.0);
+            }
+        }
+        if (phi >= real(0.0)) cgemm(G, fVh, fV, cb, KEND, (int)plo >> 4, (int)phi >> 4);
 #pragma unroll
         for (int rt = 0; rt < NB; ++rt) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = rt * 16 + g4 + 4 * v;
                 const real li = r.lam[i < N ? i : 0];
-                const cx t = tile_get(T[rt], v), g = tile_get(G[rt], v);
-                cx e = g * real(-0.5);
                 const real gap = lj - li;
-                if (i == j) e = cm(real(0.5) * (real(1.0) - g.x), 0.0);
-                else if (!(fabs(gap) < delta)) e = e + (t - g * (real(0.5) * (li + lj))) * rcp_nr(gap);
+                cx e = tile_get(Y[rt], v) * rcp_nr(gap);
+                if (i == j) e = cm(real(0.5) * (real(1.0) - gj), 0.0);
+                else if (fabs(gap) < delta) e = tile_get(G[rt], v) * real(-0.5);
                 if (i >= N || !jin) e = cm(0.0, 0.0);
                 if (it == iters - 1) emax = fmax(emax, fmax(fabs(e.x), fabs(e.y)));
-                T[rt].re[v] = e.x;
-                T[rt].im[v] = e.y;
+                Y[rt].re[v] = e.x;
+                Y[rt].im[v] = e.y;
             }
         }
-        cgemm_reg(G, fV, T);                     // V E
+        cgemm_reg(X, fV, Y);                     // V E
 #pragma unroll
         for (int rt = 0; rt < NB; ++rt) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = rt * 16 + g4 + 4 * v;
-                const cx u = elem(r.U, LDR, i, j) + tile_get(G[rt], v);
-                G[rt].re[v] = u.x;
-                G[rt].im[v] = u.y;
+                const cx u = elem(r.U, LDR, i, j) + tile_get(X[rt], v);
+                X[rt].re[v] = u.x;
+                X[rt].im[v] = u.y;
             }
         }
         __syncthreads();     // every wave is done reading V
@@ -2221,7 +2261,7 @@ geneig_mx_refine(const cx* __restrict__ Mp, const cx* __restrict__ Lp, real* __r
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = rt * 16 + g4 + 4 * v;
-                if (i < N && jin) r.U[j * LDR + i] = tile_get(G[rt], v);
+                if (i < N && jin) r.U[j * LDR + i] = tile_get(X[rt], v);
             }
         }
         __syncthreads();
