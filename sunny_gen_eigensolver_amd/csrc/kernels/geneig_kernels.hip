@@ -2203,31 +2203,15 @@ geneig_mx_refine(const cx* __restrict__ Mp, const cx* __restrict__ Lp, real* __r
         // Ogita-Aishima's correction E_ij = (T_ij - lambda_j G_ij) /
         // (lambda_j - lambda_i) with T = V^H M V and G = V^H V is
         // Y_ij / (lambda_j - lambda_i): for separated pairs only Y is needed.
-        // Pairs closer than delta take E_ij = -G_ij / 2 and the diagonal
-        // E_jj = (1 - G_jj) / 2, so G is formed only over the row tiles that
-        // hold this block's close pairs (adjacent to the diagonal tile when
-        // the eigenvalues are sorted; found here pair by pair, whatever the order).
-        real plo = real(N), phi = real(-1.0);
-        if (jin) {
-            for (int i = 0; i < N; ++i) {
-                if (i != j && fabs(r.lam[i] - lj) < delta) {
-                    plo = fmin(plo, (real)i);
-                    phi = fmax(phi, (real)i);
-                }
-            }
+        // The diagonal takes E_jj = (1 - G_jj) / 2 and pairs closer than
+        // delta E_ij = -G_ij / 2, each G_ij of those a dot product of two
+        // columns (the pairs are few: found per column against every
+        // eigenvalue, whatever their order).
+        unsigned long long pm = 0;               // close partners of column j (lanes g4 == 0)
+        if (jin && g4 == 0) {
+            for (int i = 0; i < N; ++i)
+                if (i != j && fabs(r.lam[i] - lj) < delta) pm |= 1ull << i;
         }
-        plo = -wave_max(-plo);
-        phi = wave_max(phi);
-        ctile G[NB];
-#pragma unroll
-        for (int rt = 0; rt < NB; ++rt) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                G[rt].re[v] = real(0.0);
-                G[rt].im[v] = real(0.0);
-            }
-        }
-        if (phi >= real(0.0)) cgemm(G, fVh, fV, cb, KEND, (int)plo >> 4, (int)phi >> 4);
 #pragma unroll
         for (int rt = 0; rt < NB; ++rt) {
 #pragma unroll
@@ -2237,12 +2221,39 @@ geneig_mx_refine(const cx* __restrict__ Mp, const cx* __restrict__ Lp, real* __r
                 const real gap = lj - li;
                 cx e = tile_get(Y[rt], v) * rcp_nr(gap);
                 if (i == j) e = cm(real(0.5) * (real(1.0) - gj), 0.0);
-                else if (fabs(gap) < delta) e = tile_get(G[rt], v) * real(-0.5);
+                else if (fabs(gap) < delta) e = cm(0.0, 0.0);    // set below
                 if (i >= N || !jin) e = cm(0.0, 0.0);
-                if (it == iters - 1) emax = fmax(emax, fmax(fabs(e.x), fabs(e.y)));
                 Y[rt].re[v] = e.x;
                 Y[rt].im[v] = e.y;
             }
+        }
+        for (;;) {
+            const unsigned long long any = __ballot(pm != 0);
+            if (any == 0) break;
+            const int src = (int)__builtin_ctzll(any);
+            const int jj = __builtin_amdgcn_readlane(j, src);
+            const unsigned lo = __builtin_amdgcn_readlane((unsigned)pm, src);
+            const unsigned hi = __builtin_amdgcn_readlane((unsigned)(pm >> 32), src);
+            const int ii = lo ? (int)__builtin_ctz(lo) : 32 + (int)__builtin_ctz(hi);
+            if (l == src) pm &= pm - 1;
+            cx g = (l < N) ? cconjmul(r.U[ii * LDR + l], r.U[jj * LDR + l]) : cm(0.0, 0.0);
+            g = wave_sum_cx(g) * real(-0.5);
+#pragma unroll
+            for (int rt = 0; rt < NB; ++rt) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    if (j == jj && rt * 16 + g4 + 4 * v == ii) {
+                        Y[rt].re[v] = g.x;
+                        Y[rt].im[v] = g.y;
+                    }
+                }
+            }
+        }
+        if (it == iters - 1) {
+#pragma unroll
+            for (int rt = 0; rt < NB; ++rt)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) emax = fmax(emax, fmax(fabs(Y[rt].re[v]), fabs(Y[rt].im[v])));
         }
         cgemm_reg(X, fV, Y);                     // V E
 #pragma unroll
