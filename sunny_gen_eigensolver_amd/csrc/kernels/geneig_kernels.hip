@@ -1906,6 +1906,79 @@ __device__ __forceinline__ void cgemm_reg(ctile (&acc)[NB], FA fa, const ctile (
     }
 }
 
+// The refinement's two products whose results are corrections (Y = V^H R
+// with the residual R = M V - V diag(lambda), and V E) on the FP32 matrix
+// cores (v_mfma_f32_16x16x4f32, twice the FP64 rate), operands rounded to
+// FP32 and sums in FP32: R and E are small, so the rounding is relative to
+// them (about 6e-8 of a correction of 1e-6 in the first step and far less
+// in the second), below what the step's FP64 parts leave. M V, the
+// Rayleigh quotients, the residual itself, the Gram entries and V + V E are
+// FP64. FP32 result layout: D(i, j) in lane j + 16 (i / 4), element i % 4.
+template <bool B>
+struct BoolC {
+    static constexpr bool value = B;
+};
+typedef float f4 __attribute__((ext_vector_type(4)));
+struct ftile {
+    f4 re, im;
+};
+struct ftile3 {
+    f4 p, q, s;
+};
+__device__ __forceinline__ void cmfma3f(ftile3& c, float ax, float ay, float bx, float by, float bs) {
+    c.p = __builtin_amdgcn_mfma_f32_16x16x4f32(ax, bx, c.p, 0, 0, 0);
+    c.q = __builtin_amdgcn_mfma_f32_16x16x4f32(ay, by, c.q, 0, 0, 0);
+    c.s = __builtin_amdgcn_mfma_f32_16x16x4f32(ax + ay, bs, c.s, 0, 0, 0);
+}
+// Row of element e of an FP32 result tile held by lane group g.
+__device__ __forceinline__ constexpr int frow(int e, int g) { return 4 * g + e; }
+
+// acc[rt] = A Bt in FP32, Bt this wave's column block in registers: an FP64
+// result (B64, element v of lane group g = row 16 q + 4 v + g) or an FP32 one
+// (element v of lane group g = row 16 q + 4 g + v). Each (q, v) is one k-step
+// over the four rows that lane groups 0..3 hold as element v; A is fetched
+// for the same rows.
+template <bool B64, class FA, class BT>
+__device__ __forceinline__ void cgemm_reg_f32(ftile (&acc)[NB], FA fa, const BT (&Bt)[NB]) {
+    const int l = lane_id();
+    const int c16 = l & 15, k4 = l >> 4;
+#pragma unroll
+    for (int g0 = 0; g0 < NB; g0 += RGR) {
+        ftile3 a3[RGR];
+#pragma unroll
+        for (int g = 0; g < RGR; ++g) {
+            a3[g].p = f4{0.f, 0.f, 0.f, 0.f};
+            a3[g].q = a3[g].p;
+            a3[g].s = a3[g].p;
+        }
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                if (B64 && 16 * q + 4 * v >= KEND) continue;     // k-steps entirely beyond N
+                if (!B64 && 16 * q >= N) continue;
+                const int k = 16 * q + (B64 ? 4 * v + k4 : 4 * k4 + v);
+                const float bx = (float)Bt[q].re[v], by = (float)Bt[q].im[v];
+                const float bs = bx + by;
+#pragma unroll
+                for (int g = 0; g < RGR; ++g) {
+                    const int rt = g0 + g;
+                    if (rt >= NB) continue;
+                    const cx a = fa(rt * 16 + c16, k);
+                    cmfma3f(a3[g], (float)a.x, (float)a.y, bx, by, bs);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);      // bounds the operand loads in flight
+        }
+#pragma unroll
+        for (int g = 0; g < RGR; ++g)
+            if (g0 + g < NB) {
+                acc[g0 + g].re = a3[g].p - a3[g].q;
+                acc[g0 + g].im = a3[g].s - a3[g].p - a3[g].q;
+            }
+    }
+}
+
 // Sum over the four lanes that hold one column (lanes c, c + 16, c + 32, c + 48).
 __device__ __forceinline__ real col_sum(real v) {
     v += __shfl_xor(v, 16, 64);
@@ -2164,8 +2237,11 @@ geneig_mx_refine(const cx* __restrict__ Mp, const cx* __restrict__ Lp, real* __r
     }
     real delta = real(0.0);
     real emax = real(0.0);                       // this lane's largest |E| entry, last step
-    for (int it = 0; it < iters; ++it) {
-        ctile X[NB], Y[NB];
+    // One step; the last (LAST) and the earlier ones are separate
+    // instances, so that each is register-allocated on its own.
+    auto step = [&](int it, auto last_c) {
+        constexpr bool LAST = decltype(last_c)::value;
+        ctile X[NB];
         cgemm(X, fM, fV, cb);                    // X = M V
         // Rayleigh quotients of this wave's columns, lambda_j = v_j^H x_j /
         // v_j^H v_j (the four lanes of column j hold a quarter of its rows each).
@@ -2199,84 +2275,112 @@ geneig_mx_refine(const cx* __restrict__ Mp, const cx* __restrict__ Lp, real* __r
                 X[rt].im[v] = rr.y;
             }
         }
-        cgemm_reg(Y, fVh, X);
         // Ogita-Aishima's correction E_ij = (T_ij - lambda_j G_ij) /
         // (lambda_j - lambda_i) with T = V^H M V and G = V^H V is
         // Y_ij / (lambda_j - lambda_i): for separated pairs only Y is needed.
         // The diagonal takes E_jj = (1 - G_jj) / 2 and pairs closer than
         // delta E_ij = -G_ij / 2, each G_ij of those a dot product of two
         // columns (the pairs are few: found per column against every
-        // eigenvalue, whatever their order).
+        // eigenvalue, whatever their order). Yt holds Y on entry and E on
+        // return, element v of lane group g at row rowf(v, g) of its tile.
         unsigned long long pm = 0;               // close partners of column j (lanes g4 == 0)
         if (jin && g4 == 0) {
             for (int i = 0; i < N; ++i)
                 if (i != j && fabs(r.lam[i] - lj) < delta) pm |= 1ull << i;
         }
-#pragma unroll
-        for (int rt = 0; rt < NB; ++rt) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = rt * 16 + g4 + 4 * v;
-                const real li = r.lam[i < N ? i : 0];
-                const real gap = lj - li;
-                cx e = tile_get(Y[rt], v) * rcp_nr(gap);
-                if (i == j) e = cm(real(0.5) * (real(1.0) - gj), 0.0);
-                else if (fabs(gap) < delta) e = cm(0.0, 0.0);    // set below
-                if (i >= N || !jin) e = cm(0.0, 0.0);
-                Y[rt].re[v] = e.x;
-                Y[rt].im[v] = e.y;
-            }
-        }
-        for (;;) {
-            const unsigned long long any = __ballot(pm != 0);
-            if (any == 0) break;
-            const int src = (int)__builtin_ctzll(any);
-            const int jj = __builtin_amdgcn_readlane(j, src);
-            const unsigned lo = __builtin_amdgcn_readlane((unsigned)pm, src);
-            const unsigned hi = __builtin_amdgcn_readlane((unsigned)(pm >> 32), src);
-            const int ii = lo ? (int)__builtin_ctz(lo) : 32 + (int)__builtin_ctz(hi);
-            if (l == src) pm &= pm - 1;
-            cx g = (l < N) ? cconjmul(r.U[ii * LDR + l], r.U[jj * LDR + l]) : cm(0.0, 0.0);
-            g = wave_sum_cx(g) * real(-0.5);
+        auto correction = [&](auto& Yt, auto rowf) {
 #pragma unroll
             for (int rt = 0; rt < NB; ++rt) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    if (j == jj && rt * 16 + g4 + 4 * v == ii) {
-                        Y[rt].re[v] = g.x;
-                        Y[rt].im[v] = g.y;
+                    const int i = rt * 16 + rowf(v, g4);
+                    const real li = r.lam[i < N ? i : 0];
+                    const real gap = lj - li;
+                    cx e = cm((real)Yt[rt].re[v], (real)Yt[rt].im[v]) * rcp_nr(gap);
+                    if (i == j) e = cm(real(0.5) * (real(1.0) - gj), 0.0);
+                    else if (fabs(gap) < delta) e = cm(0.0, 0.0);    // set below
+                    if (i >= N || !jin) e = cm(0.0, 0.0);
+                    Yt[rt].re[v] = e.x;
+                    Yt[rt].im[v] = e.y;
+                }
+            }
+            for (;;) {
+                const unsigned long long any = __ballot(pm != 0);
+                if (any == 0) break;
+                const int src = (int)__builtin_ctzll(any);
+                const int jj = __builtin_amdgcn_readlane(j, src);
+                const unsigned lo = __builtin_amdgcn_readlane((unsigned)pm, src);
+                const unsigned hi = __builtin_amdgcn_readlane((unsigned)(pm >> 32), src);
+                const int ii = lo ? (int)__builtin_ctz(lo) : 32 + (int)__builtin_ctz(hi);
+                if (l == src) pm &= pm - 1;
+                cx g = (l < N) ? cconjmul(r.U[ii * LDR + l], r.U[jj * LDR + l]) : cm(0.0, 0.0);
+                g = wave_sum_cx(g) * real(-0.5);
+#pragma unroll
+                for (int rt = 0; rt < NB; ++rt) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        if (j == jj && rt * 16 + rowf(v, g4) == ii) {
+                            Yt[rt].re[v] = g.x;
+                            Yt[rt].im[v] = g.y;
+                        }
                     }
                 }
             }
-        }
-        if (it == iters - 1) {
+        };
+        // The earlier steps take Y and V E on the FP32 matrix cores (their
+        // rounding is removed by the next step); the last one is FP64, since
+        // near the closeness threshold the FP32 rounding of Y over a gap of
+        // delta would stay in V (S-orthogonality 14.6 -> 251 N eps at N = 54).
+        if constexpr (!LAST) {
+            ftile Y[NB];
+            cgemm_reg_f32<true>(Y, fVh, X);
+            correction(Y, [](int v, int g) { return frow(v, g); });
+            ftile D[NB];
+            cgemm_reg_f32<false>(D, fV, Y);      // V E
+#pragma unroll
+            for (int rt = 0; rt < NB; ++rt) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = rt * 16 + frow(v, g4);
+                    const cx u = elem(r.U, LDR, i, j) + cm((real)D[rt].re[v], (real)D[rt].im[v]);
+                    X[rt].re[v] = u.x;
+                    X[rt].im[v] = u.y;
+                }
+            }
+        } else {
+            ctile Y[NB];
+            cgemm_reg(Y, fVh, X);
+            correction(Y, [](int v, int g) { return 4 * v + g; });
 #pragma unroll
             for (int rt = 0; rt < NB; ++rt)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) emax = fmax(emax, fmax(fabs(Y[rt].re[v]), fabs(Y[rt].im[v])));
-        }
-        cgemm_reg(X, fV, Y);                     // V E
+            cgemm_reg(X, fV, Y);                 // V E
 #pragma unroll
-        for (int rt = 0; rt < NB; ++rt) {
+            for (int rt = 0; rt < NB; ++rt) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = rt * 16 + g4 + 4 * v;
-                const cx u = elem(r.U, LDR, i, j) + tile_get(X[rt], v);
-                X[rt].re[v] = u.x;
-                X[rt].im[v] = u.y;
+                for (int v = 0; v < 4; ++v) {
+                    const int i = rt * 16 + g4 + 4 * v;
+                    const cx u = elem(r.U, LDR, i, j) + tile_get(X[rt], v);
+                    X[rt].re[v] = u.x;
+                    X[rt].im[v] = u.y;
+                }
             }
         }
         __syncthreads();     // every wave is done reading V
+        constexpr bool f32l = !LAST;             // X in the FP32 result layout
 #pragma unroll
         for (int rt = 0; rt < NB; ++rt) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int i = rt * 16 + g4 + 4 * v;
+                const int i = rt * 16 + (f32l ? frow(v, g4) : 4 * v + g4);
                 if (i < N && jin) r.U[j * LDR + i] = tile_get(X[rt], v);
             }
         }
         __syncthreads();
-    }
+    };
+    for (int it = 0; it + 1 < iters; ++it) step(it, BoolC<false>{});
+    if (iters > 0) step(iters - 1, BoolC<true>{});
     // Clusters: runs [c0, c1] with adjacent gaps below delta. Their bounds are
     // fixed first (bit i: gap i to i + 1 is close; every wave computes the same
     // mask before any wave changes lam), then cluster number nc goes to wave
