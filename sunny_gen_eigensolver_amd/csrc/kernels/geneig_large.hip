@@ -1859,11 +1859,19 @@ __device__ __forceinline__ void ritz_pair_g(const cx* __restrict__ M, cx* V, rea
 // and V slots alternately, P in the reflector slot, E in the caller's U.
 // Convergence test as in geneig_mx_refine: a last correction with an entry
 // above `tol` marks the matrix info = N.
+namespace gk {
+// LDS of geneig_lg_refine, padded to 56 KB so that at most two workgroups
+// share a CU: its products stream their operands through L2, and more
+// matrices per CU evict each other's. N = 96, batch 16384: 27.5 ms at four
+// workgroups per CU (126 VGPRs allow it), 24.4 ms at two, 38.7 ms at one.
+constexpr int REF_GNR = (56 * 1024) / (int)sizeof(real);
+}  // namespace gk
 extern "C" __global__ void __launch_bounds__(gk::LNT, 2)
 geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, int* __restrict__ info, char* __restrict__ ws,
                  long long wss, int batch, int iters, real delta_rel, real tol) {
     using namespace gk;
     __shared__ real lam[N];
+    __shared__ real gnr[REF_GNR > N ? REF_GNR : N];    // v_j^H v_j (padded, see REF_GNR)
     __shared__ real wmax[LNW];
     const int b = blockIdx.x;
     if (b >= batch || info[b] > N) return;
@@ -1915,8 +1923,12 @@ geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, int* __restrict__ inf
                     gj += cabs2(v);
                 }
             }
-            const real lj = wave_sum(tj) / wave_sum(gj);
-            if (lane == 0) lam[j] = lj;
+            const real gs = wave_sum(gj);
+            const real lj = wave_sum(tj) / gs;
+            if (lane == 0) {
+                lam[j] = lj;
+                gnr[j] = gs;
+            }
             mx = fmax(mx, fabs(lj));
         }
         if (lane == 0) wmax[wave] = mx;
@@ -1927,15 +1939,19 @@ geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, int* __restrict__ inf
         // The threshold doubles every step, so a pair treated as close once
         // stays close and ends in Rayleigh-Ritz.
         delta = delta_rel * (real)(1 << it) * mx;
-        // E from the tiles of T = V^H P and G = V^H V.
+        // E = (T - G diag(lambda)) / gap with T = V^H M V, G = V^H V, that is
+        // Y / gap with Y = V^H R and the residual R = P - V diag(lambda)
+        // (formed as the product's right operand): one product instead of
+        // T and G. The diagonal takes (1 - G_jj) / 2 and pairs closer than
+        // delta -G_ij / 2: those entries are written after the tiles, each
+        // G_ij a dot product of two columns.
         for (int t = wave; t < NRG * LNB; t += LNW) {
             const int rt0 = RRG * (t / LNB), ct = t % LNB;
-            ctile T[RRG], G[RRG];
-            gemm2_rg<RRG>(T, G, [&](int i, int k) { return cconj(gel(V, k, i)); },
-                        [&](int k, int j) { return gel(P, k, j); }, [&](int k, int j) { return gel(V, k, j); }, rt0,
-                        ct, 0, LKEND);
             const int j = ct * 16 + (lane & 15);
             const real lj = lam[j < N ? j : 0];
+            ctile Y[RRG];
+            gemm_rg<RRG>(Y, [&](int i, int k) { return cconj(gel(V, k, i)); },
+                         [&](int k, int jj) { return gel(P, k, jj) - gel(V, k, jj) * lj; }, rt0, ct, 0, LKEND);
 #pragma unroll
             for (int g = 0; g < RRG; ++g)
 #pragma unroll
@@ -1943,15 +1959,36 @@ geneig_lg_refine(real* __restrict__ W, cx* __restrict__ U, int* __restrict__ inf
                     const int i = tile_row(rt0 + g, v);
                     if (i < N && j < N) {
                         const real li = lam[i];
-                        const cx tv = tile_get(T[g], v), gv = tile_get(G[g], v);
-                        cx e = gv * real(-0.5);
                         const real gap = lj - li;
-                        if (i == j) e = cm(real(0.5) * (real(1.0) - gv.x), 0.0);
-                        else if (!(fabs(gap) < delta)) e = e + (tv - gv * (real(0.5) * (li + lj))) * rcp_nr(gap);
+                        cx e = tile_get(Y[g], v) * rcp_nr(gap);
+                        if (i == j) e = cm(real(0.5) * (real(1.0) - gnr[j]), 0.0);
+                        else if (fabs(gap) < delta) e = cm(0.0, 0.0);
                         if (it == iters - 1) emax = fmax(emax, fmax(fabs(e.x), fabs(e.y)));
                         E[(size_t)j * N + i] = e;
                     }
                 }
+        }
+        __syncthreads();
+        // Close pairs (i, j), column j by wave j % LNW: E_ij = -v_i^H v_j / 2.
+        for (int j = wave; j < N; j += LNW) {
+            const real lj = lam[j];
+            for (int i0 = 0; i0 < N; i0 += 64) {
+                const int ir = i0 + lane;
+                unsigned long long m = __ballot(ir < N && ir != j && fabs(lam[ir < N ? ir : 0] - lj) < delta);
+                while (m) {
+                    const int ii = i0 + (int)__builtin_ctzll(m);
+                    m &= m - 1;
+                    cx g = cm(0.0, 0.0);
+#pragma unroll
+                    for (int q = 0; q < ND; ++q) {
+                        const int r = 64 * q + lane;
+                        if (r < N) g = g + cconjmul(V[(size_t)ii * N + r], V[(size_t)j * N + r]);
+                    }
+                    g = wave_sum_cx(g) * real(-0.5);
+                    if (lane == 0) E[(size_t)j * N + ii] = g;
+                    if (it == iters - 1) emax = fmax(emax, fmax(fabs(g.x), fabs(g.y)));
+                }
+            }
         }
         __syncthreads();
         // V <- V + V E, into the other buffer.
