@@ -1053,12 +1053,14 @@ namespace gk {
 // the trailing lower triangle is updated in global memory once per panel.
 // Against updating it every step, this reads the active block once per step
 // instead of twice and writes it once per panel instead of once per step
-// (N = 192, batch 2048: 17.8 -> 13.7 ms). Where K0 is at most one panel the
-// steps update in place (N = 96, K0 = 8: 16.9 ms in place, 17.4 ms as one
-// panel, whose serial column and reflector the other waves wait for).
+// (N = 192, batch 2048: 17.8 -> 14.3 ms; N = 150, batch 4096: 20.5 ->
+// 19.2 ms). Where K0 is smaller the steps update in place, since each
+// panel step's serial column and reflector, which the other waves wait for,
+// outweigh the traffic saved (N = 128, batch 8192, K0 = 44: 19.7 ms in place,
+// 20.9 ms in panels; N = 96, K0 = 8: 16.9 and 17.4 ms).
 constexpr int HT_NBP_FIT = (NL * (NL + 1) / 2) / (2 * N);
 constexpr int HT_NBP = HT_NBP_FIT < 16 ? HT_NBP_FIT : 16;
-constexpr bool HT_PANELS = K0 > HT_NBP;
+constexpr bool HT_PANELS = K0 >= 64;
 
 __device__ __forceinline__ void ht_global_panels(HtWork& w, cx* __restrict__ Mg, cx* __restrict__ R,
                                                  real* __restrict__ dg, real* __restrict__ eg) {
