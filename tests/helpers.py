@@ -14,7 +14,7 @@ TOL_EIG = 1e-10        # reference bar (tests/static/test_full_n54.cpp:144-182)
 TOL_PHASE = 1e-10
 DEGEN_GAP = 1e-6
 RESID_FACTOR = 50.0    # bar: residual and S-orthogonality <= 50 N eps (1-norms)
-EIG_NORM_BAR = 1e-12   # north-star eigenvalue bar: |dlambda| <= 1e-12 ||H||_2 per matrix
+EIG_NORM_BAR = 1e-12   # eigenvalue bar of the benchmark's pencils: |dlambda| <= 1e-12 ||H||_2 per matrix
 
 
 def fixtures_index():
@@ -64,7 +64,7 @@ def norm2(A):
 
 def eig_abs_ratio(H, W, W_ref):
     """max over each matrix of |W - W_ref| / (1e-12 ||H||_2): <= 1 passes the
-    north-star eigenvalue bar."""
+    eigenvalue bar."""
     d = np.abs(np.asarray(W) - np.asarray(W_ref)).max(axis=1)
     return d / (EIG_NORM_BAR * norm2(H))
 
