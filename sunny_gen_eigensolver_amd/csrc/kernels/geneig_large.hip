@@ -286,11 +286,7 @@ __device__ __forceinline__ void back_wy(BackWork& bw, FV vget, FT tget, FZ zget,
 // kernels', so that each step's column loops are half as long per wave and
 // a CU holds twice the waves to hide their latency; four where the FP32
 // triangle and eight waves' partial sums would not fit LDS together (N = 192).
-#ifdef GENEIG_FP32
-constexpr int HT_NW = ((size_t)N * (N + 1) / 2 + (size_t)(2 + 1 + 8 + 1) * N) * sizeof(cx) + 64 > LDS_CAP ? 4 : 8;
-#else
 constexpr int HT_NW = 8;
-#endif
 constexpr int HT_NT = 64 * HT_NW;
 // Bytes beside the packed triangle: v (two steps), w, per-wave partial sums
 // of A v and the column sums of its upper part.
@@ -301,13 +297,9 @@ constexpr size_t HT_EXTRA = (size_t)(2 + 1 + HT_NW + 1) * N * sizeof(cx) + 64;
 // 16384 / 2048: N = 96 46.8 -> 26.6 ms (K0 = 5), N = 192 39.0 -> 25.3 ms
 // (K0 = 94) against the whole LDS at one workgroup per CU; a third of the
 // LDS (three workgroups) was slower (29.5 ms at N = 96). The FP32 build
-// keeps the whole triangle in LDS (its in-place global path would overwrite
-// the FP64 M the refinement still reads).
-#ifdef GENEIG_FP32
-constexpr size_t HT_CAP = LDS_CAP;
-#else
+// reduces its first K0 columns in an FP32 copy of M in the V slot (free
+// until V = QZ), since the refinement still reads the FP64 M.
 constexpr size_t HT_CAP = LDS_CAP / 2;
-#endif
 constexpr int ht_fit() {
     int nl = N;
     while (nl > 1 && (size_t)nl * (nl + 1) / 2 * sizeof(cx) + HT_EXTRA > HT_CAP) --nl;
@@ -315,9 +307,6 @@ constexpr int ht_fit() {
 }
 constexpr int NL = ht_fit();                  // order of the LDS-resident trailing triangle
 constexpr int K0 = N - NL;                    // columns reduced in global memory first
-#ifdef GENEIG_FP32
-static_assert(K0 == 0, "the FP32 triangle fits LDS up to N = 192");
-#endif
 
 struct HtWork {
     cx P[NL * (NL + 1) / 2];                  // lower triangle of rows/columns K0.., packed by columns
@@ -1041,9 +1030,8 @@ extern "C" __device__ const unsigned long long geneig_lg_ws_bytes = gk::WS_BYTES
 // Workgroup size of geneig_lg_hetrd, read by the host before launching it.
 extern "C" __device__ const int geneig_lg_hetrd_threads = gk::HT_NT;
 
-#ifndef GENEIG_FP32
 namespace gk {
-// Columns 0 .. K0-1 of the FP64 tridiagonalisation, whose trailing triangle
+// Columns 0 .. K0-1 of the tridiagonalisation, whose trailing triangle
 // does not fit LDS yet, in panels of up to HT_NBP columns with the updates
 // deferred (the panel step of LAPACK's blocked zhetrd): within a panel the
 // matrix in global memory keeps its value from the panel's start, the
@@ -1245,7 +1233,6 @@ __device__ __forceinline__ void ht_global_panels(HtWork& w, cx* __restrict__ Mg,
     }
 }
 }  // namespace gk
-#endif
 
 // Tridiagonalisation of M (read as FP64, reduced in the module precision):
 // d, e and the reflectors to the workspace.
@@ -1262,35 +1249,42 @@ geneig_lg_hetrd(const int* __restrict__ info, char* __restrict__ ws, long long w
     real* dg = reinterpret_cast<real*>(wsb + WS_D);
     real* eg = dg + N;
     const HtLds L{w.P};
-    auto load_trailing = [&]() {
+    auto load_trailing = [&](const auto* src) {
         for (int e = tid; e < (int)NN; e += HT_NT) {
             const int i = e % N, j = e / N;
-            if (j >= K0 && i >= j) w.P[pkl(j - K0) + i - j] = to_cx(Mg[e]);
+            if (j >= K0 && i >= j) w.P[pkl(j - K0) + i - j] = to_cx(src[e]);
         }
     };
-#ifndef GENEIG_FP32
     if constexpr (K0 > 0) {
-        // Columns 0 .. K0-1 in place in global memory (the FP64 M is not
-        // needed after this stage in the FP64 modes), then the trailing
-        // triangle to LDS.
+        // Columns 0 .. K0-1 in global memory, then the trailing triangle to
+        // LDS. FP64: in place (the FP64 modes do not need M after this
+        // stage); FP32: in an FP32 copy of M's lower triangle in the V slot.
+#ifdef GENEIG_FP32
+        cx* Gm = reinterpret_cast<cx*>(wsb + WS_V);
+        for (int e = tid; e < (int)NN; e += HT_NT) {
+            const int i = e % N, j = e / N;
+            if (i >= j) Gm[e] = to_cx(Mg[e]);
+        }
+        __syncthreads();
+#else
+        cx* Gm = reinterpret_cast<cx*>(wsb + WS_M);
+#endif
         if constexpr (HT_PANELS) {
-            ht_global_panels(w, reinterpret_cast<cx*>(wsb + WS_M), R, dg, eg);
-            load_trailing();
+            ht_global_panels(w, Gm, R, dg, eg);
+            load_trailing(Gm);
             __syncthreads();
             if (wave_id() == 0) ht_house(w, L, K0, R, dg, eg);
             __syncthreads();
         } else {
-            const HtGlb G{reinterpret_cast<cx*>(wsb + WS_M)};
+            const HtGlb G{Gm};
             if (wave_id() == 0) ht_house(w, G, 0, R, dg, eg);
             __syncthreads();
             for (int k = 0; k < K0; ++k) ht_step(w, G, k, R, dg, eg);
-            load_trailing();
+            load_trailing(Gm);
             __syncthreads();
         }
-    } else
-#endif
-    {
-        load_trailing();
+    } else {
+        load_trailing(Mg);
         __syncthreads();
         if (wave_id() == 0) ht_house(w, L, 0, R, dg, eg);
         __syncthreads();
